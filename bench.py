@@ -1,0 +1,172 @@
+"""Benchmark: SRHT sketch apply on one MI355X.
+
+Workload (one step = one pass of the hot path over one batch): C = S*A with
+S an SRHT (Left) recipe, A 65536 x 512 Float32 N(0,1) (seed 1234), 2048
+sampled rows, scale sqrt(2048/65536) as the reference stores it.  Inputs are
+resident in HBM; steps rotate over 4 copies of A and C (512 MiB + 16 MiB, more
+than the 256 MB Infinity Cache) so every step streams A from HBM.
+
+Prints one JSON line: value = algorithmic GB/s of the apply
+((65536*512 + 2048*512) * 4 bytes per step / time), with the roofline of the
+dominant kernel (timed with HIP events on the stream it runs on) and the CPU
+baseline (the oracle's C restatement, all host threads) beside it.  A second
+line-internal block reports the SparseSign 8192x256 Float64 apply.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+M, N, S_DIM = 65536, 512, 2048
+COPIES = 4
+
+
+def srht_bytes():
+    return (M * N + S_DIM * N) * 4
+
+
+def gpu_srht(rla, torch, steps, warmup):
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(1234)
+    As = [torch.randn(N, M, generator=g, device=dev, dtype=torch.float32).T
+          for _ in range(COPIES)]  # column-major M x N
+    Cs = [torch.zeros(N, S_DIM, device=dev, dtype=torch.float32).T for _ in range(COPIES)]
+    rla.seed(1234)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=S_DIM, type=torch.float32), As[0])
+    for i in range(warmup):
+        rla.mul(Cs[i % COPIES], S, As[i % COPIES])
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(steps):
+        rla.mul(Cs[i % COPIES], S, As[i % COPIES])
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    # parity spot check of the last apply against the oracle (first 4 columns)
+    import numpy as np
+    from oracle import restate as R
+    k = (steps - 1) % COPIES
+    A4 = As[k][:, :4].cpu().numpy()
+    want = R.srht_left_mul(A4, S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float32)
+    exact = bool(np.array_equal(Cs[k][:, :4].cpu().numpy(), want))
+    return ms, wall, exact, S, As[0]
+
+
+def gpu_sparse_sign(rla, torch, steps, warmup):
+    """C1: SparseSign 8192x256 Float64 -> 1024x256, nnz 8, rotating 32 copies."""
+    dev = torch.device("cuda", 0)
+    m, n, s, copies = 8192, 256, 1024, 32
+    g = torch.Generator(device=dev).manual_seed(1234)
+    As = [torch.randn(n, m, generator=g, device=dev, dtype=torch.float64).T
+          for _ in range(copies)]
+    Cs = [torch.zeros(n, s, device=dev, dtype=torch.float64).T for _ in range(copies)]
+    rla.seed(1234)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=s, nnz=8), As[0])
+    for i in range(warmup):
+        rla.mul(Cs[i % copies], S, As[i % copies])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(steps, copies)
+    e0.record()
+    for i in range(reps):
+        rla.mul(Cs[i % copies], S, As[i % copies])
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    byts = (m * n + s * n) * 8
+    return {"workload": "SparseSign left apply 8192x256 f64 -> 1024x256, nnz 8",
+            "ms_per_apply": ms, "GB_s": byts / ms / 1e6,
+            "frac_hbm_peak": byts / ms / 1e6 / HBM_PEAK_GBS, "rotating_copies": copies}
+
+
+def cpu_baseline(S, A0):
+    """Oracle C restatement (OpenMP over columns, all host threads) on the full
+    workload: the same A, op, signs and scale."""
+    import ctypes
+
+    import numpy as np
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    vp, i64, f64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    lib.oracle_srht_left_f32.argtypes = [vp, i64, i64, i64, i64, i64, vp, vp, f64, vp, i64, f64,
+                                         f64, i32]
+    lib.oracle_max_threads.restype = ctypes.c_int
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    A = np.asfortranarray(A0.cpu().numpy())
+    op = S.op.cpu().numpy().astype(np.int32)
+    ch = S.signs.cpu().numpy().view(np.uint64).copy()
+    C = np.zeros((S_DIM, N), dtype=np.float32, order="F")
+    reps, t = 0, 0.0
+    while t < 10.0 and reps < 50:
+        t0 = time.perf_counter()
+        lib.oracle_srht_left_f32(A.ctypes.data, M, N, M, M, S_DIM, op.ctypes.data, ch.ctypes.data,
+                                 S.scale, C.ctypes.data, S_DIM, 1.0, 0.0, threads)
+        t += time.perf_counter() - t0
+        reps += 1
+    per = t / reps
+    return {"value": srht_bytes() / per / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full workload (65536x512 f32 SRHT apply), {reps} reps, "
+                      f"{per * 1e3:.1f} ms each, C restatement of fwht.jl/srht.jl, OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the SRHT apply has no multi-GPU path in this build "
+              "(columns shard independently; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__
+    rla = __graft_entry__.load_package()
+    ms, wall, exact, S, A0 = gpu_srht(rla, torch, args.steps, args.warmup)
+    gbs = srht_bytes() / ms / 1e6
+    extra = gpu_sparse_sign(rla, torch, args.steps, args.warmup)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "srht_f32_65536x512_traffic.txt")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
+    out = {
+        "metric": "sketch GB/s (% HBM peak)",
+        "value": gbs,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic N(0,1), seed 1234",
+        "config": {"workload": "SRHT/FJLT left apply: A 65536x512 f32 -> 2048x512 "
+                               "(Rademacher signs, FWHT, 2048 sampled rows)",
+                   "rotating_copies": COPIES, "bit_exact_vs_oracle": exact,
+                   "wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": srht_bytes()},
+        "sparse_sign_c1": extra,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(S, A0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * rla.h — C ABI of librla_mi355x.so, the gfx950 (MI355X) kernels behind the
+ * RandLinearAlgebra.jl compressor / approximator / solver surface.
+ *
+ * Conventions (all entry points):
+ *   - Return an int status (RLA_OK on success).  On failure a thread-local
+ *     message is available from rla_last_error().  Status codes map to the
+ *     reference's exception classes: RLA_EARG -> ArgumentError,
+ *     RLA_EDIM -> DimensionMismatch, RLA_EDOMAIN -> DomainError.
+ *   - Matrices are passed as (pointer, row stride, column stride) in elements,
+ *     so a Julia column-major matrix is (p, 1, ld) and its transpose/adjoint
+ *     view is (p, ld, 1).  Pointers are device pointers unless the name says
+ *     "host".
+ *   - Index arrays are 0-based int32 on the device.  Julia's 1-based Int64
+ *     arrays are converted at the boundary with rla_index_from_julia, which
+ *     range-checks them.
+ *   - Work is enqueued on the context's stream; nothing blocks unless the
+ *     entry point says so.  Reads of A are borrowed, never retained.
+ *   - One context per host thread: a context owns a stream and a workspace,
+ *     and two calls that share a context must not overlap.
+ */
+#ifndef RLA_MI355X_H
+#define RLA_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RLA_OK = 0,
+  RLA_EARG = 1,     /* ArgumentError: bad parameter / unsupported combination */
+  RLA_EDIM = 2,     /* DimensionMismatch (messages mirror src/helpers/mul_dim_checks.jl:41-159) */
+  RLA_EDOMAIN = 3,  /* DomainError */
+  RLA_ENOMEM = 4,   /* device allocation failed */
+  RLA_EADMIT = 5,   /* request over the context's HBM budget */
+  RLA_EHIP = 6,     /* HIP runtime error */
+  RLA_ENOTSUP = 7   /* size outside what the kernels support */
+};
+
+typedef struct rla_ctx rla_ctx;
+
+/* ---- context --------------------------------------------------------- */
+const char* rla_version(void);
+const char* rla_last_error(void);
+/* Work goes to hip_stream; NULL is the device's null stream (what torch's
+ * default stream is). */
+int rla_ctx_create(int device, void* hip_stream, rla_ctx** out);
+int rla_ctx_destroy(rla_ctx* ctx);
+int rla_ctx_set_stream(rla_ctx* ctx, void* hip_stream);
+int rla_ctx_sync(rla_ctx* ctx);
+/* Bytes of device workspace currently held by the context. */
+int rla_ctx_workspace_bytes(rla_ctx* ctx, size_t* out);
+
+/* ---- counter-based generator ------------------------------------------
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Host evaluation, for
+ * known-answer tests.  Every index, sign and Gaussian draw of the recipe
+ * generators below is a pure function of (seed, epoch, item), identical on
+ * host and device.  Replaces the Random/StatsBase draws at
+ * src/Compressors/sparse_sign.jl:167,252,317, count_sketch.jl:124-125,160-161,
+ * srht.jl:150-151,207-208, gaussian.jl:107,122,138. */
+void rla_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Julia 1-based Int64 index vector -> 0-based int32 (device), each entry
+ * checked to lie in 1:maxval (RLA_EARG otherwise; blocks to read the check). */
+int rla_index_from_julia(rla_ctx* ctx, const int64_t* idx1, int64_t n, int64_t maxval,
+                         int32_t* idx0);
+
+/* SparseSign recipe: for each of n_groups groups, nnz distinct sorted rows
+ * from 0:s-1 (StatsBase sample!(..., replace=false, ordered=true), as in
+ * sparse_idx_update!, src/Compressors/sparse_sign.jl:155-172) and values
+ * +-1/sqrt(nnz) in the element type (sparse_sign.jl:248-252).
+ * idx and val hold n_groups*nnz entries, group-major (CSC column order). */
+int rla_sparse_sign_gen_f32(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n_groups,
+                            int64_t s, int64_t nnz, int32_t* idx, float* val);
+int rla_sparse_sign_gen_f64(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n_groups,
+                            int64_t s, int64_t nnz, int32_t* idx, double* val);
+int rla_host_sparse_sign_gen_f64(uint64_t seed, uint64_t epoch, int64_t n_groups, int64_t s,
+                                 int64_t nnz, int32_t* idx_host, double* val_host);
+
+/* CountSketch recipe: one row uniform in 0:s-1 and one sign +-1 per group
+ * (src/Compressors/count_sketch.jl:124-127,160-161). */
+int rla_count_sketch_gen_f32(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n_groups,
+                             int64_t s, int32_t* idx, float* val);
+int rla_count_sketch_gen_f64(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n_groups,
+                             int64_t s, int32_t* idx, double* val);
+int rla_host_count_sketch_gen_f64(uint64_t seed, uint64_t epoch, int64_t n_groups, int64_t s,
+                                  int32_t* idx_host, double* val_host);
+
+/* SRHT/FJLT recipe: sign bits for n_pad entries in Julia BitVector chunk
+ * layout (bit i of chunk i>>6, LSB first; srht.jl:150 bitrand) and, when
+ * op != NULL, s distinct sampled rows of 0:n_pad-1 in random order
+ * (srht.jl:151, sample(1:n_pad, s, replace=false)). */
+int rla_srht_gen(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n_pad, int64_t s,
+                 int32_t* op, uint64_t* sign_chunks);
+int rla_host_srht_gen(uint64_t seed, uint64_t epoch, int64_t n_pad, int64_t s,
+                      int32_t* op_host, uint64_t* sign_chunks_host);
+
+/* Gaussian recipe: out[i + j*ld] = scale * N(0,1) draw (gaussian.jl:107,122). */
+int rla_gaussian_gen_f32(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t rows,
+                         int64_t cols, double scale, float* out, int64_t ld);
+int rla_gaussian_gen_f64(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t rows,
+                         int64_t cols, double scale, double* out, int64_t ld);
+int rla_host_gaussian_gen_f64(uint64_t seed, uint64_t epoch, int64_t rows, int64_t cols,
+                              double scale, double* out_host, int64_t ld);
+
+/* ---- sparse compressors (SparseSign, CountSketch, FJLT's K) ------------
+ * M is an nrows x ncols CSC matrix: colptr[ncols+1], rowidx[nnz], val[nnz],
+ * rows sorted within each column.  rla_sparse_transpose builds its CSR twin
+ * (rowptr[nrows+1], colidx[nnz] ascending within each row, val[nnz]);
+ * deterministic.  Replaces the SparseMatrixCSC the recipes hold
+ * (sparse_sign.jl:219-295, count_sketch.jl:105-147). */
+int rla_sparse_transpose_f32(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                             const int32_t* colptr, const int32_t* rowidx, const float* val,
+                             int32_t* rowptr, int32_t* colidx, float* val_t);
+int rla_sparse_transpose_f64(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                             const int32_t* colptr, const int32_t* rowidx, const double* val,
+                             int32_t* rowptr, int32_t* colidx, double* val_t);
+
+/* C = alpha * op(M) * X + beta * C, op(M) = M (trans_m = 0) or M' (trans_m = 1).
+ * X is op(M)-cols x ncols, C is op(M)-rows x ncols, both strided.  Every
+ * orientation of the reference's sparse mul! reduces to this one by taking
+ * transposed views:  A*M = (M'*A')',  A*M' = (M*A')'.
+ * Replaces mul! at sparse_sign.jl:339-361 and count_sketch.jl:166-213 (and
+ * through them the SparseArrays sparse x dense kernels).  Accumulation per
+ * output entry: c = beta*c (or 0 when beta == 0), then c += v*(x*alpha) over
+ * the nonzeros in ascending index order — the SparseArrays loop order. */
+int rla_sparse_apply_f32(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                         const int32_t* colptr, const int32_t* rowidx, const float* val,
+                         const int32_t* rowptr, const int32_t* colidx, const float* val_t,
+                         int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs,
+                         float* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
+int rla_sparse_apply_f64(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                         const int32_t* colptr, const int32_t* rowidx, const double* val,
+                         const int32_t* rowptr, const int32_t* colidx, const double* val_t,
+                         int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
+                         double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
+
+/* ---- Walsh–Hadamard / SRHT / FJLT -------------------------------------
+ * In-place FWHT of each of ncols columns of X (length n, a power of two),
+ * natural (Sylvester) order, radix-2 stages h = 1, 2, 4, ... exactly as
+ * fwht!(x, signs; scaling) / fwht!(x; scaling)
+ * (src/Compressors/helpers/fwht.jl:21-118): the first stage multiplies each
+ * entry by +-scaling (signs == NULL: +scaling).  Bit-identical to that loop. */
+int rla_fwht_f32(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_chunks,
+                 double scaling, float* X, int64_t ldx);
+int rla_fwht_f64(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_chunks,
+                 double scaling, double* X, int64_t ldx);
+
+/* SRHT operator T = (scale * H * D * pad)[op, :] (s x m, H the n_pad-point
+ * unnormalised Hadamard, D = diag(+-1) from sign_chunks, pad = zero-padding
+ * of m rows to n_pad).
+ *   adjoint = 0:  Y = alpha * T * X + beta * Y      (X m x ncols, Y s x ncols)
+ *   adjoint = 1:  Y = alpha * T' * X + beta * Y     (X s x ncols, Y m x ncols)
+ * Left S*A is adjoint 0 on A; Right A*S is adjoint 0 on A' (Y = C');
+ * Left A*S is adjoint 1 on A'; Right S*A is adjoint 1 on A.
+ * Replaces mul! at src/Compressors/srht.jl:220-504.  scale is the recipe's
+ * stored Float64 (reference: type(sqrt(s)/sqrt(n_pad)), srht.jl:152). */
+int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                       double scale, const float* X, int64_t x_rs, int64_t x_cs, float* Y,
+                       int64_t y_rs, int64_t y_cs, double alpha, double beta);
+int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                       double scale, const double* X, int64_t x_rs, int64_t x_cs, double* Y,
+                       int64_t y_rs, int64_t y_cs, double alpha, double beta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RLA_MI355X_H */

@@ -1,0 +1,285 @@
+"""CPU restatement of RandLinearAlgebra.jl's hot-path arithmetic — the oracle.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may use this module, and only as the checker.
+The product path (randlinearalgebra.jl_amd/) never imports it.
+
+Each function restates a reference loop, citing the file:line it follows
+(paths relative to the RandLinearAlgebra.jl repository).  Where the reference
+arithmetic is fixed by its code (the FWHT stage order, Julia's Float64
+promotion of alpha/beta/scaling), the restatement reproduces it operation for
+operation, so results can be compared bit for bit.  Pinning (see the tests in
+tests/test_oracle.py): every function is checked against a closed form or a
+mathematical invariant that a dropped term, wrong sign or transposed operand
+would break (dense Hadamard / sparse products, self-inverse FWHT, Philox
+known-answer vectors, Floyd/Lemire distribution checks).
+"""
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Philox4x32-10 (Salmon et al., SC'11) and the draws built on it
+
+
+M0, M1 = 0xD2511F53, 0xCD9E8D57
+W0, W1 = 0x9E3779B9, 0xBB67AE85
+MASK = 0xFFFFFFFF
+
+
+def philox4x32_10(ctr, key):
+    c0, c1, c2, c3 = ctr
+    k0, k1 = key
+    for _ in range(10):
+        p0 = M0 * c0
+        p1 = M1 * c2
+        hi0, lo0 = p0 >> 32, p0 & MASK
+        hi1, lo1 = p1 >> 32, p1 & MASK
+        c0, c1, c2, c3 = (hi1 ^ c1 ^ k0) & MASK, lo1, (hi0 ^ c3 ^ k1) & MASK, lo0
+        k0, k1 = (k0 + W0) & MASK, (k1 + W1) & MASK
+    return [c0, c1, c2, c3]
+
+
+DOM_INDEX, DOM_SIGN, DOM_GAUSS, DOM_PERM = 1, 2, 3, 4
+
+
+class Stream:
+    """32-bit word stream for (seed, epoch, domain, item): counter
+    {block, item_lo, item_hi ^ (domain << 24), epoch_lo ^ epoch_hi}."""
+
+    def __init__(self, seed, epoch, domain, item):
+        self.key = [seed & MASK, (seed >> 32) & MASK]
+        self.ctr = [0, item & MASK, ((item >> 32) ^ (domain << 24)) & MASK,
+                    (epoch & MASK) ^ ((epoch >> 32) & MASK)]
+        self.buf = None
+        self.pos = 4
+
+    def next(self):
+        if self.pos == 4:
+            self.buf = philox4x32_10(self.ctr, self.key)
+            self.ctr[0] = (self.ctr[0] + 1) & MASK
+            self.pos = 0
+        v = self.buf[self.pos]
+        self.pos += 1
+        return v
+
+    def below(self, n):
+        """Lemire's multiply-and-reject uniform integer in [0, n)."""
+        x = self.next()
+        m = x * n
+        lo = m & MASK
+        if lo < n:
+            t = ((1 << 32) - n) % n
+            while lo < t:
+                x = self.next()
+                m = x * n
+                lo = m & MASK
+        return m >> 32
+
+    def open01(self):
+        a = self.next() >> 5
+        b = self.next() >> 6
+        return ((a * 67108864.0 + b) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def sign_word(seed, epoch, word):
+    return Stream(seed, epoch, DOM_SIGN, word).next()
+
+
+def floyd_sorted(st, n, k):
+    """Floyd's uniform k-subset of [0, n), kept sorted (StatsBase
+    sample!(..., replace=false, ordered=true) as called at
+    src/Compressors/sparse_sign.jl:167)."""
+    out = []
+    for t in range(n - k, n):
+        v = st.below(t + 1)
+        if v in out:
+            v = t
+        out.append(v)
+        out.sort()
+    return out
+
+
+def sparse_sign_gen(seed, epoch, n_groups, s, nnz):
+    """sparse_idx_update! (sparse_sign.jl:155-172) + rand([-sc, sc])
+    (sparse_sign.jl:248-252): 0-based rows and signed values."""
+    idx = np.zeros(n_groups * nnz, dtype=np.int32)
+    val = np.zeros(n_groups * nnz)
+    sc = 1.0 / math.sqrt(nnz)
+    for j in range(n_groups):
+        idx[j * nnz:(j + 1) * nnz] = floyd_sorted(Stream(seed, epoch, DOM_INDEX, j), s, nnz)
+        for q in range(nnz):
+            e = j * nnz + q
+            w = sign_word(seed, epoch, e >> 5)
+            val[e] = sc if (w >> (e & 31)) & 1 else -sc
+    return idx, val
+
+
+def count_sketch_gen(seed, epoch, n_groups, s):
+    """count_sketch.jl:124-127: groups = rand(1:s), signs = rand([-1, 1])."""
+    idx = np.zeros(n_groups, dtype=np.int32)
+    val = np.zeros(n_groups)
+    for j in range(n_groups):
+        idx[j] = Stream(seed, epoch, DOM_INDEX, j).below(s)
+        w = sign_word(seed, epoch, j >> 5)
+        val[j] = 1.0 if (w >> (j & 31)) & 1 else -1.0
+    return idx, val
+
+
+def srht_gen(seed, epoch, n_pad, s):
+    """srht.jl:150-151: bitrand(n_pad) and sample(1:n_pad, s, replace=false)
+    (Floyd's subset, then a Fisher–Yates shuffle for a uniform order)."""
+    nchunks = (n_pad + 63) // 64
+    signs = np.zeros(n_pad, dtype=bool)
+    for c in range(nchunks):
+        lo = sign_word(seed, epoch, 2 * c)
+        hi = sign_word(seed, epoch, 2 * c + 1)
+        ch = lo | (hi << 32)
+        for b in range(64):
+            i = 64 * c + b
+            if i < n_pad:
+                signs[i] = bool((ch >> b) & 1)
+    st = Stream(seed, epoch, DOM_INDEX, 0)
+    chosen = set()
+    op = []
+    for t in range(n_pad - s, n_pad):
+        v = st.below(t + 1)
+        if v in chosen:
+            v = t
+        chosen.add(v)
+        op.append(v)
+    sp = Stream(seed, epoch, DOM_PERM, 0)
+    for i in range(s, 1, -1):
+        j = sp.below(i)
+        op[i - 1], op[j] = op[j], op[i - 1]
+    return np.array(op, dtype=np.int32), signs
+
+
+# ---------------------------------------------------------------------------
+# Walsh–Hadamard transform: src/Compressors/helpers/fwht.jl:21-118
+
+
+def fwht(x, signs=None, scaling=1.0, dtype=None):
+    """In-place FWHT of the columns of x (n x k or length n), natural order,
+    stages h = 1, 2, 4, ... as fwht.jl:21-70.  Stage 1 evaluates
+    x[i] * (signs[i] ? scaling : -scaling) and the sums in Float64 (scaling is
+    Float64 in the reference), the rest in the element type."""
+    x = np.asarray(x)
+    dtype = x.dtype if dtype is None else np.dtype(dtype)
+    two_d = x.ndim == 2
+    X = x.reshape(x.shape[0], -1).astype(dtype, copy=True)
+    n = X.shape[0]
+    if n < 2 or (n & (n - 1)):
+        raise ValueError("Size of vector must be power of 2.")
+    if signs is None:
+        s = np.full(n, float(scaling))
+    else:
+        signs = np.asarray(signs, dtype=bool)
+        if signs.shape[0] != n:
+            raise ValueError("Vector `x` and vector `signs` must be same length.")
+        s = np.where(signs, float(scaling), -float(scaling))
+    z = X[0::2].astype(np.float64) * s[0::2, None]
+    y = X[1::2].astype(np.float64) * s[1::2, None]
+    X[0::2] = (z + y).astype(dtype)
+    X[1::2] = (z - y).astype(dtype)
+    h = 2
+    while h < n:
+        V = X.reshape(n // (2 * h), 2, h, -1)
+        a = V[:, 0].copy()
+        b = V[:, 1].copy()
+        V[:, 0] = a + b
+        V[:, 1] = a - b
+        h *= 2
+    return X if two_d else X[:, 0]
+
+
+def padded_size(n):
+    """srht.jl:147 Int64(2^(ceil(log(2, n))))."""
+    return int(2.0 ** math.ceil(math.log(float(n)) / math.log(2.0)))
+
+
+def srht_left_mul(A, op, signs, scale, dtype, C=None, alpha=1.0, beta=0.0):
+    """mul!(C, S::SRHTRecipe{Left}, A, alpha, beta) — srht.jl:220-286: zero
+    padding, fwht!(col, signs; scaling=scale) per column, then
+    Cv .= beta .* Cv .+ alpha .* padding[op, :] (Float64 broadcast)."""
+    A = np.asarray(A)
+    m = A.shape[0]
+    k = A.shape[1] if A.ndim == 2 else 1
+    n_pad = np.asarray(signs).shape[0]
+    P = np.zeros((n_pad, k), dtype=dtype)
+    P[:m] = A.reshape(m, k)
+    P = fwht(P, signs, scale, dtype)
+    G = P[np.asarray(op)].astype(np.float64)
+    Cd = np.zeros_like(G) if C is None else np.asarray(C, dtype=np.float64).reshape(G.shape)
+    return (beta * Cd + alpha * G).astype(dtype)
+
+
+def srht_adjoint_mul(X, op, signs, scale, m, dtype, Y=None, alpha=1.0, beta=0.0):
+    """Y = alpha*T'*X + beta*Y with T = (scale*H*D*pad)[op, :] — the A*S
+    orientation of a Left recipe (srht.jl:289-363) on A' (and S*A of a Right
+    recipe, srht.jl:432-504): padding[op, :] = alpha .* X, fwht!(col;
+    scaling=scale) without signs, rows multiplied by +-1, first m rows,
+    axpby!(1, P, beta, Y)."""
+    X = np.asarray(X)
+    s = X.shape[0]
+    k = X.shape[1] if X.ndim == 2 else 1
+    n_pad = np.asarray(signs).shape[0]
+    P = np.zeros((n_pad, k), dtype=dtype)
+    P[np.asarray(op)] = (alpha * X.reshape(s, k).astype(np.float64)).astype(dtype)
+    P = fwht(P, None, scale, dtype)
+    sg = np.where(np.asarray(signs, dtype=bool), 1, -1).astype(dtype)
+    P = P * sg[:, None]
+    Yd = np.zeros((m, k)) if Y is None else np.asarray(Y, dtype=np.float64).reshape(m, k)
+    out = P[:m].astype(np.float64) if beta == 0.0 else P[:m].astype(np.float64) + beta * Yd
+    return out.astype(dtype)
+
+
+# ---------------------------------------------------------------------------
+# sparse x dense: the SparseArrays loop behind sparse_sign.jl:347,359 and
+# count_sketch.jl:174-211
+
+
+def csr_from_dense(M):
+    """CSR of a dense matrix with column indices ascending within each row."""
+    M = np.asarray(M)
+    ptr = [0]
+    idx, val = [], []
+    for i in range(M.shape[0]):
+        nz = np.nonzero(M[i])[0]
+        idx.extend(nz.tolist())
+        val.extend(M[i, nz].tolist())
+        ptr.append(len(idx))
+    return np.array(ptr, dtype=np.int64), np.array(idx, dtype=np.int64), np.array(val)
+
+
+def sparse_rowgather(ptr, idx, val, X, alpha=1.0, beta=0.0, C=None, dtype=None):
+    """C = alpha*op(M)*X + beta*C where row i of op(M) has nonzeros
+    (idx[p], val[p]) for p in ptr[i]:ptr[i+1].  Per output entry, as in
+    SparseArrays' `_spmatmul!`: C is first filled with 0 (beta == 0) or scaled
+    by beta, then C[i,k] += nzv*(X[j,k]*alpha) for the nonzeros in ascending
+    j, each step in Float64 (alpha is a Float64) rounded to the element type."""
+    X = np.asarray(X)
+    dtype = X.dtype if dtype is None else np.dtype(dtype)
+    X2 = X.reshape(X.shape[0], -1)
+    nout = len(ptr) - 1
+    k = X2.shape[1]
+    if beta == 0.0 or C is None:
+        acc = np.zeros((nout, k), dtype=dtype)
+    else:
+        acc = (beta * np.asarray(C, dtype=np.float64).reshape(nout, k)).astype(dtype)
+    ptr = np.asarray(ptr)
+    lens = np.diff(ptr)
+    Xd = X2.astype(dtype).astype(np.float64) * alpha
+    valv = np.asarray(val).astype(dtype).astype(np.float64)
+    for q in range(int(lens.max()) if nout else 0):
+        rows = np.nonzero(lens > q)[0]
+        p = ptr[rows] + q
+        term = valv[p][:, None] * Xd[np.asarray(idx)[p]]
+        acc[rows] = (acc[rows].astype(np.float64) + term).astype(dtype)
+    return acc if X.ndim == 2 else acc[:, 0]
+
+
+def sparse_mul(M, X, alpha=1.0, beta=0.0, C=None, dtype=None):
+    """C = alpha*M*X + beta*C for a dense-stored sparse M, SparseArrays order."""
+    ptr, idx, val = csr_from_dense(M)
+    return sparse_rowgather(ptr, idx, val, X, alpha, beta, C, dtype)

@@ -1,0 +1,19 @@
+"""MI355X (gfx950) implementation of RandLinearAlgebra.jl's data-parallel hot
+path: compressor application (SparseSign / CountSketch, SRHT / FJLT,
+Gaussian), the RangeFinder / RandSVD loop and block Kaczmarz, behind the
+reference's complete_compressor / update_compressor! / mul! / rapproximate /
+rsolve! surface.
+
+The directory name contains a dot, so load it with
+`__graft_entry__.load_package()` (it registers the module as
+`randlinearalgebra_amd`).
+"""
+from ._lib import (AdmissionError, ArgumentError, DeviceError, DimensionMismatch,  # noqa: F401
+                   DomainError)
+from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F401
+                          CompressorRecipe, CountSketch, CountSketchRecipe, Left, Right, SRHT,
+                          SRHTRecipe, SparseSign, SparseSignRecipe, Undef, View, adjoint,
+                          as_view, complete_compressor, fwht, left_mul_dimcheck, mul, mul_,
+                          pack_bits, padded_size, right_mul_dimcheck, seed, times,
+                          update_compressor, update_compressor_)
+from . import _lib  # noqa: F401

@@ -1,0 +1,181 @@
+"""ctypes binding of librla_mi355x.so (the C ABI declared in include/rla.h).
+
+The host mirror calls the library only through this module.  There is no
+CPU fallback: if the shared library is missing, or no GPU is visible, calls
+raise instead of computing anything on the host.
+"""
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librla_mi355x.so")
+
+RLA_OK, RLA_EARG, RLA_EDIM, RLA_EDOMAIN, RLA_ENOMEM, RLA_EADMIT, RLA_EHIP, RLA_ENOTSUP = range(8)
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError (bad parameter or unsupported combination)."""
+
+
+class DimensionMismatch(ValueError):
+    """Julia's DimensionMismatch."""
+
+
+class DomainError(ValueError):
+    """Julia's DomainError."""
+
+
+class AdmissionError(RuntimeError):
+    """Request exceeds the device-memory budget."""
+
+
+class DeviceError(RuntimeError):
+    """HIP runtime failure or device allocation failure."""
+
+
+_ERR = {
+    RLA_EARG: ArgumentError,
+    RLA_EDIM: DimensionMismatch,
+    RLA_EDOMAIN: DomainError,
+    RLA_ENOMEM: DeviceError,
+    RLA_EADMIT: AdmissionError,
+    RLA_EHIP: DeviceError,
+    RLA_ENOTSUP: ArgumentError,
+}
+
+i32, i64, u64, f64, vp = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double,
+                          ctypes.c_void_p)
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+_SIGS = {
+    "rla_version": [],
+    "rla_last_error": [],
+    "rla_ctx_create": [i32, vp, vp],
+    "rla_ctx_destroy": [vp],
+    "rla_ctx_set_stream": [vp, vp],
+    "rla_ctx_sync": [vp],
+    "rla_ctx_workspace_bytes": [vp, vp],
+    "rla_philox4x32_10": [vp, vp, vp],
+    "rla_index_from_julia": [vp, vp, i64, i64, vp],
+    "rla_sparse_sign_gen_f32": [vp, u64, u64, i64, i64, i64, vp, vp],
+    "rla_sparse_sign_gen_f64": [vp, u64, u64, i64, i64, i64, vp, vp],
+    "rla_host_sparse_sign_gen_f64": [u64, u64, i64, i64, i64, vp, vp],
+    "rla_count_sketch_gen_f32": [vp, u64, u64, i64, i64, vp, vp],
+    "rla_count_sketch_gen_f64": [vp, u64, u64, i64, i64, vp, vp],
+    "rla_host_count_sketch_gen_f64": [u64, u64, i64, i64, vp, vp],
+    "rla_srht_gen": [vp, u64, u64, i64, i64, vp, vp],
+    "rla_host_srht_gen": [u64, u64, i64, i64, vp, vp],
+    "rla_gaussian_gen_f32": [vp, u64, u64, i64, i64, f64, vp, i64],
+    "rla_gaussian_gen_f64": [vp, u64, u64, i64, i64, f64, vp, i64],
+    "rla_host_gaussian_gen_f64": [u64, u64, i64, i64, f64, vp, i64],
+    "rla_sparse_transpose_f32": [vp, i64, i64, i64, vp, vp, vp, vp, vp, vp],
+    "rla_sparse_transpose_f64": [vp, i64, i64, i64, vp, vp, vp, vp, vp, vp],
+    "rla_sparse_apply_f32": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
+                             i64, i64, f64, f64],
+    "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
+                             i64, i64, f64, f64],
+    "rla_fwht_f32": [vp, i64, i64, vp, f64, vp, i64],
+    "rla_fwht_f64": [vp, i64, i64, vp, f64, vp, i64],
+    "rla_srht_apply_f32": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, i64, i64, vp, i64, i64,
+                           f64, f64],
+    "rla_srht_apply_f64": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, i64, i64, vp, i64, i64,
+                           f64, f64],
+}
+_RESTYPE = {"rla_version": ctypes.c_char_p, "rla_last_error": ctypes.c_char_p,
+            "rla_philox4x32_10": None}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def exported_symbols():
+    """Names of every entry point the host mirror binds (all declared in rla.h)."""
+    return sorted(_SIGS)
+
+
+def load():
+    """Loads and binds the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                    "(make -C randlinearalgebra.jl_amd); there is no CPU fallback")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, args in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPE.get(name, ctypes.c_int)
+            _lib = lib
+    return _lib
+
+
+def last_error():
+    return load().rla_last_error().decode()
+
+
+def check(rc, what=""):
+    if rc != RLA_OK:
+        msg = last_error()
+        raise _ERR.get(rc, RuntimeError)(msg if msg else f"{what} failed with status {rc}")
+
+
+class _Ctx:
+    def __init__(self, device):
+        self.device = device
+        self.handle = ctypes.c_void_p()
+        self.stream = None
+
+    def bind(self):
+        lib = load()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        if not self.handle:
+            check(lib.rla_ctx_create(self.device, ctypes.c_void_p(stream),
+                                     ctypes.byref(self.handle)), "rla_ctx_create")
+        elif stream != self.stream:
+            check(lib.rla_ctx_set_stream(self.handle, ctypes.c_void_p(stream)))
+        self.stream = stream
+        return self.handle
+
+
+_ctxs = {}
+
+
+def ctx(device=None):
+    """The calling thread's context for `device`, bound to torch's current stream."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the rla kernels run on the device only")
+    if device is None:
+        device = torch.cuda.current_device()
+    elif isinstance(device, torch.device):
+        device = device.index if device.index is not None else torch.cuda.current_device()
+    key = (threading.get_ident(), device)
+    c = _ctxs.get(key)
+    if c is None:
+        c = _Ctx(device)
+        _ctxs[key] = c
+    return c.bind()
+
+
+def call(name, *args):
+    """Calls an entry point and raises the mapped exception on failure."""
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def sfx(dtype):
+    if dtype == torch.float32:
+        return "f32"
+    if dtype == torch.float64:
+        return "f64"
+    raise ArgumentError(f"element type {dtype} is not supported (Float32 / Float64)")

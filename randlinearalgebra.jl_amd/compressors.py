@@ -1,0 +1,653 @@
+"""Compressor surface of RandLinearAlgebra.jl on the MI355X kernels.
+
+Mirrors src/Compressors.jl: a `Compressor` holds user parameters, a
+`CompressorRecipe` holds the realised operator for a particular matrix (here:
+device tensors), `complete_compressor` builds a recipe, `update_compressor`
+redraws it in place, and `mul(C, S, A, alpha, beta)` / `mul(C, A, S, alpha,
+beta)` are the reference's 5-argument `mul!` (3-argument forms default
+alpha = 1.0, beta = 0.0, Compressors.jl:389-410).  `adjoint(S)` (or `S.T`)
+gives the `CompressorAdjoint` of Compressors.jl:94-104, and `S @ A` / `A @ S`
+are the allocating `*` of Compressors.jl:416-449.
+
+Matrices are torch tensors on the GPU, indexed like the Julia matrices they
+stand for (A[i, j] is row i, column j); any strides are accepted, so Julia
+views, transposes and column-major storage all map directly.  All arithmetic
+runs in librla_mi355x.so; nothing here computes on the host.
+"""
+import math
+
+import torch
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, DomainError
+
+# ---------------------------------------------------------------------------
+# cardinality (Compressors.jl:24-45)
+
+
+class Cardinality:
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self).__name__)
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+
+class Left(Cardinality):
+    pass
+
+
+class Right(Cardinality):
+    pass
+
+
+class Undef(Cardinality):
+    pass
+
+
+# ---------------------------------------------------------------------------
+# global draw state (the reference draws from Julia's task-local RNG; here a
+# recipe is a pure function of (seed, epoch), and every complete/update call
+# takes the next epoch)
+
+_rng = {"seed": 0x5EED_0000_1234, "epoch": 0}
+
+
+def seed(s):
+    """Counterpart of Random.seed!(s) for recipe generation."""
+    _rng["seed"] = int(s) & 0xFFFFFFFFFFFFFFFF
+    _rng["epoch"] = 0
+
+
+def _draw():
+    e = _rng["epoch"]
+    _rng["epoch"] = e + 1
+    return _rng["seed"], e
+
+
+# ---------------------------------------------------------------------------
+# strided views
+
+
+class View:
+    """A (rows x cols) strided window on a device tensor's storage."""
+
+    __slots__ = ("t", "rows", "cols", "rs", "cs")
+
+    def __init__(self, t, rows, cols, rs, cs):
+        self.t, self.rows, self.cols, self.rs, self.cs = t, rows, cols, rs, cs
+
+    @property
+    def T(self):
+        return View(self.t, self.cols, self.rows, self.cs, self.rs)
+
+    @property
+    def dtype(self):
+        return self.t.dtype
+
+    def size(self, d=None):
+        s = (self.rows, self.cols)
+        return s if d is None else s[d - 1]
+
+
+def as_view(x):
+    if isinstance(x, View):
+        return x
+    if not isinstance(x, torch.Tensor):
+        raise ArgumentError(f"expected a torch tensor, got {type(x).__name__}")
+    if x.dim() == 1:
+        n = x.shape[0]
+        return View(x, n, 1, x.stride(0) if n > 1 else 1, max(n, 1))
+    if x.dim() == 2:
+        return View(x, x.shape[0], x.shape[1], x.stride(0), x.stride(1))
+    raise ArgumentError("only vectors and matrices are supported")
+
+
+def _size2(x):
+    v = as_view(x)
+    return v.rows, v.cols
+
+
+def _check_device(*vs):
+    for v in vs:
+        if not v.t.is_cuda:
+            raise ArgumentError("arrays must live on the GPU (torch cuda tensors)")
+
+
+# ---------------------------------------------------------------------------
+# abstract types
+
+
+class Compressor:
+    """Parameters of a compression technique (Compressors.jl:7)."""
+
+    cardinality = Left()
+
+
+class CompressorRecipe:
+    """A realised compression operator (Compressors.jl:16); subclasses set
+    n_rows and n_cols."""
+
+    n_rows = 0
+    n_cols = 0
+
+    def size(self, dim=None):
+        if dim is None:
+            return (self.n_rows, self.n_cols)
+        if dim < 1 or dim > 2:
+            raise DomainError("`dim` must be 1 or 2.")
+        return self.n_rows if dim == 1 else self.n_cols
+
+    @property
+    def T(self):
+        return CompressorAdjoint(self)
+
+    def __matmul__(self, A):
+        return times(self, A)
+
+    def __rmatmul__(self, A):
+        return times(A, self)
+
+    # implemented by concrete recipes
+    def _left(self, C, A, alpha, beta):
+        raise ArgumentError(
+            f"No method `mul!` defined for ({type(C).__name__}, {type(self).__name__}, "
+            f"{type(A).__name__}, Float64, Float64).")
+
+    def _right(self, C, A, alpha, beta):
+        raise ArgumentError(
+            f"No method `mul!` defined for ({type(C).__name__}, {type(A).__name__}, "
+            f"{type(self).__name__}, Float64, Float64).")
+
+
+class CompressorAdjoint:
+    """Adjoint of a recipe (Compressors.jl:94-104)."""
+
+    def __init__(self, parent):
+        self.parent = parent
+
+    def size(self, dim=None):
+        p = self.parent
+        if dim is None:
+            return (p.n_cols, p.n_rows)
+        if dim < 1 or dim > 2:
+            raise DomainError("`dim` must be 1 or 2.")
+        return p.n_cols if dim == 1 else p.n_rows
+
+    @property
+    def T(self):
+        return self.parent
+
+    def __matmul__(self, A):
+        return times(self, A)
+
+    def __rmatmul__(self, A):
+        return times(A, self)
+
+
+def adjoint(S):
+    return S.T
+
+
+# ---------------------------------------------------------------------------
+# dimension checks (src/helpers/mul_dim_checks.jl:41-159), same messages
+
+
+def left_mul_dimcheck(C, S, A):
+    if isinstance(S, CompressorAdjoint):
+        return right_mul_dimcheck(as_view(C).T, as_view(A).T, S.parent)
+    s_rows, s_cols = S.size()
+    a_rows, a_cols = _size2(A)
+    c_rows, c_cols = _size2(C)
+    if a_rows != s_cols:
+        raise DimensionMismatch(f"Matrix A has {a_rows} rows while S has {s_cols} columns.")
+    if a_cols != c_cols:
+        raise DimensionMismatch(f"Matrix A has {a_cols} columns while C has {c_cols} columns.")
+    if c_rows != s_rows:
+        raise DimensionMismatch(f"Matrix C has {c_rows} rows while S has {s_rows} rows.")
+
+
+def right_mul_dimcheck(C, A, S):
+    if isinstance(S, CompressorAdjoint):
+        return left_mul_dimcheck(as_view(C).T, S.parent, as_view(A).T)
+    s_rows, s_cols = S.size()
+    a_rows, a_cols = _size2(A)
+    c_rows, c_cols = _size2(C)
+    if a_cols != s_rows:
+        raise DimensionMismatch(f"Matrix A has {a_cols} columns while S has {s_rows} rows.")
+    if c_cols != s_cols:
+        raise DimensionMismatch(f"Matrix C has {c_cols} columns while S has {s_cols} columns.")
+    if c_rows != a_rows:
+        raise DimensionMismatch(f"Matrix C has {c_rows} rows while A has {a_rows} rows.")
+
+
+# ---------------------------------------------------------------------------
+# generic entry points
+
+
+def complete_compressor(compressor, *args):
+    """complete_compressor(compressor, A) / (compressor, A, b) / (compressor, x, A, b)
+    (Compressors.jl:126-206: the longer forms forward to the matrix form unless
+    the technique needs them)."""
+    if not isinstance(compressor, Compressor):
+        raise ArgumentError(f"No method `complete_compressor` defined for {type(compressor)}.")
+    return compressor._complete(*args)
+
+
+def update_compressor(S, *args):
+    """update_compressor!(S[, A[, b]] / [x, A, b]) (Compressors.jl:225-305)."""
+    if isinstance(S, CompressorAdjoint):
+        S = S.parent
+    S._update(*args)
+
+
+update_compressor_ = update_compressor
+
+
+def mul(C, X, Y, alpha=1.0, beta=0.0):
+    """mul!(C, S, A, alpha, beta) or mul!(C, A, S, alpha, beta)."""
+    alpha, beta = float(alpha), float(beta)
+    if isinstance(X, CompressorAdjoint):
+        # alpha*S'*A + beta*C -> C  ==  alpha*A'*S + beta*C' -> C'
+        return mul(as_view(C).T, as_view(Y).T, X.parent, alpha, beta)
+    if isinstance(Y, CompressorAdjoint):
+        return mul(as_view(C).T, Y.parent, as_view(X).T, alpha, beta)
+    if isinstance(X, CompressorRecipe):
+        left_mul_dimcheck(C, X, Y)
+        Cv, Av = as_view(C), as_view(Y)
+        _check_device(Cv, Av)
+        _check_dtypes(Cv, Av)
+        X._left(Cv, Av, alpha, beta)
+        return None
+    if isinstance(Y, CompressorRecipe):
+        right_mul_dimcheck(C, X, Y)
+        Cv, Av = as_view(C), as_view(X)
+        _check_device(Cv, Av)
+        _check_dtypes(Cv, Av)
+        Y._right(Cv, Av, alpha, beta)
+        return None
+    raise ArgumentError("mul needs a CompressorRecipe or CompressorAdjoint operand")
+
+
+mul_ = mul
+
+
+def times(X, Y):
+    """The allocating product `S * A` / `A * S` (Compressors.jl:416-449)."""
+    if isinstance(X, (CompressorRecipe, CompressorAdjoint)):
+        A = Y
+        s_rows = X.size(1)
+        a_cols = 1 if A.dim() == 1 else A.shape[1]
+        C = (torch.zeros(s_rows, dtype=A.dtype, device=A.device) if a_cols == 1 and A.dim() == 1
+             else _colmajor_zeros(s_rows, a_cols, A))
+        mul(C, X, A)
+        return C
+    A, S = X, Y
+    s_cols = S.size(2)
+    a_rows = A.shape[0]
+    C = _colmajor_zeros(a_rows, s_cols, A)
+    mul(C, A, S)
+    return C
+
+
+def _colmajor_zeros(r, c, like):
+    return torch.zeros(c, r, dtype=like.dtype, device=like.device).T
+
+
+def _check_dtypes(Cv, Av):
+    if Cv.dtype != Av.dtype:
+        raise ArgumentError(f"C and A must share an element type ({Cv.dtype} vs {Av.dtype})")
+    _lib.sfx(Av.dtype)
+
+
+def _dtype(type_):
+    if type_ in (torch.float64, float, "Float64", None):
+        return torch.float64
+    if type_ in (torch.float32, "Float32"):
+        return torch.float32
+    raise ArgumentError(f"unsupported element type {type_}")
+
+
+def _device_of(*xs):
+    for x in xs:
+        if isinstance(x, torch.Tensor) and x.is_cuda:
+            return x.device
+        if isinstance(x, View) and x.t.is_cuda:
+            return x.t.device
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+# ---------------------------------------------------------------------------
+# sparse operators: SparseSign, CountSketch
+
+
+class _SparseOp:
+    """An nrows x ncols CSC matrix M on the device and its CSR twin."""
+
+    def __init__(self, nrows, ncols, colptr, rowidx, val):
+        self.nrows, self.ncols = nrows, ncols
+        self.colptr, self.rowidx, self.val = colptr, rowidx, val
+        self.nnz = rowidx.numel()
+        self._vals = {}
+        self.rebuild_csr()
+
+    def rebuild_csr(self):
+        dev = self.rowidx.device
+        self.rowptr = torch.empty(self.nrows + 1, dtype=torch.int32, device=dev)
+        self.colidx = torch.empty(self.nnz, dtype=torch.int32, device=dev)
+        self.val_t = torch.empty(self.nnz, dtype=self.val.dtype, device=dev)
+        _lib.call(f"rla_sparse_transpose_{_lib.sfx(self.val.dtype)}", _lib.ctx(dev), self.nrows,
+                  self.ncols, self.nnz, _lib.ptr(self.colptr), _lib.ptr(self.rowidx),
+                  _lib.ptr(self.val), _lib.ptr(self.rowptr), _lib.ptr(self.colidx),
+                  _lib.ptr(self.val_t))
+        self._vals = {}
+
+    def _vals_for(self, dtype):
+        if dtype == self.val.dtype:
+            return self.val, self.val_t
+        if dtype not in self._vals:
+            self._vals[dtype] = (self.val.to(dtype), self.val_t.to(dtype))
+        return self._vals[dtype]
+
+    def apply(self, trans_m, X, C, alpha, beta):
+        """C = alpha*op(M)*X + beta*C on strided views (op = M or M')."""
+        dt = X.dtype
+        val, val_t = self._vals_for(dt)
+        _lib.call(f"rla_sparse_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(trans_m),
+                  self.nrows, self.ncols, _lib.ptr(self.colptr), _lib.ptr(self.rowidx),
+                  _lib.ptr(val), _lib.ptr(self.rowptr), _lib.ptr(self.colidx), _lib.ptr(val_t),
+                  X.cols, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
+
+    def dense(self):
+        """M as a dense host float64 array (for inspection and tests)."""
+        import numpy as np
+        out = np.zeros((self.nrows, self.ncols))
+        cp = self.colptr.cpu().numpy()
+        ri = self.rowidx.cpu().numpy()
+        v = self.val.double().cpu().numpy()
+        for j in range(self.ncols):
+            out[ri[cp[j]:cp[j + 1]], j] = v[cp[j]:cp[j + 1]]
+        return out
+
+
+class _SparseRecipe(CompressorRecipe):
+    """Shared mul! plumbing: Left recipes hold S = M (s x m); Right recipes
+    hold S = M' with M (s x n) — the reference's `op` / `adjoint(op)`
+    (sparse_sign.jl:229-295, count_sketch.jl:113-147)."""
+
+    def _left(self, C, A, alpha, beta):
+        # S*A: op(M) = M for Left, M' for Right
+        self.op.apply(0 if self.cardinality == Left() else 1, A, C, alpha, beta)
+
+    def _right(self, C, A, alpha, beta):
+        # A*S = (S'*A')'
+        self.op.apply(1 if self.cardinality == Left() else 0, A.T, C.T, alpha, beta)
+
+
+class SparseSign(Compressor):
+    """SparseSign(cardinality=Left(), compression_dim=2, nnz=min(8, compression_dim),
+    type=Float64) — src/Compressors/sparse_sign.jl:65-127."""
+
+    def __init__(self, cardinality=None, compression_dim=2, nnz=None, type=torch.float64):
+        cardinality = Left() if cardinality is None else cardinality
+        nnz = min(8, compression_dim) if nnz is None else nnz
+        if compression_dim <= 0:
+            raise ArgumentError("Field `compression_dim` must be positive.")
+        if nnz > compression_dim:
+            raise ArgumentError(
+                f"Number of non-zero indices, {nnz}, must be less than or equal to "
+                f"compression dimension, {compression_dim}.")
+        if nnz <= 0:
+            raise ArgumentError("Field `nnz` must be positive.")
+        object.__setattr__(self, "cardinality", cardinality)
+        object.__setattr__(self, "compression_dim", int(compression_dim))
+        object.__setattr__(self, "nnz", int(nnz))
+        object.__setattr__(self, "type", _dtype(type))
+
+    def __setattr__(self, name, val):
+        if name == "compression_dim":
+            if val <= 0:
+                raise ArgumentError("Field `compression_dim` must be positive.")
+            if self.nnz > val:
+                raise ArgumentError(
+                    f"New `compression_dim`, {val}, must be greater than or equal to current "
+                    f"`nnz`, {self.nnz}.")
+        elif name == "nnz":
+            if val <= 0:
+                raise ArgumentError("Field `nnz` must be positive.")
+            if val > self.compression_dim:
+                raise ArgumentError(
+                    f"New `nnz`, {val}, must be less than or equal to current "
+                    f"`compression_dim`, {self.compression_dim}.")
+        object.__setattr__(self, name, val)
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return SparseSignRecipe(self.cardinality, self.compression_dim, A, self.nnz, self.type)
+
+
+class SparseSignRecipe(_SparseRecipe):
+    """sparse_sign.jl:219-295.  Left: n_rows = s, n_cols = size(A, 1);
+    Right: n_rows = size(A, 2), n_cols = s.  `op` is M (nnz sorted rows per
+    column, values +-1/sqrt(nnz))."""
+
+    def __init__(self, cardinality, compression_dim, A, nnz, type_):
+        if cardinality == Undef():
+            raise ArgumentError("SparseSign needs cardinality Left() or Right().")
+        a_rows, a_cols = _size2(A)
+        self.cardinality = cardinality
+        self.nnz = nnz
+        if cardinality == Left():
+            self.n_rows, self.n_cols = compression_dim, a_rows
+            initial = a_rows
+        else:
+            self.n_rows, self.n_cols = a_cols, compression_dim
+            initial = a_cols
+        self.type = type_
+        sc = float(torch.tensor(1.0 / math.sqrt(nnz), dtype=type_))
+        self.scale = [-sc, sc]
+        dev = _device_of(A)
+        colptr = torch.arange(0, initial * nnz + 1, nnz, dtype=torch.int32, device=dev)
+        rowidx = torch.empty(initial * nnz, dtype=torch.int32, device=dev)
+        val = torch.empty(initial * nnz, dtype=type_, device=dev)
+        self._gen(rowidx, val, compression_dim, initial)
+        self.op = _SparseOp(compression_dim, initial, colptr, rowidx, val)
+
+    def _gen(self, rowidx, val, s, initial):
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        _lib.call(f"rla_sparse_sign_gen_{_lib.sfx(self.type)}", _lib.ctx(rowidx.device), sd, ep,
+                  initial, s, self.nnz, _lib.ptr(rowidx), _lib.ptr(val))
+
+    def _update(self, *args):
+        op = self.op
+        self._gen(op.rowidx, op.val, op.nrows, op.ncols)
+        op.rebuild_csr()
+
+
+class CountSketch(Compressor):
+    """CountSketch(cardinality=Left(), compression_dim=2, type=Float64) —
+    src/Compressors/count_sketch.jl:60-81."""
+
+    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64):
+        cardinality = Left() if cardinality is None else cardinality
+        if compression_dim <= 0:
+            raise ArgumentError("Field 'compression_dim' must be positive.")
+        if cardinality == Undef():
+            raise ArgumentError("`cardinality` must be specified as `Left()` or `Right()`."
+                                "`Undef()` is not allowed in `CountSketch` structure.")
+        self.cardinality = cardinality
+        self.compression_dim = int(compression_dim)
+        self.type = _dtype(type)
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return CountSketchRecipe(self.cardinality, self.compression_dim, A, self.type)
+
+
+class CountSketchRecipe(_SparseRecipe):
+    """count_sketch.jl:105-147: one +-1 per column of M (s x initial)."""
+
+    def __init__(self, cardinality, compression_dim, A, type_):
+        a_rows, a_cols = _size2(A)
+        self.cardinality = cardinality
+        self.compression_dim = compression_dim
+        if cardinality == Left():
+            self.n_rows, self.n_cols = compression_dim, a_rows
+            initial = a_rows
+        else:
+            self.n_rows, self.n_cols = a_cols, compression_dim
+            initial = a_cols
+        self.type = type_
+        dev = _device_of(A)
+        colptr = torch.arange(0, initial + 1, dtype=torch.int32, device=dev)
+        rowidx = torch.empty(initial, dtype=torch.int32, device=dev)
+        val = torch.empty(initial, dtype=type_, device=dev)
+        self._gen(rowidx, val, compression_dim, initial)
+        self.op = _SparseOp(compression_dim, initial, colptr, rowidx, val)
+
+    @property
+    def mat(self):
+        return self.op
+
+    def _gen(self, rowidx, val, s, initial):
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        _lib.call(f"rla_count_sketch_gen_{_lib.sfx(self.type)}", _lib.ctx(rowidx.device), sd, ep,
+                  initial, s, _lib.ptr(rowidx), _lib.ptr(val))
+
+    def _update(self, *args):
+        op = self.op
+        self._gen(op.rowidx, op.val, op.nrows, op.ncols)
+        op.rebuild_csr()
+
+
+# ---------------------------------------------------------------------------
+# SRHT
+
+
+def padded_size(n):
+    """Int64(2^(ceil(log(2, n)))) exactly as srht.jl:147 evaluates it
+    (log(2, n) = log(n)/log(2) in Float64, which overshoots at n = 2^29,
+    2^31, 2^39 and then pads to the next power)."""
+    return int(2.0 ** math.ceil(math.log(float(n)) / math.log(2.0)))
+
+
+class SRHT(Compressor):
+    """SRHT(cardinality=Left(), compression_dim=2, block_size=10, type=Float64)
+    — src/Compressors/srht.jl:60-91."""
+
+    def __init__(self, cardinality=None, compression_dim=2, block_size=10, type=torch.float64):
+        cardinality = Left() if cardinality is None else cardinality
+        if compression_dim <= 0:
+            raise ArgumentError("Field `compression_dim` must be positive.")
+        if block_size <= 0:
+            raise ArgumentError("Field `block_size` must be positive.")
+        if cardinality == Undef():
+            raise ArgumentError("Cardinality must be of type `Left` or `Right`.")
+        self.cardinality = cardinality
+        self.compression_dim = int(compression_dim)
+        self.block_size = int(block_size)
+        self.type = _dtype(type)
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return SRHTRecipe(self.compression_dim, self.block_size, self.cardinality, A, self.type)
+
+
+class SRHTRecipe(CompressorRecipe):
+    """srht.jl:125-191.  Left: S = (scale*H*D*pad)[op, :] is s x m; Right: S is
+    n x s, the transpose of the same construction on size(A, 2).
+    `op` holds 0-based sampled rows, `signs` the BitVector chunks, `scale` the
+    Float64 type(sqrt(s)/sqrt(n_pad)).  block_size is kept for parity of the
+    recipe fields only: the kernels do not block columns."""
+
+    def __init__(self, compression_dim, block_size, cardinality, A, type_):
+        a_rows, a_cols = _size2(A)
+        self.cardinality = cardinality
+        self.block_size = block_size
+        if cardinality == Left():
+            self.n_rows, self.n_cols = compression_dim, a_rows
+            self.dim = a_rows
+        else:
+            self.n_rows, self.n_cols = a_cols, compression_dim
+            self.dim = a_cols
+        self.n_pad = padded_size(self.dim)
+        if compression_dim > self.n_pad:
+            raise ArgumentError(
+                f"Cannot draw a sample of size {compression_dim} without replacement from "
+                f"1:{self.n_pad}.")
+        self.s = compression_dim
+        self.type = type_
+        self.scale = float(torch.tensor(math.sqrt(compression_dim) / math.sqrt(self.n_pad),
+                                        dtype=type_))
+        dev = _device_of(A)
+        self.op = torch.empty(compression_dim, dtype=torch.int32, device=dev)
+        self.signs = torch.empty((self.n_pad + 63) // 64, dtype=torch.int64, device=dev)
+        self._gen()
+
+    def _gen(self):
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        _lib.call("rla_srht_gen", _lib.ctx(self.op.device), sd, ep, self.n_pad, self.s,
+                  _lib.ptr(self.op), _lib.ptr(self.signs))
+
+    def _update(self, *args):
+        self._gen()
+
+    def sign_vector(self):
+        """The signs as a host bool array of length n_pad (BitVector order)."""
+        import numpy as np
+        ch = self.signs.cpu().numpy().view(np.uint64)
+        bits = ((ch[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(bool)
+        return bits.reshape(-1)[: self.n_pad]
+
+    def _apply(self, adjoint, X, Y, alpha, beta):
+        dt = X.dtype
+        _lib.call(f"rla_srht_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(adjoint), self.dim,
+                  self.n_pad, self.s, X.cols, _lib.ptr(self.op), _lib.ptr(self.signs),
+                  self.scale, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t), Y.rs, Y.cs, alpha, beta)
+
+    def _left(self, C, A, alpha, beta):
+        # Left S*A: T*A (srht.jl:220); Right S*A: T'*A (srht.jl:432)
+        self._apply(0 if self.cardinality == Left() else 1, A, C, alpha, beta)
+
+    def _right(self, C, A, alpha, beta):
+        # Left A*S = (T'*A')' (srht.jl:289); Right A*S = (T*A')' (srht.jl:366)
+        self._apply(1 if self.cardinality == Left() else 0, A.T, C.T, alpha, beta)
+
+
+def fwht(x, signs=None, scaling=1.0):
+    """fwht!(x, signs; scaling) / fwht!(x; scaling) on each column of a device
+    matrix (or a vector) — src/Compressors/helpers/fwht.jl:21-118.  `signs` is
+    a bool tensor of length n or a BitVector-chunk int64 tensor."""
+    v = as_view(x)
+    n = v.rows
+    if n < 2 or (n & (n - 1)) != 0:
+        raise DimensionMismatch("Size of vector must be power of 2.")
+    if v.rs != 1:
+        raise ArgumentError("fwht needs unit row stride")
+    chunks = None
+    if signs is not None:
+        if signs.dtype == torch.bool:
+            if signs.numel() != n:
+                raise DimensionMismatch("Vector `x` and vector `signs` must be same length.")
+            chunks = pack_bits(signs).to(x.device)
+        else:
+            chunks = signs
+    _lib.call(f"rla_fwht_{_lib.sfx(x.dtype)}", _lib.ctx(x.device), n, v.cols, _lib.ptr(chunks),
+              float(scaling), _lib.ptr(v.t), v.cs)
+
+
+def pack_bits(bits):
+    """bool vector -> Julia BitVector chunks (int64 tensor, bit i of chunk i>>6)."""
+    import numpy as np
+    b = bits.detach().cpu().numpy().astype(np.uint64)
+    n = b.shape[0]
+    pad = np.zeros(((n + 63) // 64) * 64, dtype=np.uint64)
+    pad[:n] = b
+    w = (pad.reshape(-1, 64) << np.arange(64, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+    return torch.from_numpy(w.view(np.int64).copy())

@@ -1,0 +1,313 @@
+// Walsh–Hadamard transforms and the SRHT operator (generic path).
+//
+// The transform of a length n = 2^L column is done as the reference's
+// radix-2 network (src/Compressors/helpers/fwht.jl:21-118): stage b pairs
+// entries whose index differs in bit b, for b = 0, 1, ..., L-1, and writes
+// (z + y, z - y).  Stage 0 first multiplies each entry by +-scaling; the
+// reference evaluates that product and the stage-0 sum in Float64 (scaling is
+// a Float64), so we do too, then round to the element type.  Every later
+// stage is a plain add/subtract in the element type.  Doing the same stages in
+// the same order makes the result bit-identical to the reference loop,
+// whatever grouping of stages into passes is used.
+//
+// Generic path: a pass handles a run of nb consecutive bits [lo, lo + nb) for
+// every column.  A workgroup loads gpw butterfly groups (2^nb entries each,
+// stride 2^lo) into LDS with coalesced reads, runs the nb stages with a
+// barrier between stages, and writes them back.  The first pass may read
+// straight from the (unpadded, strided) input and write to the padded
+// workspace.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+
+#include "rla_internal.hpp"
+
+namespace rla {
+
+constexpr int kPassThreads = 512;
+constexpr size_t kPassLds = 64 * 1024;
+
+__device__ inline bool sign_bit(const uint64_t* __restrict__ s, int64_t i) {
+  return (s[i >> 6] >> (i & 63)) & 1ull;
+}
+
+template <class T>
+__global__ __launch_bounds__(kPassThreads) void fwht_pass_kernel(
+    const T* __restrict__ src, int64_t src_rs, int64_t src_cs, int64_t m_valid, T* P,
+    int64_t ldp, int64_t ncols, int L, int lo, int nb, int lg_gpw,
+    const uint64_t* __restrict__ signs, double scaling, int scale_stage0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
+  const int gpw = 1 << lg_gpw;
+  const int64_t groups_per_col = int64_t(1) << (L - nb);
+  const int64_t total_groups = groups_per_col * ncols;
+  const int64_t G0 = int64_t(blockIdx.x) * gpw;
+  if (G0 >= total_groups) return;
+  const int ne = gpw << nb;
+  const int tmask = (1 << nb) - 1;
+  // tile index of (t, gl): contiguous groups when lo == 0, interleaved
+  // groups (consecutive lanes -> consecutive groups) when lo > 0.
+  const int tstride = lo == 0 ? 1 : gpw;
+  const int gstride = lo == 0 ? (1 << nb) : 1;
+
+  auto addr_of = [&](int t, int gl, int64_t& col, int64_t& pos) {
+    const int64_t G = G0 + gl;
+    col = G >> (L - nb);
+    const int64_t g = G & (groups_per_col - 1);
+    if (lo == 0) {
+      pos = (g << nb) + t;
+    } else {
+      const int64_t glo = g & ((int64_t(1) << lo) - 1), ghi = g >> lo;
+      pos = (ghi << (lo + nb)) + (int64_t(t) << lo) + glo;
+    }
+  };
+
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int t = lo == 0 ? (e & tmask) : (e >> lg_gpw);
+    const int gl = lo == 0 ? (e >> nb) : (e & (gpw - 1));
+    int64_t col, pos;
+    addr_of(t, gl, col, pos);
+    T v;
+    if (col >= ncols) {
+      v = T(0);
+    } else if (src) {
+      v = pos < m_valid ? src[pos * src_rs + col * src_cs] : T(0);
+    } else {
+      v = P[col * ldp + pos];
+    }
+    tile[t * tstride + gl * gstride] = v;
+  }
+
+  const int npairs = ne >> 1;
+  for (int b = 0; b < nb; ++b) {
+    __syncthreads();
+    const int hmask = (1 << b) - 1;
+    for (int q = threadIdx.x; q < npairs; q += blockDim.x) {
+      const int qt = lo == 0 ? (q & (tmask >> 1)) : (q >> lg_gpw);
+      const int gl = lo == 0 ? (q >> (nb - 1)) : (q & (gpw - 1));
+      const int t0 = ((qt >> b) << (b + 1)) | (qt & hmask);
+      const int t1 = t0 | (1 << b);
+      T* a0 = &tile[t0 * tstride + gl * gstride];
+      T* a1 = &tile[t1 * tstride + gl * gstride];
+      if (b == 0 && scale_stage0) {
+        // global positions of the pair (lo == 0 here)
+        const int64_t G = G0 + gl;
+        const int64_t g = G & (groups_per_col - 1);
+        const int64_t p0 = (g << nb) + t0;
+        const double s0 = (!signs || sign_bit(signs, p0)) ? scaling : -scaling;
+        const double s1 = (!signs || sign_bit(signs, p0 + 1)) ? scaling : -scaling;
+        const double z = double(*a0) * s0;
+        const double y = double(*a1) * s1;
+        *a0 = static_cast<T>(z + y);
+        *a1 = static_cast<T>(z - y);
+      } else {
+        const T z = *a0, y = *a1;
+        *a0 = z + y;
+        *a1 = z - y;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int t = lo == 0 ? (e & tmask) : (e >> lg_gpw);
+    const int gl = lo == 0 ? (e >> nb) : (e & (gpw - 1));
+    int64_t col, pos;
+    addr_of(t, gl, col, pos);
+    if (col < ncols) P[col * ldp + pos] = tile[t * tstride + gl * gstride];
+  }
+}
+
+// Y[r, k] = beta*Y[r, k] + alpha*P[op[r], k] in double (the reference's
+// broadcast `beta .* Cv .+ alpha .* padding[op, :]`, srht.jl:259); beta == 0
+// does not read Y.
+template <class T>
+__global__ void srht_gather_kernel(const T* __restrict__ P, int64_t ldp, int64_t s,
+                                   int64_t ncols, const int32_t* __restrict__ op,
+                                   T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha,
+                                   double beta) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= s * ncols) return;
+  const int64_t r = e % s, k = e / s;
+  T* yp = Y + r * y_rs + k * y_cs;
+  const double p = double(P[k * ldp + op[r]]);
+  *yp = static_cast<T>(beta == 0.0 ? alpha * p : beta * double(*yp) + alpha * p);
+}
+
+// P[op[r], k] = alpha * X[r, k] (srht.jl:311 `pv' .= alpha .* Av`, rounded to
+// the element type); P is zero elsewhere.
+template <class T>
+__global__ void srht_scatter_kernel(const T* __restrict__ X, int64_t x_rs, int64_t x_cs,
+                                    int64_t s, int64_t ncols, const int32_t* __restrict__ op,
+                                    T* __restrict__ P, int64_t ldp, double alpha) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= s * ncols) return;
+  const int64_t r = e % s, k = e / s;
+  P[k * ldp + op[r]] = static_cast<T>(alpha * double(X[r * x_rs + k * x_cs]));
+}
+
+// Y[i, k] = (+-1)*P[i, k] + beta*Y[i, k] for i < m (srht.jl:321-327: the
+// sign flip after the transform, then axpby!(1, pv', beta, Cv)).
+template <class T>
+__global__ void srht_adjoint_epilogue_kernel(const T* __restrict__ P, int64_t ldp, int64_t m,
+                                             int64_t ncols, const uint64_t* __restrict__ signs,
+                                             T* __restrict__ Y, int64_t y_rs, int64_t y_cs,
+                                             double beta) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= m * ncols) return;
+  const int64_t i = e % m, k = e / m;
+  T p = P[k * ldp + i];
+  if (!sign_bit(signs, i)) p = -p;
+  T* yp = Y + i * y_rs + k * y_cs;
+  *yp = static_cast<T>(beta == 0.0 ? double(p) : double(p) + beta * double(*yp));
+}
+
+template <class T>
+__global__ void zero_kernel(T* p, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+// Runs every pass of the transform of ncols columns of length 2^L held in P
+// (or, for the first pass, read from src with m_valid rows).
+template <class T>
+static int fwht_passes(rla_ctx* ctx, const T* src, int64_t src_rs, int64_t src_cs,
+                       int64_t m_valid, T* P, int64_t ldp, int64_t ncols, int L,
+                       const uint64_t* signs, double scaling) {
+  const int cap_lg = ilog2(static_cast<int64_t>(kPassLds / sizeof(T)));  // 14 f32, 13 f64
+  int lo = 0;
+  while (lo < L) {
+    int nb, lg_gpw;
+    if (lo == 0) {
+      nb = std::min(L, cap_lg);
+      lg_gpw = cap_lg - nb;  // several short columns per workgroup
+      const int64_t total_groups = (int64_t(1) << (L - nb)) * ncols;
+      while (lg_gpw > 0 && (int64_t(1) << lg_gpw) > total_groups) --lg_gpw;
+    } else {
+      nb = std::min(L - lo, cap_lg - 6);
+      lg_gpw = std::min(lo, cap_lg - nb);
+    }
+    const int64_t total_groups = (int64_t(1) << (L - nb)) * ncols;
+    const int64_t blocks = ceil_div(total_groups, int64_t(1) << lg_gpw);
+    const size_t lds = (size_t(1) << (nb + lg_gpw)) * sizeof(T);
+    const bool first = lo == 0;
+    fwht_pass_kernel<T><<<dim3(static_cast<unsigned>(blocks)), kPassThreads, lds,
+                          ctx->stream>>>(first ? src : nullptr, src_rs, src_cs, m_valid, P, ldp,
+                                         ncols, L, lo, nb, lg_gpw, signs, scaling,
+                                         first ? 1 : 0);
+    RLA_LAUNCH_CHECK("fwht_pass");
+    lo += nb;
+  }
+  return RLA_OK;
+}
+
+template <class T>
+static int fwht(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* signs, double scaling,
+                T* X, int64_t ldx) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n) || n < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (ncols < 0 || ldx < n) return fail(RLA_EARG, "bad column count or leading dimension");
+  if (ncols == 0) return RLA_OK;
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  return fwht_passes<T>(ctx, nullptr, 0, 0, n, X, ldx, ncols, ilog2(n), signs, scaling);
+}
+
+// Columns per chunk so that the padded workspace stays below ~512 MiB.
+template <class T>
+static int64_t srht_chunk_cols(int64_t n_pad, int64_t ncols) {
+  const int64_t budget = int64_t(512) << 20;
+  int64_t c = budget / (n_pad * int64_t(sizeof(T)));
+  if (c < 1) c = 1;
+  return std::min(c, ncols);
+}
+
+template <class T>
+int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                       const int32_t* op, const uint64_t* signs, double scale, const T* X,
+                       int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta, bool* handled);
+
+template <class T>
+static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                      int64_t ncols, const int32_t* op, const uint64_t* signs, double scale,
+                      const T* X, int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs,
+                      double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (m < 1 || m > n_pad || s < 1 || s > n_pad || ncols < 0)
+    return fail(RLA_EARG, "SRHT sizes must satisfy 1 <= m <= n_pad and 1 <= s <= n_pad");
+  if (ncols == 0) return RLA_OK;
+  if (!op || !signs || !X || !Y) return fail(RLA_EARG, "SRHT apply: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int L = ilog2(n_pad);
+  if (!adjoint) {
+    bool handled = false;
+    rc = srht_fused_forward<T>(ctx, m, n_pad, s, ncols, op, signs, scale, X, x_rs, x_cs, Y, y_rs,
+                               y_cs, alpha, beta, &handled);
+    if (rc || handled) return rc;
+  }
+  const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
+  void* ws = nullptr;
+  rc = workspace(ctx, static_cast<size_t>(n_pad * cc) * sizeof(T), &ws);
+  if (rc) return rc;
+  T* P = static_cast<T*>(ws);
+  for (int64_t k0 = 0; k0 < ncols; k0 += cc) {
+    const int64_t kc = std::min(cc, ncols - k0);
+    const T* Xk = X + k0 * x_cs;
+    T* Yk = Y + k0 * y_cs;
+    if (!adjoint) {
+      rc = fwht_passes<T>(ctx, Xk, x_rs, x_cs, m, P, n_pad, kc, L, signs, scale);
+      if (rc) return rc;
+      srht_gather_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(s * kc, 256))), 256, 0,
+                              ctx->stream>>>(P, n_pad, s, kc, op, Yk, y_rs, y_cs, alpha, beta);
+      RLA_LAUNCH_CHECK("srht_gather");
+    } else {
+      zero_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(n_pad * kc, 256))), 256, 0,
+                       ctx->stream>>>(P, n_pad * kc);
+      srht_scatter_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(s * kc, 256))), 256, 0,
+                               ctx->stream>>>(Xk, x_rs, x_cs, s, kc, op, P, n_pad, alpha);
+      rc = fwht_passes<T>(ctx, nullptr, 0, 0, n_pad, P, n_pad, kc, L, nullptr, scale);
+      if (rc) return rc;
+      srht_adjoint_epilogue_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(m * kc, 256))), 256,
+                                        0, ctx->stream>>>(P, n_pad, m, kc, signs, Yk, y_rs, y_cs,
+                                                          beta);
+      RLA_LAUNCH_CHECK("srht_adjoint");
+    }
+  }
+  return RLA_OK;
+}
+
+}  // namespace rla
+
+using namespace rla;
+
+extern "C" {
+
+int rla_fwht_f32(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_chunks,
+                 double scaling, float* X, int64_t ldx) {
+  return fwht<float>(ctx, n, ncols, sign_chunks, scaling, X, ldx);
+}
+
+int rla_fwht_f64(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_chunks,
+                 double scaling, double* X, int64_t ldx) {
+  return fwht<double>(ctx, n, ncols, sign_chunks, scaling, X, ldx);
+}
+
+int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                       double scale, const float* X, int64_t x_rs, int64_t x_cs, float* Y,
+                       int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  return srht_apply<float>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, X, x_rs,
+                           x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                       double scale, const double* X, int64_t x_rs, int64_t x_cs, double* Y,
+                       int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  return srht_apply<double>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, X, x_rs,
+                            x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+}  // extern "C"

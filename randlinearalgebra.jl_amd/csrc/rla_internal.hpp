@@ -1,0 +1,69 @@
+// Internal helpers shared by the librla_mi355x translation units: context,
+// error reporting, workspace, launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/rla.h"
+
+struct rla_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* ws = nullptr;       // grow-only device workspace
+  size_t ws_bytes = 0;
+  int* flag = nullptr;      // small device word for kernel-side checks
+  int* flag_host = nullptr; // pinned mirror of flag
+};
+
+namespace rla {
+
+// Thread-local last-error text behind rla_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+// HIP call check: on error records the message and returns RLA_EHIP.
+#define RLA_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return ::rla::fail(RLA_EHIP, std::string(#call " failed: ") +          \
+                                       hipGetErrorString(_e));               \
+  } while (0)
+
+// After a launch: picks up configuration errors without synchronising.
+#define RLA_LAUNCH_CHECK(name)                                               \
+  do {                                                                       \
+    hipError_t _e = hipGetLastError();                                       \
+    if (_e != hipSuccess)                                                    \
+      return ::rla::fail(RLA_EHIP, std::string(name " launch failed: ") +    \
+                                       hipGetErrorString(_e));               \
+  } while (0)
+
+#define RLA_CHECK_CTX(ctx)                                                   \
+  do {                                                                       \
+    if (!(ctx)) return ::rla::fail(RLA_EARG, "null rla_ctx");                \
+  } while (0)
+
+// Makes ctx->device current for the calling thread.
+int use_device(rla_ctx* ctx);
+
+// Returns a device workspace of at least `bytes` (grow-only, owned by ctx).
+// Growth synchronises the context stream before freeing the old block.
+int workspace(rla_ctx* ctx, size_t bytes, void** out);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+inline int ilog2(int64_t n) {
+  int l = 0;
+  while ((int64_t(1) << l) < n) ++l;
+  return l;
+}
+
+}  // namespace rla

@@ -1,0 +1,348 @@
+// Sparse compressor application (SparseSign, CountSketch, FJLT's K).
+//
+// Every orientation of the reference's sparse mul! is C = alpha*op(M)*X +
+// beta*C on (possibly transposed) strided views, and op(M)*X is a row gather:
+// output row i sums v_p * X[idx_p, :] over the nonzeros p of row i of op(M),
+// i.e. over the CSR of M (op = M) or the CSC of M (op = M').  Each output
+// entry is owned by one thread and accumulated sequentially in ascending
+// index order, as the SparseArrays loop does, so results are deterministic
+// and independent of the launch shape.  No floating-point atomics.
+//
+// Arithmetic follows Julia's promotion in that loop: alpha and beta are
+// Float64, so `alpha*x` and `acc + v*(alpha*x)` are evaluated in double and
+// the sum is rounded back to the element type after every step
+// (C[rv[j],k] += nzv[j]*αxj with αxj = B[col,k]*α).
+//
+// Two gather kernels:
+//  * rowgather_lds: X column-major (unit row stride).  A workgroup stages a
+//    block of rows of KT adjacent columns of X in LDS (coalesced reads,
+//    interleaved [row][KT]) and every thread gathers its output rows from
+//    LDS.  Rows of X beyond one LDS block are handled block by block, with
+//    the partial sums carried in C (the workgroup owns its C tile).
+//  * rowgather_direct: X row-major view (unit column stride).  One wavefront
+//    per output row, lanes over the right-hand sides, gathered rows read
+//    straight from global memory (each gathered row is contiguous).
+#pragma clang fp contract(off)
+
+#include <algorithm>
+
+#include "rla_internal.hpp"
+
+namespace rla {
+
+constexpr int kGatherThreads = 512;
+constexpr size_t kGatherLds = 64 * 1024;
+
+// ---- CSC -> CSR ----------------------------------------------------------
+
+__global__ void zero_i32_kernel(int32_t* p, int64_t n) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void count_rows_kernel(const int32_t* __restrict__ rowidx, int64_t nnz,
+                                  int32_t* __restrict__ cnt /* rowptr + 1 */) {
+  int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p < nnz) atomicAdd(&cnt[rowidx[p]], 1);
+}
+
+// In-place inclusive scan of a[0..n) by one workgroup of 1024 threads:
+// each thread scans a contiguous segment, segment totals are scanned in LDS.
+__global__ __launch_bounds__(1024) void scan_inclusive_kernel(int32_t* a, int64_t n) {
+  __shared__ int32_t part[1024];
+  const int T = blockDim.x;
+  int64_t seg = (n + T - 1) / T;
+  int64_t b = threadIdx.x * seg, e = b + seg < n ? b + seg : n;
+  int32_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += a[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    int32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int64_t i = b; i < e; ++i) {
+    run += a[i];
+    a[i] = run;
+  }
+}
+
+template <class T>
+__global__ void fill_csr_kernel(int64_t ncols, const int32_t* __restrict__ colptr,
+                                const int32_t* __restrict__ rowidx, const T* __restrict__ val,
+                                int32_t* __restrict__ cursor, int32_t* __restrict__ colidx,
+                                T* __restrict__ val_t) {
+  int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (j >= ncols) return;
+  for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+    int32_t pos = atomicAdd(&cursor[rowidx[p]], 1);
+    colidx[pos] = static_cast<int32_t>(j);
+    val_t[pos] = val[p];
+  }
+}
+
+// Sorts each CSR row by column index (shell sort with Ciura's gaps; keys are
+// distinct, so the result does not depend on the fill order above).
+template <class T>
+__global__ void sort_rows_kernel(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                 int32_t* __restrict__ colidx, T* __restrict__ val_t) {
+  int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= nrows) return;
+  int32_t b = rowptr[i], n = rowptr[i + 1] - b;
+  int32_t* c = colidx + b;
+  T* v = val_t + b;
+  const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+  for (int g = 0; g < 8; ++g) {
+    int gap = gaps[g];
+    for (int32_t q = gap; q < n; ++q) {
+      int32_t kc = c[q];
+      T kv = v[q];
+      int32_t r = q;
+      while (r >= gap && c[r - gap] > kc) {
+        c[r] = c[r - gap];
+        v[r] = v[r - gap];
+        r -= gap;
+      }
+      c[r] = kc;
+      v[r] = kv;
+    }
+  }
+}
+
+template <class T>
+static int sparse_transpose(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                            const int32_t* colptr, const int32_t* rowidx, const T* val,
+                            int32_t* rowptr, int32_t* colidx, T* val_t) {
+  RLA_CHECK_CTX(ctx);
+  if (nrows < 0 || ncols < 0 || nnz < 0) return fail(RLA_EARG, "negative sparse dimensions");
+  if (nrows >= INT32_MAX || nnz >= INT32_MAX)
+    return fail(RLA_ENOTSUP, "sparse operator larger than int32 indexing");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  zero_i32_kernel<<<dim3(unsigned(ceil_div(nrows + 1, 256))), 256, 0, st>>>(rowptr, nrows + 1);
+  if (nnz > 0)
+    count_rows_kernel<<<dim3(unsigned(ceil_div(nnz, 256))), 256, 0, st>>>(rowidx, nnz,
+                                                                          rowptr + 1);
+  scan_inclusive_kernel<<<1, 1024, 0, st>>>(rowptr + 1, nrows);
+  RLA_LAUNCH_CHECK("sparse_transpose/scan");
+  if (nnz == 0 || ncols == 0) return RLA_OK;
+  void* ws = nullptr;
+  rc = workspace(ctx, static_cast<size_t>(nrows + 1) * sizeof(int32_t), &ws);
+  if (rc) return rc;
+  int32_t* cursor = static_cast<int32_t*>(ws);
+  RLA_HIP(hipMemcpyAsync(cursor, rowptr, static_cast<size_t>(nrows) * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, st));
+  fill_csr_kernel<T><<<dim3(unsigned(ceil_div(ncols, 256))), 256, 0, st>>>(
+      ncols, colptr, rowidx, val, cursor, colidx, val_t);
+  sort_rows_kernel<T><<<dim3(unsigned(ceil_div(nrows, 256))), 256, 0, st>>>(nrows, rowptr,
+                                                                           colidx, val_t);
+  RLA_LAUNCH_CHECK("sparse_transpose/fill");
+  return RLA_OK;
+}
+
+// ---- gather kernels --------------------------------------------------------
+
+__device__ inline int32_t lower_bound_i32(const int32_t* __restrict__ a, int32_t lo, int32_t hi,
+                                          int64_t key) {
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <class T, int KT>
+__global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
+    int64_t out_rows, int64_t in_rows, int64_t n_rhs, const int32_t* __restrict__ ptr,
+    const int32_t* __restrict__ idx, const T* __restrict__ val, const T* __restrict__ X,
+    int64_t x_cs, T* __restrict__ C, int64_t c_rs, int64_t c_cs, double alpha, double beta,
+    int64_t rb_rows, int nk_tiles, int rsplit, int64_t rows_per_part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  // Block -> (column tile, output-row part).  The parts of one column tile
+  // are 8 block ids apart, so they land on the same XCD (blocks are dealt
+  // round-robin over the 8 XCDs) and share that XCD's L2 for the X tile.
+  const int b = blockIdx.x;
+  const int grp = b / (8 * rsplit), within = b % (8 * rsplit);
+  const int ktile = grp * 8 + (within % 8);
+  const int part = within / 8;
+  if (ktile >= nk_tiles) return;
+  const int64_t k0 = int64_t(ktile) * KT;
+  const int kt = static_cast<int>(n_rhs - k0 < KT ? n_rhs - k0 : KT);
+  const int64_t r_begin = part * rows_per_part;
+  const int64_t r_end = r_begin + rows_per_part < out_rows ? r_begin + rows_per_part : out_rows;
+  const int64_t nblocks = in_rows > 0 ? (in_rows + rb_rows - 1) / rb_rows : 1;
+
+  for (int64_t blk = 0; blk < nblocks; ++blk) {
+    const int64_t j0 = blk * rb_rows;
+    const int64_t jn = in_rows - j0 < rb_rows ? in_rows - j0 : rb_rows;
+    __syncthreads();
+    for (int kk = 0; kk < KT; ++kk) {
+      if (kk < kt) {
+        const T* xc = X + (k0 + kk) * x_cs + j0;
+        for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j * KT + kk] = xc[j];
+      } else {
+        for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j * KT + kk] = T(0);
+      }
+    }
+    __syncthreads();
+    for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
+      int32_t p = ptr[i], pe = ptr[i + 1];
+      if (nblocks > 1) {
+        p = lower_bound_i32(idx, p, pe, j0);
+        pe = lower_bound_i32(idx, p, pe, j0 + jn);
+      }
+      T acc[KT];
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        if (kk < kt) {
+          T* cp = C + i * c_rs + (k0 + kk) * c_cs;
+          if (blk > 0) acc[kk] = *cp;
+          else acc[kk] = (beta == 0.0) ? T(0) : static_cast<T>(beta * double(*cp));
+        } else {
+          acc[kk] = T(0);
+        }
+      }
+      for (; p < pe; ++p) {
+        const int64_t j = idx[p] - j0;
+        const T v = val[p];
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk)
+          acc[kk] = static_cast<T>(double(acc[kk]) + double(v) * (double(lds[j * KT + kk]) * alpha));
+      }
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk)
+        if (kk < kt) C[i * c_rs + (k0 + kk) * c_cs] = acc[kk];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void rowgather_direct_kernel(
+    int64_t out_rows, int64_t n_rhs, const int32_t* __restrict__ ptr,
+    const int32_t* __restrict__ idx, const T* __restrict__ val, const T* __restrict__ X,
+    int64_t x_rs, int64_t x_cs, T* __restrict__ C, int64_t c_rs, int64_t c_cs, double alpha,
+    double beta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (i >= out_rows) return;
+  const int32_t p0 = ptr[i], pe = ptr[i + 1];
+  for (int64_t k = int64_t(blockIdx.y) * 64 + lane; k < n_rhs; k += int64_t(gridDim.y) * 64) {
+    T* cp = C + i * c_rs + k * c_cs;
+    T acc = (beta == 0.0) ? T(0) : static_cast<T>(beta * double(*cp));
+    for (int32_t p = p0; p < pe; ++p)
+      acc = static_cast<T>(double(acc) + double(val[p]) * (double(X[idx[p] * x_rs + k * x_cs]) * alpha));
+    *cp = acc;
+  }
+}
+
+template <class T, int KT>
+static int launch_rowgather_lds(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, int64_t n_rhs,
+                                const int32_t* ptr, const int32_t* idx, const T* val,
+                                const T* X, int64_t x_cs, T* C, int64_t c_rs, int64_t c_cs,
+                                double alpha, double beta) {
+  int64_t rb_rows = static_cast<int64_t>(kGatherLds / (KT * sizeof(T)));
+  if (in_rows < rb_rows) rb_rows = in_rows > 0 ? in_rows : 1;
+  size_t lds = static_cast<size_t>(rb_rows) * KT * sizeof(T);
+  int nk = static_cast<int>(ceil_div(n_rhs, KT));
+  // Split output rows so that the launch has about two workgroups per CU.
+  int64_t max_parts = ceil_div(out_rows, kGatherThreads);
+  int64_t want = ceil_div(512, nk);
+  int rsplit = static_cast<int>(std::max<int64_t>(1, std::min(max_parts, want)));
+  int64_t rows_per_part = ceil_div(out_rows, rsplit);
+  int64_t groups = ceil_div(nk, 8);
+  int64_t blocks = groups * 8 * rsplit;
+  rowgather_lds_kernel<T, KT><<<dim3(static_cast<unsigned>(blocks)), kGatherThreads, lds,
+                                ctx->stream>>>(out_rows, in_rows, n_rhs, ptr, idx, val, X, x_cs,
+                                               C, c_rs, c_cs, alpha, beta, rb_rows, nk, rsplit,
+                                               rows_per_part);
+  RLA_LAUNCH_CHECK("rowgather_lds");
+  return RLA_OK;
+}
+
+template <class T>
+static int sparse_apply(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                        const int32_t* colptr, const int32_t* rowidx, const T* val,
+                        const int32_t* rowptr, const int32_t* colidx, const T* val_t,
+                        int64_t n_rhs, const T* X, int64_t x_rs, int64_t x_cs, T* C,
+                        int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (nrows < 0 || ncols < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  const int64_t out_rows = trans_m ? ncols : nrows;
+  const int64_t in_rows = trans_m ? nrows : ncols;
+  const int32_t* ptr = trans_m ? colptr : rowptr;
+  const int32_t* idx = trans_m ? rowidx : colidx;
+  const T* v = trans_m ? val : val_t;
+  if (out_rows == 0 || n_rhs == 0) return RLA_OK;
+  if (!ptr || !idx || !v || !C || (in_rows > 0 && !X))
+    return fail(RLA_EARG, trans_m ? "sparse apply with op = M' needs the CSC arrays"
+                                  : "sparse apply with op = M needs the CSR arrays "
+                                    "(rla_sparse_transpose)");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const double a = alpha, bt = beta;
+  if (x_rs == 1 && in_rows > 0) {
+    // Wider column tiles amortise the index reads; narrower ones keep enough
+    // workgroups in flight.
+    const int64_t kt_target = n_rhs / 512;
+    if (sizeof(T) == 4 && kt_target >= 4)
+      return launch_rowgather_lds<T, 4>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
+                                        c_rs, c_cs, a, bt);
+    if (kt_target >= 2)
+      return launch_rowgather_lds<T, 2>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
+                                        c_rs, c_cs, a, bt);
+    return launch_rowgather_lds<T, 1>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
+                                      c_rs, c_cs, a, bt);
+  }
+  int64_t gy = std::min<int64_t>(ceil_div(n_rhs, 64), 1024);
+  rowgather_direct_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(out_rows, 4)),
+                                    static_cast<unsigned>(gy)),
+                               256, 0, ctx->stream>>>(out_rows, n_rhs, ptr, idx, v, X, x_rs,
+                                                      x_cs, C, c_rs, c_cs, a, bt);
+  RLA_LAUNCH_CHECK("rowgather_direct");
+  return RLA_OK;
+}
+
+}  // namespace rla
+
+using namespace rla;
+
+extern "C" {
+
+int rla_sparse_transpose_f32(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                             const int32_t* colptr, const int32_t* rowidx, const float* val,
+                             int32_t* rowptr, int32_t* colidx, float* val_t) {
+  return sparse_transpose<float>(ctx, nrows, ncols, nnz, colptr, rowidx, val, rowptr, colidx,
+                                 val_t);
+}
+
+int rla_sparse_transpose_f64(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                             const int32_t* colptr, const int32_t* rowidx, const double* val,
+                             int32_t* rowptr, int32_t* colidx, double* val_t) {
+  return sparse_transpose<double>(ctx, nrows, ncols, nnz, colptr, rowidx, val, rowptr, colidx,
+                                  val_t);
+}
+
+int rla_sparse_apply_f32(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                         const int32_t* colptr, const int32_t* rowidx, const float* val,
+                         const int32_t* rowptr, const int32_t* colidx, const float* val_t,
+                         int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                         int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply<float>(ctx, trans_m, nrows, ncols, colptr, rowidx, val, rowptr, colidx,
+                             val_t, n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_f64(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                         const int32_t* colptr, const int32_t* rowidx, const double* val,
+                         const int32_t* rowptr, const int32_t* colidx, const double* val_t,
+                         int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs, double* C,
+                         int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply<double>(ctx, trans_m, nrows, ncols, colptr, rowidx, val, rowptr, colidx,
+                              val_t, n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""The C ABI loads and exports everything include/rla.h declares; host twins
+of the recipe generators match the Python restatement bit for bit; host-side
+validation mirrors the reference's errors.  No GPU needed."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import restate as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "rla.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rla_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(rla):
+    lib = rla._lib.load()
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bound_symbols_are_declared(rla):
+    assert set(rla._lib.exported_symbols()) <= set(_declared())
+
+
+def test_version_string(rla):
+    assert b"gfx950" in rla._lib.load().rla_version()
+
+
+def test_host_sparse_sign_gen_matches_restatement(rla):
+    lib = rla._lib.load()
+    n, s, nnz = 300, 50, 7
+    idx = np.zeros(n * nnz, np.int32)
+    val = np.zeros(n * nnz)
+    assert lib.rla_host_sparse_sign_gen_f64(77, 3, n, s, nnz, idx.ctypes.data,
+                                            val.ctypes.data) == 0
+    ridx, rval = R.sparse_sign_gen(77, 3, n, s, nnz)
+    assert np.array_equal(idx, ridx) and np.array_equal(val, rval)
+
+
+def test_host_count_sketch_gen_matches_restatement(rla):
+    lib = rla._lib.load()
+    n, s = 1000, 37
+    idx = np.zeros(n, np.int32)
+    val = np.zeros(n)
+    assert lib.rla_host_count_sketch_gen_f64(5, 9, n, s, idx.ctypes.data, val.ctypes.data) == 0
+    ridx, rval = R.count_sketch_gen(5, 9, n, s)
+    assert np.array_equal(idx, ridx) and np.array_equal(val, rval)
+
+
+@pytest.mark.parametrize("n_pad,s", [(2, 1), (64, 64), (1000 and 1024, 300), (8, 5)])
+def test_host_srht_gen_matches_restatement(rla, n_pad, s):
+    lib = rla._lib.load()
+    op = np.zeros(s, np.int32)
+    ch = np.zeros((n_pad + 63) // 64, np.uint64)
+    assert lib.rla_host_srht_gen(12, 4, n_pad, s, op.ctypes.data, ch.ctypes.data) == 0
+    rop, rsg = R.srht_gen(12, 4, n_pad, s)
+    assert np.array_equal(op, rop)
+    bits = ((ch[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(bool).ravel()
+    assert np.array_equal(bits[:n_pad], rsg) and not bits[n_pad:].any()
+
+
+def test_host_gaussian_gen_is_standard_normal(rla):
+    lib = rla._lib.load()
+    r, c = 400, 250
+    out = np.zeros((c, r))  # column-major r x c
+    assert lib.rla_host_gaussian_gen_f64(3, 0, r, c, 1.0, out.ctypes.data, r) == 0
+    z = out.ravel()
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    # Box–Muller pair 0 from the restated stream
+    st = R.Stream(3, 0, R.DOM_GAUSS, 0)
+    u1, u2 = st.open01(), st.open01()
+    rr = np.sqrt(-2 * np.log(u1))
+    assert abs(out[0, 0] - rr * np.cos(2 * np.pi * u2)) < 1e-14
+    assert abs(out[0, 1] - rr * np.sin(2 * np.pi * u2)) < 1e-14
+
+
+def test_host_gen_rejects_bad_arguments(rla):
+    lib = rla._lib.load()
+    idx = np.zeros(16, np.int32)
+    val = np.zeros(16)
+    assert lib.rla_host_sparse_sign_gen_f64(1, 0, 2, 4, 8, idx.ctypes.data,
+                                            val.ctypes.data) == rla._lib.RLA_EARG
+    assert b"must be less than or equal" in lib.rla_last_error()
+    assert lib.rla_host_srht_gen(1, 0, 12, 3, idx.ctypes.data, None) == rla._lib.RLA_EDIM
+
+
+def test_technique_validation_mirrors_reference(rla):
+    with pytest.raises(rla.ArgumentError, match="compression_dim` must be positive"):
+        rla.SparseSign(compression_dim=0)
+    with pytest.raises(rla.ArgumentError, match="Number of non-zero indices, 9"):
+        rla.SparseSign(compression_dim=4, nnz=9)
+    with pytest.raises(rla.ArgumentError, match="`nnz` must be positive"):
+        rla.SparseSign(compression_dim=4, nnz=0)
+    s = rla.SparseSign(compression_dim=10, nnz=5)
+    with pytest.raises(rla.ArgumentError, match="New `compression_dim`, 3"):
+        s.compression_dim = 3
+    with pytest.raises(rla.ArgumentError, match="New `nnz`, 11"):
+        s.nnz = 11
+    assert rla.SparseSign().nnz == 2 and rla.SparseSign(compression_dim=20).nnz == 8
+    with pytest.raises(rla.ArgumentError):
+        rla.CountSketch(cardinality=rla.Undef())
+    with pytest.raises(rla.ArgumentError, match="block_size"):
+        rla.SRHT(block_size=0)
+    with pytest.raises(rla.ArgumentError, match="Left` or `Right"):
+        rla.SRHT(cardinality=rla.Undef())
+
+
+class _FakeRecipe:
+    pass
+
+
+def test_dimension_checks_mirror_reference_messages(rla):
+    class Rec(rla.CompressorRecipe):
+        n_rows, n_cols = 4, 10
+    S = Rec()
+    A = torch.zeros(9, 3)
+    C = torch.zeros(4, 3)
+    with pytest.raises(rla.DimensionMismatch, match="Matrix A has 9 rows while S has 10 columns."):
+        rla.left_mul_dimcheck(C, S, A)
+    with pytest.raises(rla.DimensionMismatch,
+                       match="Matrix A has 3 columns while C has 2 columns."):
+        rla.left_mul_dimcheck(torch.zeros(4, 2), S, torch.zeros(10, 3))
+    with pytest.raises(rla.DimensionMismatch, match="Matrix C has 5 rows while S has 4 rows."):
+        rla.left_mul_dimcheck(torch.zeros(5, 3), S, torch.zeros(10, 3))
+    with pytest.raises(rla.DimensionMismatch, match="Matrix A has 3 columns while S has 4 rows."):
+        rla.right_mul_dimcheck(torch.zeros(2, 10), torch.zeros(2, 3), S)
+    with pytest.raises(rla.DimensionMismatch,
+                       match="Matrix C has 9 columns while S has 10 columns."):
+        rla.right_mul_dimcheck(torch.zeros(2, 9), torch.zeros(2, 4), S)
+    with pytest.raises(rla.DimensionMismatch, match="Matrix C has 3 rows while A has 2 rows."):
+        rla.right_mul_dimcheck(torch.zeros(3, 10), torch.zeros(2, 4), S)
+    # adjoint forwarding: S' is 10 x 4
+    rla.left_mul_dimcheck(torch.zeros(10, 2), S.T, torch.zeros(4, 2))
+    with pytest.raises(rla.DomainError):
+        S.size(3)
+    assert S.T.size() == (10, 4)
+
+
+def test_mul_without_recipe_is_argument_error(rla):
+    with pytest.raises(rla.ArgumentError):
+        rla.mul(torch.zeros(2, 2), torch.zeros(2, 2), torch.zeros(2, 2))

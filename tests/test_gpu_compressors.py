@@ -1,0 +1,295 @@
+"""GPU parity of the compressor kernels against the oracle (bit for bit where
+the reference arithmetic is fixed) and against closed forms."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import restate as R
+
+pytestmark = pytest.mark.gpu
+
+F64, F32 = torch.float64, torch.float32
+NP = {F64: np.float64, F32: np.float32}
+
+
+def colmajor(a, dev):
+    """Device copy of a host array with Julia (column-major) storage."""
+    t = torch.as_tensor(np.asarray(a))
+    if t.dim() == 1:
+        return t.to(dev)
+    return t.T.contiguous().T.to(dev)
+
+
+def rnd(shape, seed, dtype=np.float64):
+    return np.random.default_rng(seed).standard_normal(shape).astype(dtype)
+
+
+# ---- generators --------------------------------------------------------------
+
+def test_device_sparse_sign_gen_equals_host(rla, dev):
+    lib = rla._lib
+    n, s, nnz = 5000, 300, 8
+    idx = torch.empty(n * nnz, dtype=torch.int32, device=dev)
+    val = torch.empty(n * nnz, dtype=F64, device=dev)
+    lib.call("rla_sparse_sign_gen_f64", lib.ctx(dev), 1234, 7, n, s, nnz, lib.ptr(idx),
+             lib.ptr(val))
+    hi = np.zeros(n * nnz, np.int32)
+    hv = np.zeros(n * nnz)
+    lib.load().rla_host_sparse_sign_gen_f64(1234, 7, n, s, nnz, hi.ctypes.data, hv.ctypes.data)
+    assert np.array_equal(idx.cpu().numpy(), hi) and np.array_equal(val.cpu().numpy(), hv)
+
+
+def test_device_count_sketch_gen_equals_host(rla, dev):
+    lib = rla._lib
+    n, s = 20000, 123
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    val = torch.empty(n, dtype=F64, device=dev)
+    lib.call("rla_count_sketch_gen_f64", lib.ctx(dev), 9, 1, n, s, lib.ptr(idx), lib.ptr(val))
+    hi = np.zeros(n, np.int32)
+    hv = np.zeros(n)
+    lib.load().rla_host_count_sketch_gen_f64(9, 1, n, s, hi.ctypes.data, hv.ctypes.data)
+    assert np.array_equal(idx.cpu().numpy(), hi) and np.array_equal(val.cpu().numpy(), hv)
+
+
+@pytest.mark.parametrize("n_pad,s", [(65536, 2048), (8, 3), (2, 2)])
+def test_device_srht_gen_equals_host(rla, dev, n_pad, s):
+    lib = rla._lib
+    op = torch.empty(s, dtype=torch.int32, device=dev)
+    ch = torch.empty((n_pad + 63) // 64, dtype=torch.int64, device=dev)
+    lib.call("rla_srht_gen", lib.ctx(dev), 42, 3, n_pad, s, lib.ptr(op), lib.ptr(ch))
+    hop = np.zeros(s, np.int32)
+    hch = np.zeros((n_pad + 63) // 64, np.uint64)
+    lib.load().rla_host_srht_gen(42, 3, n_pad, s, hop.ctypes.data, hch.ctypes.data)
+    assert np.array_equal(op.cpu().numpy(), hop)
+    assert np.array_equal(ch.cpu().numpy().view(np.uint64), hch)
+
+
+def test_device_gaussian_gen_matches_host(rla, dev):
+    lib = rla._lib
+    r, c = 301, 77
+    out = torch.empty(c, r, dtype=F64, device=dev).T
+    lib.call("rla_gaussian_gen_f64", lib.ctx(dev), 5, 0, r, c, 0.5, lib.ptr(out), r)
+    h = np.zeros((c, r))
+    lib.load().rla_host_gaussian_gen_f64(5, 0, r, c, 0.5, h.ctypes.data, r)
+    assert np.max(np.abs(out.cpu().numpy() - h.T)) < 1e-14
+
+
+# ---- FWHT --------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+@pytest.mark.parametrize("n,k", [(2, 3), (16, 5), (4096, 7), (16384, 2), (65536, 3),
+                                 (1 << 18, 2)])
+@pytest.mark.parametrize("with_signs", [True, False])
+def test_fwht_bit_exact(rla, dev, dtype, n, k, with_signs):
+    x = rnd((n, k), n + k, NP[dtype])
+    sg = np.random.default_rng(n).random(n) < 0.5 if with_signs else None
+    sc = 0.125 if with_signs else 1.0
+    xd = colmajor(x, dev)
+    rla.fwht(xd, torch.from_numpy(sg) if sg is not None else None, sc)
+    want = R.fwht(x, sg, sc, NP[dtype])
+    assert np.array_equal(xd.cpu().numpy(), want)
+
+
+def test_fwht_rejects_non_power_of_two(rla, dev):
+    with pytest.raises(rla.DimensionMismatch):
+        rla.fwht(torch.zeros(12, 2, device=dev, dtype=F64))
+
+
+# ---- SRHT ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+@pytest.mark.parametrize("m,n,s", [(8, 3, 4), (1000, 7, 64), (4096, 33, 512), (5000, 4, 100),
+                                   (32768, 6, 1024), (65536, 16, 2048), (100000, 2, 300)])
+def test_srht_left_bit_exact(rla, dev, dtype, m, n, s):
+    rla.seed(m + n)
+    A = rnd((m, n), m, NP[dtype])
+    Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=dtype), Ad)
+    assert S.n_pad == R.padded_size(m) and S.size() == (s, m)
+    C0 = rnd((s, n), s, NP[dtype])
+    Cd = colmajor(C0, dev)
+    rla.mul(Cd, S, Ad, 1.5, 0.25)
+    want = R.srht_left_mul(A, S.op.cpu().numpy(), S.sign_vector(), S.scale, NP[dtype], C0, 1.5,
+                           0.25)
+    assert np.array_equal(Cd.cpu().numpy(), want)
+
+
+def _srht_dense(S, m):
+    """The s x m operator T from the oracle, float64."""
+    return R.srht_left_mul(np.eye(m), S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float64)
+
+
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+def test_srht_all_orientations(rla, dev, dtype, tol):
+    rla.seed(3)
+    m, n, s = 300, 5, 40
+    A = rnd((m, n), 1, NP[dtype])
+    # Left recipe on A (m x n): S is s x m
+    SL = rla.complete_compressor(rla.SRHT(compression_dim=s, type=dtype), colmajor(A, dev))
+    T = _srht_dense(SL, m)
+    B = rnd((n, m), 2, NP[dtype])       # B*S: (n x m)*(m?) no: A*S needs A cols == s rows
+    X = rnd((7, s), 3, NP[dtype])      # X*S with S s x m -> 7 x m
+    C = torch.zeros(7, m, dtype=dtype, device=dev)
+    rla.mul(C, colmajor(X, dev), SL)
+    assert np.max(np.abs(C.cpu().numpy() - X @ T)) < tol * np.max(np.abs(X @ T))
+    Y = rnd((s, 4), 4, NP[dtype])     # S'*Y -> m x 4
+    C2 = torch.zeros(m, 4, dtype=dtype, device=dev)
+    rla.mul(C2, SL.T, colmajor(Y, dev))
+    assert np.max(np.abs(C2.cpu().numpy() - T.T @ Y)) < tol * np.max(np.abs(T.T @ Y))
+    C3 = torch.zeros(n, s, dtype=dtype, device=dev)  # B*S' -> n x s
+    rla.mul(C3, colmajor(B, dev), SL.T)
+    assert np.max(np.abs(C3.cpu().numpy() - B @ T.T)) < tol * np.max(np.abs(B @ T.T))
+    # Right recipe on A: S is n x s; S = T_R' with T_R over size(A, 2)
+    A2 = rnd((9, m), 5, NP[dtype])
+    SR = rla.complete_compressor(rla.SRHT(cardinality=rla.Right(), compression_dim=s,
+                                          type=dtype), colmajor(A2, dev))
+    assert SR.size() == (m, s)
+    TR = _srht_dense(SR, m)             # s x m, S = TR'
+    C4 = torch.zeros(9, s, dtype=dtype, device=dev)
+    rla.mul(C4, colmajor(A2, dev), SR)
+    want4 = A2 @ TR.T
+    assert np.max(np.abs(C4.cpu().numpy() - want4)) < tol * np.max(np.abs(want4))
+    Z = rnd((s, 3), 6, NP[dtype])
+    C5 = torch.zeros(m, 3, dtype=dtype, device=dev)
+    rla.mul(C5, SR, colmajor(Z, dev))
+    want5 = TR.T @ Z
+    assert np.max(np.abs(C5.cpu().numpy() - want5)) < tol * np.max(np.abs(want5))
+
+
+def test_srht_right_orientation_bit_exact(rla, dev):
+    # A*S with a Right recipe is the forward transform on A' (srht.jl:366-429)
+    rla.seed(8)
+    A = rnd((6, 1000), 9, np.float32)
+    Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SRHT(cardinality=rla.Right(), compression_dim=50,
+                                         type=F32), Ad)
+    C = torch.zeros(6, 50, dtype=F32, device=dev)
+    rla.mul(C, Ad, S)
+    want = R.srht_left_mul(A.T, S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float32).T
+    assert np.array_equal(C.cpu().numpy(), want)
+
+
+def test_srht_update_changes_operator(rla, dev):
+    A = colmajor(rnd((256, 3), 0), dev)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=32), A)
+    op0, sg0 = S.op.clone(), S.signs.clone()
+    rla.update_compressor(S)
+    assert not torch.equal(op0, S.op) and not torch.equal(sg0, S.signs)
+
+
+# ---- SparseSign / CountSketch ----------------------------------------------------
+
+def _dense_op(S):
+    M = S.op.dense()
+    return M if S.cardinality == rla_left() else M.T
+
+
+def rla_left():
+    import __graft_entry__
+    return __graft_entry__.load_package().Left()
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("kind", ["sparse_sign", "count_sketch"])
+def test_sparse_left_bit_exact(rla, dev, dtype, kind):
+    rla.seed(11)
+    m, n, s = 3000, 37, 200
+    A = rnd((m, n), 2, NP[dtype])
+    Ad = colmajor(A, dev)
+    tech = (rla.SparseSign(compression_dim=s, nnz=8, type=dtype) if kind == "sparse_sign"
+            else rla.CountSketch(compression_dim=s, type=dtype))
+    S = rla.complete_compressor(tech, Ad)
+    M = S.op.dense()
+    C0 = rnd((s, n), 3, NP[dtype])
+    Cd = colmajor(C0, dev)
+    rla.mul(Cd, S, Ad, 0.5, 1.5)
+    want = R.sparse_mul(M, A, 0.5, 1.5, C0, NP[dtype])
+    assert np.array_equal(Cd.cpu().numpy(), want)
+    dense = 0.5 * M @ A.astype(np.float64) + 1.5 * C0.astype(np.float64)
+    tol = 1e-12 if dtype == F64 else 1e-5
+    assert np.max(np.abs(Cd.cpu().numpy() - dense)) < tol * np.max(np.abs(dense))
+
+
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+@pytest.mark.parametrize("kind", ["sparse_sign", "count_sketch"])
+@pytest.mark.parametrize("card", ["left", "right"])
+def test_sparse_all_orientations(rla, dev, dtype, tol, kind, card):
+    # the four products of test/Compressors/sparse_sign.jl:299-351 for each cardinality
+    rla.seed(21)
+    cardinality = rla.Left() if card == "left" else rla.Right()
+    m, n, s = 400, 300, 64
+    A = rnd((m, n), 4, NP[dtype])
+    tech = (rla.SparseSign(cardinality=cardinality, compression_dim=s, nnz=5, type=dtype)
+            if kind == "sparse_sign" else
+            rla.CountSketch(cardinality=cardinality, compression_dim=s, type=dtype))
+    S = rla.complete_compressor(tech, colmajor(A, dev))
+    M = S.op.dense()
+    Sd = M if card == "left" else M.T           # the operator S itself
+    r, c = Sd.shape
+    X = rnd((c, 6), 5, NP[dtype])
+    Y = rnd((7, r), 6, NP[dtype])
+    cases = [
+        (lambda C, X_: rla.mul(C, S, X_), X, Sd @ X),                 # S*X
+        (lambda C, X_: rla.mul(C, X_, S), rnd((7, r), 7, NP[dtype]), None),  # X*S
+        (lambda C, X_: rla.mul(C, S.T, X_), rnd((r, 3), 8, NP[dtype]), None),  # S'*X
+        (lambda C, X_: rla.mul(C, X_, S.T), rnd((5, c), 9, NP[dtype]), None),  # X*S'
+    ]
+    wants = [Sd @ cases[0][1], cases[1][1] @ Sd, Sd.T @ cases[2][1], cases[3][1] @ Sd.T]
+    for (fn, Xh, _), want in zip(cases, wants):
+        C = torch.full(want.shape, float("nan"), dtype=dtype, device=dev)
+        fn(C, colmajor(Xh, dev))
+        got = C.cpu().numpy()
+        assert np.max(np.abs(got - want)) < tol * np.max(np.abs(want)), (kind, card)
+    del Y
+
+
+def test_sparse_vector_and_beta(rla, dev):
+    rla.seed(4)
+    b = rnd(500, 1)
+    bd = torch.as_tensor(b, device=dev)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=30), colmajor(rnd((500, 2), 1),
+                                                                             dev))
+    c = torch.as_tensor(rnd(30, 2), device=dev)
+    c0 = c.cpu().numpy().copy()
+    rla.mul(c, S, bd, 2.0, -1.0)
+    want = R.sparse_mul(S.op.dense(), b, 2.0, -1.0, c0)
+    assert np.array_equal(c.cpu().numpy(), want)
+    out = S @ bd
+    assert out.shape == (30,)
+
+
+def test_sparse_sign_config_c1_bit_exact(rla, dev):
+    # 8192 x 256 Float64 N(0,1) (seed 1234), s = 1024, nnz = 8
+    rla.seed(1234)
+    A = rnd((8192, 256), 1234)
+    Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=1024, nnz=8), Ad)
+    C = torch.zeros(256, 1024, dtype=F64, device=dev).T
+    rla.mul(C, S, Ad)
+    M = S.op.dense()
+    assert np.all((M != 0).sum(axis=0) == 8)
+    got = C.cpu().numpy()
+    assert np.array_equal(got, R.sparse_mul(M, A))
+    dense = M @ A
+    assert np.max(np.abs(got - dense)) <= 1e-12 * np.max(np.abs(dense))
+
+
+def test_sparse_update_changes_operator(rla, dev):
+    A = colmajor(rnd((100, 3), 0), dev)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=20), A)
+    r0 = S.op.rowidx.clone()
+    rla.update_compressor(S)
+    assert not torch.equal(r0, S.op.rowidx)
+    C = torch.zeros(20, 3, dtype=F64, device=dev)
+    rla.mul(C, S, A)
+    want = S.op.dense() @ A.cpu().numpy()
+    assert np.allclose(C.cpu().numpy(), want, rtol=0, atol=1e-12)
+
+
+def test_mul_dimension_mismatch_on_device(rla, dev):
+    A = colmajor(rnd((100, 3), 0), dev)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=20), A)
+    with pytest.raises(rla.DimensionMismatch, match="Matrix C has 19 rows while S has 20 rows."):
+        rla.mul(torch.zeros(19, 3, dtype=F64, device=dev), S, A)

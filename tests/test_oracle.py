@@ -1,0 +1,219 @@
+"""Pins the oracle (oracle/restate.py, oracle/rla_oracle.c) against closed
+forms, invariants and known-answer vectors, independently of the GPU."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import restate as R
+
+
+# Philox4x32-10 known-answer vectors published with Random123 (kat_vectors).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_kat_python(ctr, key, want):
+    assert tuple(R.philox4x32_10(list(ctr), list(key))) == want
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_kat_library(rla, ctr, key, want):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    out = (ctypes.c_uint32 * 4)()
+    rla._lib.load().rla_philox4x32_10(c, k, out)
+    assert tuple(out) == want
+
+
+def _signs(n, seed):
+    return np.random.default_rng(seed).random(n) < 0.5
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 256, 1024])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_fwht_matches_dense_hadamard(n, dtype, tol):
+    # fwht.jl tests compare against Hadamard.hadamard(n) * x (Sylvester order)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, 3)).astype(dtype)
+    sg = _signs(n, n + 1)
+    sc = 0.37
+    got = R.fwht(x, sg, sc, dtype)
+    H = scipy.linalg.hadamard(n).astype(np.float64)
+    want = H @ (np.where(sg, sc, -sc)[:, None] * x.astype(np.float64))
+    assert np.max(np.abs(got - want)) <= tol * np.max(np.abs(want))
+    # sign flipping really matters: the unsigned transform differs
+    if n >= 4:
+        assert not np.allclose(R.fwht(x, None, sc, dtype), got)
+
+
+def test_fwht_self_inverse():
+    # fwht.jl test: H(Hx)/n == x  (scaling = 1/n on the second application)
+    n = 4096
+    x = np.random.default_rng(3).standard_normal((n, 2))
+    y = R.fwht(R.fwht(x, None, 1.0), None, 1.0 / n)
+    assert np.max(np.abs(y - x)) < 1e-12
+
+
+def test_fwht_stage_order_matters_in_float32():
+    # a different stage order would change float32 rounding; the restatement
+    # follows h = 1, 2, 4, ...  Check against a scalar transcription of
+    # fwht.jl:21-70 on one column.
+    n = 256
+    x = np.random.default_rng(5).standard_normal(n).astype(np.float32) * 1e3
+    sg = _signs(n, 6)
+    sc = float(np.float32(math.sqrt(16) / math.sqrt(n)))
+    ref = x.copy()
+    for i in range(0, n, 2):
+        z = float(ref[i]) * (sc if sg[i] else -sc)
+        y = float(ref[i + 1]) * (sc if sg[i + 1] else -sc)
+        ref[i], ref[i + 1] = np.float32(z + y), np.float32(z - y)
+    h = 2
+    while h < n:
+        for i in range(0, n, 2 * h):
+            for j in range(i, i + h):
+                z, y = ref[j], ref[j + h]
+                ref[j], ref[j + h] = np.float32(z + y), np.float32(z - y)
+        h *= 2
+    assert np.array_equal(R.fwht(x, sg, sc, np.float32), ref)
+
+
+def _pack(sg):
+    n = sg.shape[0]
+    pad = np.zeros(((n + 63) // 64) * 64, dtype=np.uint64)
+    pad[:n] = sg
+    return (pad.reshape(-1, 64) << np.arange(64, dtype=np.uint64)).sum(1, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fwht_numpy_equals_c_restatement(coracle, dtype):
+    n, k = 2048, 5
+    x = np.random.default_rng(7).standard_normal((n, k)).astype(dtype)
+    sg = _signs(n, 8)
+    ch = _pack(sg)
+    want = R.fwht(x, sg, 0.25, dtype)
+    xf = np.asfortranarray(x.copy())
+    fn = coracle.oracle_fwht_f32 if dtype == np.float32 else coracle.oracle_fwht_f64
+    fn(xf.ctypes.data, n, k, n, ch.ctypes.data, 0.25, 2)
+    assert np.array_equal(xf, want)
+
+
+@pytest.mark.parametrize("m,s", [(8, 4), (5, 3), (1000, 64), (4096, 4096)])
+def test_srht_left_matches_reference_test_formula(m, s):
+    # test/Compressors/srht.jl:291-296: (H * (signs .* pad(A)) * alpha * scale)[op, :] + beta*C
+    rng = np.random.default_rng(m)
+    A = rng.standard_normal((m, 4))
+    n_pad = R.padded_size(m)
+    op = rng.choice(n_pad, s, replace=False)
+    sg = _signs(n_pad, m + 2)
+    scale = math.sqrt(s) / math.sqrt(n_pad)
+    C0 = rng.standard_normal((s, 4))
+    got = R.srht_left_mul(A, op, sg, scale, np.float64, C0, 1.5, 0.5)
+    P = np.zeros((n_pad, 4))
+    P[:m] = A
+    H = scipy.linalg.hadamard(n_pad).astype(float)
+    want = (H @ (np.where(sg, 1.0, -1.0)[:, None] * P) * 1.5 * scale)[op] + 0.5 * C0
+    assert np.max(np.abs(got - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
+    # textbook special case: s = n_pad, identity sample, all signs +: scaled Hadamard
+    if m == s == n_pad:
+        got2 = R.srht_left_mul(A, np.arange(n_pad), np.ones(n_pad, bool), 1.0, np.float64)
+        assert np.allclose(got2, H @ A, rtol=0, atol=1e-9)
+
+
+def test_srht_adjoint_is_transpose_of_forward():
+    rng = np.random.default_rng(11)
+    m, s = 300, 40
+    n_pad = R.padded_size(m)
+    op = rng.choice(n_pad, s, replace=False)
+    sg = _signs(n_pad, 12)
+    scale = math.sqrt(s) / math.sqrt(n_pad)
+    T = R.srht_left_mul(np.eye(m), op, sg, scale, np.float64)   # s x m operator
+    X = rng.standard_normal((s, 3))
+    got = R.srht_adjoint_mul(X, op, sg, scale, m, np.float64, None, 2.0, 0.0)
+    assert np.max(np.abs(got - 2.0 * T.T @ X)) < 1e-12
+
+
+@pytest.mark.parametrize("m,s,nnz", [(10, 6, 3), (300, 40, 8), (64, 64, 64)])
+def test_sparse_rowgather_matches_dense_product(m, s, nnz):
+    idx, val = R.sparse_sign_gen(99, 0, m, s, nnz)
+    M = np.zeros((s, m))
+    for j in range(m):
+        M[idx[j * nnz:(j + 1) * nnz], j] = val[j * nnz:(j + 1) * nnz]
+    rng = np.random.default_rng(m)
+    X = rng.standard_normal((m, 5))
+    C0 = rng.standard_normal((s, 5))
+    got = R.sparse_mul(M, X, 0.75, 2.0, C0)
+    want = 0.75 * M @ X + 2.0 * C0
+    assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    # M' * Y (the Right / adjoint orientation) through the CSC lists
+    Y = rng.standard_normal((s, 2))
+    gotT = R.sparse_mul(M.T, Y)
+    assert np.max(np.abs(gotT - M.T @ Y)) < 1e-12 * max(1, np.max(np.abs(M.T @ Y)))
+
+
+def test_sparse_gather_equals_c_scatter_loop(coracle):
+    # the gather restatement and the CSC scatter loop of SparseArrays produce
+    # the same sums in the same order, hence identical bits
+    m, s, nnz, k = 2000, 128, 8, 6
+    idx, val = R.sparse_sign_gen(5, 1, m, s, nnz)
+    M = np.zeros((s, m))
+    for j in range(m):
+        M[idx[j * nnz:(j + 1) * nnz], j] = val[j * nnz:(j + 1) * nnz]
+    X = np.asfortranarray(np.random.default_rng(1).standard_normal((m, k)))
+    C = np.asfortranarray(np.random.default_rng(2).standard_normal((s, k)))
+    want = R.sparse_mul(M, X, 1.25, 0.5, C)
+    colptr = np.arange(0, m * nnz + 1, nnz, dtype=np.int32)
+    coracle.oracle_sparse_left_f64(colptr.ctypes.data, idx.ctypes.data, val.ctypes.data, s, m,
+                                   X.ctypes.data, k, m, C.ctypes.data, s, 1.25, 0.5, 2)
+    assert np.array_equal(C, want)
+
+
+def test_sparse_sign_gen_structure():
+    idx, val = R.sparse_sign_gen(1234, 0, 500, 32, 8)
+    for j in range(500):
+        r = idx[j * 8:(j + 1) * 8]
+        assert np.all(np.diff(r) > 0) and r.min() >= 0 and r.max() < 32   # distinct, sorted
+    assert set(np.unique(np.abs(val))) == {1 / math.sqrt(8)}
+    frac = np.mean(val > 0)
+    assert 0.45 < frac < 0.55
+    # rows uniform: chi-square over 32 bins, 4000 draws
+    counts = np.bincount(idx, minlength=32)
+    chi2 = ((counts - 125.0) ** 2 / 125.0).sum()
+    assert chi2 < 70  # 31 dof, p ~ 1e-4
+
+
+def test_floyd_is_uniform_over_subsets():
+    # all C(5,2) = 10 subsets of a 2-sample from 5 should appear equally often
+    from collections import Counter
+    c = Counter(tuple(R.floyd_sorted(R.Stream(7, 0, 1, j), 5, 2)) for j in range(5000))
+    assert len(c) == 10
+    chi2 = sum((v - 500.0) ** 2 / 500.0 for v in c.values())
+    assert chi2 < 33  # 9 dof, p ~ 1e-4
+
+
+def test_lemire_below_is_unbiased():
+    st = R.Stream(3, 0, 1, 0)
+    draws = np.array([st.below(7) for _ in range(14000)])
+    counts = np.bincount(draws, minlength=7)
+    assert counts.min() > 1800 and counts.max() < 2200
+
+
+def test_srht_gen_structure():
+    op, sg = R.srht_gen(1, 2, 1024, 100)
+    assert len(set(op.tolist())) == 100 and op.min() >= 0 and op.max() < 1024
+    assert not np.all(np.diff(op) > 0)  # unordered, like sample(..., replace=false)
+    assert 0.4 < sg.mean() < 0.6
+
+
+@pytest.mark.parametrize("n,want", [(1, 1), (2, 2), (3, 4), (8, 8), (1000, 1024), (65536, 65536),
+                                    (2 ** 29, 2 ** 30), (2 ** 29 + 1, 2 ** 30),
+                                    (2 ** 28, 2 ** 28)])
+def test_padded_size_matches_log_ratio_quirk(n, want):
+    assert R.padded_size(n) == want

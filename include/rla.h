@@ -157,15 +157,30 @@ int rla_fwht_f64(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_ch
  * Left S*A is adjoint 0 on A; Right A*S is adjoint 0 on A' (Y = C');
  * Left A*S is adjoint 1 on A'; Right S*A is adjoint 1 on A.
  * Replaces mul! at src/Compressors/srht.jl:220-504.  scale is the recipe's
- * stored Float64 (reference: type(sqrt(s)/sqrt(n_pad)), srht.jl:152). */
+ * stored Float64 (reference: type(sqrt(s)/sqrt(n_pad)), srht.jl:152).
+ * plan: from rla_srht_plan for this op (NULL: built in the workspace on each
+ * call).  adjoint = 0 with unit row stride and 2^11 <= n_pad <= 2^16 (Float32)
+ * or 2^15 (Float64) runs as one fused kernel; other shapes use the multi-pass
+ * path.  beta == 0 does not read Y.  Bit-identical to the reference loops for
+ * adjoint = 0. */
 int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
-                       double scale, const float* X, int64_t x_rs, int64_t x_cs, float* Y,
-                       int64_t y_rs, int64_t y_cs, double alpha, double beta);
+                       double scale, const void* plan, const float* X, int64_t x_rs,
+                       int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta);
 int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
-                       double scale, const double* X, int64_t x_rs, int64_t x_cs, double* Y,
-                       int64_t y_rs, int64_t y_cs, double alpha, double beta);
+                       double scale, const void* plan, const double* X, int64_t x_rs,
+                       int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta);
+
+/* Sampled-row plan of an SRHT recipe (bitmap of needed positions, slot ->
+ * output-row map) for the fused forward kernel; rebuild after the recipe's op
+ * changes (update_compressor!).  plan_bytes is 0 when the shape has no fused
+ * kernel (then pass plan = NULL). */
+int rla_srht_plan_bytes(int elem_bytes, int64_t n_pad, int64_t s, size_t* bytes);
+int rla_srht_plan(rla_ctx* ctx, int elem_bytes, int64_t n_pad, int64_t s, const int32_t* op,
+                  void* plan);
 
 #ifdef __cplusplus
 }

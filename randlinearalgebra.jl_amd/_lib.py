@@ -79,10 +79,12 @@ _SIGS = {
                              i64, i64, f64, f64],
     "rla_fwht_f32": [vp, i64, i64, vp, f64, vp, i64],
     "rla_fwht_f64": [vp, i64, i64, vp, f64, vp, i64],
-    "rla_srht_apply_f32": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, i64, i64, vp, i64, i64,
-                           f64, f64],
-    "rla_srht_apply_f64": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, i64, i64, vp, i64, i64,
-                           f64, f64],
+    "rla_srht_apply_f32": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, vp, i64, i64, vp, i64,
+                           i64, f64, f64],
+    "rla_srht_apply_f64": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, vp, i64, i64, vp, i64,
+                           i64, f64, f64],
+    "rla_srht_plan_bytes": [i32, i64, i64, vp],
+    "rla_srht_plan": [vp, i32, i64, i64, vp, vp],
 }
 _RESTYPE = {"rla_version": ctypes.c_char_p, "rla_last_error": ctypes.c_char_p,
             "rla_philox4x32_10": None}

@@ -14,6 +14,7 @@ stand for (A[i, j] is row i, column j); any strides are accepted, so Julia
 views, transposes and column-major storage all map directly.  All arithmetic
 runs in librla_mi355x.so; nothing here computes on the host.
 """
+import ctypes
 import math
 
 import torch
@@ -594,9 +595,25 @@ class SRHTRecipe(CompressorRecipe):
         self.seed, self.epoch = sd, ep
         _lib.call("rla_srht_gen", _lib.ctx(self.op.device), sd, ep, self.n_pad, self.s,
                   _lib.ptr(self.op), _lib.ptr(self.signs))
+        self._plans = {}
 
     def _update(self, *args):
         self._gen()
+
+    def plan(self, dtype):
+        """Device plan of the fused forward kernel for element type `dtype`
+        (None when the shape has no fused kernel); rebuilt after updates."""
+        if dtype not in self._plans:
+            elem = 4 if dtype == torch.float32 else 8
+            nb = ctypes.c_size_t(0)
+            _lib.call("rla_srht_plan_bytes", elem, self.n_pad, self.s, ctypes.byref(nb))
+            pl = None
+            if nb.value:
+                pl = torch.empty((nb.value + 3) // 4, dtype=torch.int32, device=self.op.device)
+                _lib.call("rla_srht_plan", _lib.ctx(self.op.device), elem, self.n_pad, self.s,
+                          _lib.ptr(self.op), _lib.ptr(pl))
+            self._plans[dtype] = pl
+        return self._plans[dtype]
 
     def sign_vector(self):
         """The signs as a host bool array of length n_pad (BitVector order)."""
@@ -607,9 +624,11 @@ class SRHTRecipe(CompressorRecipe):
 
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
+        plan = self.plan(dt) if not adjoint else None
         _lib.call(f"rla_srht_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(adjoint), self.dim,
                   self.n_pad, self.s, X.cols, _lib.ptr(self.op), _lib.ptr(self.signs),
-                  self.scale, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t), Y.rs, Y.cs, alpha, beta)
+                  self.scale, _lib.ptr(plan), _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t), Y.rs, Y.cs,
+                  alpha, beta)
 
     def _left(self, C, A, alpha, beta):
         # Left S*A: T*A (srht.jl:220); Right S*A: T'*A (srht.jl:432)

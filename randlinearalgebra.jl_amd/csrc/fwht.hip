@@ -223,15 +223,15 @@ static int64_t srht_chunk_cols(int64_t n_pad, int64_t ncols) {
 
 template <class T>
 int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
-                       const int32_t* op, const uint64_t* signs, double scale, const T* X,
-                       int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs, double alpha,
-                       double beta, bool* handled);
+                       const int32_t* op, const uint64_t* signs, double scale, const void* plan,
+                       const T* X, int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs,
+                       double alpha, double beta, bool* handled);
 
 template <class T>
 static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                       int64_t ncols, const int32_t* op, const uint64_t* signs, double scale,
-                      const T* X, int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs,
-                      double alpha, double beta) {
+                      const void* plan, const T* X, int64_t x_rs, int64_t x_cs, T* Y,
+                      int64_t y_rs, int64_t y_cs, double alpha, double beta) {
   RLA_CHECK_CTX(ctx);
   if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
   if (m < 1 || m > n_pad || s < 1 || s > n_pad || ncols < 0)
@@ -243,8 +243,8 @@ static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64
   const int L = ilog2(n_pad);
   if (!adjoint) {
     bool handled = false;
-    rc = srht_fused_forward<T>(ctx, m, n_pad, s, ncols, op, signs, scale, X, x_rs, x_cs, Y, y_rs,
-                               y_cs, alpha, beta, &handled);
+    rc = srht_fused_forward<T>(ctx, m, n_pad, s, ncols, op, signs, scale, plan, X, x_rs, x_cs, Y,
+                               y_rs, y_cs, alpha, beta, &handled);
     if (rc || handled) return rc;
   }
   const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
@@ -296,18 +296,20 @@ int rla_fwht_f64(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_ch
 
 int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
-                       double scale, const float* X, int64_t x_rs, int64_t x_cs, float* Y,
-                       int64_t y_rs, int64_t y_cs, double alpha, double beta) {
-  return srht_apply<float>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, X, x_rs,
-                           x_cs, Y, y_rs, y_cs, alpha, beta);
+                       double scale, const void* plan, const float* X, int64_t x_rs,
+                       int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta) {
+  return srht_apply<float>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
+                           x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
 }
 
 int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
-                       double scale, const double* X, int64_t x_rs, int64_t x_cs, double* Y,
-                       int64_t y_rs, int64_t y_cs, double alpha, double beta) {
-  return srht_apply<double>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, X, x_rs,
-                            x_cs, Y, y_rs, y_cs, alpha, beta);
+                       double scale, const void* plan, const double* X, int64_t x_rs,
+                       int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta) {
+  return srht_apply<double>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
+                            x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
 }
 
 }  // extern "C"

@@ -10,17 +10,18 @@
 // order (fwht.jl:21-70), so results are bit-identical to it.
 //
 // Only the sampled outputs are needed.  With two chunks the last stage
-// (bit LC) combines entry e of chunk 0 with entry e of chunk 1; a per-recipe
-// plan lists the "needed" positions e (those with e or e + 2^LC sampled) as a
-// bitmap + per-word prefix counts (slot numbers) and maps each slot to its
-// output rows.  Chunk 0's needed values wait in LDS (one float per slot);
-// chunk 1 finishes the last stage for exactly those slots and writes the
-// rows.  Global traffic is one read of X and one write of the s sampled rows.
+// (bit LC) combines entry e of chunk 0 with entry e of chunk 1.  A per-recipe
+// plan says, for every thread, which of its 32 registers hold a "needed"
+// entry after the last phase (e or e + 2^LC sampled) as a 32-bit mask, and
+// where its needed entries start in a compact slot numbering; each slot maps
+// to its output rows.  Chunk 0's needed values wait in LDS (one value per
+// slot); chunk 1 finishes the last stage for exactly those slots and writes
+// the rows.  Global traffic is one read of X and one write of the s rows.
 //
-// LDS layout: the chunk (2^LC entries) with an XOR swizzle of the 16-byte
-// granule index by bits 5..7 of the entry index, which makes the contiguous
-// (phase 0) and strided (later phases) register<->LDS moves conflict-free,
-// followed by the slot buffer.
+// LDS layout: entry e at e + (e >> 5) (one pad word per 32 entries).  Every
+// register<->LDS move is then thread base + compile-time offset, and the
+// contiguous (phase 0), strided (later phases) and load-order moves all hit
+// 32 distinct banks per 32 lanes.  The slot buffer follows the chunk.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -32,11 +33,7 @@ namespace rla {
 
 constexpr int kVals = 32;  // entries per thread in a chunk
 
-template <class T>
-__device__ inline int swz(int e) {
-  constexpr int lg_epg = sizeof(T) == 4 ? 2 : 1;  // entries per 16-byte granule
-  return e ^ (((e >> 5) & 7) << lg_epg);
-}
+__device__ __forceinline__ int pad(int e) { return e + (e >> 5); }
 
 // entry index of register v of thread t when the register window starts at bit w
 template <int W>
@@ -45,7 +42,7 @@ __device__ inline int map_entry(int t, int v) {
 }
 
 template <int RB, class T>
-__device__ inline void stage(T (&x)[kVals]) {
+__device__ __forceinline__ void stage(T (&x)[kVals]) {
 #pragma unroll
   for (int v = 0; v < kVals; ++v) {
     if (!(v & (1 << RB))) {
@@ -57,7 +54,7 @@ __device__ inline void stage(T (&x)[kVals]) {
 }
 
 template <class T>
-__device__ inline void stage_range(T (&x)[kVals], int b0, int b1) {
+__device__ __forceinline__ void stage_range(T (&x)[kVals], int b0, int b1) {
   // b0/b1 are compile-time constants at every call site, so the branches fold
   if (b0 <= 0 && 0 < b1) stage<0>(x);
   if (b0 <= 1 && 1 < b1) stage<1>(x);
@@ -69,7 +66,7 @@ __device__ inline void stage_range(T (&x)[kVals], int b0, int b1) {
 // Stage 0 of the transform with the sign flips and scaling, in Float64 as in
 // the reference (x[i] * (signs[i] ? scaling : -scaling), then z + y, z - y).
 template <class T>
-__device__ inline void stage0_signed(T (&x)[kVals], uint32_t sw, double sc) {
+__device__ __forceinline__ void stage0_signed(T (&x)[kVals], uint32_t sw, double sc) {
 #pragma unroll
   for (int v = 0; v < kVals; v += 2) {
     const double s0 = ((sw >> v) & 1u) ? sc : -sc;
@@ -88,9 +85,9 @@ struct Phases {
   static constexpr int B0 = 5 * P - W;
   static constexpr int B1 = ((5 * P + 5 < LC) ? 5 * P + 5 : LC) - W;
 
-  static __device__ inline void run(T (&x)[kVals], T* lds, int t, uint32_t sw, double sc) {
+  static __device__ __forceinline__ void run(T (&x)[kVals], T* lds, int t, uint32_t sw, double sc) {
 #pragma unroll
-    for (int v = 0; v < kVals; ++v) x[v] = lds[swz<T>(map_entry<W>(t, v))];
+    for (int v = 0; v < kVals; ++v) x[v] = lds[pad(map_entry<W>(t, v))];
     if (P == 0) {
       stage0_signed<T>(x, sw, sc);
       stage_range<T>(x, 1, B1);
@@ -100,7 +97,7 @@ struct Phases {
     if constexpr (P + 1 < NPH) {
       __syncthreads();
 #pragma unroll
-      for (int v = 0; v < kVals; ++v) lds[swz<T>(map_entry<W>(t, v))] = x[v];
+      for (int v = 0; v < kVals; ++v) lds[pad(map_entry<W>(t, v))] = x[v];
       __syncthreads();
       Phases<T, LC, P + 1>::run(x, lds, t, sw, sc);
     }
@@ -112,14 +109,6 @@ struct LastWindow {
   static constexpr int value = ((LC + 4) / 5 - 1) * 5 < LC - 5 ? ((LC + 4) / 5 - 1) * 5 : LC - 5;
 };
 
-__device__ inline int slot_of(const uint32_t* __restrict__ mask,
-                              const uint32_t* __restrict__ prefix, int e) {
-  const uint32_t w = mask[e >> 5];
-  const uint32_t b = 1u << (e & 31);
-  if (!(w & b)) return -1;
-  return static_cast<int>(prefix[e >> 5]) + __popc(w & (b - 1u));
-}
-
 template <class T>
 __device__ inline void emit(T* __restrict__ Y, int64_t y_rs, int64_t ycol, int r, double val,
                             double alpha, double beta) {
@@ -130,21 +119,23 @@ __device__ inline void emit(T* __restrict__ Y, int64_t y_rs, int64_t ycol, int r
 template <class T, int LC, int NCH>
 __global__ __launch_bounds__(1 << (LC - 5)) void srht_fused_kernel(
     const T* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
-    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ mask,
-    const uint32_t* __restrict__ prefix, const int32_t* __restrict__ rowmap,
+    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ tmask,
+    const uint32_t* __restrict__ tstart, const int32_t* __restrict__ rowmap,
     T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta, int vec_ok) {
   constexpr int TB = LC - 5;
   constexpr int CH = 1 << LC;
   constexpr int W = LastWindow<T, LC>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  T* tbuf = lds + CH;
+  T* tbuf = lds + CH + CH / 32;
   const int t = threadIdx.x;
+  const uint32_t my_mask = tmask[t];
+  const int my_start = static_cast<int>(tstart[t]);
   T x[kVals];
 
   for (int64_t col = blockIdx.x; col < ncols; col += gridDim.x) {
     const T* xc = X + col * x_cs;
-#pragma unroll
+#pragma unroll 1
     for (int ch = 0; ch < NCH; ++ch) {
       const int64_t base = int64_t(ch) << LC;
       // coalesced load: register (vh, q) holds entry (vh << (TB+2)) | (t << 2) | q
@@ -177,33 +168,35 @@ __global__ __launch_bounds__(1 << (LC - 5)) void srht_fused_kernel(
 #pragma unroll
       for (int vh = 0; vh < 8; ++vh)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) lds[swz<T>((vh << (TB + 2)) | (t << 2) | q)] = x[4 * vh + q];
+        for (int q = 0; q < 4; ++q) lds[pad((vh << (TB + 2)) | (t << 2) | q)] = x[4 * vh + q];
       __syncthreads();
       Phases<T, LC, 0>::run(x, lds, t, sw, scale);
 
       const int64_t ycol = col * y_cs;
+      int slot = my_start;
       if constexpr (NCH == 1) {
 #pragma unroll
         for (int v = 0; v < kVals; ++v) {
-          const int slot = slot_of(mask, prefix, map_entry<W>(t, v));
-          if (slot >= 0) emit<T>(Y, y_rs, ycol, rowmap[slot], double(x[v]), alpha, beta);
+          if ((my_mask >> v) & 1u) {
+            emit<T>(Y, y_rs, ycol, rowmap[slot], double(x[v]), alpha, beta);
+            ++slot;
+          }
         }
       } else {
         if (ch == 0) {
 #pragma unroll
           for (int v = 0; v < kVals; ++v) {
-            const int slot = slot_of(mask, prefix, map_entry<W>(t, v));
-            if (slot >= 0) tbuf[slot] = x[v];
+            if ((my_mask >> v) & 1u) tbuf[slot++] = x[v];
           }
         } else {
 #pragma unroll
           for (int v = 0; v < kVals; ++v) {
-            const int slot = slot_of(mask, prefix, map_entry<W>(t, v));
-            if (slot >= 0) {
+            if ((my_mask >> v) & 1u) {
               const T z = tbuf[slot], y = x[v];
               const int r0 = rowmap[2 * slot], r1 = rowmap[2 * slot + 1];
               if (r0 >= 0) emit<T>(Y, y_rs, ycol, r0, double(T(z + y)), alpha, beta);
               if (r1 >= 0) emit<T>(Y, y_rs, ycol, r1, double(T(z - y)), alpha, beta);
+              ++slot;
             }
           }
         }
@@ -214,10 +207,13 @@ __global__ __launch_bounds__(1 << (LC - 5)) void srht_fused_kernel(
 
 // ---- plan ------------------------------------------------------------------
 // layout (32-bit words): [0] magic, [1] LC, [2] NCH, [3] s, [4] n_pad lo,
-// [5] n_pad hi, [6] elem bytes, [7] reserved; then mask[2^LC/32],
-// prefix[2^LC/32], rowmap[s*NCH].
+// [5] n_pad hi, [6] elem bytes, [7] reserved; then tmask[2^(LC-5)],
+// tstart[2^(LC-5)], rowmap[s*NCH].  tmask[t] bit v: register v of thread t
+// holds a needed entry after the last phase; tstart[t]: slot of the first one
+// (exclusive scan of the mask popcounts); rowmap[slot*NCH + c]: output row of
+// position c*2^LC + e, or -1.
 
-constexpr uint32_t kPlanMagic = 0x52484C50u;  // "PLHR"
+constexpr uint32_t kPlanMagic = 0x52484C50u;
 constexpr int kHdr = 8;
 
 static bool fused_shape(int elem, int64_t n_pad, int& LC, int& NCH) {
@@ -233,13 +229,16 @@ static size_t plan_words(int LC, int NCH, int64_t s) {
   return kHdr + 2 * (size_t(1) << (LC - 5)) + static_cast<size_t>(s) * NCH;
 }
 
+// register window of the last phase (matches LastWindow<T, LC>)
+static int last_window(int LC) { return std::min(((LC + 4) / 5 - 1) * 5, LC - 5); }
+
 __global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int64_t s, int64_t n_pad,
                                  int elem) {
-  const int64_t words = int64_t(1) << (LC - 5);
-  const int64_t total = 2 * words + s * NCH;
+  const int64_t nt = int64_t(1) << (LC - 5);
+  const int64_t total = 2 * nt + s * NCH;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total;
        i += int64_t(gridDim.x) * blockDim.x) {
-    plan[kHdr + i] = i < 2 * words ? 0u : 0xFFFFFFFFu;  // rowmap = -1
+    plan[kHdr + i] = i < 2 * nt ? 0u : 0xFFFFFFFFu;  // rowmap = -1
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     plan[0] = kPlanMagic;
@@ -253,21 +252,27 @@ __global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int64_t s, int
   }
 }
 
-__global__ void plan_mark_kernel(uint32_t* mask, const int32_t* __restrict__ op, int64_t s,
-                                 int LC) {
-  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (r >= s) return;
-  const int e = op[r] & ((1 << LC) - 1);
-  atomicOr(&mask[e >> 5], 1u << (e & 31));
+__device__ inline void owner_of(int e, int W, int& t, int& v) {
+  v = (e >> W) & 31;
+  t = ((e >> (W + 5)) << W) | (e & ((1 << W) - 1));
 }
 
-__global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __restrict__ mask,
-                                                           uint32_t* prefix, int words) {
+__global__ void plan_mark_kernel(uint32_t* tmask, const int32_t* __restrict__ op, int64_t s,
+                                 int LC, int W) {
+  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (r >= s) return;
+  int t, v;
+  owner_of(op[r] & ((1 << LC) - 1), W, t, v);
+  atomicOr(&tmask[t], 1u << v);
+}
+
+__global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __restrict__ tmask,
+                                                           uint32_t* tstart, int nt) {
   __shared__ uint32_t part[1024];
-  const int per = (words + 1023) / 1024;
+  const int per = (nt + 1023) / 1024;
   const int b = threadIdx.x * per;
   uint32_t s = 0;
-  for (int i = b; i < b + per && i < words; ++i) s += __popc(mask[i]);
+  for (int i = b; i < b + per && i < nt; ++i) s += __popc(tmask[i]);
   part[threadIdx.x] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -277,21 +282,22 @@ __global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __res
     __syncthreads();
   }
   uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (int i = b; i < b + per && i < words; ++i) {
-    prefix[i] = run;
-    run += __popc(mask[i]);
+  for (int i = b; i < b + per && i < nt; ++i) {
+    tstart[i] = run;
+    run += __popc(tmask[i]);
   }
 }
 
-__global__ void plan_rowmap_kernel(const uint32_t* __restrict__ mask,
-                                   const uint32_t* __restrict__ prefix, int32_t* rowmap,
-                                   const int32_t* __restrict__ op, int64_t s, int LC, int NCH) {
+__global__ void plan_rowmap_kernel(const uint32_t* __restrict__ tmask,
+                                   const uint32_t* __restrict__ tstart, int32_t* rowmap,
+                                   const int32_t* __restrict__ op, int64_t s, int LC, int NCH,
+                                   int W) {
   const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (r >= s) return;
   const int64_t pos = op[r];
-  const int e = static_cast<int>(pos & ((1 << LC) - 1));
-  const uint32_t w = mask[e >> 5];
-  const int slot = static_cast<int>(prefix[e >> 5]) + __popc(w & ((1u << (e & 31)) - 1u));
+  int t, v;
+  owner_of(static_cast<int>(pos & ((1 << LC) - 1)), W, t, v);
+  const int slot = static_cast<int>(tstart[t]) + __popc(tmask[t] & ((1u << v) - 1u));
   rowmap[int64_t(slot) * NCH + (pos >> LC)] = static_cast<int32_t>(r);
 }
 
@@ -299,16 +305,17 @@ static int build_plan(rla_ctx* ctx, int elem, int64_t n_pad, int64_t s, const in
                       uint32_t* plan) {
   int LC = 0, NCH = 0;
   if (!fused_shape(elem, n_pad, LC, NCH)) return RLA_OK;
-  const int words = 1 << (LC - 5);
+  const int nt = 1 << (LC - 5);
+  const int W = last_window(LC);
   hipStream_t st = ctx->stream;
   plan_init_kernel<<<64, 256, 0, st>>>(plan, LC, NCH, s, n_pad, elem);
-  uint32_t* mask = plan + kHdr;
-  uint32_t* prefix = mask + words;
-  int32_t* rowmap = reinterpret_cast<int32_t*>(prefix + words);
-  plan_mark_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(mask, op, s, LC);
-  plan_prefix_kernel<<<1, 1024, 0, st>>>(mask, prefix, words);
-  plan_rowmap_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(mask, prefix, rowmap, op,
-                                                                      s, LC, NCH);
+  uint32_t* tmask = plan + kHdr;
+  uint32_t* tstart = tmask + nt;
+  int32_t* rowmap = reinterpret_cast<int32_t*>(tstart + nt);
+  plan_mark_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(tmask, op, s, LC, W);
+  plan_prefix_kernel<<<1, 1024, 0, st>>>(tmask, tstart, nt);
+  plan_rowmap_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(tmask, tstart, rowmap,
+                                                                      op, s, LC, NCH, W);
   RLA_LAUNCH_CHECK("srht_plan");
   return RLA_OK;
 }
@@ -326,12 +333,13 @@ template <class T, int LC, int NCH>
 static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64_t ncols,
                         const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
                         T* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
-  const int words = 1 << (LC - 5);
-  const uint32_t* mask = plan + kHdr;
-  const uint32_t* prefix = mask + words;
-  const int32_t* rowmap = reinterpret_cast<const int32_t*>(prefix + words);
+  const int nt = 1 << (LC - 5);
+  const uint32_t* tmask = plan + kHdr;
+  const uint32_t* tstart = tmask + nt;
+  const int32_t* rowmap = reinterpret_cast<const int32_t*>(tstart + nt);
   const int64_t nslots = std::min<int64_t>(s, int64_t(1) << LC);
-  const size_t lds = (size_t(1) << LC) * sizeof(T) + (NCH > 1 ? nslots * sizeof(T) : 0);
+  const size_t lds = ((size_t(1) << LC) + (size_t(1) << (LC - 5))) * sizeof(T) +
+                     (NCH > 1 ? nslots * sizeof(T) : 0);
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
@@ -349,7 +357,7 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
   srht_fused_kernel<T, LC, NCH><<<dim3(static_cast<unsigned>(grid)), threads, lds,
                                   ctx->stream>>>(X, x_cs, m, ncols,
                                                  reinterpret_cast<const uint32_t*>(signs), scale,
-                                                 mask, prefix, rowmap, Y, y_rs, y_cs, alpha,
+                                                 tmask, tstart, rowmap, Y, y_rs, y_cs, alpha,
                                                  beta, vec_ok);
   RLA_LAUNCH_CHECK("srht_fused");
   return RLA_OK;
@@ -364,8 +372,8 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
   int LC = 0, NCH = 0;
   if (x_rs != 1 || !fused_shape(sizeof(T), n_pad, LC, NCH)) return RLA_OK;
   // the slot buffer of a 2-chunk column must fit next to the chunk in LDS
-  if (NCH == 2 && (size_t(1) << LC) * sizeof(T) + size_t(std::min<int64_t>(s, 1 << LC)) *
-                                                    sizeof(T) > 160 * 1024)
+  if (NCH == 2 && ((size_t(1) << LC) + (size_t(1) << (LC - 5)) +
+                   size_t(std::min<int64_t>(s, 1 << LC))) * sizeof(T) > 160 * 1024)
     return RLA_OK;
   const uint32_t* pl = static_cast<const uint32_t*>(plan);
   if (!pl) {

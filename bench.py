@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--skip-sparse", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SRHT apply has no multi-GPU path in this build "
@@ -134,7 +137,13 @@ def main():
     rla = __graft_entry__.load_package()
     ms, wall, exact, S, A0 = gpu_srht(rla, torch, args.steps, args.warmup)
     gbs = srht_bytes() / ms / 1e6
-    extra = gpu_sparse_sign(rla, torch, args.steps, args.warmup)
+    extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
+    if args.calib_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "srht_f32_65536x512_traffic.txt")
     if os.path.exists(tpath):

@@ -181,12 +181,21 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
     const int64_t j0 = blk * rb_rows;
     const int64_t jn = in_rows - j0 < rb_rows ? in_rows - j0 : rb_rows;
     __syncthreads();
+    // stage X[j0:j0+jn, k0:k0+kt] as lds[j*KT + kk], 8 loads in flight per thread
     for (int kk = 0; kk < KT; ++kk) {
-      if (kk < kt) {
-        const T* xc = X + (k0 + kk) * x_cs + j0;
-        for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j * KT + kk] = xc[j];
-      } else {
-        for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j * KT + kk] = T(0);
+      const T* xc = X + (k0 + kk) * x_cs + j0;
+      for (int64_t jb = 0; jb < jn; jb += 8 * int64_t(blockDim.x)) {
+        T tmp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t j = jb + u * int64_t(blockDim.x) + threadIdx.x;
+          tmp[u] = (kk < kt && j < jn) ? xc[j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t j = jb + u * int64_t(blockDim.x) + threadIdx.x;
+          if (j < jn) lds[j * KT + kk] = tmp[u];
+        }
       }
     }
     __syncthreads();
@@ -206,6 +215,27 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
         } else {
           acc[kk] = T(0);
         }
+      }
+      // 8 nonzeros per batch: their index/value loads and LDS reads are issued
+      // together; the sums stay in ascending order.
+      for (; p + 8 <= pe; p += 8) {
+        int32_t jj[8];
+        T vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          jj[u] = idx[p + u];
+          vv[u] = val[p + u];
+        }
+        T xv[8][KT];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) xv[u][kk] = lds[(jj[u] - j0) * KT + kk];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk)
+            acc[kk] = static_cast<T>(double(acc[kk]) + double(vv[u]) * (double(xv[u][kk]) * alpha));
       }
       for (; p < pe; ++p) {
         const int64_t j = idx[p] - j0;
@@ -234,7 +264,19 @@ __global__ __launch_bounds__(256) void rowgather_direct_kernel(
   for (int64_t k = int64_t(blockIdx.y) * 64 + lane; k < n_rhs; k += int64_t(gridDim.y) * 64) {
     T* cp = C + i * c_rs + k * c_cs;
     T acc = (beta == 0.0) ? T(0) : static_cast<T>(beta * double(*cp));
-    for (int32_t p = p0; p < pe; ++p)
+    int32_t p = p0;
+    for (; p + 8 <= pe; p += 8) {
+      T xv[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        vv[u] = val[p + u];
+        xv[u] = X[idx[p + u] * x_rs + k * x_cs];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        acc = static_cast<T>(double(acc) + double(vv[u]) * (double(xv[u]) * alpha));
+    }
+    for (; p < pe; ++p)
       acc = static_cast<T>(double(acc) + double(val[p]) * (double(X[idx[p] * x_rs + k * x_cs]) * alpha));
     *cp = acc;
   }

@@ -198,15 +198,18 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
   T x[R];
   T nx[PF ? R : 1];
 
-  const int64_t items = ncols * NCH;
-  int64_t it = blockIdx.x;
+  // This workgroup's items: columns blockIdx.x, blockIdx.x + gridDim.x, ...,
+  // each as its NCH chunks in order (chunk 0's slot values wait in LDS for
+  // chunk 1 of the same column).
+  const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t my_items = my_cols * NCH;
+  auto item_col = [&](int64_t k) { return blockIdx.x + (k / NCH) * int64_t(gridDim.x); };
   if constexpr (PF) {
-    if (it < items)
-      load_chunk<T, R, TB>(nx, X + (it / NCH) * x_cs, int64_t(it % NCH) << LC, m, t, vec_ok);
+    if (my_items > 0) load_chunk<T, R, TB>(nx, X + item_col(0) * x_cs, 0, m, t, vec_ok);
   }
-  for (; it < items; it += gridDim.x) {
-    const int64_t col = it / NCH;
-    const int ch = static_cast<int>(it % NCH);
+  for (int64_t k = 0; k < my_items; ++k) {
+    const int64_t col = item_col(k);
+    const int ch = static_cast<int>(k % NCH);
     const int64_t base = int64_t(ch) << LC;
     if constexpr (PF) {
 #pragma unroll
@@ -231,9 +234,9 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
         for (int q = 0; q < 4; ++q) lb[load_off<TB>(vh, q)] = x[4 * vh + q];
     }
     if constexpr (PF) {
-      const int64_t nit = it + gridDim.x;
-      if (nit < items)
-        load_chunk<T, R, TB>(nx, X + (nit / NCH) * x_cs, int64_t(nit % NCH) << LC, m, t, vec_ok);
+      if (k + 1 < my_items)
+        load_chunk<T, R, TB>(nx, X + item_col(k + 1) * x_cs, int64_t((k + 1) % NCH) << LC, m, t,
+                             vec_ok);
     }
     __syncthreads();
     Phases<T, LC, VB, 0>::run(x, lds, t, sw, scale);
@@ -435,7 +438,7 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
   // workgroups resident per CU, limited by LDS (160 KiB) and by 16 waves / CU
   const int per_cu = std::max<int>(1, std::min<int>(static_cast<int>((160 * 1024) / lds),
                                                     std::max(1, 1024 / nt)));
-  const int64_t grid = std::min<int64_t>(ncols * NCH, int64_t(num_cus(ctx->device)) * per_cu);
+  const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
   const int vec_ok = (x_cs % (sizeof(T) == 4 ? 4 : 2) == 0) &&
                      (reinterpret_cast<uintptr_t>(X) % 16 == 0);
   srht_fused_kernel<T, LC, NCH, VB, PF><<<dim3(static_cast<unsigned>(grid)), nt, lds,

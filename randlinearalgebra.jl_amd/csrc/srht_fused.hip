@@ -179,19 +179,23 @@ __device__ __forceinline__ void load_chunk(T (&x)[R], const T* __restrict__ xc, 
   }
 }
 
-template <class T, int LC, int NCH, int VB, bool PF>
+template <class T, int LC, int NCH, int VB, bool PF, bool SCR_LDS>
 __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
     const T* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
     const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ tmask,
     const uint32_t* __restrict__ tstart, const int32_t* __restrict__ rowmap,
-    T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta, int vec_ok) {
+    T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta, int vec_ok,
+    T* __restrict__ gscr, int nslots_cap) {
   using G = Geometry<LC, VB>;
   constexpr int R = G::R;
   constexpr int TB = G::TB;
   constexpr int CH = 1 << LC;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  T* tbuf = lds + CH + CH / 32;
+  // needed-slot values of chunks 0 .. NCH-2, tb[c * nslots_cap + slot]; each
+  // thread only reads back what it wrote itself
+  T* tb = SCR_LDS ? lds + CH + CH / 32
+                  : gscr + int64_t(blockIdx.x) * (NCH > 1 ? NCH - 1 : 0) * nslots_cap;
   const int t = threadIdx.x;
   const uint64_t my_mask = uint64_t(tmask[2 * t]) | (uint64_t(tmask[2 * t + 1]) << 32);
   const int my_start = static_cast<int>(tstart[t]);
@@ -252,19 +256,35 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
         }
       }
     } else {
-      if (ch == 0) {
+      if (ch < NCH - 1) {
+        T* tc = tb + int64_t(ch) * nslots_cap;
 #pragma unroll
         for (int v = 0; v < R; ++v) {
-          if ((my_mask >> v) & 1u) tbuf[slot++] = x[v];
+          if ((my_mask >> v) & 1u) tc[slot++] = x[v];
         }
       } else {
 #pragma unroll
         for (int v = 0; v < R; ++v) {
           if ((my_mask >> v) & 1u) {
-            const T z = tbuf[slot], y = x[v];
-            const int r0 = rowmap[2 * slot], r1 = rowmap[2 * slot + 1];
-            if (r0 >= 0) emit<T>(Y, y_rs, ycol, r0, double(T(z + y)), alpha, beta);
-            if (r1 >= 0) emit<T>(Y, y_rs, ycol, r1, double(T(z - y)), alpha, beta);
+            T a[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH - 1; ++c) a[c] = tb[int64_t(c) * nslots_cap + slot];
+            a[NCH - 1] = x[v];
+            // the stages of bits LC, LC+1, ... pair chunks c and c|h, in order
+#pragma unroll
+            for (int h = 1; h < NCH; h <<= 1)
+#pragma unroll
+              for (int c = 0; c < NCH; ++c)
+                if (!(c & h)) {
+                  const T z = a[c], y = a[c | h];
+                  a[c] = z + y;
+                  a[c | h] = z - y;
+                }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+              const int r = rowmap[slot * NCH + c];
+              if (r >= 0) emit<T>(Y, y_rs, ycol, r, double(a[c]), alpha, beta);
+            }
             ++slot;
           }
         }
@@ -279,15 +299,15 @@ struct FusedShape {
   int LC, NCH, VB;
 };
 
-// Float32: chunks up to 2^15 (128 KiB + pad in LDS), 64 registers per thread
-// from 2^12 up; Float64: chunks up to 2^14, 32 registers per thread.
+// Chunks of up to 64 KiB (2^14 Float32 / 2^13 Float64 entries, plus padding)
+// so that two workgroups share a CU; columns of up to 8 chunks.
 static bool fused_shape(int elem, int64_t n_pad, FusedShape& f) {
   const int L = ilog2(n_pad);
-  const int lc_max = elem == 4 ? 15 : 14;
-  if (L < 11 || L > lc_max + 1) return false;
+  const int lc_max = elem == 4 ? 14 : 13;
+  if (L < 11 || L > lc_max + 3) return false;
   f.LC = std::min(L, lc_max);
-  f.NCH = L > lc_max ? 2 : 1;
-  f.VB = (elem == 4 && f.LC >= 12) ? 6 : 5;
+  f.NCH = 1 << (L - f.LC);
+  f.VB = 5;
   return true;
 }
 
@@ -412,15 +432,27 @@ static int num_cus(int device) {
   return p.multiProcessorCount;
 }
 
-static size_t fused_lds_bytes(const FusedShape& f, int elem, int64_t s) {
-  const int64_t nslots = std::min<int64_t>(s, int64_t(1) << f.LC);
-  return ((size_t(1) << f.LC) + (size_t(1) << (f.LC - 5)) + (f.NCH > 1 ? nslots : 0)) * elem;
+static int64_t nslots_cap(const FusedShape& f, int64_t s) {
+  return std::min<int64_t>(s, int64_t(1) << f.LC);
 }
 
-template <class T, int LC, int NCH, int VB, bool PF>
+// slot values of the first NCH-1 chunks live in LDS when that still lets
+// two workgroups share a CU, otherwise in a per-workgroup global scratch
+static bool scratch_in_lds(const FusedShape& f, int elem, int64_t s) {
+  const size_t chunk = ((size_t(1) << f.LC) + (size_t(1) << (f.LC - 5))) * elem;
+  return f.NCH > 1 && 2 * (chunk + (f.NCH - 1) * nslots_cap(f, s) * elem) <= 160 * 1024;
+}
+
+static size_t fused_lds_bytes(const FusedShape& f, int elem, int64_t s) {
+  const size_t chunk = ((size_t(1) << f.LC) + (size_t(1) << (f.LC - 5))) * elem;
+  return chunk + (scratch_in_lds(f, elem, s) ? (f.NCH - 1) * nslots_cap(f, s) * elem : 0);
+}
+
+template <class T, int LC, int NCH, int VB, bool PF, bool SCR_LDS>
 static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64_t ncols,
                         const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
-                        T* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+                        T* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta,
+                        T* gscr) {
   const int nt = 1 << (LC - VB);
   const uint32_t* tmask = plan + kHdr;
   const uint32_t* tstart = tmask + 2 * nt;
@@ -431,20 +463,19 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
     attr_err = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&srht_fused_kernel<T, LC, NCH, VB, PF>),
+        reinterpret_cast<const void*>(&srht_fused_kernel<T, LC, NCH, VB, PF, SCR_LDS>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   });
   RLA_HIP(attr_err);
   // workgroups resident per CU, limited by LDS (160 KiB) and by 16 waves / CU
-  const int per_cu = std::max<int>(1, std::min<int>(static_cast<int>((160 * 1024) / lds),
-                                                    std::max(1, 1024 / nt)));
+  const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / lds));
   const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
   const int vec_ok = (x_cs % (sizeof(T) == 4 ? 4 : 2) == 0) &&
                      (reinterpret_cast<uintptr_t>(X) % 16 == 0);
-  srht_fused_kernel<T, LC, NCH, VB, PF><<<dim3(static_cast<unsigned>(grid)), nt, lds,
-                                          ctx->stream>>>(
+  srht_fused_kernel<T, LC, NCH, VB, PF, SCR_LDS><<<dim3(static_cast<unsigned>(grid)), nt, lds,
+                                                   ctx->stream>>>(
       X, x_cs, m, ncols, reinterpret_cast<const uint32_t*>(signs), scale, tmask, tstart, rowmap,
-      Y, y_rs, y_cs, alpha, beta, vec_ok);
+      Y, y_rs, y_cs, alpha, beta, vec_ok, gscr, static_cast<int>(nslots_cap(f, s)));
   RLA_LAUNCH_CHECK("srht_fused");
   return RLA_OK;
 }
@@ -458,27 +489,40 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
   FusedShape f;
   if (x_rs != 1 || !fused_shape(sizeof(T), n_pad, f)) return RLA_OK;
   if (fused_lds_bytes(f, sizeof(T), s) > 160 * 1024) return RLA_OK;
+  if (s > (int64_t(1) << 30)) return RLA_OK;
+  const bool lds_scr = scratch_in_lds(f, sizeof(T), s);
+  const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / fused_lds_bytes(f, sizeof(T), s)));
+  const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
+  const size_t scr_bytes =
+      (f.NCH > 1 && !lds_scr) ? size_t(grid) * (f.NCH - 1) * nslots_cap(f, s) * sizeof(T) : 0;
+  const size_t plan_bytes = plan ? 0 : plan_words(f, s) * 4;
+  void* ws = nullptr;
+  if (scr_bytes + plan_bytes > 0) {
+    int rc = workspace(ctx, ((plan_bytes + 255) & ~size_t(255)) + scr_bytes, &ws);
+    if (rc) return rc;
+  }
   const uint32_t* pl = static_cast<const uint32_t*>(plan);
   if (!pl) {
-    void* ws = nullptr;
-    int rc = workspace(ctx, plan_words(f, s) * 4, &ws);
-    if (rc) return rc;
-    rc = build_plan(ctx, sizeof(T), n_pad, s, op, static_cast<uint32_t*>(ws));
+    int rc = build_plan(ctx, sizeof(T), n_pad, s, op, static_cast<uint32_t*>(ws));
     if (rc) return rc;
     pl = static_cast<const uint32_t*>(ws);
   }
+  T* gscr = scr_bytes ? reinterpret_cast<T*>(static_cast<char*>(ws) +
+                                             ((plan_bytes + 255) & ~size_t(255)))
+                      : nullptr;
   *handled = true;
-#define RLA_FUSED(LCV, NCHV, VBV, PFV)                                                    \
-  if (f.LC == LCV && f.NCH == NCHV && f.VB == VBV)                                         \
-    return launch_fused<T, LCV, NCHV, VBV, PFV>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, Y, \
-                                                y_rs, y_cs, alpha, beta);
+#define RLA_FUSED(LCV, NCHV, SCRV)                                                            \
+  if (f.LC == LCV && f.NCH == NCHV && lds_scr == SCRV)                                          \
+    return launch_fused<T, LCV, NCHV, 5, false, SCRV>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, \
+                                                      Y, y_rs, y_cs, alpha, beta, gscr);
   if constexpr (sizeof(T) == 4) {
-    RLA_FUSED(11, 1, 5, false)
-    RLA_FUSED(12, 1, 6, true) RLA_FUSED(13, 1, 6, true) RLA_FUSED(14, 1, 6, true)
-    RLA_FUSED(15, 1, 6, false) RLA_FUSED(15, 2, 6, true)
+    RLA_FUSED(11, 1, false) RLA_FUSED(12, 1, false) RLA_FUSED(13, 1, false)
+    RLA_FUSED(14, 1, false) RLA_FUSED(14, 2, true) RLA_FUSED(14, 2, false)
+    RLA_FUSED(14, 4, false) RLA_FUSED(14, 8, false)
   } else {
-    RLA_FUSED(11, 1, 5, false) RLA_FUSED(12, 1, 5, false) RLA_FUSED(13, 1, 5, false)
-    RLA_FUSED(14, 1, 5, false) RLA_FUSED(14, 2, 5, false)
+    RLA_FUSED(11, 1, false) RLA_FUSED(12, 1, false) RLA_FUSED(13, 1, false)
+    RLA_FUSED(13, 2, true) RLA_FUSED(13, 2, false) RLA_FUSED(13, 4, false)
+    RLA_FUSED(13, 8, false)
   }
 #undef RLA_FUSED
   *handled = false;

@@ -102,7 +102,8 @@ def test_fwht_rejects_non_power_of_two(rla, dev):
 @pytest.mark.parametrize("dtype", [F32, F64])
 @pytest.mark.parametrize("m,n,s", [(8, 3, 4), (1000, 7, 64), (4096, 33, 512), (5000, 4, 100),
                                    (32768, 6, 1024), (65536, 16, 2048), (100000, 2, 300),
-                                   (32768, 300, 64), (65536, 300, 256)])
+                                   (32768, 300, 64), (65536, 300, 256), (2000, 3, 2048),
+                                   (16384, 5, 700)])
 def test_srht_left_bit_exact(rla, dev, dtype, m, n, s):
     rla.seed(m + n)
     A = rnd((m, n), m, NP[dtype])

@@ -13,14 +13,15 @@
 // global loads of the next item are issued into a second register set before
 // the current item is transformed, so HBM streaming overlaps the LDS/ALU work.
 //
-// Only the sampled outputs are needed.  With two chunks the last stage
-// (bit LC) combines entry e of chunk 0 with entry e of chunk 1.  A per-recipe
-// plan says, for every thread, which of its R registers hold a "needed" entry
-// after the last phase (e or e + 2^LC sampled) as a bit mask, and where its
-// needed entries start in a compact slot numbering; each slot maps to its
-// output rows.  Chunk 0's needed values wait in LDS (one value per slot);
-// chunk 1 finishes the last stage for exactly those slots and writes the
-// rows.  Global traffic is one read of X and one write of the s rows.
+// Only the sampled outputs are needed.  With several chunks the last stages
+// (bits LC, LC+1, ...) combine entry e of every chunk.  A per-recipe plan
+// lists the "needed" positions e (e + c*2^LC sampled for some chunk c) in
+// ascending order as slots, and maps each (slot, c) to its output row.  After
+// a chunk's last phase the chunk goes back to LDS and the threads walk the
+// slots: earlier chunks park their slot values (in LDS when it fits, else in
+// a per-workgroup global scratch that the same thread reads back), the last
+// chunk finishes the remaining stages per slot and writes the rows.  Global
+// traffic is one read of X, one write of the s rows, plus the parked slots.
 //
 // LDS layout: entry e at e + (e >> 5) (one pad word per 32 entries).  Every
 // register<->LDS move is then thread base + compile-time offset, with at most
@@ -34,6 +35,8 @@
 #include "rla_internal.hpp"
 
 namespace rla {
+
+constexpr int kPlanHdr = 8;  // words before the plan's bitmap
 
 __device__ __forceinline__ int pad(int e) { return e + (e >> 5); }
 
@@ -182,29 +185,31 @@ __device__ __forceinline__ void load_chunk(T (&x)[R], const T* __restrict__ xc, 
 template <class T, int LC, int NCH, int VB, bool PF, bool SCR_LDS>
 __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
     const T* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
-    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ tmask,
-    const uint32_t* __restrict__ tstart, const int32_t* __restrict__ rowmap,
+    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ plan,
     T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta, int vec_ok,
     T* __restrict__ gscr, int nslots_cap) {
   using G = Geometry<LC, VB>;
   constexpr int R = G::R;
   constexpr int TB = G::TB;
+  constexpr int NT = 1 << TB;
   constexpr int CH = 1 << LC;
+  constexpr int WL = G::WLAST;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  // needed-slot values of chunks 0 .. NCH-2, tb[c * nslots_cap + slot]; each
-  // thread only reads back what it wrote itself
+  const int t = threadIdx.x;
+  const int nslots = static_cast<int>(plan[7]);
+  const int32_t* __restrict__ slot_pos =
+      reinterpret_cast<const int32_t*>(plan + kPlanHdr + 2 * (CH >> 5));
+  const int32_t* __restrict__ rowmap = slot_pos + nslots_cap;
+  // values parked by chunks 0 .. NCH-2, tb[c * nslots_cap + slot]; a slot is
+  // always handled by thread slot % NT, which reads back only its own writes
   T* tb = SCR_LDS ? lds + CH + CH / 32
                   : gscr + int64_t(blockIdx.x) * (NCH > 1 ? NCH - 1 : 0) * nslots_cap;
-  const int t = threadIdx.x;
-  const uint64_t my_mask = uint64_t(tmask[2 * t]) | (uint64_t(tmask[2 * t + 1]) << 32);
-  const int my_start = static_cast<int>(tstart[t]);
   T x[R];
   T nx[PF ? R : 1];
 
   // This workgroup's items: columns blockIdx.x, blockIdx.x + gridDim.x, ...,
-  // each as its NCH chunks in order (chunk 0's slot values wait in LDS for
-  // chunk 1 of the same column).
+  // each as its NCH chunks in order.
   const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
   const int64_t my_items = my_cols * NCH;
   auto item_col = [&](int64_t k) { return blockIdx.x + (k / NCH) * int64_t(gridDim.x); };
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
     } else {
       sw = signs32[(base >> 5) + t];
     }
-    __syncthreads();  // the previous item's readers of lds / tbuf are done
+    __syncthreads();  // the previous item's readers of lds are done
     {
       T* lb = lds + load_base<TB>(t);
 #pragma unroll
@@ -244,48 +249,42 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
     }
     __syncthreads();
     Phases<T, LC, VB, 0>::run(x, lds, t, sw, scale);
-
-    const int64_t ycol = col * y_cs;
-    int slot = my_start;
-    if constexpr (NCH == 1) {
+    // the transformed chunk back to LDS, then slot-parallel output
+    __syncthreads();
+    {
+      T* lb = lds + lds_base<WL, VB>(t);
 #pragma unroll
-      for (int v = 0; v < R; ++v) {
-        if ((my_mask >> v) & 1u) {
-          emit<T>(Y, y_rs, ycol, rowmap[slot], double(x[v]), alpha, beta);
-          ++slot;
-        }
-      }
+      for (int v = 0; v < R; ++v) lb[lds_off<WL, VB>(v)] = x[v];
+    }
+    __syncthreads();
+    const int64_t ycol = col * y_cs;
+    if constexpr (NCH == 1) {
+      for (int sl = t; sl < nslots; sl += NT)
+        emit<T>(Y, y_rs, ycol, rowmap[sl], double(lds[pad(slot_pos[sl])]), alpha, beta);
     } else {
       if (ch < NCH - 1) {
         T* tc = tb + int64_t(ch) * nslots_cap;
-#pragma unroll
-        for (int v = 0; v < R; ++v) {
-          if ((my_mask >> v) & 1u) tc[slot++] = x[v];
-        }
+        for (int sl = t; sl < nslots; sl += NT) tc[sl] = lds[pad(slot_pos[sl])];
       } else {
+        for (int sl = t; sl < nslots; sl += NT) {
+          T a[NCH];
 #pragma unroll
-        for (int v = 0; v < R; ++v) {
-          if ((my_mask >> v) & 1u) {
-            T a[NCH];
+          for (int c = 0; c < NCH - 1; ++c) a[c] = tb[int64_t(c) * nslots_cap + sl];
+          a[NCH - 1] = lds[pad(slot_pos[sl])];
+          // the stages of bits LC, LC+1, ... pair chunks c and c|h, in order
 #pragma unroll
-            for (int c = 0; c < NCH - 1; ++c) a[c] = tb[int64_t(c) * nslots_cap + slot];
-            a[NCH - 1] = x[v];
-            // the stages of bits LC, LC+1, ... pair chunks c and c|h, in order
+          for (int h = 1; h < NCH; h <<= 1)
 #pragma unroll
-            for (int h = 1; h < NCH; h <<= 1)
+            for (int c = 0; c < NCH; ++c)
+              if (!(c & h)) {
+                const T z = a[c], y = a[c | h];
+                a[c] = z + y;
+                a[c | h] = z - y;
+              }
 #pragma unroll
-              for (int c = 0; c < NCH; ++c)
-                if (!(c & h)) {
-                  const T z = a[c], y = a[c | h];
-                  a[c] = z + y;
-                  a[c | h] = z - y;
-                }
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-              const int r = rowmap[slot * NCH + c];
-              if (r >= 0) emit<T>(Y, y_rs, ycol, r, double(a[c]), alpha, beta);
-            }
-            ++slot;
+          for (int c = 0; c < NCH; ++c) {
+            const int r = rowmap[int64_t(sl) * NCH + c];
+            if (r >= 0) emit<T>(Y, y_rs, ycol, r, double(a[c]), alpha, beta);
           }
         }
       }
@@ -311,34 +310,31 @@ static bool fused_shape(int elem, int64_t n_pad, FusedShape& f) {
   return true;
 }
 
-static int window_last(const FusedShape& f) {
-  const int nph = (f.LC + f.VB - 1) / f.VB;
-  return std::min(f.VB * (nph - 1), f.LC - f.VB);
-}
 
 // ---- plan ------------------------------------------------------------------
 // layout (32-bit words): [0] magic, [1] LC, [2] NCH, [3] s, [4] n_pad lo,
-// [5] n_pad hi, [6] elem bytes, [7] VB; then tmask[2 * 2^(LC-VB)] (64-bit
-// per thread), tstart[2^(LC-VB)], rowmap[s*NCH].  Bit v of tmask[t]: register
-// v of thread t holds a needed entry after the last phase; tstart[t]: slot of
-// the first one (exclusive scan of the mask popcounts); rowmap[slot*NCH + c]:
-// output row of position c*2^LC + e, or -1.
+// [5] n_pad hi, [6] elem bytes, [7] nslots; [8 .. 8+W) needed-position
+// bitmap, [8+W .. 8+2W) its exclusive per-word prefix counts (W = 2^LC/32);
+// then slot_pos[nslots_cap] (position in the chunk of each slot, ascending)
+// and rowmap[nslots_cap * NCH] (output row of position c*2^LC + slot_pos, or
+// -1).  nslots_cap = min(s, 2^LC).
 
 constexpr uint32_t kPlanMagic = 0x52484C50u;
-constexpr int kHdr = 8;
 
 static size_t plan_words(const FusedShape& f, int64_t s) {
-  const size_t nt = size_t(1) << (f.LC - f.VB);
-  return kHdr + 3 * nt + static_cast<size_t>(s) * f.NCH;
+  const size_t w = size_t(1) << (f.LC - 5);
+  const size_t cap = static_cast<size_t>(std::min<int64_t>(s, int64_t(1) << f.LC));
+  return kPlanHdr + 2 * w + cap * (1 + f.NCH);
 }
 
-__global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int VB, int64_t s,
-                                 int64_t n_pad, int elem) {
-  const int64_t nt = int64_t(1) << (LC - VB);
-  const int64_t total = 3 * nt + s * NCH;
+__global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int64_t s, int64_t n_pad,
+                                 int elem, int64_t cap) {
+  const int64_t w = int64_t(1) << (LC - 5);
+  const int64_t total = 2 * w + cap * (1 + NCH);
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total;
        i += int64_t(gridDim.x) * blockDim.x) {
-    plan[kHdr + i] = i < 3 * nt ? 0u : 0xFFFFFFFFu;  // rowmap = -1
+    // slot_pos and rowmap start at -1
+    plan[kPlanHdr + i] = i < 2 * w ? 0u : 0xFFFFFFFFu;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     plan[0] = kPlanMagic;
@@ -348,31 +344,26 @@ __global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int VB, int64_
     plan[4] = static_cast<uint32_t>(n_pad);
     plan[5] = static_cast<uint32_t>(n_pad >> 32);
     plan[6] = elem;
-    plan[7] = VB;
+    plan[7] = 0;
   }
 }
 
-__device__ inline void owner_of(int e, int W, int VB, int& t, int& v) {
-  v = (e >> W) & ((1 << VB) - 1);
-  t = ((e >> (W + VB)) << W) | (e & ((1 << W) - 1));
-}
-
-__global__ void plan_mark_kernel(uint32_t* tmask, const int32_t* __restrict__ op, int64_t s,
-                                 int LC, int W, int VB) {
+__global__ void plan_mark_kernel(uint32_t* bitmap, const int32_t* __restrict__ op, int64_t s,
+                                 int LC) {
   const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (r >= s) return;
-  int t, v;
-  owner_of(op[r] & ((1 << LC) - 1), W, VB, t, v);
-  atomicOr(&tmask[2 * t + (v >> 5)], 1u << (v & 31));
+  const int e = op[r] & ((1 << LC) - 1);
+  atomicOr(&bitmap[e >> 5], 1u << (e & 31));
 }
 
-__global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __restrict__ tmask,
-                                                           uint32_t* tstart, int nt) {
+__global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __restrict__ bitmap,
+                                                           uint32_t* prefix, int words,
+                                                           uint32_t* nslots_out) {
   __shared__ uint32_t part[1024];
-  const int per = (nt + 1023) / 1024;
+  const int per = (words + 1023) / 1024;
   const int b = threadIdx.x * per;
   uint32_t s = 0;
-  for (int i = b; i < b + per && i < nt; ++i) s += __popc(tmask[2 * i]) + __popc(tmask[2 * i + 1]);
+  for (int i = b; i < b + per && i < words; ++i) s += __popc(bitmap[i]);
   part[threadIdx.x] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -382,23 +373,24 @@ __global__ __launch_bounds__(1024) void plan_prefix_kernel(const uint32_t* __res
     __syncthreads();
   }
   uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (int i = b; i < b + per && i < nt; ++i) {
-    tstart[i] = run;
-    run += __popc(tmask[2 * i]) + __popc(tmask[2 * i + 1]);
+  for (int i = b; i < b + per && i < words; ++i) {
+    prefix[i] = run;
+    run += __popc(bitmap[i]);
   }
+  if (threadIdx.x == 1023) *nslots_out = part[1023];
 }
 
-__global__ void plan_rowmap_kernel(const uint32_t* __restrict__ tmask,
-                                   const uint32_t* __restrict__ tstart, int32_t* rowmap,
-                                   const int32_t* __restrict__ op, int64_t s, int LC, int NCH,
-                                   int W, int VB) {
+__global__ void plan_rowmap_kernel(const uint32_t* __restrict__ bitmap,
+                                   const uint32_t* __restrict__ prefix, int32_t* slot_pos,
+                                   int32_t* rowmap, const int32_t* __restrict__ op, int64_t s,
+                                   int LC, int NCH) {
   const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (r >= s) return;
   const int64_t pos = op[r];
-  int t, v;
-  owner_of(static_cast<int>(pos & ((1 << LC) - 1)), W, VB, t, v);
-  const uint64_t mk = uint64_t(tmask[2 * t]) | (uint64_t(tmask[2 * t + 1]) << 32);
-  const int slot = static_cast<int>(tstart[t]) + __popcll(mk & ((uint64_t(1) << v) - 1u));
+  const int e = static_cast<int>(pos & ((1 << LC) - 1));
+  const int slot =
+      static_cast<int>(prefix[e >> 5]) + __popc(bitmap[e >> 5] & ((1u << (e & 31)) - 1u));
+  slot_pos[slot] = e;
   rowmap[int64_t(slot) * NCH + (pos >> LC)] = static_cast<int32_t>(r);
 }
 
@@ -406,19 +398,18 @@ static int build_plan(rla_ctx* ctx, int elem, int64_t n_pad, int64_t s, const in
                       uint32_t* plan) {
   FusedShape f;
   if (!fused_shape(elem, n_pad, f)) return RLA_OK;
-  const int nt = 1 << (f.LC - f.VB);
-  const int W = window_last(f);
+  const int w = 1 << (f.LC - 5);
+  const int64_t cap = std::min<int64_t>(s, int64_t(1) << f.LC);
   hipStream_t st = ctx->stream;
-  plan_init_kernel<<<64, 256, 0, st>>>(plan, f.LC, f.NCH, f.VB, s, n_pad, elem);
-  uint32_t* tmask = plan + kHdr;
-  uint32_t* tstart = tmask + 2 * nt;
-  int32_t* rowmap = reinterpret_cast<int32_t*>(tstart + nt);
-  plan_mark_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(tmask, op, s, f.LC, W,
-                                                                    f.VB);
-  plan_prefix_kernel<<<1, 1024, 0, st>>>(tmask, tstart, nt);
-  plan_rowmap_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(tmask, tstart, rowmap,
-                                                                      op, s, f.LC, f.NCH, W,
-                                                                      f.VB);
+  plan_init_kernel<<<64, 256, 0, st>>>(plan, f.LC, f.NCH, s, n_pad, elem, cap);
+  uint32_t* bitmap = plan + kPlanHdr;
+  uint32_t* prefix = bitmap + w;
+  int32_t* slot_pos = reinterpret_cast<int32_t*>(prefix + w);
+  int32_t* rowmap = slot_pos + cap;
+  plan_mark_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(bitmap, op, s, f.LC);
+  plan_prefix_kernel<<<1, 1024, 0, st>>>(bitmap, prefix, w, plan + 7);
+  plan_rowmap_kernel<<<dim3(unsigned(ceil_div(s, 256))), 256, 0, st>>>(bitmap, prefix, slot_pos,
+                                                                      rowmap, op, s, f.LC, f.NCH);
   RLA_LAUNCH_CHECK("srht_plan");
   return RLA_OK;
 }
@@ -454,9 +445,7 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
                         T* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta,
                         T* gscr) {
   const int nt = 1 << (LC - VB);
-  const uint32_t* tmask = plan + kHdr;
-  const uint32_t* tstart = tmask + 2 * nt;
-  const int32_t* rowmap = reinterpret_cast<const int32_t*>(tstart + nt);
+  const int64_t cap = std::min<int64_t>(s, int64_t(1) << LC);
   const FusedShape f{LC, NCH, VB};
   const size_t lds = fused_lds_bytes(f, sizeof(T), s);
   static std::once_flag once;
@@ -474,8 +463,8 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
                      (reinterpret_cast<uintptr_t>(X) % 16 == 0);
   srht_fused_kernel<T, LC, NCH, VB, PF, SCR_LDS><<<dim3(static_cast<unsigned>(grid)), nt, lds,
                                                    ctx->stream>>>(
-      X, x_cs, m, ncols, reinterpret_cast<const uint32_t*>(signs), scale, tmask, tstart, rowmap,
-      Y, y_rs, y_cs, alpha, beta, vec_ok, gscr, static_cast<int>(nslots_cap(f, s)));
+      X, x_cs, m, ncols, reinterpret_cast<const uint32_t*>(signs), scale, plan, Y, y_rs, y_cs,
+      alpha, beta, vec_ok, gscr, static_cast<int>(cap));
   RLA_LAUNCH_CHECK("srht_fused");
   return RLA_OK;
 }

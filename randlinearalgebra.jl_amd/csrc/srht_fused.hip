@@ -92,16 +92,23 @@ __device__ __forceinline__ void stage_range(T (&x)[R]) {
   if constexpr (B0 <= 5 && 5 < B1) stage<5, R>(x);
 }
 
+__device__ __forceinline__ float flip_sign(float a, uint32_t neg) {
+  return __uint_as_float(__float_as_uint(a) ^ (neg << 31));
+}
+__device__ __forceinline__ double flip_sign(double a, uint32_t neg) {
+  return __longlong_as_double(__double_as_longlong(a) ^ (static_cast<long long>(neg) << 63));
+}
+
 // Stage 0 of the transform with the sign flips and scaling, in Float64 as in
 // the reference (x[i] * (signs[i] ? scaling : -scaling), then z + y, z - y).
+// The sign is applied to x (exact) so the product is x' * scaling.
 template <int R, class T>
 __device__ __forceinline__ void stage0_signed(T (&x)[R], uint64_t sw, double sc) {
+  const uint64_t neg = ~sw;
 #pragma unroll
   for (int v = 0; v < R; v += 2) {
-    const double s0 = ((sw >> v) & 1u) ? sc : -sc;
-    const double s1 = ((sw >> (v + 1)) & 1u) ? sc : -sc;
-    const double z = double(x[v]) * s0;
-    const double y = double(x[v + 1]) * s1;
+    const double z = double(flip_sign(x[v], uint32_t(neg >> v) & 1u)) * sc;
+    const double y = double(flip_sign(x[v + 1], uint32_t(neg >> (v + 1)) & 1u)) * sc;
     x[v] = static_cast<T>(z + y);
     x[v + 1] = static_cast<T>(z - y);
   }
@@ -156,29 +163,36 @@ __device__ __forceinline__ void emit(T* __restrict__ Y, int64_t y_rs, int64_t yc
 template <class T, int R, int TB>
 __device__ __forceinline__ void load_chunk(T (&x)[R], const T* __restrict__ xc, int64_t base,
                                            int64_t m, int t, int vec_ok) {
+  constexpr int64_t CH = int64_t(R) << TB;
+  if (vec_ok && base + CH <= m) {
+    // whole chunk inside the matrix: unguarded vector loads
+    const T* p = xc + base + (t << 2);
 #pragma unroll
-  for (int vh = 0; vh < R / 4; ++vh) {
-    const int64_t pos = base + ((vh << (TB + 2)) | (t << 2));
-    if (vec_ok && pos + 3 < m) {
+    for (int vh = 0; vh < R / 4; ++vh) {
       if constexpr (sizeof(T) == 4) {
         typedef float v4f __attribute__((ext_vector_type(4)));
-        const v4f q4 = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(xc + pos));
+        const v4f q4 = __builtin_nontemporal_load(
+            reinterpret_cast<const v4f*>(p + (int64_t(vh) << (TB + 2))));
         x[4 * vh + 0] = q4.x;
         x[4 * vh + 1] = q4.y;
         x[4 * vh + 2] = q4.z;
         x[4 * vh + 3] = q4.w;
       } else {
-        const double2 a = *reinterpret_cast<const double2*>(xc + pos);
-        const double2 b = *reinterpret_cast<const double2*>(xc + pos + 2);
+        const double2 a = *reinterpret_cast<const double2*>(p + (int64_t(vh) << (TB + 2)));
+        const double2 b = *reinterpret_cast<const double2*>(p + (int64_t(vh) << (TB + 2)) + 2);
         x[4 * vh + 0] = a.x;
         x[4 * vh + 1] = a.y;
         x[4 * vh + 2] = b.x;
         x[4 * vh + 3] = b.y;
       }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) x[4 * vh + q] = (pos + q < m) ? xc[pos + q] : T(0);
     }
+    return;
+  }
+#pragma unroll
+  for (int vh = 0; vh < R / 4; ++vh) {
+    const int64_t pos = base + ((vh << (TB + 2)) | (t << 2));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[4 * vh + q] = (pos + q < m) ? xc[pos + q] : T(0);
   }
 }
 
@@ -500,18 +514,20 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
                                              ((plan_bytes + 255) & ~size_t(255)))
                       : nullptr;
   *handled = true;
-#define RLA_FUSED(LCV, NCHV, SCRV)                                                            \
+#define RLA_FUSED(LCV, NCHV, SCRV, PFV)                                                       \
   if (f.LC == LCV && f.NCH == NCHV && lds_scr == SCRV)                                          \
-    return launch_fused<T, LCV, NCHV, 5, false, SCRV>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, \
-                                                      Y, y_rs, y_cs, alpha, beta, gscr);
+    return launch_fused<T, LCV, NCHV, 5, PFV, SCRV>(ctx, X, x_cs, m, ncols, signs, scale, pl, s,   \
+                                                    Y, y_rs, y_cs, alpha, beta, gscr);
   if constexpr (sizeof(T) == 4) {
-    RLA_FUSED(11, 1, false) RLA_FUSED(12, 1, false) RLA_FUSED(13, 1, false)
-    RLA_FUSED(14, 1, false) RLA_FUSED(14, 2, true) RLA_FUSED(14, 2, false)
-    RLA_FUSED(14, 4, false) RLA_FUSED(14, 8, false)
+    RLA_FUSED(11, 1, false, false) RLA_FUSED(12, 1, false, false)
+    RLA_FUSED(13, 1, false, false) RLA_FUSED(14, 1, false, false)
+    RLA_FUSED(14, 2, true, true) RLA_FUSED(14, 2, false, true) RLA_FUSED(14, 4, false, true)
+    RLA_FUSED(14, 8, false, true)
   } else {
-    RLA_FUSED(11, 1, false) RLA_FUSED(12, 1, false) RLA_FUSED(13, 1, false)
-    RLA_FUSED(13, 2, true) RLA_FUSED(13, 2, false) RLA_FUSED(13, 4, false)
-    RLA_FUSED(13, 8, false)
+    RLA_FUSED(11, 1, false, false) RLA_FUSED(12, 1, false, false)
+    RLA_FUSED(13, 1, false, false) RLA_FUSED(13, 2, true, false)
+    RLA_FUSED(13, 2, false, false) RLA_FUSED(13, 4, false, false)
+    RLA_FUSED(13, 8, false, false)
   }
 #undef RLA_FUSED
   *handled = false;

@@ -182,6 +182,74 @@ int rla_srht_plan_bytes(int elem_bytes, int64_t n_pad, int64_t s, size_t* bytes)
 int rla_srht_plan(rla_ctx* ctx, int elem_bytes, int64_t n_pad, int64_t s, const int32_t* op,
                   void* plan);
 
+/* ---- dense products and conversions ------------------------------------
+ * Column-major GEMM C = alpha*op(A)*op(B) + beta*C (vendor library: the plain
+ * Float32/Float64 products of the Gaussian compressor, gaussian.jl:145-170,
+ * and of the plain-precision RangeFinder, power_its.jl:21-72). */
+int rla_gemm_f32(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,
+                 double alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+                 double beta, float* C, int64_t ldc);
+int rla_gemm_f64(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                 double beta, double* C, int64_t ldc);
+
+/* BF16 copies: dst[l + i*ldd] = bf16(src[i, l]) (round to nearest even) for
+ * the rows x cols strided view src (rows become contiguous K-runs). */
+int rla_to_bf16_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t s_rs,
+                    int64_t s_cs, uint16_t* dst, int64_t ldd);
+int rla_to_bf16_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src, int64_t s_rs,
+                    int64_t s_cs, uint16_t* dst, int64_t ldd);
+
+/* BF16 MFMA GEMM, FP32 accumulation (v_mfma_f32_32x32x16_bf16):
+ * C[i,j] = alpha * sum_l A[l + i*lda] * B[l + j*ldb] + beta * C[i*c_rs + j*c_cs],
+ * i < M, j < N, l < K; A and B 16-byte aligned, lda and ldb multiples of 8.
+ * The mixed-precision Gaussian sketch and power-iteration products
+ * (power_its.jl:21,28-29,59,68,72). */
+int rla_gemm_bf16(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, const uint16_t* A,
+                  int64_t lda, const uint16_t* B, int64_t ldb, float beta, float* C, int64_t c_rs,
+                  int64_t c_cs);
+
+int rla_convert_f32_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t lds,
+                        double* dst, int64_t ldd);
+int rla_convert_f64_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src, int64_t lds,
+                        float* dst, int64_t ldd);
+/* dst = src * diag(d) (RandSVD's S scaling, randsvd.jl:224,270). */
+int rla_scale_cols_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src, int64_t lds,
+                       const double* d, double* dst, int64_t ldd);
+
+/* ---- QR, RangeFinder, RandSVD -------------------------------------------
+ * FP64 CholeskyQR2 of the m x k (m >= k) column-major Y in place (Y <- Q,
+ * R optional k x k upper with Y_in = Q R), falling back to Householder QR
+ * when the Cholesky factor breaks down; *used_householder reports which.
+ * Replaces the qr! calls of power_its.jl:35,60,70,73. */
+int rla_qr_workspace_bytes(int64_t k, size_t* bytes);
+int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R,
+               int64_t ldr, void* work, size_t work_bytes, int* used_householder);
+
+/* RangeFinder iterations on Y = A*S (m x k, FP64, column-major, in/out):
+ * orthogonalize != 0 runs rand_ortho_it (power_its.jl:53-80), otherwise
+ * rand_power_it (power_its.jl:15-36); Y leaves as Q.
+ * precision 0: products in A's type (a_elem 4 or 8; A column-major, lda);
+ * precision 1: products on the BF16 MFMA with FP32 accumulation, reading the
+ * two resident BF16 copies of A (A_rm: rows contiguous, ld_rm; A_cm:
+ * columns contiguous, ld_cm; from rla_to_bf16_*).  QR is FP64 throughout. */
+int rla_rangefinder_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,
+                                    size_t* bytes);
+int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
+                    int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
+                    const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
+                    double* Y, int64_t ldy, void* work, size_t work_bytes);
+
+/* RandSVD finish (randsvd.jl:174-189) from the range Q (m x k): B' = A'Q,
+ * B' = Qb Rb (CholeskyQR2), Rb' = Ur S Vr' (FP64 SVD), U = Q Ur, V = Qb Vr.
+ * Outputs U (m x k), S (k), V (n x k), FP64. */
+int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,
+                                size_t* bytes);
+int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_t k, int a_elem,
+                       const void* A, int64_t lda, const uint16_t* A_cm, int64_t ld_cm,
+                       const double* Q, int64_t ldq, double* U, int64_t ldu, double* S, double* V,
+                       int64_t ldv, void* work, size_t work_bytes);
+
 #ifdef __cplusplus
 }
 #endif

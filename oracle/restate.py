@@ -283,3 +283,39 @@ def sparse_mul(M, X, alpha=1.0, beta=0.0, C=None, dtype=None):
     """C = alpha*M*X + beta*C for a dense-stored sparse M, SparseArrays order."""
     ptr, idx, val = csr_from_dense(M)
     return sparse_rowgather(ptr, idx, val, X, alpha, beta, C, dtype)
+
+
+# ---------------------------------------------------------------------------
+# RangeFinder / RandSVD: src/Approximators/RangeApproximators/helpers/power_its.jl
+
+
+def _q(Y):
+    """Thin Q of Y with a positive-diagonal R (LAPACK's Q up to column signs)."""
+    Q, Rr = np.linalg.qr(Y)
+    s = np.sign(np.diag(Rr))
+    s[s == 0] = 1
+    return Q * s
+
+
+def rand_power_it(A, Omega, power_its):
+    """power_its.jl:15-36: Y = A*S; power_its x (Z = A'Y; Y = A Z); Q = qr(Y)."""
+    Y = A @ Omega
+    for _ in range(power_its):
+        Z = A.T @ Y
+        Y = A @ Z
+    return _q(Y)
+
+
+def rand_ortho_it(A, Omega, power_its):
+    """power_its.jl:53-80: a QR after every product."""
+    Q = _q(A @ Omega)
+    for _ in range(power_its):
+        Q = _q(A.T @ Q)
+        Q = _q(A @ Q)
+    return Q
+
+
+def randsvd(A, Q):
+    """randsvd.jl:174-189: QA = Q'A (Float64), U, S, V = svd(QA), U = Q*U."""
+    Ub, S, Vt = np.linalg.svd(Q.T @ A, full_matrices=False)
+    return Q @ Ub, S, Vt.T

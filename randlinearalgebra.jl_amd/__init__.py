@@ -11,9 +11,13 @@ The directory name contains a dot, so load it with
 from ._lib import (AdmissionError, ArgumentError, DeviceError, DimensionMismatch,  # noqa: F401
                    DomainError)
 from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F401
-                          CompressorRecipe, CountSketch, CountSketchRecipe, Left, Right, SRHT,
+                          CompressorRecipe, CountSketch, CountSketchRecipe, Gaussian,
+                          GaussianRecipe, Left, Right, SRHT,
                           SRHTRecipe, SparseSign, SparseSignRecipe, Undef, View, adjoint,
                           as_view, complete_compressor, fwht, left_mul_dimcheck, mul, mul_,
                           pack_bits, padded_size, right_mul_dimcheck, seed, times,
                           update_compressor, update_compressor_)
-from . import _lib  # noqa: F401
+from .approximators import (Approximator, ApproximatorAdjoint, ApproximatorRecipe,  # noqa: F401
+                            RandSVD, RandSVDRecipe, RangeFinder, RangeFinderRecipe,
+                            complete_approximator, rapproximate, rapproximate_)
+from . import _dense, _lib  # noqa: F401

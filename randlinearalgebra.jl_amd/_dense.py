@@ -1,0 +1,84 @@
+"""Dense plumbing for the host mirror: strided views -> column-major GEMM
+calls, BF16 copies and precision conversions, all through librla_mi355x."""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import ArgumentError
+
+
+def _colmajor(v):
+    """(pointer tensor, transposed?, leading dim) for a View usable by BLAS."""
+    if v.rs == 1 and v.cs >= max(1, v.rows):
+        return v.t, 0, v.cs
+    if v.cs == 1 and v.rs >= max(1, v.cols):
+        return v.t, 1, v.rs
+    if v.cols == 1 and v.rs == 1:
+        return v.t, 0, max(1, v.rows)
+    return None
+
+
+def gemm(C, A, B, alpha=1.0, beta=0.0):
+    """C = alpha*A*B + beta*C on strided Views (Float32 / Float64)."""
+    if C.rs != 1:
+        if C.cs == 1:
+            return gemm(C.T, B.T, A.T, alpha, beta)
+        raise ArgumentError("output must have a unit stride in one dimension")
+    m, n, k = C.rows, C.cols, A.cols
+    if m == 0 or n == 0:
+        return
+    a, b = _colmajor(A), _colmajor(B)
+    if a is None or b is None:
+        raise ArgumentError("operands must have a unit stride in one dimension")
+    name = f"rla_gemm_{_lib.sfx(C.dtype)}"
+    _lib.call(name, _lib.ctx(C.t.device), a[1], b[1], m, n, k, float(alpha), _lib.ptr(a[0]),
+              max(1, a[2]), _lib.ptr(b[0]), max(1, b[2]), float(beta), _lib.ptr(C.t),
+              max(1, C.cs))
+
+
+def ld8(n):
+    return (n + 7) & ~7
+
+
+def bf16_rows(v):
+    """BF16 copy of the (rows x cols) View with each row contiguous
+    (row i at out[i, :cols]); rows padded to a multiple of 8 entries."""
+    out = torch.empty(v.rows, ld8(v.cols), dtype=torch.bfloat16, device=v.t.device)
+    _lib.call(f"rla_to_bf16_{_lib.sfx(v.dtype)}", _lib.ctx(v.t.device), v.rows, v.cols,
+              _lib.ptr(v.t), v.rs, v.cs, _lib.ptr(out), ld8(v.cols))
+    return out
+
+
+def gemm_bf16(M, N, K, A_rows, B_cols, C, alpha=1.0, beta=0.0):
+    """C (Float32 View, M x N) = alpha * A_rows[:M, :K] @ B_cols[:N, :K]' + beta*C,
+    with both operands BF16 and K-contiguous (from bf16_rows)."""
+    _lib.call("rla_gemm_bf16", _lib.ctx(C.t.device), M, N, K, float(alpha), _lib.ptr(A_rows),
+              A_rows.shape[1], _lib.ptr(B_cols), B_cols.shape[1], float(beta), _lib.ptr(C.t),
+              C.rs, C.cs)
+
+
+def convert(src, dtype):
+    """Column-major copy of a column-major device matrix in another precision."""
+    from .compressors import as_view
+    v = as_view(src)
+    if v.rs != 1:
+        raise ArgumentError("convert needs a column-major source")
+    if src.dtype == dtype:
+        return src.clone() if src.dim() == 1 else src.T.contiguous().T
+    shape = (v.cols, v.rows)
+    out = torch.empty(shape, dtype=dtype, device=src.device).T
+    name = "rla_convert_f32_f64" if dtype == torch.float64 else "rla_convert_f64_f32"
+    _lib.call(name, _lib.ctx(src.device), v.rows, v.cols, _lib.ptr(src), max(1, v.cs),
+              _lib.ptr(out), max(1, v.rows))
+    return out if src.dim() == 2 else out[:, 0]
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(1, (int(nbytes) + 7) // 8), dtype=torch.float64, device=device)
+
+
+def query_bytes(name, *args):
+    out = ctypes.c_size_t(0)
+    _lib.call(name, *args, ctypes.byref(out))
+    return out.value

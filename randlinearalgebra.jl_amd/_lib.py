@@ -85,6 +85,23 @@ _SIGS = {
                            i64, f64, f64],
     "rla_srht_plan_bytes": [i32, i64, i64, vp],
     "rla_srht_plan": [vp, i32, i64, i64, vp, vp],
+    "rla_gemm_f32": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
+    "rla_gemm_f64": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
+    "rla_to_bf16_f32": [vp, i64, i64, vp, i64, i64, vp, i64],
+    "rla_to_bf16_f64": [vp, i64, i64, vp, i64, i64, vp, i64],
+    "rla_gemm_bf16": [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64, ctypes.c_float, vp,
+                      i64, i64],
+    "rla_convert_f32_f64": [vp, i64, i64, vp, i64, vp, i64],
+    "rla_convert_f64_f32": [vp, i64, i64, vp, i64, vp, i64],
+    "rla_scale_cols_f64": [vp, i64, i64, vp, i64, vp, vp, i64],
+    "rla_qr_workspace_bytes": [i64, vp],
+    "rla_qr_f64": [vp, i64, i64, vp, i64, vp, i64, vp, ctypes.c_size_t, vp],
+    "rla_rangefinder_workspace_bytes": [i32, i32, i64, i64, i64, vp],
+    "rla_rangefinder": [vp, i32, i32, i64, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp,
+                        i64, vp, ctypes.c_size_t],
+    "rla_randsvd_workspace_bytes": [i32, i32, i64, i64, i64, vp],
+    "rla_randsvd_finish": [vp, i32, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, i64, vp,
+                           vp, i64, vp, ctypes.c_size_t],
 }
 _RESTYPE = {"rla_version": ctypes.c_char_p, "rla_last_error": ctypes.c_char_p,
             "rla_philox4x32_10": None}

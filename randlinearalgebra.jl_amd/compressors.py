@@ -248,9 +248,20 @@ def update_compressor(S, *args):
 update_compressor_ = update_compressor
 
 
+_mul_hooks = []
+
+
+def _register_mul_hook(h):
+    """Lets other operator families (approximators) join `mul`."""
+    _mul_hooks.append(h)
+
+
 def mul(C, X, Y, alpha=1.0, beta=0.0):
     """mul!(C, S, A, alpha, beta) or mul!(C, A, S, alpha, beta)."""
     alpha, beta = float(alpha), float(beta)
+    for h in _mul_hooks:
+        if h(C, X, Y, alpha, beta):
+            return None
     if isinstance(X, CompressorAdjoint):
         # alpha*S'*A + beta*C -> C  ==  alpha*A'*S + beta*C' -> C'
         return mul(as_view(C).T, as_view(Y).T, X.parent, alpha, beta)
@@ -277,19 +288,17 @@ mul_ = mul
 
 
 def times(X, Y):
-    """The allocating product `S * A` / `A * S` (Compressors.jl:416-449)."""
-    if isinstance(X, (CompressorRecipe, CompressorAdjoint)):
+    """The allocating product `S * A` / `A * S` (Compressors.jl:416-449); also
+    for approximator recipes (Approximators.jl:420-453)."""
+    if not isinstance(X, torch.Tensor):
         A = Y
         s_rows = X.size(1)
-        a_cols = 1 if A.dim() == 1 else A.shape[1]
-        C = (torch.zeros(s_rows, dtype=A.dtype, device=A.device) if a_cols == 1 and A.dim() == 1
-             else _colmajor_zeros(s_rows, a_cols, A))
+        C = (torch.zeros(s_rows, dtype=A.dtype, device=A.device) if A.dim() == 1
+             else _colmajor_zeros(s_rows, A.shape[1], A))
         mul(C, X, A)
         return C
     A, S = X, Y
-    s_cols = S.size(2)
-    a_rows = A.shape[0]
-    C = _colmajor_zeros(a_rows, s_cols, A)
+    C = _colmajor_zeros(A.shape[0], S.size(2), A)
     mul(C, A, S)
     return C
 
@@ -670,3 +679,64 @@ def pack_bits(bits):
     pad[:n] = b
     w = (pad.reshape(-1, 64) << np.arange(64, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
     return torch.from_numpy(w.view(np.int64).copy())
+
+
+# ---------------------------------------------------------------------------
+# Gaussian
+
+
+class Gaussian(Compressor):
+    """Gaussian(cardinality=Left(), compression_dim=2, type=Float64) —
+    src/Compressors/gaussian.jl:49-61."""
+
+    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64):
+        cardinality = Left() if cardinality is None else cardinality
+        if compression_dim <= 0:
+            raise ArgumentError("Field 'compression_dim' must be positive.")
+        self.cardinality = cardinality
+        self.compression_dim = int(compression_dim)
+        self.type = _dtype(type)
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return GaussianRecipe(self.cardinality, self.compression_dim, self.type, A)
+
+
+class GaussianRecipe(CompressorRecipe):
+    """gaussian.jl:87-124: op = scale .* randn(type, n_rows, n_cols) with
+    scale = type(1/sqrt(compression_dim)); Left: s x size(A, 1), Right:
+    size(A, 2) x s.  `op` is the dense operator S itself (column-major)."""
+
+    def __init__(self, cardinality, compression_dim, type_, A):
+        a_rows, a_cols = _size2(A)
+        self.cardinality = cardinality
+        self.compression_dim = compression_dim
+        if cardinality == Right():
+            self.n_rows, self.n_cols = a_cols, compression_dim
+        else:
+            self.n_rows, self.n_cols = compression_dim, a_rows
+        self.type = type_
+        self.scale = float(torch.tensor(1.0 / math.sqrt(compression_dim), dtype=type_))
+        dev = _device_of(A)
+        self.op = torch.empty(self.n_cols, self.n_rows, dtype=type_, device=dev).T
+        self._gen()
+
+    def _gen(self):
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        _lib.call(f"rla_gaussian_gen_{_lib.sfx(self.type)}", _lib.ctx(self.op.device), sd, ep,
+                  self.n_rows, self.n_cols, self.scale, _lib.ptr(self.op), max(1, self.n_rows))
+
+    def _update(self, *args):
+        self._gen()
+
+    def _op_as(self, dtype):
+        return self.op if dtype == self.op.dtype else self.op.to(dtype)
+
+    def _left(self, C, A, alpha, beta):
+        from ._dense import gemm
+        gemm(C, as_view(self._op_as(A.dtype)), A, alpha, beta)
+
+    def _right(self, C, A, alpha, beta):
+        from ._dense import gemm
+        gemm(C, A, as_view(self._op_as(A.dtype)), alpha, beta)

@@ -18,6 +18,7 @@ struct rla_ctx {
   size_t ws_bytes = 0;
   int* flag = nullptr;      // small device word for kernel-side checks
   int* flag_host = nullptr; // pinned mirror of flag
+  void* blas = nullptr;     // rocblas_handle, created on first use
 };
 
 namespace rla {
@@ -48,6 +49,9 @@ int fail(int code, const std::string& msg);
   do {                                                                       \
     if (!(ctx)) return ::rla::fail(RLA_EARG, "null rla_ctx");                \
   } while (0)
+
+// Releases the context's rocBLAS handle (linalg.hip).
+void destroy_blas(rla_ctx* ctx);
 
 // Makes ctx->device current for the calling thread.
 int use_device(rla_ctx* ctx);

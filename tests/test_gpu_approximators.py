@@ -1,0 +1,210 @@
+"""GPU parity: Gaussian compressor, BF16 MFMA GEMM, FP64 CholeskyQR2,
+RangeFinder and RandSVD against the oracle and closed forms."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import restate as R
+
+pytestmark = pytest.mark.gpu
+F64, F32 = torch.float64, torch.float32
+
+
+def colmajor(a, dev, dtype=None):
+    t = torch.as_tensor(np.asarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.T.contiguous().T.to(dev) if t.dim() == 2 else t.to(dev)
+
+
+def rnd(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape)
+
+
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+@pytest.mark.parametrize("card", ["left", "right"])
+def test_gaussian_all_orientations(rla, dev, dtype, tol, card):
+    # test/Compressors/gaussian.jl:216-280: mul! against S.op * A with the recipe's op
+    rla.seed(5)
+    cardinality = rla.Left() if card == "left" else rla.Right()
+    A = rnd((50, 40), 1)
+    S = rla.complete_compressor(rla.Gaussian(cardinality=cardinality, compression_dim=9,
+                                             type=dtype), colmajor(A, dev, dtype))
+    op = S.op.double().cpu().numpy()
+    assert op.shape == S.size()
+    r, c = op.shape
+    # column scale ~ 1/sqrt(s): variance of the entries
+    assert abs(np.var(op) * 9 - 1) < 0.3
+    X = rnd((c, 4), 2)
+    C = torch.zeros(r, 4, dtype=dtype, device=dev)
+    rla.mul(C, S, colmajor(X, dev, dtype), 2.0, 0.0)
+    assert np.max(np.abs(C.double().cpu().numpy() - 2 * op @ X)) < tol * np.max(np.abs(op @ X)) * 4
+    Y = rnd((3, r), 3)
+    C2 = torch.zeros(3, c, dtype=dtype, device=dev)
+    rla.mul(C2, colmajor(Y, dev, dtype), S)
+    assert np.max(np.abs(C2.double().cpu().numpy() - Y @ op)) < tol * np.max(np.abs(Y @ op)) * 4
+    C3 = torch.zeros(c, 2, dtype=dtype, device=dev)
+    Z = rnd((r, 2), 4)
+    rla.mul(C3, S.T, colmajor(Z, dev, dtype))
+    assert np.max(np.abs(C3.double().cpu().numpy() - op.T @ Z)) < tol * np.max(np.abs(op.T @ Z)) * 4
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 266, 8192), (100, 37, 300), (8192, 266, 4096),
+                                   (33, 300, 1000), (128, 64, 64)])
+def test_gemm_bf16_matches_fp32_reference(rla, dev, M, N, K):
+    # inputs rounded to bf16 on the host; products exact, FP32 accumulation
+    A = rnd((M, K), 1).astype(np.float32)
+    B = rnd((K, N), 2).astype(np.float32)
+    Ab = torch.as_tensor(A).to(torch.bfloat16).float().numpy().astype(np.float64)
+    Bb = torch.as_tensor(B).to(torch.bfloat16).float().numpy().astype(np.float64)
+    want = Ab @ Bb
+    Ad = colmajor(A, dev)
+    Bd = colmajor(B, dev)
+    rows = rla._dense.bf16_rows(rla.as_view(Ad))        # M x ld8(K)
+    cols = rla._dense.bf16_rows(rla.as_view(Bd).T)      # N x ld8(K)
+    C = torch.zeros(N, M, dtype=F32, device=dev).T
+    rla._dense.gemm_bf16(M, N, K, rows, cols, rla.as_view(C))
+    got = C.double().cpu().numpy()
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert err < 2e-6, err
+
+
+def test_bf16_conversion_rounds_to_nearest_even(rla, dev):
+    x = torch.tensor([[1.0, 1.00390625, 1.005859375, -3.14159, 65504.0, 1e-30]], dtype=F32)
+    got = rla._dense.bf16_rows(rla.as_view(x.to(dev)))[:, :6].cpu().float()
+    want = x.to(torch.bfloat16).float()
+    assert torch.equal(got, want)
+
+
+@pytest.mark.parametrize("m,k", [(5000, 266), (300, 40), (64, 64)])
+def test_qr_f64_cholqr2(rla, dev, m, k):
+    Y = rnd((m, k), m) @ np.diag(np.logspace(0, -4, k))  # cond 1e4
+    Yd = colmajor(Y, dev)
+    Rd = torch.zeros(k, k, dtype=F64, device=dev).T
+    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", k)
+    work = rla._dense.workspace(nb, dev)
+    import ctypes
+    hh = ctypes.c_int(-1)
+    rla._lib.call("rla_qr_f64", rla._lib.ctx(dev), m, k, rla._lib.ptr(Yd), m, rla._lib.ptr(Rd),
+                  k, rla._lib.ptr(work), work.numel() * 8, ctypes.byref(hh))
+    Q, Rm = Yd.cpu().numpy(), Rd.cpu().numpy()
+    assert hh.value == 0
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-12 * k
+    assert np.linalg.norm(Q @ Rm - Y) < 1e-12 * np.linalg.norm(Y)
+    assert np.allclose(np.triu(Rm), Rm) and np.all(np.diag(Rm) > 0)
+    # same Q as a positive-diagonal Householder QR
+    assert np.linalg.norm(Q - R._q(Y)) < 1e-8
+
+
+def test_qr_f64_householder_fallback_on_rank_deficient(rla, dev):
+    m, k = 400, 30
+    Y = rnd((m, 10), 1) @ rnd((10, k), 2)        # rank 10 < k: Cholesky breaks down
+    Yd = colmajor(Y, dev)
+    Rd = torch.zeros(k, k, dtype=F64, device=dev).T
+    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", k)
+    work = rla._dense.workspace(nb, dev)
+    import ctypes
+    hh = ctypes.c_int(-1)
+    rla._lib.call("rla_qr_f64", rla._lib.ctx(dev), m, k, rla._lib.ptr(Yd), m, rla._lib.ptr(Rd),
+                  k, rla._lib.ptr(work), work.numel() * 8, ctypes.byref(hh))
+    Q, Rm = Yd.cpu().numpy(), Rd.cpu().numpy()
+    assert hh.value == 1
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-12 * k
+    assert np.linalg.norm(Q @ Rm - Y) < 1e-12 * np.linalg.norm(Y)
+    assert np.all(np.diag(Rm) >= 0)
+
+
+def _spectrum_matrix(m, n, decay, seed):
+    rng = np.random.default_rng(seed)
+    r = min(m, n)
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = np.exp(-np.arange(r) / decay)
+    return (U * s) @ V.T, s
+
+
+@pytest.mark.parametrize("ortho", [True, False])
+@pytest.mark.parametrize("q", [0, 2])
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-9), (F32, 2e-4)])
+def test_rangefinder_plain_matches_oracle(rla, dev, ortho, q, dtype, tol):
+    A, _ = _spectrum_matrix(300, 200, 20.0, 3)
+    Ad = colmajor(A, dev, dtype)
+    rla.seed(11)
+    rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=25,
+                                                 type=dtype),
+                         power_its=q, orthogonalize=ortho)
+    rec = rla.rapproximate(rf, Ad)
+    Om = rec.compressor.op.double().cpu().numpy()
+    want = (R.rand_ortho_it if ortho else R.rand_power_it)(A, Om, q)
+    Q = rec.range.double().cpu().numpy()
+    assert rec.range.dtype == dtype and Q.shape == (300, 25)
+    assert np.linalg.norm(Q.T @ Q - np.eye(25)) < (1e-12 if dtype == F64 else 1e-5) * 25
+    assert np.linalg.norm(Q @ Q.T - want @ want.T) < tol * 25
+
+
+def test_rangefinder_exact_recovery_sparse_sign(rla, dev):
+    # default compressor SparseSign(Right()), rank 6 < k: A = QQ'A to 1e-10
+    A = R._q(rnd((400, 6), 1)) @ rnd((6, 150), 2)
+    Ad = colmajor(A, dev)
+    rec = rla.rapproximate(rla.RangeFinder(compressor=rla.SparseSign(
+        cardinality=rla.Right(), compression_dim=12)), Ad)
+    Q = rec.range.cpu().numpy()
+    assert np.linalg.norm(A - Q @ (Q.T @ A)) < 1e-10 * np.linalg.norm(A)
+    C = torch.zeros(400, 150, dtype=F64, device=dev)
+    B = torch.zeros(12, 150, dtype=F64, device=dev)
+    rla.mul(B, rec.T, Ad)       # Q'A
+    rla.mul(C, rec, B)          # Q Q'A
+    assert np.linalg.norm(C.cpu().numpy() - A) < 1e-10 * np.linalg.norm(A)
+
+
+def test_rangefinder_bf16_error_close_to_plain(rla, dev):
+    # the C3 recipe at a reduced size: sigma_i = exp(-i/64), k = 266, q = 2
+    A, s = _spectrum_matrix(4096, 1024, 64.0, 7)
+    Ad = colmajor(A, dev)
+    opt = np.sqrt(np.sum(s[266:] ** 2) / np.sum(s ** 2))
+    errs = {}
+    for prec in ("plain", "bf16"):
+        rla.seed(3)
+        rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(),
+                                                     compression_dim=266),
+                             power_its=2, orthogonalize=True, precision=prec)
+        Q = rla.rapproximate(rf, Ad).range.cpu().numpy()
+        errs[prec] = np.linalg.norm(A - Q @ (Q.T @ A)) / np.linalg.norm(A)
+    assert errs["plain"] < 1.05 * opt
+    assert errs["bf16"] <= 1.1 * errs["plain"], errs
+
+
+@pytest.mark.parametrize("prec", ["plain", "bf16"])
+def test_randsvd(rla, dev, prec):
+    A, s = _spectrum_matrix(600, 300, 8.0, 5)
+    Ad = colmajor(A, dev)
+    rla.seed(2)
+    rec = rla.rapproximate(rla.RandSVD(compressor=rla.Gaussian(cardinality=rla.Right(),
+                                                               compression_dim=40),
+                                       power_its=2, orthogonalize=True, precision=prec), Ad)
+    U, S, V = rec.U.cpu().numpy(), rec.S.cpu().numpy(), rec.V.cpu().numpy()
+    assert U.shape == (600, 40) and S.shape == (40,) and V.shape == (300, 40)
+    tol = 1e-10 if prec == "plain" else 5e-3
+    assert np.max(np.abs(S[:20] - s[:20]) / s[:20]) < tol
+    assert np.linalg.norm(U.T @ U - np.eye(40)) < 1e-10 and np.linalg.norm(V.T @ V - np.eye(40)) < 1e-10
+    if prec == "plain":
+        Qw = rla.rapproximate(rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(),
+                                                                      compression_dim=40)),
+                              Ad)  # sanity of the recipe path
+        assert Qw.range.shape == (600, 40)
+        # oracle finish from the same Q reproduces U S V'
+        rla.seed(2)
+        rec2 = rla.rapproximate(rla.RandSVD(compressor=rla.Gaussian(cardinality=rla.Right(),
+                                                                    compression_dim=40),
+                                            power_its=2, orthogonalize=True), Ad)
+        Om = rec2.compressor.op.cpu().numpy()
+        Q = R.rand_ortho_it(A, Om, 2)
+        Uo, So, Vo = R.randsvd(A, Q)
+        assert np.max(np.abs(rec2.S.cpu().numpy() - So) / So) < 1e-9
+        lowrank = (U * S) @ V.T
+        assert np.linalg.norm(lowrank - (Uo * So) @ Vo.T) < 1e-8 * np.linalg.norm(lowrank)
+    # low-rank mul!: C = U S V' X
+    X = rnd((300, 3), 1)
+    C = torch.zeros(600, 3, dtype=F64, device=dev)
+    rla.mul(C, rec, colmajor(X, dev))
+    assert np.linalg.norm(C.cpu().numpy() - (U * S) @ (V.T @ X)) < 1e-10 * np.linalg.norm(C.cpu().numpy())

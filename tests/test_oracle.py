@@ -217,3 +217,42 @@ def test_srht_gen_structure():
                                     (2 ** 28, 2 ** 28)])
 def test_padded_size_matches_log_ratio_quirk(n, want):
     assert R.padded_size(n) == want
+
+
+def _lowrank(m, n, r, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((m, r)) @ rng.standard_normal((r, n))
+
+
+@pytest.mark.parametrize("q", [0, 1, 2])
+def test_rangefinder_oracle_exact_recovery(q):
+    # test/Approximators/RangeApproximator/rangefinder.jl:190,247: with k >= rank(A),
+    # ||A - Q Q'A|| < 1e-10 and Q'Q = I
+    A = _lowrank(60, 40, 5, q)
+    Om = np.random.default_rng(9).standard_normal((40, 8))
+    for fn in (R.rand_power_it, R.rand_ortho_it):
+        Q = fn(A, Om, q)
+        assert Q.shape == (60, 8)
+        assert np.linalg.norm(Q.T @ Q - np.eye(8)) < 1e-12
+        assert np.linalg.norm(A - Q @ (Q.T @ A)) < 1e-10 * np.linalg.norm(A)
+
+
+def test_rangefinder_oracle_formula():
+    # range_finder_helpers.jl:66-68: rand_power_it spans qr(A*A'*A*A'*A*Omega)
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((50, 30))
+    Om = rng.standard_normal((30, 6))
+    Q = R.rand_power_it(A, Om, 2)
+    Qf, _ = np.linalg.qr(A @ A.T @ A @ A.T @ A @ Om)
+    assert np.linalg.norm(Q @ Q.T - Qf @ Qf.T) < 1e-10
+    Qo = R.rand_ortho_it(A, Om, 2)  # same subspace in exact arithmetic
+    assert np.linalg.norm(Qo @ Qo.T - Qf @ Qf.T) < 1e-8
+
+
+def test_randsvd_oracle_exact_recovery():
+    # randsvd tests :272-273: exact recovery of a rank-5 matrix
+    A = _lowrank(40, 30, 5, 2)
+    Q = R.rand_ortho_it(A, np.random.default_rng(1).standard_normal((30, 7)), 1)
+    U, S, V = R.randsvd(A, Q)
+    assert np.linalg.norm(A - (U * S) @ V.T) < 1e-10 * np.linalg.norm(A)
+    assert np.allclose(S[:5], np.linalg.svd(A, compute_uv=False)[:5], rtol=1e-10)

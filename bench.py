@@ -88,6 +88,53 @@ def gpu_sparse_sign(rla, torch, steps, warmup):
             "frac_hbm_peak": byts / ms / 1e6 / HBM_PEAK_GBS, "rotating_copies": copies}
 
 
+def rangefinder_c3(rla, torch, reps=3):
+    """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
+    32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
+    Haar U, V from QR of Gaussians (seed 1234); Gaussian sketch k = 256 + 10,
+    2 orthogonalised power iterations.  Reports solves/s with A (and its BF16
+    copies) resident, and ||A - QQ'A||_F / ||A||_F in FP64."""
+    import math
+    dev = torch.device("cuda", 0)
+    m, n, k, r = 32768, 8192, 266, 2560
+    g = torch.Generator(device=dev).manual_seed(1234)
+    U, _ = torch.linalg.qr(torch.randn(m, r, generator=g, device=dev, dtype=torch.float64))
+    V, _ = torch.linalg.qr(torch.randn(n, r, generator=g, device=dev, dtype=torch.float64))
+    sig = torch.exp(-torch.arange(r, device=dev, dtype=torch.float64) / 64.0)
+    A64 = ((U * sig) @ V.T).T.contiguous().T
+    del U, V
+    A32 = A64.float().T.contiguous().T
+    full = torch.exp(-torch.arange(n, device=dev, dtype=torch.float64) / 64.0)
+    opt = float(torch.sqrt((full[k:] ** 2).sum() / (full ** 2).sum()))
+    nA = float(torch.linalg.norm(A64))
+    out = {"workload": "RangeFinder 32768x8192, k=266 (256+10), q=2, orthogonalize, Gaussian",
+           "eckart_young_optimum": opt}
+    for prec in ("bf16", "plain"):
+        rla.seed(7)
+        rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=k,
+                                                     type=torch.float32),
+                             power_its=2, orthogonalize=True, precision=prec)
+        rec = rla.complete_approximator(rf, A32)
+        rla.rapproximate_(rec, A32)   # warm-up (and BF16 copies of A for the bf16 path)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rla.rapproximate_(rec, A32)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        Q = rec.range.double()
+        B = torch.zeros(n, k, dtype=torch.float64, device=dev).T
+        rla._dense.gemm(rla.as_view(B), rla.as_view(Q).T, rla.as_view(A64), 1.0, 0.0)
+        R_ = A64.clone()
+        rla._dense.gemm(rla.as_view(R_), rla.as_view(Q), rla.as_view(B), -1.0, 1.0)
+        err = float(torch.linalg.norm(R_)) / nA
+        out[prec] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "rel_error": err}
+        del rec, Q, B, R_
+    out["speedup_solves_per_s"] = out["plain"]["seconds_per_solve"] / out["bf16"]["seconds_per_solve"]
+    out["error_ratio_bf16_over_plain"] = out["bf16"]["rel_error"] / out["plain"]["rel_error"]
+    return out
+
+
 def cpu_baseline(S, A0):
     """Oracle C restatement (OpenMP over columns, all host threads) on the full
     workload: the same A, op, signs and scale."""
@@ -126,6 +173,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (counter calibration)")
     ap.add_argument("--skip-sparse", action="store_true")
+    ap.add_argument("--skip-rangefinder", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SRHT apply has no multi-GPU path in this build "
@@ -138,6 +186,7 @@ def main():
     ms, wall, exact, S, A0 = gpu_srht(rla, torch, args.steps, args.warmup)
     gbs = srht_bytes() / ms / 1e6
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
+    rf = None if args.skip_rangefinder else rangefinder_c3(rla, torch)
     if args.calib_copy:
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -171,6 +220,7 @@ def main():
                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": srht_bytes()},
         "sparse_sign_c1": extra,
+        "rangefinder_c3": rf,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, A0)

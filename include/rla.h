@@ -219,10 +219,12 @@ int rla_scale_cols_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* s
 
 /* ---- QR, RangeFinder, RandSVD -------------------------------------------
  * FP64 CholeskyQR2 of the m x k (m >= k) column-major Y in place (Y <- Q,
+ * Gram matrix and Y*R^-1 on the FP64 MFMA, Cholesky and triangular inverse of
+ * the k x k factor in the vendor solver library;
  * R optional k x k upper with Y_in = Q R), falling back to Householder QR
  * when the Cholesky factor breaks down; *used_householder reports which.
  * Replaces the qr! calls of power_its.jl:35,60,70,73. */
-int rla_qr_workspace_bytes(int64_t k, size_t* bytes);
+int rla_qr_workspace_bytes(int64_t m, int64_t k, size_t* bytes);
 int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R,
                int64_t ldr, void* work, size_t work_bytes, int* used_householder);
 

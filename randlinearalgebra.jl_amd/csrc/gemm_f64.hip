@@ -1,0 +1,138 @@
+// FP64 MFMA GEMM for the tall-skinny products of CholeskyQR2:
+//   C = alpha * op(A) * op(B) + beta * C,  op(X) = X or X' (column-major).
+// Used for the Gram matrix G = Y'Y (m x k, m >> k: split over K) and for
+// Q = Y * R^-1 (k x k right factor).
+//
+// Tile 64 x 64 per workgroup (4 wavefronts, 32 x 32 each = 2 x 2 blocks of
+// v_mfma_f64_16x16x4f64), K staged through LDS 16 at a time.  Split-K writes
+// partial tiles that a second kernel sums in split order (deterministic).
+#include <algorithm>
+
+#include "rla_internal.hpp"
+
+namespace rla {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kT = 64;    // tile rows / cols
+constexpr int kKS = 16;   // K per LDS stage
+
+// element (i, l) of op(X) for a column-major X with leading dimension ld
+__device__ __forceinline__ double opget(const double* __restrict__ X, int64_t ld, bool trans,
+                                        int64_t i, int64_t l) {
+  return trans ? X[l + i * ld] : X[i + l * ld];
+}
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K,
+                                                       double alpha, const double* __restrict__ A,
+                                                       int64_t lda, int ta,
+                                                       const double* __restrict__ B, int64_t ldb,
+                                                       int tb, double beta, double* __restrict__ C,
+                                                       int64_t ldc, int ksplit, int64_t kps,
+                                                       double* __restrict__ part) {
+  __shared__ double As[kKS][kT + 1];  // As[l][i]
+  __shared__ double Bs[kKS][kT + 1];  // Bs[l][j]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t tiles_m = (M + kT - 1) / kT;
+  const int split = blockIdx.x % ksplit;
+  const int64_t tile = blockIdx.x / ksplit;
+  const int64_t i0 = (tile % tiles_m) * kT, j0 = (tile / tiles_m) * kT;
+  const int64_t kb = split * kps, ke = std::min<int64_t>(K, kb + kps);
+  const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = kb; k0 < ke; k0 += kKS) {
+    __syncthreads();
+    // stage 64 x 16 of op(A) and 16 x 64 of op(B); the unit-stride index of
+    // each operand runs along consecutive threads
+    for (int e = threadIdx.x; e < kT * kKS; e += 256) {
+      int ii, ll;
+      if (!ta) { ii = e % kT; ll = e / kT; } else { ll = e % kKS; ii = e / kKS; }
+      const int64_t gi = i0 + ii, gl = k0 + ll;
+      As[ll][ii] = (gi < M && gl < ke) ? opget(A, lda, ta, gi, gl) : 0.0;
+      int jj, l2;
+      if (tb) { jj = e % kT; l2 = e / kT; } else { l2 = e % kKS; jj = e / kKS; }
+      const int64_t gj = j0 + jj, gl2 = k0 + l2;
+      Bs[l2][jj] = (gj < N && gl2 < ke) ? opget(B, ldb, tb, gl2, gj) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kKS; kk += 4) {
+      const int l = kk + (lane >> 4);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const double av = As[l][wi + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const double bv = Bs[l][wj + b * 16 + (lane & 15)];
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // acc[a][b][r]: row wi + a*16 + (lane/16)*4 + r, column wj + b*16 + lane%16
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wi + a * 16 + (lane >> 4) * 4 + r;
+        const int64_t j = j0 + wj + b * 16 + (lane & 15);
+        if (i >= M || j >= N) continue;
+        if (ksplit == 1) {
+          double* cp = C + i + j * ldc;
+          *cp = beta == 0.0 ? alpha * acc[a][b][r] : alpha * acc[a][b][r] + beta * (*cp);
+        } else {
+          part[(int64_t(split) * N + j) * M + i] = acc[a][b][r];
+        }
+      }
+}
+
+__global__ void gemm_f64_reduce_kernel(int64_t M, int64_t N, int ksplit,
+                                       const double* __restrict__ part, double alpha,
+                                       double beta, double* __restrict__ C, int64_t ldc) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= M * N) return;
+  const int64_t i = e % M, j = e / M;
+  double s = 0.0;
+  for (int q = 0; q < ksplit; ++q) s += part[(int64_t(q) * N + j) * M + i];
+  double* cp = C + i + j * ldc;
+  *cp = beta == 0.0 ? alpha * s : alpha * s + beta * (*cp);
+}
+
+// part: caller scratch of ksplit*M*N doubles when the call splits K
+// (gemm_f64_split_bytes tells how much).
+int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+             int64_t ldc, double* part, size_t part_bytes) {
+  if (M <= 0 || N <= 0) return RLA_OK;
+  const int64_t tiles = ceil_div(M, kT) * ceil_div(N, kT);
+  int ksplit = 1;
+  while (ksplit < 16 && tiles * ksplit * 2 <= 512 && K / (ksplit * 2) >= 512) ksplit *= 2;
+  if (ksplit > 1 && (!part || part_bytes < size_t(ksplit) * M * N * sizeof(double))) {
+    ksplit = static_cast<int>(std::min<int64_t>(
+        ksplit, part ? int64_t(part_bytes / (sizeof(double) * M * N)) : 1));
+    if (ksplit < 1) ksplit = 1;
+  }
+  const int64_t kps = ceil_div(ceil_div(K, ksplit), kKS) * kKS;
+  ksplit = static_cast<int>(std::max<int64_t>(1, ceil_div(K, kps)));
+  gemm_f64_kernel<<<dim3(static_cast<unsigned>(tiles * ksplit)), 256, 0, ctx->stream>>>(
+      M, N, K, alpha, A, lda, ta ? 1 : 0, B, ldb, tb ? 1 : 0, beta, C, ldc, ksplit, kps, part);
+  RLA_LAUNCH_CHECK("gemm_f64");
+  if (ksplit > 1) {
+    gemm_f64_reduce_kernel<<<dim3(static_cast<unsigned>(ceil_div(M * N, 256))), 256, 0,
+                             ctx->stream>>>(M, N, ksplit, part, alpha, beta, C, ldc);
+    RLA_LAUNCH_CHECK("gemm_f64/reduce");
+  }
+  return RLA_OK;
+}
+
+size_t gemm_f64_split_bytes(int64_t M, int64_t N) { return size_t(16) * M * N * sizeof(double); }
+
+}  // namespace rla

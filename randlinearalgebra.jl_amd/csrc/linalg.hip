@@ -23,6 +23,10 @@ int gemm_bf16_tn(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, con
 template <class T>
 int to_bf16_rows(rla_ctx* ctx, int64_t rows, int64_t cols, const T* src, int64_t s_rs,
                  int64_t s_cs, uint16_t* dst, int64_t ldd);
+int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+             int64_t ldc, double* part, size_t part_bytes);
+size_t gemm_f64_split_bytes(int64_t M, int64_t N);
 
 #define RLA_BLAS(call)                                                                       \
   do {                                                                                       \
@@ -119,14 +123,23 @@ __global__ void signed_upper_kernel(int64_t k, const double* R, int64_t ldr, dou
   dst[i + j * ldd] = R[i + i * ldr] < 0 ? -v : v;
 }
 
-size_t qr_work_bytes(int64_t k) {
+size_t qr_work_bytes(int64_t m, int64_t k) {
   Bump b{nullptr, 0};
   b.take<double>(k * k);
   b.take<double>(k * k);
   b.take<double>(k);
   b.take<double>(k);
   b.take<int>(1);
+  b.take<double>(m * k);
+  b.take<char>(gemm_f64_split_bytes(k, k));
   return b.need;
+}
+
+__global__ void zero_lower_kernel(int64_t k, double* R, int64_t ldr) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= k * k) return;
+  const int64_t i = e % k, j = e / k;
+  if (i > j) R[i + j * ldr] = 0.0;
 }
 
 // Y (m x k, ldy) <- Q; optionally R (k x k upper, ldr) with Y_in = Q R.
@@ -142,17 +155,21 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
   double* tau = b.take<double>(k);
   double* sg = b.take<double>(k);
   int* info = b.take<int>(1);
-  if (!info) return fail(RLA_EARG, "QR workspace too small");
+  double* T = b.take<double>(m * k);
+  const size_t part_bytes = gemm_f64_split_bytes(k, k);
+  double* part = reinterpret_cast<double*>(b.take<char>(part_bytes));
+  if (!part) return fail(RLA_EARG, "QR workspace too small");
   rocblas_handle h;
   int rc = blas(ctx, &h);
   if (rc) return rc;
-  const double one = 1.0, zero = 0.0;
+  const double one = 1.0;
   const int mi = static_cast<int>(m), ki = static_cast<int>(k), li = static_cast<int>(ldy);
   bool breakdown = false;
   int passes_done = 0;
   for (int pass = 0; pass < 2 && !breakdown; ++pass) {
-    RLA_BLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, ki, mi, &one, Y,
-                           li, &zero, G, ki));
+    // G = Y'Y (FP64 MFMA, split over the rows), R = chol(G)
+    rc = gemm_f64(ctx, true, false, k, k, m, 1.0, Y, ldy, Y, ldy, 0.0, G, k, part, part_bytes);
+    if (rc) return rc;
     RLA_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, ki, G, ki, info));
     int hinfo = 0;
     RLA_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -161,8 +178,6 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
       breakdown = true;
       break;
     }
-    RLA_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                           rocblas_diagonal_non_unit, mi, ki, &one, G, ki, Y, li));
     if (R) {
       if (pass == 0) {
         copy_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(
@@ -174,6 +189,13 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
                                ki));
       }
     }
+    // Y <- Y R^-1 as a GEMM with the explicit triangular inverse
+    RLA_BLAS(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, ki, G, ki, info));
+    zero_lower_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(k, G, k);
+    rc = gemm_f64(ctx, false, false, m, k, k, 1.0, Y, ldy, G, k, 0.0, T, m, nullptr, 0);
+    if (rc) return rc;
+    rc = convert<double, double>(ctx, m, k, T, m, Y, ldy);
+    if (rc) return rc;
     ++passes_done;
   }
   if (breakdown) {
@@ -235,7 +257,7 @@ static size_t rf_layout(int precision, int a_elem, int64_t m, int64_t n, int64_t
     r.Ybf = b.take<uint16_t>(ld8(m) * k);
     r.Zbf = b.take<uint16_t>(ld8(n) * k);
   }
-  r.qr_bytes = qr_work_bytes(k);
+  r.qr_bytes = qr_work_bytes(std::max(m, n), k);
   r.qr = b.take<char>(r.qr_bytes);
   if (out) *out = r;
   return b.need;
@@ -356,9 +378,9 @@ int rla_gemm_f32(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n, i
   return RLA_OK;
 }
 
-int rla_qr_workspace_bytes(int64_t k, size_t* bytes) {
-  if (!bytes || k < 0) return fail(RLA_EARG, "rla_qr_workspace_bytes: bad arguments");
-  *bytes = qr_work_bytes(k);
+int rla_qr_workspace_bytes(int64_t m, int64_t k, size_t* bytes) {
+  if (!bytes || k < 0 || m < 0) return fail(RLA_EARG, "rla_qr_workspace_bytes: bad arguments");
+  *bytes = qr_work_bytes(m, k);
   return RLA_OK;
 }
 

@@ -81,7 +81,7 @@ def test_qr_f64_cholqr2(rla, dev, m, k):
     Y = rnd((m, k), m) @ np.diag(np.logspace(0, -4, k))  # cond 1e4
     Yd = colmajor(Y, dev)
     Rd = torch.zeros(k, k, dtype=F64, device=dev).T
-    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", k)
+    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", m, k)
     work = rla._dense.workspace(nb, dev)
     import ctypes
     hh = ctypes.c_int(-1)
@@ -101,7 +101,7 @@ def test_qr_f64_householder_fallback_on_rank_deficient(rla, dev):
     Y = rnd((m, 10), 1) @ rnd((10, k), 2)        # rank 10 < k: Cholesky breaks down
     Yd = colmajor(Y, dev)
     Rd = torch.zeros(k, k, dtype=F64, device=dev).T
-    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", k)
+    nb = rla._dense.query_bytes("rla_qr_workspace_bytes", m, k)
     work = rla._dense.workspace(nb, dev)
     import ctypes
     hh = ctypes.c_int(-1)

@@ -193,6 +193,13 @@ int rla_gemm_f64(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n, i
                  double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                  double beta, double* C, int64_t ldc);
 
+/* The same product on the FP64 MFMA (v_mfma_f64_16x16x4f64; split-K for
+ * tall-skinny shapes, deterministic): the Gram and R^-1 products of
+ * CholeskyQR2. */
+int rla_gemm_f64_mfma(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,
+                      double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                      double beta, double* C, int64_t ldc);
+
 /* BF16 copies: dst[l + i*ldd] = bf16(src[i, l]) (round to nearest even) for
  * the rows x cols strided view src (rows become contiguous K-runs). */
 int rla_to_bf16_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t s_rs,

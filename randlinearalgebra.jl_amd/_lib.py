@@ -87,6 +87,7 @@ _SIGS = {
     "rla_srht_plan": [vp, i32, i64, i64, vp, vp],
     "rla_gemm_f32": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_gemm_f64": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
+    "rla_gemm_f64_mfma": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_to_bf16_f32": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_to_bf16_f64": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_gemm_bf16": [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64, ctypes.c_float, vp,

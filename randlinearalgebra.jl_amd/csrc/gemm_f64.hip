@@ -75,14 +75,15 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
       }
     }
   }
-  // acc[a][b][r]: row wi + a*16 + (lane/16)*4 + r, column wj + b*16 + lane%16
+  // acc[a][b][r]: row wi + a*16 + lane/16 + 4*r, column wj + b*16 + lane%16
+  // (the f64 16x16x4 accumulator layout, measured)
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wi + a * 16 + (lane >> 4) * 4 + r;
+        const int64_t i = i0 + wi + a * 16 + (lane >> 4) + 4 * r;
         const int64_t j = j0 + wj + b * 16 + (lane & 15);
         if (i >= M || j >= N) continue;
         if (ksplit == 1) {
@@ -136,3 +137,20 @@ int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, do
 size_t gemm_f64_split_bytes(int64_t M, int64_t N) { return size_t(16) * M * N * sizeof(double); }
 
 }  // namespace rla
+
+using namespace rla;
+
+extern "C" int rla_gemm_f64_mfma(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t n,
+                                 int64_t k, double alpha, const double* A, int64_t lda,
+                                 const double* B, int64_t ldb, double beta, double* C,
+                                 int64_t ldc) {
+  RLA_CHECK_CTX(ctx);
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  void* ws = nullptr;
+  const size_t pb = gemm_f64_split_bytes(m, n);
+  rc = workspace(ctx, pb, &ws);
+  if (rc) return rc;
+  return gemm_f64(ctx, trans_a != 0, trans_b != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                  static_cast<double*>(ws), pb);
+}

@@ -69,6 +69,23 @@ def test_gemm_bf16_matches_fp32_reference(rla, dev, M, N, K):
     assert err < 2e-6, err
 
 
+@pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 64, 64, 16), (1, 0, 64, 64, 16),
+                                         (1, 0, 266, 266, 5000), (0, 0, 5000, 266, 266),
+                                         (0, 1, 70, 33, 100), (1, 1, 17, 65, 1030)])
+def test_gemm_f64_mfma(rla, dev, ta, tb, M, N, K):
+    A = rnd((K, M) if ta else (M, K), 1)
+    B = rnd((N, K) if tb else (K, N), 2)
+    C0 = rnd((M, N), 3)
+    want = 1.5 * (A.T if ta else A) @ (B.T if tb else B) + 0.5 * C0
+    Ad, Bd, Cd = colmajor(A, dev), colmajor(B, dev), colmajor(C0, dev)
+    rla._lib.call("rla_gemm_f64_mfma", rla._lib.ctx(dev), ta, tb, M, N, K, 1.5,
+                  rla._lib.ptr(Ad), Ad.stride(1), rla._lib.ptr(Bd), Bd.stride(1), 0.5,
+                  rla._lib.ptr(Cd), Cd.stride(1))
+    got = Cd.cpu().numpy()
+    err = np.abs(got - want)
+    assert err.max() <= 1e-11 * np.abs(want).max()
+
+
 def test_bf16_conversion_rounds_to_nearest_even(rla, dev):
     x = torch.tensor([[1.0, 1.00390625, 1.005859375, -3.14159, 65504.0, 1e-30]], dtype=F32)
     got = rla._dense.bf16_rows(rla.as_view(x.to(dev)))[:, :6].cpu().float()

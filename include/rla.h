@@ -225,15 +225,26 @@ int rla_scale_cols_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* s
                        const double* d, double* dst, int64_t ldd);
 
 /* ---- QR, RangeFinder, RandSVD -------------------------------------------
- * FP64 CholeskyQR2 of the m x k (m >= k) column-major Y in place (Y <- Q,
- * Gram matrix and Y*R^-1 on the FP64 MFMA, Cholesky and triangular inverse of
- * the k x k factor in the vendor solver library;
+ * FP64 CholeskyQR2 of the m x k (m >= k) column-major Y in place (Y <- Q;
  * R optional k x k upper with Y_in = Q R), falling back to Householder QR
  * when the Cholesky factor breaks down; *used_householder reports which.
+ * k <= 288: Gram matrix in one read of Y, Cholesky in one workgroup with
+ * the trailing matrix in registers, Y*R^-1 by an in-place blocked solve (all
+ * FP64 MFMA); wider k: tiled FP64 MFMA GEMM and the vendor solver's
+ * potrf/trtri.
  * Replaces the qr! calls of power_its.jl:35,60,70,73. */
 int rla_qr_workspace_bytes(int64_t m, int64_t k, size_t* bytes);
 int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R,
                int64_t ldr, void* work, size_t work_bytes, int* used_householder);
+
+/* Cholesky factor and inverse of a k x k symmetric positive definite G
+ * (k <= 288, else RLA_ENOTSUP; upper triangle used): R upper with G = R'R
+ * (R may alias G), X = R^-1 (column-major, ldx; NULL to skip).  *info = 0,
+ * or the 1-based column whose pivot was not above k * eps * G_jj (then R and
+ * X are incomplete).
+ * The factorisation step of the CholeskyQR that replaces power_its.jl's qr!. */
+int rla_chol_inv_f64(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R,
+                     int64_t ldr, double* X, int64_t ldx, int* info);
 
 /* RangeFinder iterations on Y = A*S (m x k, FP64, column-major, in/out):
  * orthogonalize != 0 runs rand_ortho_it (power_its.jl:53-80), otherwise

@@ -97,6 +97,7 @@ _SIGS = {
     "rla_scale_cols_f64": [vp, i64, i64, vp, i64, vp, vp, i64],
     "rla_qr_workspace_bytes": [i64, i64, vp],
     "rla_qr_f64": [vp, i64, i64, vp, i64, vp, i64, vp, ctypes.c_size_t, vp],
+    "rla_chol_inv_f64": [vp, i64, vp, i64, vp, i64, vp, i64, vp],
     "rla_rangefinder_workspace_bytes": [i32, i32, i64, i64, i64, vp],
     "rla_rangefinder": [vp, i32, i32, i64, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp,
                         i64, vp, ctypes.c_size_t],

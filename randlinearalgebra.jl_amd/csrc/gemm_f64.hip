@@ -136,6 +136,177 @@ int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, do
 
 size_t gemm_f64_split_bytes(int64_t M, int64_t N) { return size_t(16) * M * N * sizeof(double); }
 
+// ---- CholeskyQR kernels for k <= 288 columns ---------------------------------
+//
+// Gram G = Y'Y: each workgroup takes a contiguous range of rows of Y and
+// computes every upper-triangular 16 x 16 block of G for it, so Y is read
+// from HBM exactly once.  Rows are staged 16 at a time in LDS (double
+// buffered, next chunk prefetched into registers); the 8 wavefronts share the
+// NBLK(NBLK+1)/2 blocks round-robin.  Per-workgroup partial blocks go to
+// scratch and a second kernel sums them in workgroup order (deterministic).
+
+constexpr int kGramRows = 16;  // rows per LDS chunk
+
+// 8 wavefronts hold up to 16 x 17 / 2 blocks in 256 registers each; the
+// 18 x 19 / 2 blocks of k <= 288 take 16 wavefronts (128 registers each)
+constexpr int gram_threads(int nblk) { return nblk > 16 ? 1024 : 512; }
+
+__device__ __forceinline__ void tri_block(int q, int& bi, int& bj) {
+  bj = 0;
+  while (q > bj) {
+    q -= bj + 1;
+    ++bj;
+  }
+  bi = q;
+}
+
+template <int NBLK>
+__global__ __launch_bounds__(gram_threads(NBLK)) void gram_f64_kernel(int64_t m, int k,
+                                                                const double* __restrict__ Y,
+                                                                int64_t ldy, int64_t rows_per_wg,
+                                                                double* __restrict__ part) {
+  constexpr int CP = NBLK * 16;
+  constexpr int LDSS = (CP + 31) / 32 * 32 + 16;  // rows 2 apart map to opposite bank halves
+  constexpr int T = NBLK * (NBLK + 1) / 2;
+  constexpr int NT = gram_threads(NBLK);
+  constexpr int NW = NT / 64;
+  constexpr int MAXB = (T + NW - 1) / NW;
+  constexpr int CSTEP = NT / kGramRows;               // chunk columns per load pass
+  constexpr int PER = (CP + CSTEP - 1) / CSTEP;
+  __shared__ double Ys[2][kGramRows * LDSS];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t r_begin = int64_t(blockIdx.x) * rows_per_wg;
+  const int64_t r_end = std::min<int64_t>(m, r_begin + rows_per_wg);
+  int offa[MAXB], offb[MAXB];
+#pragma unroll
+  for (int s = 0; s < MAXB; ++s) {
+    const int q = wave + NW * s;
+    int bi = 0, bj = 0;
+    tri_block(q, bi, bj);
+    offa[s] = q < T ? bi * 16 : -1;
+    offb[s] = bj * 16;
+  }
+  f64x4 acc[MAXB];
+#pragma unroll
+  for (int s = 0; s < MAXB; ++s) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  const int lr = t & 15, lc = t >> 4;  // chunk element (row lr, column lc + CSTEP p)
+  double v[PER];
+  // loads are unconditional from a clamped (in-bounds) address; out-of-range
+  // entries are zeroed afterwards, so the prefetch needs no branches
+  auto load = [&](int64_t r0) {
+    const int64_t gr = r0 + lr;
+    const bool row_ok = gr < r_end;
+    const double* src = Y + (row_ok ? gr : r_begin);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int c = lc + CSTEP * p;
+      const double x = src[int64_t(c < k ? c : k - 1) * ldy];
+      v[p] = (row_ok && c < k) ? x : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int c = lc + CSTEP * p;
+      if (c < CP) Ys[buf][lr * LDSS + c] = v[p];
+    }
+  };
+  if (r_begin < r_end) {
+    load(r_begin);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += kGramRows) {
+    const bool more = r0 + kGramRows < r_end;
+    if (more) load(r0 + kGramRows);
+#pragma unroll
+    for (int kk = 0; kk < kGramRows; kk += 4) {
+      const double* base = &Ys[buf][(kk + (lane >> 4)) * LDSS + (lane & 15)];
+#pragma unroll
+      for (int s = 0; s < MAXB; ++s) {
+        if (offa[s] >= 0)
+          acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(base[offa[s]], base[offb[s]], acc[s], 0,
+                                                        0, 0);
+      }
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // acc[s][r] is block element (lane/16 + 4r, lane%16), i.e. index lane + 64r
+#pragma unroll
+  for (int s = 0; s < MAXB; ++s) {
+    if (offa[s] < 0) continue;
+    double* dst = part + (int64_t(blockIdx.x) * T + wave + NW * s) * 256 + lane;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[64 * r] = acc[s][r];
+  }
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(int k, int T, int nparts,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ G, int64_t ldg) {
+  const int q = blockIdx.x, e = threadIdx.x;
+  double s = 0.0;
+  const double* p = part + int64_t(q) * 256 + e;
+#pragma unroll 8
+  for (int w = 0; w < nparts; ++w) s += p[int64_t(w) * T * 256];
+  int bi, bj;
+  tri_block(q, bi, bj);
+  const int li = e >> 4, lj = e & 15;
+  if (bi == bj && li > lj) return;  // the mirror element writes both
+  const int i = bi * 16 + li, j = bj * 16 + lj;
+  if (i >= k || j >= k) return;
+  G[i + int64_t(j) * ldg] = s;
+  G[j + int64_t(i) * ldg] = s;
+}
+
+// smallest instantiated block count covering k columns (0: k too large)
+static int qr_nblk(int64_t k) {
+  for (int nb : {4, 8, 12, 16, 17, 18})
+    if (k <= 16 * nb) return nb;
+  return 0;
+}
+
+static int64_t gram_parts(int64_t m) {
+  return std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(m, 128)));
+}
+
+size_t gram_f64_part_bytes(int64_t m, int64_t k) {
+  const int nb = qr_nblk(k);
+  if (!nb) return 0;
+  return size_t(gram_parts(m)) * (nb * (nb + 1) / 2) * 256 * sizeof(double);
+}
+
+int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, double* G,
+             int64_t ldg, double* part, size_t part_bytes) {
+  const int nb = qr_nblk(k);
+  if (!nb) return fail(RLA_ENOTSUP, "gram_f64: more than 288 columns");
+  if (part_bytes < gram_f64_part_bytes(m, k)) return fail(RLA_EARG, "gram_f64: scratch too small");
+  const int64_t P = gram_parts(m);
+  const int64_t rows = ceil_div(ceil_div(m, P), kGramRows) * kGramRows;
+  const int64_t nparts = ceil_div(m, rows);
+  const int T = nb * (nb + 1) / 2;
+  const int ki = static_cast<int>(k);
+#define RLA_GRAM(NB)                                                                          \
+  case NB:                                                                                    \
+    gram_f64_kernel<NB><<<dim3(unsigned(nparts)), gram_threads(NB), 0, ctx->stream>>>( \
+        m, ki, Y, ldy, rows, part);                                                          \
+    break;
+  switch (nb) {
+    RLA_GRAM(4) RLA_GRAM(8) RLA_GRAM(12) RLA_GRAM(16) RLA_GRAM(17) RLA_GRAM(18)
+  }
+#undef RLA_GRAM
+  RLA_LAUNCH_CHECK("gram_f64");
+  gram_reduce_kernel<<<dim3(unsigned(T)), 256, 0, ctx->stream>>>(ki, T, static_cast<int>(nparts),
+                                                                 part, G, ldg);
+  RLA_LAUNCH_CHECK("gram_f64/reduce");
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;

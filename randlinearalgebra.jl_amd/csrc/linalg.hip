@@ -3,15 +3,20 @@
 // the RandSVD finish (randsvd.jl:174-189), on top of the BF16 MFMA GEMM
 // (mixed precision) or vendor FP32/FP64 GEMMs (plain precision).
 //
-// QR: CholeskyQR2 in FP64 — G = Y'Y (SYRK), R = chol(G), Y <- Y R^-1 (TRSM),
-// twice.  If the Cholesky factorisation breaks down (Y numerically rank
-// deficient in FP64) the same call falls back to Householder QR
-// (GEQRF + ORGQR) on the current Y, which spans the same space.  Q is
-// returned with the column signs of a positive-diagonal R.
+// QR: CholeskyQR2 in FP64 — G = Y'Y, R = chol(G), Y <- Y R^-1, twice.  For
+// k <= 288 columns each pass is a few launches and no host round trip: the
+// one-read Gram kernel, a one-workgroup Cholesky, and an in-place blocked
+// triangular solve that is skipped on the device when the factorisation
+// broke down.  Wider k uses the tiled FP64 GEMM with rocSOLVER's potrf/trtri.
+// If the Cholesky factorisation breaks down (Y numerically rank deficient in
+// FP64) the QR falls back to Householder (GEQRF + ORGQR) on the current Y,
+// which spans the same space.  Q is returned with the column signs of a
+// positive-diagonal R.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "rla_internal.hpp"
 
@@ -27,6 +32,14 @@ int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, do
              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
              int64_t ldc, double* part, size_t part_bytes);
 size_t gemm_f64_split_bytes(int64_t M, int64_t N);
+bool cholqr_small_k(int64_t k);
+size_t gram_f64_part_bytes(int64_t m, int64_t k);
+int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, double* G,
+             int64_t ldg, double* part, size_t part_bytes);
+int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R, int64_t ldr,
+                double* Rt, int64_t ldt, double* Dinv, int* info_dev);
+int trsm_ru_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, const double* Rt,
+                int64_t ldt, const double* Dinv, const int* info);
 
 #define RLA_BLAS(call)                                                                       \
   do {                                                                                       \
@@ -123,17 +136,38 @@ __global__ void signed_upper_kernel(int64_t k, const double* R, int64_t ldr, dou
   dst[i + j * ldd] = R[i + i * ldr] < 0 ? -v : v;
 }
 
-size_t qr_work_bytes(int64_t m, int64_t k) {
-  Bump b{nullptr, 0};
-  b.take<double>(k * k);
-  b.take<double>(k * k);
-  b.take<double>(k);
-  b.take<double>(k);
-  b.take<int>(1);
-  b.take<double>(m * k);
-  b.take<char>(gemm_f64_split_bytes(k, k));
+struct QrBufs {
+  double* G;     // k x k Gram, then the Cholesky factor (in place)
+  double* Racc;  // k x k accumulated R
+  double* X;     // k x k: R row-major (small k) or R^-1 (wide k)
+  double* Dinv;  // inverses of R's 16 x 16 diagonal blocks (small k)
+  double* tau;
+  double* sg;
+  int* info;     // 2 slots, one per pass
+  double* T;     // m x k product temporary (wide-k path only)
+  double* part;  // split-K / Gram partials
+  size_t part_bytes;
+};
+
+static size_t qr_layout(int64_t m, int64_t k, void* work, size_t bytes, QrBufs* out) {
+  Bump b{static_cast<char*>(work), bytes};
+  QrBufs q{};
+  q.G = b.take<double>(k * k);
+  q.Racc = b.take<double>(k * k);
+  q.X = b.take<double>(k * k);
+  q.Dinv = b.take<double>(ceil_div(k, 16) * 256);
+  q.tau = b.take<double>(k);
+  q.sg = b.take<double>(k);
+  q.info = b.take<int>(2);
+  const bool small = cholqr_small_k(k);
+  q.T = small ? nullptr : b.take<double>(m * k);
+  q.part_bytes = small ? gram_f64_part_bytes(m, k) : gemm_f64_split_bytes(k, k);
+  q.part = reinterpret_cast<double*>(b.take<char>(q.part_bytes));
+  if (out) *out = q;
   return b.need;
 }
+
+size_t qr_work_bytes(int64_t m, int64_t k) { return qr_layout(m, k, nullptr, 0, nullptr); }
 
 __global__ void zero_lower_kernel(int64_t k, double* R, int64_t ldr) {
   const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -142,87 +176,120 @@ __global__ void zero_lower_kernel(int64_t k, double* R, int64_t ldr) {
   if (i > j) R[i + j * ldr] = 0.0;
 }
 
+// One CholeskyQR pass on Y: G = Y'Y, G <- chol(G) (upper, in place), Y <- Y G^-1.
+// The small-k path leaves Y untouched on the device when *info != 0 (the
+// solve is skipped there); the wide path checks on the host.
+static int cholqr_pass(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, QrBufs& q,
+                       int* info, bool* broke) {
+  if (cholqr_small_k(k)) {
+    int rc = gram_f64(ctx, m, k, Y, ldy, q.G, k, q.part, q.part_bytes);
+    if (rc) return rc;
+    rc = chol_blocks(ctx, k, q.G, k, q.G, k, q.X, k, q.Dinv, info);
+    if (rc) return rc;
+    return trsm_ru_f64(ctx, m, k, Y, ldy, q.X, k, q.Dinv, info);
+  }
+  rocblas_handle h;
+  int rc = blas(ctx, &h);
+  if (rc) return rc;
+  const int ki = static_cast<int>(k);
+  rc = gemm_f64(ctx, true, false, k, k, m, 1.0, Y, ldy, Y, ldy, 0.0, q.G, k, q.part,
+                q.part_bytes);
+  if (rc) return rc;
+  RLA_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, ki, q.G, ki, info));
+  int hinfo = 0;
+  RLA_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (hinfo != 0) {
+    *broke = true;
+    return RLA_OK;
+  }
+  zero_lower_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(k, q.G, k);
+  RLA_HIP(hipMemcpyAsync(q.X, q.G, sizeof(double) * k * k, hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  RLA_BLAS(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, ki, q.X, ki, info));
+  rc = gemm_f64(ctx, false, false, m, k, k, 1.0, Y, ldy, q.X, k, 0.0, q.T, m, nullptr, 0);
+  if (rc) return rc;
+  return convert<double, double>(ctx, m, k, q.T, m, Y, ldy);
+}
+
 // Y (m x k, ldy) <- Q; optionally R (k x k upper, ldr) with Y_in = Q R.
+// info_async == nullptr: the call checks each pass on the host and falls back
+// to Householder QR itself.  Otherwise (small k, R == nullptr) nothing is
+// synchronised: the two passes' breakdown flags land in info_async[0..1] and a
+// failed pass leaves Y as it was before that pass; the caller checks later.
 int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R, int64_t ldr,
-           void* work, size_t work_bytes, bool* used_householder) {
+           void* work, size_t work_bytes, bool* used_householder, int* info_async) {
   if (used_householder) *used_householder = false;
   if (k == 0 || m == 0) return RLA_OK;
   if (m < k) return fail(RLA_EARG, "QR of a wide matrix (m < k) is not supported");
   if (m > INT32_MAX || k > INT32_MAX) return fail(RLA_ENOTSUP, "QR dimensions exceed int32");
-  Bump b{static_cast<char*>(work), work_bytes};
-  double* G = b.take<double>(k * k);
-  double* Racc = b.take<double>(k * k);
-  double* tau = b.take<double>(k);
-  double* sg = b.take<double>(k);
-  int* info = b.take<int>(1);
-  double* T = b.take<double>(m * k);
-  const size_t part_bytes = gemm_f64_split_bytes(k, k);
-  double* part = reinterpret_cast<double*>(b.take<char>(part_bytes));
-  if (!part) return fail(RLA_EARG, "QR workspace too small");
+  QrBufs q;
+  if (qr_layout(m, k, work, work_bytes, &q) > work_bytes) return fail(RLA_EARG, "QR workspace too small");
+  const bool small = cholqr_small_k(k);
+  if (info_async && small && !R) {
+    for (int pass = 0; pass < 2; ++pass) {
+      bool broke = false;
+      int rc = cholqr_pass(ctx, m, k, Y, ldy, q, info_async + pass, &broke);
+      if (rc) return rc;
+    }
+    return RLA_OK;
+  }
   rocblas_handle h;
   int rc = blas(ctx, &h);
   if (rc) return rc;
   const double one = 1.0;
   const int mi = static_cast<int>(m), ki = static_cast<int>(k), li = static_cast<int>(ldy);
-  bool breakdown = false;
   int passes_done = 0;
+  bool breakdown = false;
   for (int pass = 0; pass < 2 && !breakdown; ++pass) {
-    // G = Y'Y (FP64 MFMA, split over the rows), R = chol(G)
-    rc = gemm_f64(ctx, true, false, k, k, m, 1.0, Y, ldy, Y, ldy, 0.0, G, k, part, part_bytes);
+    rc = cholqr_pass(ctx, m, k, Y, ldy, q, q.info + pass, &breakdown);
     if (rc) return rc;
-    RLA_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, ki, G, ki, info));
-    int hinfo = 0;
-    RLA_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    RLA_HIP(hipStreamSynchronize(ctx->stream));
-    if (hinfo != 0) {
-      breakdown = true;
-      break;
+    if (small) {
+      int hinfo = 0;
+      RLA_HIP(hipMemcpyAsync(&hinfo, q.info + pass, sizeof(int), hipMemcpyDeviceToHost,
+                             ctx->stream));
+      RLA_HIP(hipStreamSynchronize(ctx->stream));
+      breakdown = hinfo != 0;  // Y was left as it was before this pass
     }
+    if (breakdown) break;
     if (R) {
       if (pass == 0) {
         copy_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(
-            k, G, k, Racc, k);
+            k, q.G, k, q.Racc, k);
       } else {
         // Racc <- R2 * Racc
         RLA_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                               rocblas_diagonal_non_unit, ki, ki, &one, G, ki, Racc, ki, Racc,
-                               ki));
+                               rocblas_diagonal_non_unit, ki, ki, &one, q.G, ki, q.Racc, ki,
+                               q.Racc, ki));
       }
     }
-    // Y <- Y R^-1 as a GEMM with the explicit triangular inverse
-    RLA_BLAS(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, ki, G, ki, info));
-    zero_lower_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(k, G, k);
-    rc = gemm_f64(ctx, false, false, m, k, k, 1.0, Y, ldy, G, k, 0.0, T, m, nullptr, 0);
-    if (rc) return rc;
-    rc = convert<double, double>(ctx, m, k, T, m, Y, ldy);
-    if (rc) return rc;
     ++passes_done;
   }
   if (breakdown) {
     if (used_householder) *used_householder = true;
-    RLA_BLAS(rocsolver_dgeqrf(h, mi, ki, Y, li, tau));
+    RLA_BLAS(rocsolver_dgeqrf(h, mi, ki, Y, li, q.tau));
     // D = sign(diag(R_h)); keep D*R_h (positive diagonal) in G, D in sg
-    diag_sign_kernel<<<dim3(unsigned(ceil_div(k, 256))), 256, 0, ctx->stream>>>(k, Y, ldy, sg);
-    signed_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(k, Y, ldy,
-                                                                                     G, k);
-    RLA_BLAS(rocsolver_dorgqr(h, mi, ki, ki, Y, li, tau));
+    diag_sign_kernel<<<dim3(unsigned(ceil_div(k, 256))), 256, 0, ctx->stream>>>(k, Y, ldy, q.sg);
+    signed_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(
+        k, Y, ldy, q.G, k);
+    RLA_BLAS(rocsolver_dorgqr(h, mi, ki, ki, Y, li, q.tau));
     // Q = Q_h * D, so Q * (D * R_h) = Q_h * R_h
     fix_signs_kernel<<<dim3(unsigned(ceil_div(m * k, 256))), 256, 0, ctx->stream>>>(m, k, Y, ldy,
-                                                                                  sg);
+                                                                                  q.sg);
     if (R) {
       if (passes_done == 0) {
         copy_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(
-            k, G, k, Racc, k);
+            k, q.G, k, q.Racc, k);
       } else {
         RLA_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                               rocblas_diagonal_non_unit, ki, ki, &one, G, ki, Racc, ki, Racc,
-                               ki));
+                               rocblas_diagonal_non_unit, ki, ki, &one, q.G, ki, q.Racc, ki,
+                               q.Racc, ki));
       }
     }
   }
   if (R) {
-    copy_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(k, Racc, k,
-                                                                                   R, ldr);
+    copy_upper_kernel<<<dim3(unsigned(ceil_div(k * k, 256))), 256, 0, ctx->stream>>>(
+        k, q.Racc, k, R, ldr);
   }
   RLA_LAUNCH_CHECK("qr_f64");
   return RLA_OK;
@@ -240,6 +307,9 @@ struct RfBufs {
   float* Z32;     // n x k
   uint16_t* Ybf;  // m x k, ld8(m)
   uint16_t* Zbf;  // n x k, ld8(n)
+  double* Ybak;   // m x k copy of the sketch, for a rerun after a QR breakdown
+  int* info;      // Cholesky breakdown flags, 2 per QR of the run
+  int ninfo;
   void* qr;
   size_t qr_bytes;
 };
@@ -257,6 +327,9 @@ static size_t rf_layout(int precision, int a_elem, int64_t m, int64_t n, int64_t
     r.Ybf = b.take<uint16_t>(ld8(m) * k);
     r.Zbf = b.take<uint16_t>(ld8(n) * k);
   }
+  r.Ybak = b.take<double>(m * k);
+  r.ninfo = 2 * 64;
+  r.info = b.take<int>(r.ninfo);
   r.qr_bytes = qr_work_bytes(std::max(m, n), k);
   r.qr = b.take<char>(r.qr_bytes);
   if (out) *out = r;
@@ -339,6 +412,44 @@ static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, 
   return RLA_OK;
 }
 
+// The iterations of power_its.jl.  info == nullptr: QRs check breakdown on
+// the host; otherwise QR number i records its flags in info[2i], info[2i+1].
+static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
+                  int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
+                  const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
+                  double* Y, int64_t ldy, RfBufs& w, int* info) {
+  int rc;
+  int64_t nq = 0;
+  auto qr = [&](int64_t rows, double* M, int64_t ld) {
+    int* slot = info ? info + 2 * nq++ : nullptr;
+    return qr_f64(ctx, rows, k, M, ld, nullptr, 0, w.qr, w.qr_bytes, nullptr, slot);
+  };
+  if (orthogonalize) {
+    // rand_ortho_it: QR after every product (power_its.jl:53-80)
+    rc = qr(m, Y, ldy);
+    if (rc) return rc;
+    for (int64_t it = 0; it < power_its; ++it) {
+      rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
+      if (rc) return rc;
+      rc = qr(n, w.Z64, n);
+      if (rc) return rc;
+      rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
+      if (rc) return rc;
+      rc = qr(m, Y, ldy);
+      if (rc) return rc;
+    }
+    return RLA_OK;
+  }
+  // rand_power_it: products only, one QR at the end (power_its.jl:15-36)
+  for (int64_t it = 0; it < power_its; ++it) {
+    rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
+    if (rc) return rc;
+    rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
+    if (rc) return rc;
+  }
+  return qr(m, Y, ldy);
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -390,7 +501,7 @@ int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, doubl
   int rc = use_device(ctx);
   if (rc) return rc;
   bool hh = false;
-  rc = qr_f64(ctx, m, k, Y, ldy, R, ldr, work, work_bytes, &hh);
+  rc = qr_f64(ctx, m, k, Y, ldy, R, ldr, work, work_bytes, &hh, nullptr);
   if (used_householder) *used_householder = hh ? 1 : 0;
   return rc;
 }
@@ -415,30 +526,28 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
   RfBufs w;
   if (rf_layout(precision, a_elem, m, n, k, work, work_bytes, &w) > work_bytes)
     return fail(RLA_EARG, "RangeFinder workspace too small");
-  if (orthogonalize) {
-    // rand_ortho_it: QR after every product (power_its.jl:53-80)
-    rc = qr_f64(ctx, m, k, Y, ldy, nullptr, 0, w.qr, w.qr_bytes, nullptr);
+  const int64_t nqr = orthogonalize ? 1 + 2 * power_its : 1;
+  const bool async = cholqr_small_k(k) && 2 * nqr <= w.ninfo;
+  if (async) {
+    // no host round trip inside the loop; a breakdown anywhere reruns the
+    // whole solve with host-checked QRs (Householder fallback) from the copy
+    RLA_HIP(hipMemcpy2DAsync(w.Ybak, sizeof(double) * m, Y, sizeof(double) * ldy,
+                             sizeof(double) * m, k, hipMemcpyDeviceToDevice, ctx->stream));
+    rc = rf_run(ctx, precision, orthogonalize, m, n, k, power_its, a_elem, A, lda, A_rm, ld_rm,
+                A_cm, ld_cm, Y, ldy, w, w.info);
     if (rc) return rc;
-    for (int64_t it = 0; it < power_its; ++it) {
-      rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
-      if (rc) return rc;
-      rc = qr_f64(ctx, n, k, w.Z64, n, nullptr, 0, w.qr, w.qr_bytes, nullptr);
-      if (rc) return rc;
-      rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
-      if (rc) return rc;
-      rc = qr_f64(ctx, m, k, Y, ldy, nullptr, 0, w.qr, w.qr_bytes, nullptr);
-      if (rc) return rc;
-    }
-    return RLA_OK;
+    std::vector<int> h(2 * nqr);
+    RLA_HIP(hipMemcpyAsync(h.data(), w.info, sizeof(int) * 2 * nqr, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    RLA_HIP(hipStreamSynchronize(ctx->stream));
+    bool ok = true;
+    for (int v : h) ok = ok && v == 0;
+    if (ok) return RLA_OK;
+    RLA_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, w.Ybak, sizeof(double) * m,
+                             sizeof(double) * m, k, hipMemcpyDeviceToDevice, ctx->stream));
   }
-  // rand_power_it: products only, one QR at the end (power_its.jl:15-36)
-  for (int64_t it = 0; it < power_its; ++it) {
-    rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
-    if (rc) return rc;
-    rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
-    if (rc) return rc;
-  }
-  return qr_f64(ctx, m, k, Y, ldy, nullptr, 0, w.qr, w.qr_bytes, nullptr);
+  return rf_run(ctx, precision, orthogonalize, m, n, k, power_its, a_elem, A, lda, A_rm, ld_rm,
+                A_cm, ld_cm, Y, ldy, w, nullptr);
 }
 
 // RandSVD finish (randsvd.jl:174-189): with B = Q'A (k x n),
@@ -481,7 +590,7 @@ int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_
   if (rc) return rc;
   rc = convert<double, double>(ctx, n, k, w.Z64, n, V, ldv);
   if (rc) return rc;
-  rc = qr_f64(ctx, n, k, V, ldv, Rb, k, w.qr, w.qr_bytes, nullptr);
+  rc = qr_f64(ctx, n, k, V, ldv, Rb, k, w.qr, w.qr_bytes, nullptr, nullptr);
   if (rc) return rc;
   // SVD of Rb' (k x k): transpose Rb into Ur's slot first
   rocblas_handle h;

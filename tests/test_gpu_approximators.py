@@ -93,24 +93,80 @@ def test_bf16_conversion_rounds_to_nearest_even(rla, dev):
     assert torch.equal(got, want)
 
 
-@pytest.mark.parametrize("m,k", [(5000, 266), (300, 40), (64, 64)])
-def test_qr_f64_cholqr2(rla, dev, m, k):
+@pytest.mark.parametrize("m,k,ld", [(5000, 266, 5000), (300, 40, 300), (64, 64, 64),
+                                    (33000, 17, 33000), (2000, 288, 2000), (2000, 289, 2000),
+                                    (1000, 1, 1000), (4100, 130, 4100), (70, 65, 70),
+                                    (500, 100, 512), (8192, 272, 8192), (3000, 200, 3001)])
+def test_qr_f64_cholqr2(rla, dev, m, k, ld):
+    # k <= 288: one-read Gram, one-workgroup Cholesky/inverse, in-place Y R^-1;
+    # k = 289: tiled GEMM path.  ld > m checks the leading dimension.
     Y = rnd((m, k), m) @ np.diag(np.logspace(0, -4, k))  # cond 1e4
-    Yd = colmajor(Y, dev)
+    Yp = np.zeros((ld, k))
+    Yp[:m] = Y
+    Yfull = colmajor(Yp, dev)
+    Yd = Yfull[:m]
     Rd = torch.zeros(k, k, dtype=F64, device=dev).T
     nb = rla._dense.query_bytes("rla_qr_workspace_bytes", m, k)
     work = rla._dense.workspace(nb, dev)
     import ctypes
     hh = ctypes.c_int(-1)
-    rla._lib.call("rla_qr_f64", rla._lib.ctx(dev), m, k, rla._lib.ptr(Yd), m, rla._lib.ptr(Rd),
+    rla._lib.call("rla_qr_f64", rla._lib.ctx(dev), m, k, rla._lib.ptr(Yd), ld, rla._lib.ptr(Rd),
                   k, rla._lib.ptr(work), work.numel() * 8, ctypes.byref(hh))
     Q, Rm = Yd.cpu().numpy(), Rd.cpu().numpy()
     assert hh.value == 0
+    assert np.all(Yfull[m:].cpu().numpy() == 0)  # rows past m untouched
     assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-12 * k
     assert np.linalg.norm(Q @ Rm - Y) < 1e-12 * np.linalg.norm(Y)
     assert np.allclose(np.triu(Rm), Rm) and np.all(np.diag(Rm) > 0)
     # same Q as a positive-diagonal Householder QR
     assert np.linalg.norm(Q - R._q(Y)) < 1e-8
+
+
+def _chol_inv(rla, dev, G):
+    import ctypes
+    k = G.shape[0]
+    Gd = colmajor(G, dev)
+    Rd = torch.full((k, k), np.nan, dtype=F64, device=dev).T
+    Xd = torch.full((k, k), np.nan, dtype=F64, device=dev).T
+    info = ctypes.c_int(-1)
+    rla._lib.call("rla_chol_inv_f64", rla._lib.ctx(dev), k, rla._lib.ptr(Gd), k,
+                  rla._lib.ptr(Rd), k, rla._lib.ptr(Xd), k, ctypes.byref(info))
+    return Rd.cpu().numpy(), Xd.cpu().numpy(), info.value
+
+
+@pytest.mark.parametrize("k", [1, 2, 15, 16, 17, 31, 33, 100, 200, 255, 266, 272, 288])
+def test_chol_inv_matches_lapack(rla, dev, k):
+    B = rnd((k + 8, k), k) @ np.diag(np.logspace(0, -2, k))
+    G = B.T @ B
+    Rm, Xm, info = _chol_inv(rla, dev, G)
+    assert info == 0
+    Rref = np.linalg.cholesky(G).T            # upper, positive diagonal
+    assert np.array_equal(np.triu(Rm), Rm) and np.array_equal(np.triu(Xm), Xm)
+    assert np.linalg.norm(Rm - Rref) < 1e-12 * np.linalg.norm(Rref) * max(1, np.sqrt(k))
+    # X R = I to the conditioning of R (cond(R) <= 1e2 * cond(B)-ish)
+    assert np.linalg.norm(Xm @ Rm - np.eye(k)) < 1e-11 * np.sqrt(k)
+    assert np.linalg.norm(Xm - np.linalg.inv(Rref)) < 1e-10 * np.linalg.norm(Xm)
+
+
+def test_chol_inv_rejects_k_above_288(rla, dev):
+    with pytest.raises(rla._lib.ArgumentError, match="288"):
+        _chol_inv(rla, dev, np.eye(289))
+
+
+@pytest.mark.parametrize("k,bad", [(5, 3), (40, 1), (40, 17), (288, 280), (266, 266)])
+def test_chol_inv_reports_breakdown_column(rla, dev, k, bad):
+    # G = diag(d) with d[bad-1] <= 0: the factorisation stops at column `bad`
+    d = np.linspace(1.0, 2.0, k)
+    d[bad - 1] = -1.0 if bad % 2 else 0.0
+    _, _, info = _chol_inv(rla, dev, np.diag(d))
+    assert info == bad
+
+
+def test_chol_inv_flags_numerically_singular(rla, dev):
+    # rank 10 of 30 in FP64: pivots at rounding level are a breakdown
+    B = rnd((400, 10), 1) @ rnd((10, 30), 2)
+    _, _, info = _chol_inv(rla, dev, B.T @ B)
+    assert 10 < info <= 30
 
 
 def test_qr_f64_householder_fallback_on_rank_deficient(rla, dev):
@@ -189,6 +245,23 @@ def test_rangefinder_bf16_error_close_to_plain(rla, dev):
         errs[prec] = np.linalg.norm(A - Q @ (Q.T @ A)) / np.linalg.norm(A)
     assert errs["plain"] < 1.05 * opt
     assert errs["bf16"] <= 1.1 * errs["plain"], errs
+
+
+@pytest.mark.parametrize("prec", ["plain", "bf16"])
+@pytest.mark.parametrize("ortho", [True, False])
+def test_rangefinder_rank_deficient_reruns_with_householder(rla, dev, prec, ortho):
+    # rank 8 < k = 30: the device-side Cholesky breakdown is caught after the
+    # run, which is redone from the saved sketch with host-checked QRs
+    A = R._q(rnd((700, 8), 4)) @ rnd((8, 260), 5)
+    Ad = colmajor(A, dev)
+    rla.seed(9)
+    rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=30),
+                         power_its=2, orthogonalize=ortho, precision=prec)
+    Q = rla.rapproximate(rf, Ad).range.cpu().numpy()
+    assert np.isfinite(Q).all()
+    assert np.linalg.norm(Q.T @ Q - np.eye(30)) < 1e-10
+    tol = 1e-10 if prec == "plain" else 5e-2   # BF16 rounding of A is full rank
+    assert np.linalg.norm(A - Q @ (Q.T @ A)) < tol * np.linalg.norm(A)
 
 
 @pytest.mark.parametrize("prec", ["plain", "bf16"])

@@ -3,17 +3,18 @@
 // solve Y <- Y R^-1 that completes a CholeskyQR pass.
 //
 // Factorisation: one workgroup of 8 wavefronts.  The upper-triangular
-// 16 x 16 blocks of G are spread round-robin over the wavefronts and stay in
+// 16 x 16 blocks of G are spread round-robin over 7 wavefronts and stay in
 // their MFMA accumulator registers for the whole factorisation (at most
-// 18 x 19 / 2 blocks, 22 per wavefront), so the k^3/3 flops of the trailing
+// 18 x 19 / 2 blocks, 25 per wavefront), so the k^3/3 flops of the trailing
 // updates touch no memory.  Block step s:
-//   1. the owners of block row s copy it to LDS,
-//   2. wavefront 0 factors the 16 x 16 diagonal block in LDS,
-//   3. one thread per column solves R_ss' p = g for the panel row,
-//   4. every wavefront subtracts P_i' P_j from its trailing blocks (MFMA).
-// R is written column-major and row-major (the solve's operand layout), and
-// the inverses of the diagonal blocks are formed at the end, one block per
-// wavefront.
+//   1. the owner of the diagonal block G_ss copies it to LDS,
+//   2. the eighth wavefront factors it in registers (lane c holds column c;
+//      each row of R_ss is passed through LDS) and inverts R_ss,
+//   3. the owners of G_sj (j > s) form R_sj = inv(R_ss)' G_sj on the MFMA
+//      and publish it in LDS,
+//   4. every wavefront subtracts R_si' R_sj from its trailing blocks (MFMA).
+// R is written column-major and row-major (the solve's operand layout), with
+// the inverses of the diagonal blocks for the solve.
 //
 // A pivot that is not finite or not above k * eps * G_jj stops the
 // factorisation and reports the 1-based column in *info (0 on success).  The
@@ -38,11 +39,10 @@ constexpr int kCholThreads = 512;
 constexpr int kCholWaves = kCholThreads / 64;
 constexpr int kMaxCholK = 288;
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
+// LDS exchanges between the lanes of one wavefront need no fence: the LDS
+// executes a wavefront's instructions in order, and these accesses may alias
+// so the compiler keeps their order.  The wave barrier only pins the code.
+__device__ __forceinline__ void lane_exchange() { __builtin_amdgcn_wave_barrier(); }
 
 __device__ __forceinline__ void upper_block(int q, int& bi, int& bj) {
   bj = 0;
@@ -53,149 +53,241 @@ __device__ __forceinline__ void upper_block(int q, int& bi, int& bj) {
   bi = q;
 }
 
-template <int NBLK>
-__global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
-    int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
-    double* __restrict__ Dinv, int* __restrict__ info) {
-  constexpr int T = NBLK * (NBLK + 1) / 2;
-  constexpr int MAXB = (T + kCholWaves - 1) / kCholWaves;
-  constexpr int CP = NBLK * 16;
-  __shared__ double P[16][CP + 1];  // block row s: G after updates, then R
-  __shared__ double fl[CP];         // pivot floors k * eps * G_jj
-  __shared__ int fail;
-  const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int lr = lane >> 4, lc = lane & 15;
+// value of v in lane src (src uniform), for every lane
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
 
-  int bi[MAXB], bj[MAXB];
-  f64x4 acc[MAXB];
+// Upper-triangular 16 x 16 block number q -> (block row, block column),
+// enumerated column by column.
+template <int NBLK>
+struct UpperBlocks {
+  int bi[NBLK * (NBLK + 1) / 2];
+  int bj[NBLK * (NBLK + 1) / 2];
+  constexpr UpperBlocks() : bi(), bj() {
+    int q = 0;
+    for (int j = 0; j < NBLK; ++j)
+      for (int i = 0; i <= j; ++i) {
+        bi[q] = i;
+        bj[q] = j;
+        ++q;
+      }
+  }
+};
+
+constexpr int kCholBlockWaves = kCholWaves - 1;  // the last wavefront factors
+
+// The trailing-matrix role of block wavefront W: its blocks are
+// W, W + 7, W + 14, ... of the column-by-column enumeration.  W is a template
+// parameter so every block coordinate and LDS offset is a compile-time
+// constant (no address or index registers per block).
+template <int NBLK, int W>
+__device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg, double* R,
+                                                int ldr, double* __restrict__ Rt, int ldt,
+                                                double* __restrict__ Dinv,
+                                                double (*P)[NBLK * 16 + 1], double (*Ds)[17],
+                                                double (*Dv)[17], volatile int* fail, int t) {
+  constexpr int T = NBLK * (NBLK + 1) / 2;
+  constexpr int BW = kCholBlockWaves;
+  constexpr int MAXB = (T - W + BW - 1) / BW;  // blocks of this wavefront
+  constexpr UpperBlocks<NBLK> tab{};
+  const int lane = t & 63, lr = lane >> 4, lc = lane & 15;
+  f64x4 acc[MAXB > 0 ? MAXB : 1];
 #pragma unroll
   for (int u = 0; u < MAXB; ++u) {
-    const int q = wave + kCholWaves * u;
-    upper_block(q < T ? q : 0, bi[u], bj[u]);
-    if (q >= T) bi[u] = bj[u] = NBLK;  // never active
+    const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+    // clamped (in-bounds) loads zeroed afterwards; issued four blocks at a
+    // time so the prologue does not hold an address per element in flight
+    if (u % 4 == 0) asm volatile("" ::: "memory");
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = bi[u] * 16 + lr + 4 * r, j = bj[u] * 16 + lc;
-      acc[u][r] = (q < T && i < k && j < k) ? G[i + j * ldg] : 0.0;
+      const int i = bi * 16 + lr + 4 * r, j = bj * 16 + lc;
+      const double x = G[min(i, k - 1) + min(j, k - 1) * ldg];
+      acc[u][r] = (i < k && j < k) ? x : 0.0;
     }
   }
-  for (int j = t; j < CP; j += kCholThreads)
-    fl[j] = j < k ? k * 2.220446049250313e-16 * G[j + j * ldg] : 0.0;
-  if (t == 0) fail = 0;
   // zero R below the block diagonal (the diagonal blocks are written whole);
   // G's lower blocks are never read, so R may alias G
-  for (int e = t; e < k * k; e += kCholThreads) {
+  for (int e = t; e < k * k; e += BW * 64) {
     const int i = e % k, j = e / k;
     if ((i >> 4) > (j >> 4)) R[i + j * ldr] = 0.0;
   }
   __syncthreads();
-
-  double* pl = &P[lr][lc];  // this lane's element of a 4 x 16 slab; slab rows are CP+1 apart
+  double* pl = &P[lr][lc];  // this lane's element of a 4 x 16 slab (rows NBLK*16+1 apart)
+  constexpr int SR = NBLK * 16 + 1;
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
     if (js >= k) break;
-    const int nb = min(16, k - js);
-    // re-derive the lane pointer each step so the compiler does not keep one
-    // address register per block alive across the loop
-    asm volatile("" : "+v"(pl));
-    // 1. block row s to LDS
+    // 1. the diagonal block to LDS
 #pragma unroll
     for (int u = 0; u < MAXB; ++u)
-      if (bi[u] == s) {
+      if (tab.bi[W + BW * u] == s && tab.bj[W + BW * u] == s) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) pl[4 * r * (CP + 1) + bj[u] * 16] = acc[u][r];
+        for (int r = 0; r < 4; ++r) Ds[lr + 4 * r][lc] = acc[u][r];
       }
     __syncthreads();
-    // 2. diagonal block, wavefront 0; lanes own elements (i, c) = (e / 16, e % 16)
-    if (wave == 0) {
-      for (int j = 0; j < nb; ++j) {
-        const double dj = P[j][js + j];
-        if (!(dj > fl[js + j]) || !(dj > 0.0) || !isfinite(dj)) {
-          if (lane == 0) fail = js + j + 1;
-          break;
-        }
-        const double d = sqrt(dj), rd = 1.0 / d;
-        wave_sync();
-        if (lane >= j && lane < nb) P[j][js + lane] = lane == j ? d : P[j][js + lane] * rd;
-        wave_sync();
+    __syncthreads();  // 2. factored by the last wavefront
+    if (*fail) break;
+    // 3. panel row R_sj = inv(R_ss)' G_sj by its owners (MFMA); the
+    //    accumulator layout of G_sj is the operand layout the product needs.
+    //    The owner of G_ss stores R_ss and inv(R_ss).
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int e = lane + 64 * u, i = e >> 4, c = e & 15;
-          if (i > j && i <= c && c < nb) P[i][js + c] -= P[j][js + i] * P[j][js + c];
+    for (int u = 0; u < MAXB; ++u) {
+      const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+      if (bi == s && bj == s) {
+        const int c = lc, nb = min(16, k - js);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = lr + 4 * r;
+          Dinv[s * 256 + i * 16 + c] = Dv[i][c];
+          if (i < nb && c < nb) {
+            R[(js + i) + int64_t(js + c) * ldr] = Ds[i][c];
+            Rt[int64_t(js + i) * ldt + js + c] = Ds[i][c];
+          }
         }
-        wave_sync();
       }
-    }
-    __syncthreads();
-    if (fail) break;
-    // 3. panel row (columns past the diagonal block) and R / Rt for block row s
-    const int n2 = k - js - 16;
-    if (t < n2) {
-      // forward substitution in place in LDS (the trailing blocks hold the
-      // registers); column c belongs to this thread alone
-      const int c = js + 16 + t;
-      for (int i = 0; i < 16; ++i) {
-        double v = P[i][c];
-        for (int l = 0; l < i; ++l) v -= P[l][js + i] * P[l][c];
-        v /= P[i][js + i];
-        P[i][c] = v;
-        R[(js + i) + int64_t(c) * ldr] = v;
-        Rt[int64_t(js + i) * ldt + c] = v;
-      }
-    }
-    if (t < 256) {
-      const int i = t >> 4, c = t & 15;
-      if (i < nb && c < nb) {
-        const double v = i <= c ? P[i][js + c] : 0.0;
-        R[(js + i) + int64_t(js + c) * ldr] = v;
-        Rt[int64_t(js + i) * ldt + js + c] = v;
+      if (bi == s && bj > s) {
+        f64x4 o = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          o = __builtin_amdgcn_mfma_f64_16x16x4f64(Dv[4 * q + lr][lc], acc[u][q], o, 0, 0, 0);
+        const int j = bj * 16 + lc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = lr + 4 * r;
+          pl[4 * r * SR + bj * 16] = o[r];
+          if (j < k) {
+            R[(js + i) + int64_t(j) * ldr] = o[r];
+            Rt[int64_t(js + i) * ldt + j] = o[r];
+          }
+        }
       }
     }
     __syncthreads();
     // 4. trailing update of the register-resident blocks
 #pragma unroll
-    for (int u = 0; u < MAXB; ++u)
-      if (bi[u] > s && bi[u] < NBLK) {
+    for (int u = 0; u < MAXB; ++u) {
+      const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+      if (bi > s) {
         double a[4], b[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a[q] = pl[4 * q * (CP + 1) + bi[u] * 16];
-          b[q] = pl[4 * q * (CP + 1) + bj[u] * 16];
+          a[q] = pl[4 * q * SR + bi * 16];
+          b[q] = pl[4 * q * SR + bj * 16];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[q], b[q], acc[u], 0, 0, 0);
       }
-    __syncthreads();
-  }
-  if (t == 0) *info = fail;
-  if (fail) return;
-  // inverses of the diagonal blocks: lane c < nb solves R_bb x = e_c
-  for (int b = wave; b * 16 < k; b += kCholWaves) {
-    const int js = b * 16, nb = min(16, k - js);
-    if (lane < 16) {
-      double x[16];
-#pragma unroll
-      for (int i = 15; i >= 0; --i) {
-        double v = 0.0;
-        if (i < nb && i <= lane && lane < nb) {
-          v = i == lane ? 1.0 : 0.0;
-#pragma unroll
-          for (int l = 0; l < 16; ++l)
-            if (l > i && l < nb) v -= Rt[int64_t(js + i) * ldt + js + l] * x[l];
-          v /= Rt[int64_t(js + i) * ldt + js + i];
-        }
-        x[i] = v;
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) Dinv[b * 256 + i * 16 + lane] = x[i];
     }
   }
 }
 
 template <int NBLK>
-__global__ __launch_bounds__(256) void trsm_ru_f64_kernel(int64_t m, int k, double* Y, int64_t ldy,
+__device__ __forceinline__ void chol_factor_role(int k, double (*Ds)[17], double (*Dv)[17],
+                                                 double* rowj, const double* fl,
+                                                 volatile int* fail, int lane) {
+  __syncthreads();  // G read into registers, R zeroed
+  for (int s = 0; s < NBLK; ++s) {
+    const int js = s * 16;
+    if (js >= k) break;
+    const int nb = min(16, k - js);
+    __syncthreads();  // 1. G_ss in LDS
+    // 2. factor in registers, lane c holding column c (lanes 16..63 repeat
+    //    lanes 0..15 and write nothing), then invert R_ss the same way
+    const int c = lane & 15;
+    // Lane c keeps column c in registers (lanes 16..63 repeat lanes 0..15 and
+    // write nothing).  Step j: the pivot comes from lane j, row j of R is
+    // published in LDS (rowj), and every lane updates its column with it.
+    // Past nb the block is padded with the identity, so the fully unrolled
+    // loop has no data-dependent exits.
+    double col[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double g = Ds[i][c];
+      col[i] = (i < nb && c < nb) ? g : (i == c ? 1.0 : 0.0);
+    }
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      double dj = lane_bcast(col[j], j);
+      const double floor_j = j < nb ? fl[js + j] : 0.0;
+      if (!bad && (!(dj > floor_j) || !(dj > 0.0) || !isfinite(dj))) bad = js + j + 1;
+      if (bad) dj = 1.0;
+      const double d = sqrt(dj), rd = 1.0 / d;
+      col[j] = c < j ? 0.0 : (c == j ? d : col[j] * rd);
+      if (lane < 16) rowj[c] = col[j];
+      lane_exchange();
+#pragma unroll
+      for (int i = j + 1; i < 16; ++i)
+        if (c >= i) col[i] -= rowj[i] * col[j];
+      lane_exchange();
+    }
+    if (bad) {
+      if (lane == 0) *fail = bad;
+    } else {
+      // R_ss to LDS, then column c of inv(R_ss):
+      // x_i = (e_c[i] - sum_{l>i} R_il x_l) / R_ii
+      if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Ds[i][c] = i <= c ? col[i] : 0.0;
+      }
+      lane_exchange();
+      double x[16];
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        double v = i == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int l = i + 1; l < 16; ++l) v -= Ds[i][l] * x[l];
+        x[i] = i <= c ? v / Ds[i][i] : 0.0;
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Dv[i][c] = (i < nb && c < nb) ? x[i] : 0.0;
+      }
+    }
+    __syncthreads();  // 2. done
+    if (*fail) break;
+    __syncthreads();  // 3. panel in LDS
+  }
+}
+
+template <int NBLK>
+__global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
+    int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
+    double* __restrict__ Dinv, int* __restrict__ info) {
+  constexpr int CP = NBLK * 16;
+  __shared__ double P[16][CP + 1];  // panel row R_s* of the current step
+  __shared__ double Ds[16][17];     // diagonal block G_ss
+  __shared__ double Dv[16][17];     // inv(R_ss)
+  __shared__ double rowj[16];       // row j of R_ss while factoring
+  __shared__ double fl[CP];         // pivot floors k * eps * G_jj
+  __shared__ int fail;
+  const int t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  for (int j = t; j < CP; j += kCholThreads)
+    fl[j] = j < k ? k * 2.220446049250313e-16 * G[j + j * ldg] : 0.0;
+  if (t == 0) fail = 0;
+  // The roles run separate loops with the same barriers (a workgroup barrier
+  // counts arrivals, not program points).
+  switch (wave) {
+#define RLA_ROLE(W)                                                                        \
+  case W:                                                                                  \
+    chol_block_role<NBLK, W>(k, G, ldg, R, ldr, Rt, ldt, Dinv, P, Ds, Dv, &fail, t);       \
+    break;
+    RLA_ROLE(0) RLA_ROLE(1) RLA_ROLE(2) RLA_ROLE(3) RLA_ROLE(4) RLA_ROLE(5) RLA_ROLE(6)
+#undef RLA_ROLE
+    default:
+      chol_factor_role<NBLK>(k, Ds, Dv, rowj, fl, &fail, t & 63);
+  }
+  if (t == 0) *info = fail;
+}
+
+template <int NBLK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trsm_ru_f64_kernel(
+    int64_t m, int k, double* Y, int64_t ldy,
                                                           const double* __restrict__ Rt, int ldt,
                                                           const double* __restrict__ Dinv,
                                                           const int* __restrict__ info) {
@@ -207,55 +299,76 @@ __global__ __launch_bounds__(256) void trsm_ru_f64_kernel(int64_t m, int k, doub
   const int64_t rbase = int64_t(blockIdx.x) * 64 + wave * 16;
   const int64_t row = rbase + lc;  // operand layout: row lane%16, column 4q + lane/16
   const bool row_ok = row < m;
+  // loads are unconditional from clamped (in-bounds) addresses and zeroed
+  // afterwards, so they carry no branches and can all be in flight at once
+  const double* yrow = Y + (row_ok ? row : m - 1);
   double (*ts)[17] = Ts[wave];
   double ya[NBLK][4];
-#pragma unroll
-  for (int b = 0; b < NBLK; ++b) {
-    if (b * 16 >= k) break;
-    // Y_b in the accumulator layout (through LDS)
+  double yn[4];
+  auto load_y = [&](int b) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = b * 16 + 4 * q + lr;
-      ts[lc][4 * q + lr] = (row_ok && c < k) ? Y[row + int64_t(c) * ldy] : 0.0;
+      const double x = yrow[int64_t(c < k ? c : k - 1) * ldy];
+      yn[q] = (row_ok && c < k) ? x : 0.0;
     }
-    wave_sync();
+  };
+  load_y(0);
+#pragma unroll
+  for (int b = 0; b < NBLK; ++b) {
+    if (b * 16 >= k) break;
+    const int cb = b * 16 + lc;
+    const double* rcol = Rt + (cb < k ? cb : k - 1);
+    // Y_b into the accumulator layout (through LDS)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ts[lc][4 * q + lr] = yn[q];
+    lane_exchange();
     f64x4 acc;
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = ts[lr + 4 * r][lc];
-    wave_sync();
-    // T = Y_b - sum_{l<b} Ynew_l R_lb
-    const int cb = b * 16 + lc;
+    lane_exchange();
+    if ((b + 1) * 16 < k) load_y(b + 1);
+    double dv[4];
 #pragma unroll
-    for (int l = 0; l < b; ++l) {
+    for (int q = 0; q < 4; ++q) dv[q] = Dinv[b * 256 + (4 * q + lr) * 16 + lc];
+    // T = Y_b - sum_{l<b} Ynew_l R_lb, R_lb read four row blocks at a time
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const double rv = cb < k ? Rt[int64_t(l * 16 + 4 * q + lr) * ldt + cb] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[l][q], -rv, acc, 0, 0, 0);
-      }
+    for (int l0 = 0; l0 < b; l0 += 4) {
+      double rv[4][4];
+#pragma unroll
+      for (int l = l0; l < l0 + 4 && l < b; ++l)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double x = rcol[int64_t(l * 16 + 4 * q + lr) * ldt];
+          rv[l - l0][q] = cb < k ? -x : 0.0;
+        }
+#pragma unroll
+      for (int l = l0; l < l0 + 4 && l < b; ++l)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[l][q], rv[l - l0][q], acc, 0, 0, 0);
     }
     // Ynew_b = T inv(R_bb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) ts[lr + 4 * r][lc] = acc[r];
-    wave_sync();
+    lane_exchange();
     double ta[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) ta[q] = ts[lc][4 * q + lr];
-    wave_sync();
+    lane_exchange();
     acc = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[q], Dinv[b * 256 + (4 * q + lr) * 16 + lc],
-                                                 acc, 0, 0, 0);
+    for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[q], dv[q], acc, 0, 0, 0);
 #pragma unroll
     for (int r = 0; r < 4; ++r) ts[lr + 4 * r][lc] = acc[r];
-    wave_sync();
+    lane_exchange();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       ya[b][q] = ts[lc][4 * q + lr];
       const int c = b * 16 + 4 * q + lr;
       if (row_ok && c < k) Y[row + int64_t(c) * ldy] = ya[b][q];
     }
-    wave_sync();
+    lane_exchange();
   }
 }
 

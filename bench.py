@@ -176,8 +176,8 @@ def main():
     ap.add_argument("--skip-rangefinder", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the SRHT apply has no multi-GPU path in this build "
-              "(columns shard independently; see DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one GPU: this build has no multi-GPU path (the SRHT apply "
+              "would shard by columns with no exchange). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     import torch
@@ -188,10 +188,13 @@ def main():
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
     rf = None if args.skip_rangefinder else rangefinder_c3(rla, torch)
     if args.calib_copy:
-        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
+        # counter calibration: a plain streaming kernel of known traffic
+        # (rla::convert_kernel<float, double>: reads 256 MiB, writes 512 MiB)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty(64 << 20, dtype=torch.float64, device="cuda")
         for _ in range(3):
-            dst.copy_(src)
+            rla._lib.call("rla_convert_f32_f64", rla._lib.ctx(src.device), 64 << 20, 1,
+                          rla._lib.ptr(src), 64 << 20, rla._lib.ptr(dst), 64 << 20)
         torch.cuda.synchronize()
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "srht_f32_65536x512_traffic.txt")

@@ -13,6 +13,13 @@ tests/test_oracle.py): every function is checked against a closed form or a
 mathematical invariant that a dropped term, wrong sign or transposed operand
 would break (dense Hadamard / sparse products, self-inverse FWHT, Philox
 known-answer vectors, Floyd/Lemire distribution checks).
+
+Parity unpinned: the random streams (philox4x32_10, Stream, floyd_sorted and
+the *_gen recipe generators).  The reference draws with Julia's
+Xoshiro/StatsBase (sparse_sign.jl:167,252; srht.jl:150-151; gaussian.jl:107),
+which cannot run here and which no reference test fixes; these functions
+define this build's counter-based stream, pinned only to Philox known-answer
+vectors, to the device generators (bit for bit) and to distribution tests.
 """
 import math
 

@@ -270,6 +270,55 @@ int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_
                        const double* Q, int64_t ldq, double* U, int64_t ldu, double* S, double* V,
                        int64_t ldv, void* work, size_t work_bytes);
 
+/* ---- Sampling compressor, Kaczmarz -------------------------------------
+ * Row weights w_i = sum_j A[i,j]^2 in FP64 (L2Norm, strohmer_vershynin.jl:82;
+ * column weights by passing transposed strides). */
+int rla_row_norms2_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t a_rs,
+                       int64_t a_cs, double* w);
+int rla_row_norms2_f64(rla_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                       int64_t a_cs, double* w);
+
+/* nsets weighted samples of k items from 0:n-1 (w NULL = uniform), set r
+ * drawn with epoch+r, each sorted ascending: wsample!(state_space, weights,
+ * idx; ordered=true, replace) of sample_distribution! (uniform.jl:93-95,
+ * strohmer_vershynin.jl:150-153).  replace = 0: Efraimidis–Spirakis keys,
+ * the s largest (successive sampling without replacement); replace = 1:
+ * independent inverse-CDF draws.  Needs at least k items of positive weight
+ * when replace = 0; 1 <= k <= 1024. */
+int rla_wsample(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t n, const double* w,
+                int64_t k, int replace, int64_t nsets, int32_t* sets);
+
+/* ||b - A x||_2 accumulated in FP64 (FullResidual compute_error,
+ * full_residual.jl:32-41); blocks until the value is on the host. */
+int rla_residual_norm_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t a_rs,
+                          int64_t a_cs, const float* x, const float* b, double* out_host);
+
+int rla_kaczmarz_workspace_bytes(int64_t m, int64_t n, int64_t s, int64_t every, size_t* bytes);
+
+/* One block projection x += alpha * At' (At At')^-1 (bt - At x) with an
+ * explicit s x n block At (strided) and bt (stride bt_inc): the LQ
+ * minimum-norm step of kaczmarz_update_block! (kaczmarz.jl:294-311,
+ * lq.jl:35-46); s = 1 is kaczmarz_update! (kaczmarz.jl:262-276).  A zero block
+ * is skipped (kaczmarz.jl:300); a singular Gram matrix returns RLA_EDOMAIN.
+ * s <= 64.  Workspace from rla_kaczmarz_workspace_bytes(s, n, s, 1). */
+int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const float* At,
+                                  int64_t at_rs, int64_t at_cs, const float* bt, int64_t bt_inc,
+                                  float* x, double alpha, void* work, size_t work_bytes);
+
+/* rsolve! (kaczmarz.jl:313-368) with a Sampling compressor of s rows and
+ * weights w (NULL: Uniform): for i = 1..max_it, the FP64 residual is
+ * evaluated at i = 1, 1+every, 1+2*every, ... and at max_it, recorded in hist
+ * and tested against threshold (BasicLogger threshold_stop; i == max_it always
+ * stops, basic_logger.jl:93-98); otherwise a new row set is drawn and x is
+ * projected.  A: column-major (lda) for the residual; A_rm: the same matrix
+ * row-major (ld_rm) for the block gathers.  x is updated in place.  s <= 64. */
+int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t lda,
+                           const float* A_rm, int64_t ld_rm, const float* b, float* x, int64_t s,
+                           double alpha, const double* w, int replace, uint64_t seed,
+                           uint64_t epoch0, int64_t max_it, int64_t every, double threshold,
+                           double* hist, int64_t hist_cap, int64_t* n_hist, int64_t* iterations,
+                           int* converged, void* work, size_t work_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -326,3 +326,64 @@ def randsvd(A, Q):
     """randsvd.jl:174-189: QA = Q'A (Float64), U, S, V = svd(QA), U = Q*U."""
     Ub, S, Vt = np.linalg.svd(Q.T @ A, full_matrices=False)
     return Q @ Ub, S, Vt.T
+
+
+# ---------------------------------------------------------------------------
+# Kaczmarz: src/Solvers/kaczmarz.jl, SubSolvers/lq.jl, ErrorMethods/full_residual.jl
+
+
+def lq_min_norm(At, r):
+    """ldiv!(u, qr!(At')', r) of lq.jl:35-46: with At' = Q R (thin), the
+    minimum-norm solution of At u = r is u = Q (R' \\ r)."""
+    Q, Rr = np.linalg.qr(np.asarray(At, dtype=np.float64).T)
+    z = np.linalg.solve(Rr.T, np.asarray(r, dtype=np.float64))
+    return Q @ z
+
+
+def kaczmarz_update_block(x, At, bt, alpha):
+    """kaczmarz_update_block! (kaczmarz.jl:294-311): r = bt - At x, u from the
+    LQ sub-solver, x + alpha u.  A zero block leaves x unchanged (:300)."""
+    x = np.asarray(x, dtype=np.float64)
+    At = np.asarray(At, dtype=np.float64)
+    if not At.any():
+        return x.copy()
+    r = np.asarray(bt, dtype=np.float64) - At @ x
+    return x + alpha * lq_min_norm(At, r)
+
+
+def kaczmarz_update(x, a, c, alpha):
+    """kaczmarz_update! (kaczmarz.jl:262-276): x - alpha (a.x - c)/||a||^2 a."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.asarray(a, dtype=np.float64).ravel()
+    if not a.any():
+        return x.copy()
+    scaling = alpha * (a @ x - float(np.asarray(c).ravel()[0])) / (a @ a)
+    return x - scaling * a
+
+
+def full_residual(A, x, b):
+    """compute_error of full_residual.jl:32-41: ||b - A x||_2 (FP64)."""
+    return float(np.linalg.norm(np.asarray(b, np.float64) - np.asarray(A, np.float64) @
+                                np.asarray(x, np.float64)))
+
+
+def successive_inclusion(w, k):
+    """Inclusion probability of each item in a weighted sample of k without
+    replacement drawn item by item (P(next = i) = w_i / remaining weight),
+    the law of StatsBase wsample!(replace=false); exact enumeration, small n."""
+    w = np.asarray(w, dtype=np.float64)
+    n = len(w)
+    incl = np.zeros(n)
+
+    def rec(chosen, prob, depth):
+        if depth == k:
+            for i in chosen:
+                incl[i] += prob
+            return
+        rest = w.sum() - sum(w[i] for i in chosen)
+        for i in range(n):
+            if i not in chosen and w[i] > 0:
+                rec(chosen | {i}, prob * w[i] / rest, depth + 1)
+
+    rec(frozenset(), 1.0, 0)
+    return incl

@@ -20,4 +20,9 @@ from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F
 from .approximators import (Approximator, ApproximatorAdjoint, ApproximatorRecipe,  # noqa: F401
                             RandSVD, RandSVDRecipe, RangeFinder, RangeFinderRecipe,
                             complete_approximator, rapproximate, rapproximate_)
+from .solvers import (BasicLogger, BasicLoggerRecipe, DistributionRecipe,  # noqa: F401
+                      FullResidual, FullResidualRecipe, Kaczmarz, KaczmarzRecipe, L2Norm,
+                      LQSolver, LQSolverRecipe, Sampling, SamplingRecipe, Uniform,
+                      complete_distribution, complete_logger, complete_solver, compute_error,
+                      rsolve, rsolve_, threshold_stop, update_distribution)
 from . import _dense, _lib  # noqa: F401

@@ -98,6 +98,16 @@ _SIGS = {
     "rla_qr_workspace_bytes": [i64, i64, vp],
     "rla_qr_f64": [vp, i64, i64, vp, i64, vp, i64, vp, ctypes.c_size_t, vp],
     "rla_chol_inv_f64": [vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "rla_row_norms2_f32": [vp, i64, i64, vp, i64, i64, vp],
+    "rla_row_norms2_f64": [vp, i64, i64, vp, i64, i64, vp],
+    "rla_wsample": [vp, u64, u64, i64, vp, i64, ctypes.c_int, i64, vp],
+    "rla_residual_norm_f32": [vp, i64, i64, vp, i64, i64, vp, vp, vp],
+    "rla_kaczmarz_workspace_bytes": [i64, i64, i64, i64, vp],
+    "rla_kaczmarz_block_update_f32": [vp, i64, i64, vp, i64, i64, vp, i64, vp, ctypes.c_double,
+                                      vp, ctypes.c_size_t],
+    "rla_kaczmarz_solve_f32": [vp, i64, i64, vp, i64, vp, i64, vp, vp, i64, ctypes.c_double, vp,
+                               ctypes.c_int, u64, u64, i64, i64, ctypes.c_double, vp, i64, vp,
+                               vp, vp, vp, ctypes.c_size_t],
     "rla_rangefinder_workspace_bytes": [i32, i32, i64, i64, i64, vp],
     "rla_rangefinder": [vp, i32, i32, i64, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp,
                         i64, vp, ctypes.c_size_t],

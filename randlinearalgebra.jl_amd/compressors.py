@@ -63,9 +63,10 @@ def seed(s):
     _rng["epoch"] = 0
 
 
-def _draw():
+def _draw(count=1):
+    """(seed, first epoch) for `count` consecutive draws."""
     e = _rng["epoch"]
-    _rng["epoch"] = e + 1
+    _rng["epoch"] = e + count
     return _rng["seed"], e
 
 

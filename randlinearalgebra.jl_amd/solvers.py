@@ -1,0 +1,479 @@
+"""Solver surface of RandLinearAlgebra.jl on the MI355X kernels: the Sampling
+compressor with its Uniform / L2Norm distributions, block Kaczmarz with the LQ
+sub-solver, FullResidual and BasicLogger.
+
+Mirrors src/Solvers.jl (`complete_solver`, `rsolve!` returning `(x, recipe)`
+for a solver and nothing for a recipe), src/Solvers/kaczmarz.jl,
+src/Compressors/sampling.jl, src/Compressors/Distributions/uniform.jl and
+strohmer_vershynin.jl, src/Solvers/Loggers/basic_logger.jl and
+src/Solvers/ErrorMethods/full_residual.jl.
+
+A Kaczmarz recipe whose compressor is a Left Sampling recipe of at most 64
+rows on Float32 data runs the whole solve in librla_mi355x.so
+(rla_kaczmarz_solve_f32: row sets drawn on the device, fused gather/update
+kernels, FP64 residual); other compressors run the reference loop here with
+each step on the device (update_compressor, mul, rla_kaczmarz_block_update_f32,
+rla_residual_norm_f32).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _dense, _lib
+from ._lib import ArgumentError, DimensionMismatch
+from .compressors import (Compressor, CompressorRecipe, Left, Right, SparseSign, Undef,
+                          _SparseOp, _SparseRecipe, _draw, _size2, as_view, complete_compressor,
+                          mul, update_compressor)
+
+# ---------------------------------------------------------------------------
+# distributions (src/Compressors/Distributions/)
+
+
+class Distribution:
+    cardinality = Undef()
+    replace = False
+
+
+class Uniform(Distribution):
+    """Uniform(cardinality=Undef(), replace=False) — uniform.jl:7-22."""
+
+    def __init__(self, cardinality=None, replace=False):
+        self.cardinality = Undef() if cardinality is None else cardinality
+        self.replace = bool(replace)
+
+
+class L2Norm(Distribution):
+    """L2Norm(cardinality=Undef(), replace=False) — strohmer_vershynin.jl:20-37:
+    rows (Left) or columns (Right) drawn with probability proportional to
+    their squared Euclidean norm."""
+
+    def __init__(self, cardinality=None, replace=False):
+        self.cardinality = Undef() if cardinality is None else cardinality
+        self.replace = bool(replace)
+
+
+class DistributionRecipe:
+    """Weights over the state space 0:n-1 (device FP64, None = uniform)."""
+
+    def __init__(self, dist, A):
+        self.cardinality = dist.cardinality
+        self.replace = dist.replace
+        self.kind = type(dist)
+        self._key = None
+        self.weights = None
+        self.n = 0
+        self._compute(A)
+
+    def _compute(self, A):
+        name = "complete_distribution" if self._key is None else "update_distribution!"
+        if self.cardinality == Undef():
+            raise ArgumentError(
+                f"`{self.kind.__name__}` cardinality must be specified as `Left()` or "
+                f"`Right()`.`Undef()` is not allowed in `{name}`.")
+        a_rows, a_cols = _size2(A)
+        self.n = a_rows if self.cardinality == Left() else a_cols
+        if self.kind is Uniform:
+            self.weights = None
+            return
+        v = as_view(A)
+        key = (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
+        if key == self._key:
+            return  # A unchanged since the weights were computed
+        w = torch.empty(self.n, dtype=torch.float64, device=v.t.device)
+        rs, cs = (v.rs, v.cs) if self.cardinality == Left() else (v.cs, v.rs)
+        cols = v.cols if self.cardinality == Left() else v.rows
+        _lib.call(f"rla_row_norms2_{_lib.sfx(v.dtype)}", _lib.ctx(v.t.device), self.n, cols,
+                  _lib.ptr(v.t), rs, cs, _lib.ptr(w))
+        self.weights = w
+        self._key = key
+
+    def positive(self):
+        if self.weights is None:
+            return self.n
+        return int((self.weights > 0).sum())
+
+
+def complete_distribution(dist, A):
+    return DistributionRecipe(dist, A)
+
+
+def update_distribution(recipe, *args):
+    """update_distribution!(recipe, A) or (recipe, x, A, b): L2Norm recomputes
+    its weights from A (strohmer_vershynin.jl:115-134); they are cached while
+    A is unchanged."""
+    A = args[0] if len(args) == 1 else args[1]
+    recipe._compute(A)
+
+
+# ---------------------------------------------------------------------------
+# Sampling compressor (src/Compressors/sampling.jl)
+
+
+class Sampling(Compressor):
+    """Sampling(cardinality=Left(), compression_dim=2, distribution=Uniform()) —
+    sampling.jl:39-74."""
+
+    def __init__(self, cardinality=None, compression_dim=2, distribution=None):
+        cardinality = Left() if cardinality is None else cardinality
+        if compression_dim <= 0:
+            raise ArgumentError("Field `compression_dim` must be positive.")
+        if cardinality == Undef():
+            raise ArgumentError("`cardinality` must be specified as `Left()` or `Right()`."
+                                "`Undef()` is not allowed in `Sampling` structure.")
+        self.cardinality = cardinality
+        self.compression_dim = int(compression_dim)
+        self.distribution = Uniform() if distribution is None else distribution
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return SamplingRecipe(self, A)
+
+
+class SamplingRecipe(_SparseRecipe):
+    """sampling.jl:76-123.  `idx` is the sorted index set (0-based, device
+    int32).  As an operator, Left: S (s x m) selects rows, S*A = A[idx, :];
+    Right: S (n x s), A*S = A[:, idx]; other products follow from the
+    selection matrix M (s x n_items, M[r, idx[r]] = 1)."""
+
+    def __init__(self, sampling, A):
+        a_rows, a_cols = _size2(A)
+        self.cardinality = sampling.cardinality
+        self.compression_dim = sampling.compression_dim
+        if self.cardinality == Left():
+            self.n_rows, self.n_cols = sampling.compression_dim, a_rows
+        else:
+            self.n_rows, self.n_cols = a_cols, sampling.compression_dim
+        sampling.distribution.cardinality = sampling.cardinality
+        self.distribution_recipe = complete_distribution(sampling.distribution, A)
+        self.dtype = as_view(A).dtype
+        self.device = as_view(A).t.device
+        self.idx = torch.empty(self.compression_dim, dtype=torch.int32, device=self.device)
+        self._sample()
+
+    def _sample(self):
+        d = self.distribution_recipe
+        k = self.compression_dim
+        if not d.replace and d.positive() < k:
+            raise ArgumentError(
+                f"cannot draw {k} distinct indices from {d.positive()} with positive weight")
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        _lib.call("rla_wsample", _lib.ctx(self.device), sd, ep, d.n,
+                  _lib.ptr(d.weights) if d.weights is not None else None, k,
+                  1 if d.replace else 0, 1, _lib.ptr(self.idx))
+        self._build_op()
+
+    def _build_op(self):
+        # M (k x n_items) with M[r, idx[r]] = 1: CSR of M is (0:k, idx); its CSC
+        # is the transpose of M' = CSC (0:k, idx)
+        k, n = self.compression_dim, self.distribution_recipe.n
+        dev, dt = self.device, self.dtype
+        rowptr = torch.arange(0, k + 1, dtype=torch.int32, device=dev)
+        ones = torch.ones(k, dtype=dt, device=dev)
+        colptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        rowidx = torch.empty(k, dtype=torch.int32, device=dev)
+        val = torch.empty(k, dtype=dt, device=dev)
+        _lib.call(f"rla_sparse_transpose_{_lib.sfx(dt)}", _lib.ctx(dev), n, k, k,
+                  _lib.ptr(rowptr), _lib.ptr(self.idx), _lib.ptr(ones), _lib.ptr(colptr),
+                  _lib.ptr(rowidx), _lib.ptr(val))
+        op = _SparseOp.__new__(_SparseOp)
+        op.nrows, op.ncols, op.nnz = k, n, k
+        op.colptr, op.rowidx, op.val = colptr, rowidx, val
+        op.rowptr, op.colidx, op.val_t = rowptr, self.idx, ones
+        op._vals = {}
+        self.op = op
+
+    def _update(self, *args):
+        # update_compressor!(S, x, A, b): update_distribution! then a new draw
+        if args:
+            update_distribution(self.distribution_recipe, *args)
+        self._sample()
+
+
+# ---------------------------------------------------------------------------
+# loggers, errors, sub-solvers
+
+
+def threshold_stop(log):
+    """basic_logger.jl:130-134."""
+    return log.error < log.threshold_info
+
+
+class BasicLogger:
+    """BasicLogger(max_it=0, collection_rate=1, threshold=0.0,
+    stopping_criterion=threshold_stop) — basic_logger.jl:15-45."""
+
+    def __init__(self, max_it=0, collection_rate=1, threshold=0.0,
+                 stopping_criterion=threshold_stop):
+        if max_it < 0:
+            raise ArgumentError("Field `max_it` must be positive or 0.")
+        if collection_rate < 1:
+            raise ArgumentError("Field `colection_rate` must be positive.")
+        if collection_rate > max_it and max_it > 0:
+            raise ArgumentError("Field `colection_rate` must be smaller than `max_it`.")
+        self.max_it = int(max_it)
+        self.collection_rate = int(collection_rate)
+        self.threshold_info = threshold
+        self.stopping_criterion = stopping_criterion
+
+
+class BasicLoggerRecipe:
+    """basic_logger.jl:47-125."""
+
+    def __init__(self, log):
+        self.max_it = log.max_it
+        self.error = 0.0
+        self.threshold_info = log.threshold_info
+        self.iteration = 1
+        self.record_location = 1
+        self.collection_rate = log.collection_rate
+        self.converged = False
+        self.stopping_criterion = log.stopping_criterion
+        self.hist = np.zeros(int(np.ceil(log.max_it / log.collection_rate)) + 1)
+
+    def update(self, error, iteration):
+        """update_logger! (basic_logger.jl:93-109)."""
+        self.iteration = iteration
+        self.error = float(error)
+        self.converged = (self.stopping_criterion(self) if iteration < self.max_it else True)
+        if self.converged:
+            self.hist[self.record_location - 1] = error
+        elif iteration % self.collection_rate == 0:
+            self.hist[self.record_location - 1] = error
+            self.record_location += 1
+
+    def reset(self):
+        """reset_logger! (basic_logger.jl:111-118)."""
+        self.error = 0.0
+        self.iteration = 1
+        self.record_location = 1
+        self.converged = False
+        self.hist[:] = 0.0
+
+
+def complete_logger(log):
+    return BasicLoggerRecipe(log)
+
+
+class FullResidual:
+    """FullResidual() — full_residual.jl:1-20: error = ||b - A x||_2, here
+    accumulated in FP64.  `every` (an extension; 1 is the reference's
+    behaviour) evaluates it only at iterations 1, 1+every, ... and at max_it;
+    the logger sees those evaluations only."""
+
+    def __init__(self, every=1):
+        if every < 1:
+            raise ArgumentError("Field `every` must be positive.")
+        self.every = int(every)
+
+
+class FullResidualRecipe:
+    def __init__(self, err, b):
+        self.every = err.every
+        self.residual = None  # the norm is reduced on the device
+
+
+def compute_error(err, solver, A, b):
+    """compute_error (full_residual.jl:32-41) in FP64."""
+    Av = as_view(A)
+    out = ctypes.c_double(0.0)
+    _lib.call("rla_residual_norm_f32", _lib.ctx(Av.t.device), Av.rows, Av.cols, _lib.ptr(Av.t),
+              Av.rs, Av.cs, _lib.ptr(solver.solution_vec), _lib.ptr(b), ctypes.byref(out))
+    return out.value
+
+
+class LQSolver:
+    """LQSolver() — lq.jl:1-12: the minimum-norm solution of the block system."""
+
+
+class LQSolverRecipe:
+    def __init__(self, A=None):
+        self.A = A
+
+
+# ---------------------------------------------------------------------------
+# Kaczmarz (src/Solvers/kaczmarz.jl)
+
+
+class Solver:
+    pass
+
+
+class Kaczmarz(Solver):
+    """Kaczmarz(compressor=SparseSign(cardinality=Left()), log=BasicLogger(),
+    error=FullResidual(), sub_solver=LQSolver(), alpha=1.0) — kaczmarz.jl:68-101."""
+
+    def __init__(self, compressor=None, log=None, error=None, sub_solver=None, alpha=1.0):
+        self.compressor = SparseSign(cardinality=Left()) if compressor is None else compressor
+        if self.compressor.cardinality != Left():
+            import warnings
+            warnings.warn("Compressor has cardinality `Right` but kaczmarzcompresses  from the"
+                          "  `Left`.")
+        self.log = BasicLogger() if log is None else log
+        self.error = FullResidual() if error is None else error
+        self.sub_solver = LQSolver() if sub_solver is None else sub_solver
+        self.alpha = float(alpha)
+
+
+class KaczmarzRecipe:
+    """kaczmarz.jl:103-160.  Holds the compressor/logger/error/sub-solver
+    recipes, alpha, the compressed block buffers and, for the device loop,
+    the row-major copy of A."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def _check_float32(*ts):
+    for t in ts:
+        if t.dtype != torch.float32:
+            raise ArgumentError("the GPU Kaczmarz solver runs on Float32 data "
+                                f"(got {t.dtype})")
+
+
+def complete_solver(ingredients, x, A, b):
+    """complete_solver(::Kaczmarz, x, A, b) — kaczmarz.jl:162-246."""
+    if not isinstance(ingredients, Kaczmarz):
+        raise ArgumentError(f"No method `complete_solver` defined for {type(ingredients)}.")
+    compressor = complete_compressor(ingredients.compressor, x, A, b)
+    logger = complete_logger(ingredients.log)
+    if not isinstance(ingredients.error, FullResidual):
+        raise ArgumentError(
+            f"ErrorRecipe {type(ingredients.error).__name__} does not contain the field "
+            f"'residual' and is not valid for a kaczmarz solver.")
+    error = FullResidualRecipe(ingredients.error, b)
+    if not isinstance(ingredients.sub_solver, LQSolver):
+        raise ArgumentError("the GPU Kaczmarz solver projects with LQSolver() only")
+    _check_float32(as_view(A).t, b, x)
+    sample_size = compressor.n_rows
+    n = _size2(A)[1]
+    dev = as_view(A).t.device
+    # compressed block buffers (kaczmarz.jl:207-224), column-major
+    compressed_mat = torch.zeros(n, sample_size, dtype=torch.float32, device=dev).T
+    compressed_vec = torch.zeros(sample_size, dtype=torch.float32, device=dev)
+    return KaczmarzRecipe(compressor=compressor, log=logger, error=error,
+                          sub_solver=LQSolverRecipe(compressed_mat), alpha=ingredients.alpha,
+                          compressed_mat=compressed_mat, compressed_vec=compressed_vec,
+                          solution_vec=x, A_rm=None, _rm_key=None)
+
+
+def _row_major(recipe, A):
+    """A with each row contiguous (the block gathers read rows): A itself, or
+    a copy kept while A is unchanged."""
+    v = as_view(A)
+    if v.cs == 1 and v.rs >= v.cols:
+        return v.t, v.rs
+    key = (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
+    if recipe._rm_key != key:
+        recipe.A_rm = torch.empty(v.rows, v.cols, dtype=v.dtype, device=v.t.device)
+        recipe.A_rm.copy_(A)
+        recipe._rm_key = key
+    return recipe.A_rm, v.cols
+
+
+def _fast_path(recipe):
+    S = recipe.compressor
+    return (isinstance(S, SamplingRecipe) and S.cardinality == Left()
+            and S.compression_dim <= 64 and isinstance(recipe.log, BasicLoggerRecipe)
+            and recipe.log.stopping_criterion is threshold_stop
+            and np.isscalar(recipe.log.threshold_info))
+
+
+def rsolve(solver, x, A, b):
+    """rsolve!(solver, x, A, b) -> (x, recipe) for a Kaczmarz solver
+    (Solvers.jl:145-149), or rsolve!(recipe, x, A, b) -> None
+    (kaczmarz.jl:313-368): x is updated in place."""
+    if isinstance(solver, Solver):
+        recipe = complete_solver(solver, x, A, b)
+        _rsolve(recipe, x, A, b)
+        return x, recipe
+    _rsolve(solver, x, A, b)
+    return None
+
+
+rsolve_ = rsolve
+
+
+def _rsolve(recipe, x, A, b):
+    log = recipe.log
+    log.reset()
+    a_rows, a_cols = _size2(A)
+    if x.shape[0] != a_cols:
+        raise DimensionMismatch(
+            f"Dimension of `x`, {x.shape[0]} is different from number of columns in `A`,"
+            f"{a_cols}.")
+    if b.shape[0] != a_rows:
+        raise DimensionMismatch(
+            f"Dimension of `b`, {b.shape[0]} is different from number of rows in `A`,"
+            f"{a_rows}.")
+    _check_float32(as_view(A).t, b, x)
+    recipe.solution_vec = x
+    if log.max_it == 0:
+        return
+    if _fast_path(recipe):
+        return _rsolve_device(recipe, x, A, b)
+    return _rsolve_loop(recipe, x, A, b)
+
+
+def _rsolve_device(recipe, x, A, b):
+    S, log = recipe.compressor, recipe.log
+    Av = as_view(A)
+    if Av.rs != 1:
+        raise ArgumentError("the device Kaczmarz loop reads A column-major (unit row stride)")
+    A_rm, ld_rm = _row_major(recipe, A)
+    d = S.distribution_recipe
+    update_distribution(d, x, A, b)
+    if not d.replace and d.positive() < S.compression_dim:
+        raise ArgumentError(f"cannot draw {S.compression_dim} distinct indices from "
+                            f"{d.positive()} with positive weight")
+    m, n = Av.rows, Av.cols
+    s, every = S.compression_dim, recipe.error.every
+    nb = _dense.query_bytes("rla_kaczmarz_workspace_bytes", m, n, s, every)
+    work = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=Av.t.device)
+    cap = log.max_it // every + 2
+    hist = np.zeros(cap)
+    n_hist, iters, conv = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int(0)
+    sd, ep = _draw(log.max_it)  # one epoch per iteration's row set
+    _lib.call("rla_kaczmarz_solve_f32", _lib.ctx(Av.t.device), m, n, _lib.ptr(Av.t), Av.cs,
+              _lib.ptr(A_rm), ld_rm, _lib.ptr(b), _lib.ptr(x), s, recipe.alpha,
+              _lib.ptr(d.weights) if d.weights is not None else None, 1 if d.replace else 0,
+              sd, ep, log.max_it, every, float(log.threshold_info),
+              hist.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n_hist),
+              ctypes.byref(iters), ctypes.byref(conv), _lib.ptr(work), work.numel() * 8)
+    # replay the evaluated errors through the logger (same bookkeeping as
+    # update_logger! at each evaluated iteration)
+    k = int(n_hist.value)
+    its = [1 + j * every for j in range(k)]
+    if k and its[-1] != int(iters.value):
+        its[-1] = int(iters.value)
+    for j in range(k):
+        log.update(hist[j], its[j])
+    log.iteration = int(iters.value)
+    log.converged = bool(conv.value)
+
+
+def _rsolve_loop(recipe, x, A, b):
+    S, log = recipe.compressor, recipe.log
+    every = recipe.error.every
+    n = _size2(A)[1]
+    nb = _dense.query_bytes("rla_kaczmarz_workspace_bytes", 64, n, 64, 1)
+    work = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=x.device)
+    for i in range(1, log.max_it + 1):
+        if (i - 1) % every == 0 or i == log.max_it:
+            err = compute_error(recipe.error, recipe, A, b)
+            log.update(err, i)
+            if log.converged:
+                return
+        update_compressor(S, x, A, b)
+        rows_s = S.size(1)
+        if rows_s > 64:
+            raise ArgumentError("the GPU Kaczmarz block update takes at most 64 rows")
+        Mt = recipe.compressed_mat[:rows_s]
+        bt = recipe.compressed_vec[:rows_s]
+        mul(Mt, S, A)
+        mul(bt, S, b)
+        Mv = as_view(Mt)
+        _lib.call("rla_kaczmarz_block_update_f32", _lib.ctx(x.device), rows_s, n,
+                  _lib.ptr(Mv.t), Mv.rs, Mv.cs, _lib.ptr(bt), bt.stride(0), _lib.ptr(x),
+                  recipe.alpha, _lib.ptr(work), work.numel() * 8)

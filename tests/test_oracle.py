@@ -256,3 +256,49 @@ def test_randsvd_oracle_exact_recovery():
     U, S, V = R.randsvd(A, Q)
     assert np.linalg.norm(A - (U * S) @ V.T) < 1e-10 * np.linalg.norm(A)
     assert np.allclose(S[:5], np.linalg.svd(A, compute_uv=False)[:5], rtol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# Kaczmarz oracle (kaczmarz.jl:262-311, lq.jl:35-46)
+
+
+@pytest.mark.parametrize("s,n", [(1, 7), (5, 12), (16, 40), (40, 40)])
+def test_kaczmarz_block_update_is_pseudoinverse_projection(s, n):
+    rng = np.random.default_rng(s + n)
+    At, x, bt = rng.standard_normal((s, n)), rng.standard_normal(n), rng.standard_normal(s)
+    got = R.kaczmarz_update_block(x, At, bt, 0.9)
+    want = x + 0.9 * np.linalg.pinv(At) @ (bt - At @ x)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    # alpha = 1 lands on the block's solution set, orthogonally (minimum norm)
+    x1 = R.kaczmarz_update_block(x, At, bt, 1.0)
+    assert np.allclose(At @ x1, bt, atol=1e-10)
+    u = x1 - x
+    assert np.allclose(u, At.T @ np.linalg.lstsq(At.T, u, rcond=None)[0], atol=1e-10)
+
+
+def test_kaczmarz_single_row_matches_block_and_zero_block_is_skipped():
+    rng = np.random.default_rng(3)
+    a, x, c = rng.standard_normal(9), rng.standard_normal(9), rng.standard_normal(1)
+    assert np.allclose(R.kaczmarz_update(x, a, c, 1.4),
+                       R.kaczmarz_update_block(x, a[None, :], c, 1.4), rtol=1e-13)
+    # x - alpha*(a.x - c)/||a||^2 a, written out
+    assert np.allclose(R.kaczmarz_update(x, a, c, 1.4),
+                       x - 1.4 * (a @ x - c[0]) / (a @ a) * a, rtol=1e-14)
+    assert np.array_equal(R.kaczmarz_update_block(x, np.zeros((3, 9)), np.ones(3), 1.0), x)
+
+
+def test_successive_inclusion_closed_forms():
+    w = np.array([1.0, 2.0, 3.0, 4.0, 0.0])
+    W = w.sum()
+    assert np.allclose(R.successive_inclusion(w, 1), w / W)
+    # k = 2: P(i) = w_i/W + sum_{j != i} (w_j/W) (w_i/(W - w_j))
+    p2 = np.array([w[i] / W + sum(w[j] / W * w[i] / (W - w[j]) for j in range(5) if j != i)
+                   for i in range(5)])
+    assert np.allclose(R.successive_inclusion(w, 2), p2)
+    assert np.isclose(R.successive_inclusion(w, 3).sum(), 3.0)
+    assert R.successive_inclusion(w, 3)[4] == 0.0
+
+
+def test_full_residual_closed_form():
+    A = np.array([[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]])
+    assert R.full_residual(A, [1.0, 1.0], [3.0, 7.0, 2.0]) == 1.0

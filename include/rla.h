@@ -319,6 +319,46 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
                            double* hist, int64_t hist_cap, int64_t* n_hist, int64_t* iterations,
                            int* converged, void* work, size_t work_bytes);
 
+/* ---- batched sketch-and-solve least squares ------------------------------
+ * One problem: x = argmin || S A x - S b || with S the s x m CountSketch
+ * given as CSR (s_rowptr[s+1], s_colidx (the rows of A in bucket r, ascending),
+ * s_val (+-1)), i.e. CountSketch mul! (count_sketch.jl:166-213) followed by
+ * the QRSolver solve x = qr!(SA) \ Sb (src/Solvers/SubSolvers/qr.jl:36-43).
+ * A: m x n Float32 column-major (lda); b: m; x: n (written).  n <= 271,
+ * n <= s <= 1024. */
+typedef struct rla_ls_problem {
+  int64_t m, n, lda;
+  const float* A;
+  const float* b;
+  float* x;
+  const int32_t* s_rowptr;
+  const int32_t* s_colidx;
+  const float* s_val;
+  int64_t s;
+} rla_ls_problem;
+
+typedef struct rla_ls_plan rla_ls_plan;
+
+/* Plans a batch: validates, allocates the per-problem state (SA, Gram,
+ * factor) in one device allocation, picks the sketch tile shape (autotune: 0
+ * default, 1 time the three candidates on this batch, 2..4 force candidate
+ * 0..2), and captures the batch (persistent
+ * sketch kernel, batched FP64 Gram, batched Cholesky, back-substitution) in a
+ * hipGraph.  The problem data must stay valid while the plan is used. */
+int rla_ls_plan_create(rla_ctx* ctx, int64_t nprob, const rla_ls_problem* probs, int autotune,
+                       rla_ls_plan** plan);
+/* Replays the graph on the context's stream (asynchronous; no allocation). */
+int rla_ls_plan_execute(rla_ls_plan* plan);
+/* Per-problem status of the last execution: 0, or the 1-based column whose
+ * pivot broke down (x not written).  Blocks. */
+int rla_ls_plan_info(rla_ls_plan* plan, int32_t* info_host);
+/* Chosen tile shape and the autotuner's timings (ms, -1 if not timed). */
+int rla_ls_plan_config(rla_ls_plan* plan, int* cols_per_tile, int* rows_per_chunk,
+                       float* tuned_ms3);
+/* Copies problem p's sketched [SA Sb] (s x (n+1), column-major) to out. */
+int rla_ls_plan_sketch(rla_ls_plan* plan, int64_t p, float* out);
+int rla_ls_plan_destroy(rla_ls_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

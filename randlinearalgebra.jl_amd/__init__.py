@@ -25,4 +25,6 @@ from .solvers import (BasicLogger, BasicLoggerRecipe, DistributionRecipe,  # noq
                       LQSolver, LQSolverRecipe, Sampling, SamplingRecipe, Uniform,
                       complete_distribution, complete_logger, complete_solver, compute_error,
                       rsolve, rsolve_, threshold_stop, update_distribution)
+from .batched import (QRSolver, QRSolverRecipe, SketchSolveBatch,  # noqa: F401
+                      complete_sub_solver, ldiv, update_sub_solver)
 from . import _dense, _lib  # noqa: F401

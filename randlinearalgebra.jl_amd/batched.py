@@ -1,0 +1,201 @@
+"""Batched sketch-and-solve least squares and the QR sub-solver.
+
+`SketchSolveBatch` runs many problems x_p = argmin ||S_p A_p x - S_p b_p||
+(S_p a CountSketch, count_sketch.jl) as one replayable hipGraph
+(rla_ls_plan_*): the reference's per-problem
+`mul!(SA, S, A); mul!(Sb, S, b); ldiv!(x, QRSolverRecipe, Sb)`
+(src/Solvers/SubSolvers/qr.jl:36-43) for a whole batch at once.
+
+`QRSolver` / `complete_sub_solver` / `ldiv` mirror qr.jl for one (already
+sketched) matrix: the same plan with the identity as the sketch.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import ArgumentError
+from .compressors import CountSketch, Left, as_view, complete_compressor
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int64), ("n", ctypes.c_int64), ("lda", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("s_rowptr", ctypes.c_void_p), ("s_colidx", ctypes.c_void_p),
+                ("s_val", ctypes.c_void_p), ("s", ctypes.c_int64)]
+
+
+def _addr(t):
+    return t.data_ptr()
+
+
+def _plan_fns():
+    lib = _lib.load()
+    if not getattr(lib, "_ls_sigs", False):
+        vp = ctypes.c_void_p
+        lib.rla_ls_plan_create.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_Problem),
+                                           ctypes.c_int, ctypes.POINTER(vp)]
+        lib.rla_ls_plan_execute.argtypes = [vp]
+        lib.rla_ls_plan_info.argtypes = [vp, vp]
+        lib.rla_ls_plan_config.argtypes = [vp, vp, vp, vp]
+        lib.rla_ls_plan_sketch.argtypes = [vp, ctypes.c_int64, vp]
+        lib.rla_ls_plan_destroy.argtypes = [vp]
+        for f in ("create", "execute", "info", "config", "sketch", "destroy"):
+            getattr(lib, f"rla_ls_plan_{f}").restype = ctypes.c_int
+        lib._ls_sigs = True
+    return lib
+
+
+class _Plan:
+    """Owns one rla_ls_plan and keeps every tensor it points at alive."""
+
+    def __init__(self, device, probs, keep, autotune):
+        self.lib = _plan_fns()
+        self.device = device
+        self.keep = keep
+        arr = (_Problem * len(probs))(*probs)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.rla_ls_plan_create(_lib.ctx(device), len(probs), arr, int(autotune),
+                                               ctypes.byref(h)), "rla_ls_plan_create")
+        self.h = h
+        self.n = len(probs)
+
+    def execute(self):
+        _lib.check(self.lib.rla_ls_plan_execute(self.h), "rla_ls_plan_execute")
+
+    def info(self):
+        out = (ctypes.c_int32 * self.n)()
+        _lib.check(self.lib.rla_ls_plan_info(self.h, out), "rla_ls_plan_info")
+        return list(out)
+
+    def config(self):
+        cb, rows = ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_float * 3)()
+        _lib.check(self.lib.rla_ls_plan_config(self.h, ctypes.byref(cb), ctypes.byref(rows), ms),
+                   "rla_ls_plan_config")
+        return {"cols_per_tile": cb.value, "rows_per_chunk": rows.value,
+                "tuned_ms": [float(v) for v in ms]}
+
+    def sketch(self, p, s, n):
+        out = torch.empty(n + 1, s, dtype=torch.float32, device=self.device).T
+        _lib.check(self.lib.rla_ls_plan_sketch(self.h, p, _lib.ptr(out)), "rla_ls_plan_sketch")
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.rla_ls_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _check_problem(A, b, x):
+    Av = as_view(A)
+    if Av.dtype != torch.float32 or b.dtype != torch.float32 or x.dtype != torch.float32:
+        raise ArgumentError("batched least squares runs on Float32 A, b and x")
+    if Av.rs != 1:
+        raise ArgumentError("A must be column-major (unit row stride)")
+    if b.dim() != 1 or x.dim() != 1 or b.stride(0) != 1 or x.stride(0) != 1:
+        raise ArgumentError("b and x must be contiguous vectors")
+    if b.shape[0] != Av.rows or x.shape[0] != Av.cols:
+        raise ArgumentError("b must have size(A, 1) entries and x size(A, 2)")
+    return Av
+
+
+class SketchSolveBatch:
+    """x_p = argmin ||S_p A_p x - S_p b_p|| for every p, S_p =
+    CountSketch(compression_dim = s_p) drawn by complete_compressor (default
+    s_p = 4 n_p).  A_p: m_p x n_p Float32 column-major; b_p, x_p vectors.
+    `run()` replays the plan (asynchronous); `info()` gives per-problem
+    breakdown flags."""
+
+    def __init__(self, As, bs, xs, compression_dims=None, autotune=1):
+        if not (len(As) == len(bs) == len(xs)) or not As:
+            raise ArgumentError("As, bs and xs must be non-empty and of equal length")
+        probs, keep = [], [As, bs, xs]
+        self.recipes = []
+        self.shapes = []
+        for p, (A, b, x) in enumerate(zip(As, bs, xs)):
+            Av = _check_problem(A, b, x)
+            s = 4 * Av.cols if compression_dims is None else int(compression_dims[p])
+            S = complete_compressor(CountSketch(cardinality=Left(), compression_dim=s,
+                                                type=torch.float32), A)
+            self.recipes.append(S)
+            op = S.op  # s x m; CSR rows = buckets
+            lda = Av.cs if Av.cols > 1 else max(Av.cs, Av.rows)  # a single column has no stride
+            probs.append(_Problem(Av.rows, Av.cols, lda, _addr(Av.t), _addr(b), _addr(x),
+                                  _addr(op.rowptr), _addr(op.colidx), _addr(op.val_t), s))
+            self.shapes.append((Av.rows, Av.cols, s))
+        keep.append(self.recipes)
+        self.plan = _Plan(As[0].device, probs, keep, autotune)
+
+    def run(self):
+        self.plan.execute()
+
+    def info(self):
+        return self.plan.info()
+
+    def config(self):
+        return self.plan.config()
+
+    def sketch(self, p):
+        m, n, s = self.shapes[p]
+        return self.plan.sketch(p, s, n)
+
+    def close(self):
+        self.plan.close()
+
+
+# ---------------------------------------------------------------------------
+# QR sub-solver (src/Solvers/SubSolvers/qr.jl)
+
+
+class QRSolver:
+    """QRSolver() — qr.jl:1-13."""
+
+
+class QRSolverRecipe:
+    """qr.jl:15-34: holds the matrix whose least-squares problems ldiv solves."""
+
+    def __init__(self, A):
+        self.A = A
+        self._plan = None
+        self._key = None
+
+
+def complete_sub_solver(solver, A, b=None):
+    if not isinstance(solver, QRSolver):
+        raise ArgumentError(f"No method `complete_sub_solver` defined for {type(solver)}.")
+    return QRSolverRecipe(A)
+
+
+def update_sub_solver(recipe, A):
+    recipe.A = A
+
+
+def ldiv(x, recipe, b):
+    """ldiv!(x, ::QRSolverRecipe, b) (qr.jl:36-43): x = argmin ||A x - b||."""
+    A = recipe.A
+    Av = _check_problem(A, b, x)
+    if Av.rows < Av.cols:
+        raise ArgumentError("QRSolver needs size(A, 1) >= size(A, 2)")
+    key = (Av.t.data_ptr(), Av.rows, Av.cols, Av.cs, b.data_ptr(), x.data_ptr())
+    if recipe._key != key:
+        dev = Av.t.device
+        s = Av.rows
+        rowptr = torch.arange(0, s + 1, dtype=torch.int32, device=dev)
+        colidx = torch.arange(0, s, dtype=torch.int32, device=dev)
+        ones = torch.ones(s, dtype=torch.float32, device=dev)
+        lda = Av.cs if Av.cols > 1 else max(Av.cs, Av.rows)
+        prob = _Problem(Av.rows, Av.cols, lda, _addr(Av.t), _addr(b), _addr(x),
+                        _addr(rowptr), _addr(colidx), _addr(ones), s)
+        recipe._plan = _Plan(dev, [prob], [A, b, x, rowptr, colidx, ones], 0)
+        recipe._key = key
+    recipe._plan.execute()
+    info = recipe._plan.info()[0]
+    if info:
+        raise _lib.DomainError(f"least-squares matrix is rank deficient (column {info})")

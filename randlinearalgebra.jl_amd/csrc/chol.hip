@@ -110,9 +110,11 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
   }
   // zero R below the block diagonal (the diagonal blocks are written whole);
   // G's lower blocks are never read, so R may alias G
-  for (int e = t; e < k * k; e += BW * 64) {
-    const int i = e % k, j = e / k;
-    if ((i >> 4) > (j >> 4)) R[i + j * ldr] = 0.0;
+  if (R) {
+    for (int e = t; e < k * k; e += BW * 64) {
+      const int i = e % k, j = e / k;
+      if ((i >> 4) > (j >> 4)) R[i + j * ldr] = 0.0;
+    }
   }
   __syncthreads();
   double* pl = &P[lr][lc];  // this lane's element of a 4 x 16 slab (rows NBLK*16+1 apart)
@@ -143,7 +145,7 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
           const int i = lr + 4 * r;
           Dinv[s * 256 + i * 16 + c] = Dv[i][c];
           if (i < nb && c < nb) {
-            R[(js + i) + int64_t(js + c) * ldr] = Ds[i][c];
+            if (R) R[(js + i) + int64_t(js + c) * ldr] = Ds[i][c];
             Rt[int64_t(js + i) * ldt + js + c] = Ds[i][c];
           }
         }
@@ -159,7 +161,7 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
           const int i = lr + 4 * r;
           pl[4 * r * SR + bj * 16] = o[r];
           if (j < k) {
-            R[(js + i) + int64_t(j) * ldr] = o[r];
+            if (R) R[(js + i) + int64_t(j) * ldr] = o[r];
             Rt[int64_t(js + i) * ldt + j] = o[r];
           }
         }
@@ -186,8 +188,8 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
 }
 
 template <int NBLK>
-__device__ __forceinline__ void chol_factor_role(int k, double (*Ds)[17], double (*Dv)[17],
-                                                 double* rowj, const double* fl,
+__device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds)[17],
+                                                 double (*Dv)[17], double* rowj, const double* fl,
                                                  volatile int* fail, int lane) {
   __syncthreads();  // G read into registers, R zeroed
   for (int s = 0; s < NBLK; ++s) {
@@ -214,8 +216,9 @@ __device__ __forceinline__ void chol_factor_role(int k, double (*Ds)[17], double
     for (int j = 0; j < 16; ++j) {
       double dj = lane_bcast(col[j], j);
       const double floor_j = j < nb ? fl[js + j] : 0.0;
-      if (!bad && (!(dj > floor_j) || !(dj > 0.0) || !isfinite(dj))) bad = js + j + 1;
-      if (bad) dj = 1.0;
+      const bool ok = (dj > floor_j) && (dj > 0.0) && isfinite(dj);
+      if (!bad && !ok && js + j < check_k) bad = js + j + 1;
+      if (bad || !ok) dj = 1.0;  // unchecked columns past check_k continue
       const double d = sqrt(dj), rd = 1.0 / d;
       col[j] = c < j ? 0.0 : (c == j ? d : col[j] * rd);
       if (lane < 16) rowj[c] = col[j];
@@ -254,10 +257,13 @@ __device__ __forceinline__ void chol_factor_role(int k, double (*Ds)[17], double
   }
 }
 
+// One workgroup factors one matrix; pivots of columns >= check_k are not
+// tested (an augmented column whose pivot is a residual may vanish).
 template <int NBLK>
-__global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
-    int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
-    double* __restrict__ Dinv, int* __restrict__ info) {
+__device__ __forceinline__ void chol_blocks_body(int k, int check_k, const double* G, int ldg,
+                                                 double* R, int ldr, double* __restrict__ Rt,
+                                                 int ldt, double* __restrict__ Dinv,
+                                                 int* __restrict__ info) {
   constexpr int CP = NBLK * 16;
   __shared__ double P[16][CP + 1];  // panel row R_s* of the current step
   __shared__ double Ds[16][17];     // diagonal block G_ss
@@ -280,9 +286,25 @@ __global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
     RLA_ROLE(0) RLA_ROLE(1) RLA_ROLE(2) RLA_ROLE(3) RLA_ROLE(4) RLA_ROLE(5) RLA_ROLE(6)
 #undef RLA_ROLE
     default:
-      chol_factor_role<NBLK>(k, Ds, Dv, rowj, fl, &fail, t & 63);
+      chol_factor_role<NBLK>(k, check_k, Ds, Dv, rowj, fl, &fail, t & 63);
   }
   if (t == 0) *info = fail;
+}
+
+template <int NBLK>
+__global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
+    int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
+    double* __restrict__ Dinv, int* __restrict__ info) {
+  chol_blocks_body<NBLK>(k, k, G, ldg, R, ldr, Rt, ldt, Dinv, info);
+}
+
+// Batched form: workgroup p factors matrix p of a descriptor array.
+template <int NBLK>
+__global__ __launch_bounds__(kCholThreads) void chol_blocks_batched_kernel(
+    const CholBatchItem* __restrict__ items) {
+  const CholBatchItem& it = items[blockIdx.x];
+  chol_blocks_body<NBLK>(it.k, it.check_k, it.G, it.k, nullptr, 0, it.Rt, it.k, it.Dinv,
+                         it.info);
 }
 
 template <int NBLK>
@@ -399,6 +421,13 @@ int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R
   }
 #undef RLA_CHOL
   RLA_LAUNCH_CHECK("chol_blocks");
+  return RLA_OK;
+}
+
+int chol_blocks_batched(hipStream_t stream, int64_t nitems, const CholBatchItem* items) {
+  if (nitems <= 0) return RLA_OK;
+  chol_blocks_batched_kernel<17><<<dim3(unsigned(nitems)), kCholThreads, 0, stream>>>(items);
+  RLA_LAUNCH_CHECK("chol_blocks_batched");
   return RLA_OK;
 }
 

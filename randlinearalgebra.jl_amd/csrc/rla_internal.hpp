@@ -23,6 +23,19 @@ struct rla_ctx {
 
 namespace rla {
 
+// one matrix of a batched factorisation (chol.hip): G (k x k, ld k) -> Rt
+// (row-major, ld k) and the inverses of its 16 x 16 diagonal blocks; pivots of
+// columns >= check_k are not tested
+struct CholBatchItem {
+  int k, check_k;
+  const double* G;
+  double* Rt;
+  double* Dinv;
+  int* info;
+};
+// items: device array; every k <= 272
+int chol_blocks_batched(hipStream_t stream, int64_t nitems, const CholBatchItem* items);
+
 // Thread-local last-error text behind rla_last_error().
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);

@@ -27,4 +27,5 @@ from .solvers import (BasicLogger, BasicLoggerRecipe, DistributionRecipe,  # noq
                       rsolve, rsolve_, threshold_stop, update_distribution)
 from .batched import (QRSolver, QRSolverRecipe, SketchSolveBatch,  # noqa: F401
                       complete_sub_solver, ldiv, update_sub_solver)
+from .serving import KaczmarzRequest, RangeFinderRequest, Server, poisson_arrivals  # noqa: F401
 from . import _dense, _lib  # noqa: F401

@@ -434,7 +434,9 @@ def _rsolve_device(recipe, x, A, b):
     cap = log.max_it // every + 2
     hist = np.zeros(cap)
     n_hist, iters, conv = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int(0)
-    sd, ep = _draw(log.max_it)  # one epoch per iteration's row set
+    # one epoch per iteration's row set; a recipe may carry a fixed (seed,
+    # epoch) so its solve does not depend on other draws
+    sd, ep = recipe.rng if getattr(recipe, "rng", None) else _draw(log.max_it)
     _lib.call("rla_kaczmarz_solve_f32", _lib.ctx(Av.t.device), m, n, _lib.ptr(Av.t), Av.cs,
               _lib.ptr(A_rm), ld_rm, _lib.ptr(b), _lib.ptr(x), s, recipe.alpha,
               _lib.ptr(d.weights) if d.weights is not None else None, 1 if d.replace else 0,

@@ -1,6 +1,6 @@
-"""Benchmark: SRHT sketch apply on one MI355X.
+"""Benchmark of the sketching hot path on one MI355X.
 
-Workload (one step = one pass of the hot path over one batch): C = S*A with
+Headline (one step = one pass of the hot path over one batch): C = S*A with
 S an SRHT (Left) recipe, A 65536 x 512 Float32 N(0,1) (seed 1234), 2048
 sampled rows, scale sqrt(2048/65536) as the reference stores it.  Inputs are
 resident in HBM; steps rotate over 4 copies of A and C (512 MiB + 16 MiB, more
@@ -9,8 +9,12 @@ than the 256 MB Infinity Cache) so every step streams A from HBM.
 Prints one JSON line: value = algorithmic GB/s of the apply
 ((65536*512 + 2048*512) * 4 bytes per step / time), with the roofline of the
 dominant kernel (timed with HIP events on the stream it runs on) and the CPU
-baseline (the oracle's C restatement, all host threads) beside it.  A second
-line-internal block reports the SparseSign 8192x256 Float64 apply.
+baseline (the oracle's C restatement, all host threads) beside it.  Nested
+blocks report the other workloads: the SparseSign 8192x256 Float64 apply,
+the mixed-precision vs plain RangeFinder 32768x8192 (solves/s and FP64 error),
+512 batched sketch-and-solve least-squares problems, the 100000x1000 block
+Kaczmarz solve, and a served Poisson stream of RangeFinder and Kaczmarz
+requests (p50/p99 latency and solves/s, served vs a plain single queue).
 """
 import argparse
 import json
@@ -135,6 +139,163 @@ def rangefinder_c3(rla, torch, reps=3):
     return out
 
 
+def ls_batched_c4(rla, torch, reps=5):
+    """Batched sketch-and-solve least squares: 512 problems, m log-uniform in
+    [2000, 50000], n uniform in [32, 256], N(0,1) columns scaled by 10^U(-3,0),
+    b = A x* + 1e-6 noise (seed 1234); CountSketch to 4n rows, then the FP64
+    normal-equation solve of the sketched problem, all one hipGraph replay.
+    Reports solves/s over timed replays, the sketch kernel's algorithmic GB/s
+    and ||x - x*|| / ||x*|| per problem."""
+    import numpy as np
+    dev = torch.device("cuda", 0)
+    F32 = torch.float32
+    rng = np.random.default_rng(1234)
+    g = torch.Generator(device=dev).manual_seed(1234)
+    ms = np.floor(np.exp(rng.uniform(np.log(2000), np.log(50000), 512))).astype(int)
+    ns = rng.integers(32, 257, 512)
+    As, bs, xs, xstars = [], [], [], []
+    for m, n in zip(ms, ns):
+        sc = torch.as_tensor(10.0 ** rng.uniform(-3, 0, n), dtype=F32, device=dev)
+        A = (torch.randn(int(n), int(m), generator=g, device=dev, dtype=F32).T) * sc
+        xsr = torch.randn(int(n), generator=g, device=dev, dtype=F32)
+        b = A @ xsr + 1e-6 * torch.randn(int(m), generator=g, device=dev, dtype=F32)
+        As.append(A)
+        bs.append(b)
+        xs.append(torch.zeros(int(n), dtype=F32, device=dev))
+        xstars.append(xsr)
+    rla.seed(1234)
+    batch = rla.SketchSolveBatch(As, bs, xs, autotune=1)
+    batch.run()  # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        batch.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / reps
+    info = batch.info()
+    err = np.array([float(torch.linalg.vector_norm((x - xsr).double()) /
+                          torch.linalg.vector_norm(xsr.double())) for x, xsr in zip(xs, xstars)])
+    cfg = batch.config()
+    sketch_ms = min(t for t in cfg["tuned_ms"] if t > 0)
+    # sketch kernel: read A and b, the row's bucket and sign, write [SA Sb]
+    sbytes = int(sum(m * (n + 1) * 4 + m * 8 + 4 * n * (n + 1) * 4 for m, n in zip(ms, ns)))
+    batch.close()
+    return {"workload": "512 CountSketch+solve LS problems, m in [2000,50000] log-uniform, "
+                        "n in [32,256], s = 4n, f32 data, FP64 solve",
+            "ms_per_batch": ms_batch, "solves_per_s": 512 / ms_batch * 1e3,
+            "sketch_kernel_ms": sketch_ms, "sketch_GB_s": sbytes / sketch_ms / 1e6,
+            "sketch_frac_hbm_peak": sbytes / sketch_ms / 1e6 / HBM_PEAK_GBS,
+            "tile_config": {"cols_per_tile": cfg["cols_per_tile"],
+                            "rows_per_chunk": cfg["rows_per_chunk"]},
+            "breakdowns": int(sum(1 for i in info if i)),
+            "rel_error_median": float(np.median(err)), "rel_error_max": float(err.max())}
+
+
+def kaczmarz_c5(rla, torch, reps=3):
+    """Block Kaczmarz on a consistent 100000 x 1000 N(0,1) system, b = A x*
+    (seed 1234): 64-row blocks drawn by squared row norm, FP32 updates, FP64
+    residual every 100 iterations, stop at ||r|| <= 1e-6 ||b||."""
+    dev = torch.device("cuda", 0)
+    F32 = torch.float32
+    m, n, s = 100000, 1000, 64
+    g = torch.Generator(device=dev).manual_seed(1234)
+    A = torch.randn(n, m, generator=g, device=dev, dtype=F32).T
+    xs = torch.randn(n, generator=g, device=dev, dtype=F32)
+    b = A @ xs
+    bn = float(torch.linalg.vector_norm(b.double()))
+    solver = rla.Kaczmarz(compressor=rla.Sampling(compression_dim=s, distribution=rla.L2Norm()),
+                          log=rla.BasicLogger(max_it=20000, threshold=1e-6 * bn),
+                          error=rla.FullResidual(every=100))
+    x = torch.zeros(n, dtype=F32, device=dev)
+    rla.seed(1234)
+    x, rec = rla.rsolve(solver, x, A, b)  # warm-up: row-major copy and row norms
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(reps):
+        x.zero_()
+        rla.rsolve(rec, x, A, b)
+        iters += rec.log.iteration
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    it = iters / reps
+    rel = float(torch.linalg.vector_norm((x - xs).double()) / torch.linalg.vector_norm(xs.double()))
+    # per iteration: gather 64 rows (row-major, 64*1000*4 B), read and write x
+    byts = s * n * 4 + 2 * n * 4
+    return {"workload": "block Kaczmarz 100000x1000 f32, 64-row L2Norm blocks, FP64 residual "
+                        "every 100 iterations, to ||r||/||b|| <= 1e-6",
+            "seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "iterations": it,
+            "us_per_iteration": dt / it * 1e6, "gather_GB_s": byts * it / dt / 1e9,
+            "converged": bool(rec.log.converged), "rel_residual": rec.log.error / bn,
+            "rel_error_x": rel}
+
+
+def lowrank(torch, m, n, r, decay, g, dev):
+    U = torch.randn(r, m, generator=g, device=dev, dtype=torch.float32).T
+    V = torch.randn(n, r, generator=g, device=dev, dtype=torch.float32)
+    s = torch.exp(-torch.arange(r, device=dev, dtype=torch.float32) / decay)
+    return ((U * s) @ V.T / n ** 0.5).T.contiguous().T
+
+
+def serving_c5(rla, torch, qps=40.0, n_req=64):
+    """Served mixed stream: Poisson arrivals at `qps` mixing RangeFinder
+    requests (k = 266, q = 2, on resident low-rank matrices 4096x1024,
+    8192x2048, 16384x4096, 32768x8192 with sigma_i = exp(-i/64)) and block
+    Kaczmarz solves of a resident 100000x1000 system (64-row L2Norm blocks,
+    FP64 residual every 100, rtol 1e-6).  Run twice on the same arrival
+    trace: mixed precision with 4 slots (continuous batching), and the plain
+    FP32 path with one slot (a queue)."""
+    import numpy as np
+    dev = torch.device("cuda", 0)
+    F32 = torch.float32
+    g = torch.Generator(device=dev).manual_seed(1234)
+    shapes = [(4096, 1024), (8192, 2048), (16384, 4096), (32768, 8192)]
+    mats = {f"rf{m}x{n}": lowrank(torch, m, n, 1024, 64.0, g, dev) for m, n in shapes}
+    Akz = torch.randn(1000, 100000, generator=g, device=dev, dtype=F32).T
+    bkz = [Akz @ torch.randn(1000, generator=g, device=dev, dtype=F32) for _ in range(4)]
+    rng = np.random.default_rng(1234)
+    kinds = rng.integers(0, 2, n_req)
+    which = rng.integers(0, len(shapes), n_req)
+    offs = rla.poisson_arrivals(qps, n_req, rng)
+    out = {"workload": f"Poisson {qps:g} QPS, {n_req} requests, RangeFinder 4096..32768 x "
+                       "1024..8192 (k=266, q=2) and Kaczmarz 100000x1000 (s=64, rtol 1e-6)",
+           "rangefinder_requests": int(n_req - kinds.sum()), "kaczmarz_requests": int(kinds.sum())}
+    for label, prec, slots in (("served_bf16_4slots", "bf16", 4), ("queue_plain_1slot", "plain", 1)):
+        with rla.Server(device=dev, slots=slots) as srv:
+            for name, A in mats.items():
+                srv.register(name, A, "rangefinder")
+            srv.register("kz", Akz, "kaczmarz")
+            # warm-up: one request of each kind and size
+            for name in mats:
+                srv.submit(rla.RangeFinderRequest(name, k=266, precision=prec, seed=1)).result()
+            srv.submit(rla.KaczmarzRequest("kz", bkz[0], seed=1)).result()
+            srv.records.clear()
+            futs = []
+            t0 = time.perf_counter()
+            for i in range(n_req):
+                while time.perf_counter() < t0 + offs[i]:
+                    time.sleep(0.0002)
+                if kinds[i]:
+                    req = rla.KaczmarzRequest("kz", bkz[i % 4], seed=1000 + i)
+                else:
+                    name = f"rf{shapes[which[i]][0]}x{shapes[which[i]][1]}"
+                    req = rla.RangeFinderRequest(name, k=266, precision=prec, seed=1000 + i)
+                futs.append(srv.submit(req))
+            res = [f.result() for f in futs]
+            st = srv.stats()
+            st["max_in_flight"] = srv.max_in_flight
+            st["kaczmarz_all_converged"] = all(r[1].converged for r, k in zip(res, kinds) if k)
+            out[label] = st
+            del res, futs
+    out["p99_ratio_plain_over_served"] = (out["queue_plain_1slot"]["p99_ms"] /
+                                          out["served_bf16_4slots"]["p99_ms"])
+    out["solves_per_s_ratio_served_over_plain"] = (out["served_bf16_4slots"]["solves_per_s"] /
+                                                   out["queue_plain_1slot"]["solves_per_s"])
+    return out
+
+
 def cpu_baseline(S, A0):
     """Oracle C restatement (OpenMP over columns, all host threads) on the full
     workload: the same A, op, signs and scale."""
@@ -174,6 +335,9 @@ def main():
                     help="also run a plain 256 MiB device copy (counter calibration)")
     ap.add_argument("--skip-sparse", action="store_true")
     ap.add_argument("--skip-rangefinder", action="store_true")
+    ap.add_argument("--skip-ls", action="store_true")
+    ap.add_argument("--skip-kaczmarz", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the SRHT apply "
@@ -187,6 +351,9 @@ def main():
     gbs = srht_bytes() / ms / 1e6
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
     rf = None if args.skip_rangefinder else rangefinder_c3(rla, torch)
+    ls = None if args.skip_ls else ls_batched_c4(rla, torch)
+    kz = None if args.skip_kaczmarz else kaczmarz_c5(rla, torch)
+    srv = None if args.skip_serving else serving_c5(rla, torch)
     if args.calib_copy:
         # counter calibration: a plain streaming kernel of known traffic
         # (rla::convert_kernel<float, double>: reads 256 MiB, writes 512 MiB)
@@ -203,7 +370,7 @@ def main():
             if line.startswith("bytes_per_launch:"):
                 traffic = float(line.split(":", 1)[1])
     out = {
-        "metric": "sketch GB/s (% HBM peak)",
+        "metric": "sketch GB/s (% HBM peak); solves/s + p99 latency vs plain at equal error",
         "value": gbs,
         "unit": "GB/s",
         "n_gpus": 1,
@@ -224,6 +391,9 @@ def main():
                      "algorithmic_bytes_per_launch": srht_bytes()},
         "sparse_sign_c1": extra,
         "rangefinder_c3": rf,
+        "ls_batched_c4": ls,
+        "kaczmarz_c5": kz,
+        "serving_c5": srv,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(S, A0)

@@ -6,16 +6,13 @@
 // One iteration projects x onto the solution set of a block of s rows,
 //   x <- x + alpha * A~' (A~ A~')^-1 (b~ - A~ x),
 // the minimum-norm correction the LQ sub-solver computes (lq.jl:35-46).  On the
-// device an iteration is two launches:
-//   * kz_fused_kernel, one workgroup per 64 columns: first the pending update
-//     of the previous block (x_c += alpha * sum_i A~_prev[i,c] y_i), then the
-//     gather of the new block's columns from the row-major copy of A and its
-//     partial Gram matrix and partial A~ x over those columns (FP64 sums of
-//     FP32 products, which are exact);
-//   * kz_solve_kernel, one workgroup: sums the partials in column-chunk order,
-//     r = b~ - A~ x, Cholesky of the s x s Gram matrix in FP64, y = G^-1 r.
-// The update of block t is folded into the sweep that builds block t+1, so A~
-// is read twice per iteration and x once.
+// device an iteration is one launch of kz_step_kernel: its workgroups apply
+// the pending update of the previous block to their columns of x, gather the
+// new block's columns from the row-major copy of A and form partial Gram
+// matrices (FP64 MFMA on FP32 entries, exact products) and partial A~x; the
+// last workgroup to arrive sums the partials, factors the Gram matrix and
+// solves for y.  The update of block t is folded into the launch that builds
+// block t+1, so A~ is read twice per iteration and x once.
 //
 // Sampling: the row sets of a whole run of iterations are drawn at once.
 // Without replacement the draw is the Efraimidis–Spirakis A-Res scheme
@@ -34,8 +31,8 @@ namespace rla {
 constexpr uint32_t DOM_WSAMPLE = 5u;
 constexpr int kSortN = 4096;      // candidates per top-k workgroup
 constexpr int kSortThreads = 512;
-constexpr int kKzCols = 64;       // columns per fused workgroup
-constexpr int kKzMaxBlock = 64;   // rows per Kaczmarz block on the fused path
+constexpr int kKzCols = 64;       // columns per chunk of the Kaczmarz step
+constexpr int kKzMaxBlock = 64;   // rows per Kaczmarz block
 
 // ---- row weights ----------------------------------------------------------
 
@@ -232,132 +229,402 @@ struct RowSrc {
   }
 };
 
-__global__ __launch_bounds__(256) void kz_fused_kernel(int64_t n, RowSrc prev,
-                                                       const double* __restrict__ y, float alpha,
-                                                       float* __restrict__ x, RowSrc cur,
-                                                       double* __restrict__ part_G,
-                                                       double* __restrict__ part_d) {
-  __shared__ float T[kKzMaxBlock][kKzCols + 1];
-  __shared__ float xs[kKzCols];
-  const int t = threadIdx.x, w = blockIdx.x;
-  const int64_t c0 = int64_t(w) * kKzCols;
-  const int ncol = static_cast<int>(min<int64_t>(kKzCols, n - c0));
-  // pending update of the previous block: x_c += alpha * sum_i A~[i,c] y_i
-  if (t < ncol) {
-    const int64_t c = c0 + t;
-    float xc = x[c];
-    if (prev.s > 0) {
-      double u = 0.0;
-      for (int i = 0; i < prev.s; ++i) u += static_cast<double>(prev.at(i, c)) * y[i];
-      xc = alpha * static_cast<float>(u) + xc;
-      x[c] = xc;
-    }
-    xs[t] = xc;
-  }
-  if (cur.s == 0) return;
-  // gather the new block's columns c0 .. c0+ncol
-  for (int e = t; e < cur.s * kKzCols; e += blockDim.x) {
-    const int i = e / kKzCols, cc = e % kKzCols;
-    T[i][cc] = cc < ncol ? cur.at(i, c0 + cc) : 0.0f;
-  }
-  if (t >= ncol && t < kKzCols) xs[t] = 0.0f;
-  __syncthreads();
-  const int s = cur.s;
-  for (int i = t; i < s; i += blockDim.x) {
-    double d = 0.0;
-    for (int cc = 0; cc < kKzCols; ++cc) d += static_cast<double>(T[i][cc]) * xs[cc];
-    part_d[int64_t(w) * s + i] = d;
-  }
-  for (int e = t; e < s * s; e += blockDim.x) {
-    const int i = e / s, j = e % s;
-    if (i > j) continue;
-    double g = 0.0;
-    for (int cc = 0; cc < kKzCols; ++cc)
-      g += static_cast<double>(T[i][cc]) * static_cast<double>(T[j][cc]);
-    part_G[(int64_t(w) * s + i) * s + j] = g;
+typedef double kz_f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kKzTiles = 10;              // upper 16 x 16 tiles of a 64 x 64 Gram matrix
+constexpr int kKzTs = kKzCols + 4;        // LDS block-tile row stride (floats, 16-byte rows)
+constexpr int kKzGs = kKzMaxBlock + 1;    // LDS Gram row stride (doubles)
+constexpr int kKzMaxWG = 8;               // workgroups per step: partials the solver sums
+constexpr int kKzTileDoubles = 256;       // one accumulator tile: 4 x 64 lanes
+
+// One Kaczmarz iteration (one launch of kz_step_kernel).
+struct KzStep {
+  int64_t n;
+  int chunks_per_wg;     // 64-column chunks each workgroup sweeps
+  RowSrc prev;           // the pending block (prev.s == 0: none)
+  const double* y_prev;  // its coefficients
+  float alpha;
+  float* x;
+  RowSrc cur;            // the new block (cur.s == 0: apply the pending update only)
+  RowSrc bsrc;           // b~ of the new block, element (i, 0)
+  double* part_G;        // per workgroup: kKzTiles accumulator tiles
+  double* part_d;        // per workgroup: kKzMaxBlock partial sums of A~x
+  unsigned* ticket;      // arrival counter; 0 between launches
+  double* y_out;         // coefficients of the new block
+  int* info;             // sticky: 1-based pivot of a numerically singular block
+};
+
+// upper tile b = 0..9 of a 4 x 4 tile grid -> (tile row, tile column)
+__device__ __forceinline__ int kz_ti(int b) { return int((0x3221110000ull >> (4 * b)) & 15u); }
+__device__ __forceinline__ int kz_tj(int b) { return int((0x3323213210ull >> (4 * b)) & 15u); }
+
+// value of v in lane src (src uniform), in every lane
+__device__ __forceinline__ double kz_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+// LDS traffic between lanes of one wavefront needs no fence (the LDS runs a
+// wavefront's instructions in order); this only keeps the compiler from
+// moving accesses across the exchange point.
+__device__ __forceinline__ void kz_lane_order() { __builtin_amdgcn_wave_barrier(); }
+
+// x <- x + alpha A~_prev' y_prev, then for the new block A~ (rows cur, s <= 64)
+// y_out = (A~ A~')^-1 (b~ - A~ x) — the minimum-norm correction of the LQ
+// sub-solve (lq.jl:35-46, kaczmarz.jl:294-311) — in one launch:
+//   * every workgroup sweeps its 64-column chunks (the next chunk's loads in
+//     flight while the current one is processed): the pending update of its
+//     columns of x, the new block's columns gathered from the row-major copy
+//     of A into LDS, the partial A~x, and the partial Gram matrix as ten
+//     upper 16 x 16 tiles on the FP64 MFMA (FP32 entries: exact products);
+//   * the last workgroup to arrive sums the partials in workgroup order
+//     (deterministic; all loads of the sum in flight at once), and one
+//     wavefront factors G = L D L' (unit lower L; left-looking over 16-column
+//     block columns: the update of a block column from the factored ones on
+//     the MFMA, then a right-looking factor of its 16 columns with lane i
+//     holding row i, the forward solve L z = r folded in), then solves
+//     L' y = D^-1 z.
+// A pivot d_j not above 4 s eps G_jj (a numerically dependent row) gives
+// y = 0 and sets *info to its 1-based position; an all-zero block gives
+// y = 0 (kaczmarz.jl:300 skips it).
+// Gram tiles of wavefront W: b = W, W + 4, W + 8 (tile row, tile column from
+// kz_ti / kz_tj), as compile-time lists so the operands stay in registers.
+template <int W>
+__device__ __forceinline__ void kz_gram_tiles(const float (&op)[4][16], kz_f64x4 (&acc)[3]) {
+  constexpr int nt = W < 2 ? 3 : 2;
+  constexpr int ti[3] = {0, W == 0 ? 1 : (W == 3 ? 2 : 1), W == 0 ? 2 : 3};
+  constexpr int tj[3] = {W, W == 0 ? 1 : (W == 1 ? 2 : (W == 2 ? 3 : 2)), W == 0 ? 3 : 3};
+#pragma unroll
+  for (int kq = 0; kq < 16; ++kq) {
+    double o[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = static_cast<double>(op[r][kq]);
+#pragma unroll
+    for (int q = 0; q < nt; ++q)
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(o[ti[q]], o[tj[q]], acc[q], 0, 0, 0);
   }
 }
 
-// y = (A~ A~')^-1 (b~ - A~ x) from the column-chunk partials.  A pivot not
-// above 4 s eps G_jj (a numerically dependent row) gives y = 0 and sets *info
-// to its 1-based position (never cleared here); an all-zero block gives y = 0
-// (kaczmarz.jl:300 skips it).
-__global__ __launch_bounds__(256) void kz_solve_kernel(int s, int nparts,
-                                                       const double* __restrict__ part_G,
-                                                       const double* __restrict__ part_d,
-                                                       RowSrc bsrc, double* __restrict__ y,
-                                                       int* __restrict__ info) {
-  __shared__ double G[kKzMaxBlock][kKzMaxBlock + 1];
-  __shared__ double r[kKzMaxBlock];
-  __shared__ double fl[kKzMaxBlock];  // pivot floors 4 s eps G_jj
-  __shared__ int bad;
-  const int t = threadIdx.x;
-  for (int e = t; e < s * s; e += blockDim.x) {
-    const int i = e / s, j = e % s;
-    if (i > j) continue;
-    double g = 0.0;
-    for (int q = 0; q < nparts; ++q) g += part_G[(int64_t(q) * s + i) * s + j];
-    G[i][j] = g;
-    G[j][i] = g;
-  }
-  for (int i = t; i < s; i += blockDim.x) {
-    double d = 0.0;
-    for (int q = 0; q < nparts; ++q) d += part_d[int64_t(q) * s + i];
-    // b~ - A~ x (kaczmarz.jl:305, mul!(vec_view, mat_view, x, -1.0, 1.0))
-    r[i] = static_cast<double>(bsrc.at(i, 0)) - d;
-  }
-  __syncthreads();
-  for (int i = t; i < s; i += blockDim.x) fl[i] = 4.0 * s * 2.220446049250313e-16 * G[i][i];
-  if (t == 0) {
-    bool zero = true;
-    for (int i = 0; i < s; ++i) zero = zero && G[i][i] == 0.0;
-    bad = zero ? -1 : 0;
+template <bool VEC>
+__global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
+  __shared__ __align__(16) float T[kKzMaxBlock * kKzTs];
+  __shared__ __align__(16) float Tp[kKzMaxBlock * kKzTs];
+  __shared__ float xs[kKzCols];
+  __shared__ double ured[4][kKzCols];
+  __shared__ double Gs[kKzMaxBlock * kKzGs];
+  __shared__ double Dv[kKzMaxBlock];
+  __shared__ double rv[kKzMaxBlock];
+  __shared__ int64_t prow[kKzMaxBlock], crow[kKzMaxBlock];
+  __shared__ double yprev[kKzMaxBlock];
+  __shared__ int last;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int s = p.cur.s, ps = p.prev.s;
+  const int nb = (s + 15) >> 4;  // 16-row tiles of the new block
+  // Row offsets of both blocks, clamped (rows past a block repeat its last).
+  // Loads never depend on which blocks are present: a missing block reads
+  // the other block's rows, and its values go unused.
+  __shared__ int cidx[kKzMaxBlock];
+  const RowSrc P = ps > 0 ? p.prev : p.cur;
+  const RowSrc C = s > 0 ? p.cur : p.prev;
+  if (t < kKzMaxBlock) {
+    const int ip = min(t, P.s - 1), ic = min(t, C.s - 1);
+    const int64_t rp = P.idx ? int64_t(P.idx[ip]) : int64_t(ip);
+    const int64_t rc = C.idx ? int64_t(C.idx[ic]) : int64_t(ic);
+    prow[t] = rp * P.rs;
+    crow[t] = rc * C.rs;
+    cidx[t] = static_cast<int>(rc);
+    yprev[t] = p.y_prev[min(t, max(ps, 1) - 1)];
   }
   __syncthreads();
-  if (bad == -1) {  // A~ == 0: no update
-    for (int i = t; i < s; i += blockDim.x) y[i] = 0.0;
+
+  kz_f64x4 acc[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) acc[q] = kz_f64x4{0.0, 0.0, 0.0, 0.0};
+  double dpart = 0.0;  // lanes < 16: row 16 wv + lane of A~x
+  const int64_t nchunks = (p.n + kKzCols - 1) / kKzCols;
+  // Loads of a chunk (64 rows x 64 columns of each block): with VEC, float4
+  // e = t + 256 k covers row e/16, columns 4 (e%16) ..; otherwise element
+  // e = t + 256 k is row e/64, column e%64.  Rows past a block repeat its
+  // last row and columns past n are clamped, so every load is in bounds.
+  float sp[2][16], sc[2][16];
+  float xv[2] = {0.f, 0.f};
+  auto load_chunk = [&](int64_t ch, float(&lp)[16], float(&lc)[16], float& lx) {
+    const int64_t c0 = ch * kKzCols;
+    if constexpr (VEC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = t + 256 * k, row = e >> 4;
+        const int64_t c = min<int64_t>(c0 + 4 * (e & 15), p.n - 4);
+        const float4 v = *reinterpret_cast<const float4*>(P.base + prow[row] + c);
+        const float4 w = *reinterpret_cast<const float4*>(C.base + crow[row] + c);
+        lp[4 * k] = v.x, lp[4 * k + 1] = v.y, lp[4 * k + 2] = v.z, lp[4 * k + 3] = v.w;
+        lc[4 * k] = w.x, lc[4 * k + 1] = w.y, lc[4 * k + 2] = w.z, lc[4 * k + 3] = w.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, row = e >> 6;
+        const int64_t c = c0 + min<int64_t>(e & 63, p.n - c0 - 1);
+        lp[k] = P.base[prow[row] + c * P.cs];
+        lc[k] = C.base[crow[row] + c * C.cs];
+      }
+    }
+    lx = p.x[c0 + min<int64_t>(lane, p.n - c0 - 1)];
+  };
+  auto run_chunk = [&](int64_t ch, const float(&lp)[16], const float(&lc)[16], float lx) {
+    const int64_t c0 = ch * kKzCols;
+    const int ncol = static_cast<int>(min<int64_t>(kKzCols, p.n - c0));
+    // stage both blocks in LDS (new block: rows >= s and columns >= ncol are 0)
+    if constexpr (VEC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = t + 256 * k, row = e >> 4, cb = 4 * (e & 15);
+        if (ps > 0)
+          *reinterpret_cast<float4*>(&Tp[row * kKzTs + cb]) =
+              float4{lp[4 * k], lp[4 * k + 1], lp[4 * k + 2], lp[4 * k + 3]};
+        if (s > 0) {
+          const bool rok = row < s;
+          *reinterpret_cast<float4*>(&T[row * kKzTs + cb]) =
+              float4{rok && cb < ncol ? lc[4 * k] : 0.f, rok && cb + 1 < ncol ? lc[4 * k + 1] : 0.f,
+                     rok && cb + 2 < ncol ? lc[4 * k + 2] : 0.f,
+                     rok && cb + 3 < ncol ? lc[4 * k + 3] : 0.f};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, row = e >> 6, cc = e & 63;
+        if (ps > 0) Tp[row * kKzTs + cc] = lp[k];
+        if (s > 0) T[row * kKzTs + cc] = (row < s && cc < ncol) ? lc[k] : 0.f;
+      }
+    }
+    __syncthreads();
+    // pending update: rows 16 wv .. 16 wv + 15 of the previous block, column lane
+    if (ps > 0) {
+      double u = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (16 * wv + q < ps)
+          u += static_cast<double>(Tp[(16 * wv + q) * kKzTs + lane]) * yprev[16 * wv + q];
+      ured[wv][lane] = u;
+      __syncthreads();
+    }
+    if (t < kKzCols) {
+      float xc = lx;
+      if (lane < ncol && ps > 0) {
+        const double u = ((ured[0][lane] + ured[1][lane]) + ured[2][lane]) + ured[3][lane];
+        xc = p.alpha * static_cast<float>(u) + xc;
+        p.x[c0 + lane] = xc;
+      }
+      xs[lane] = lane < ncol ? xc : 0.f;
+    }
+    if (s > 0) {
+      __syncthreads();
+      // MFMA operands of this lane: T[16 rb + lane%16][4 kq + lane/16]
+      float op[4][16];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int kq = 0; kq < 16; ++kq)
+          op[rb][kq] = T[(16 * rb + (lane & 15)) * kKzTs + 4 * kq + (lane >> 4)];
+      // partial A~x of rows 16 wv .. 16 wv + 15: 16 columns per lane, then
+      // the four column groups ((g0 + g1) + (g2 + g3))
+      {
+        double d = 0.0;
+#pragma unroll
+        for (int kq = 0; kq < 16; ++kq)
+          d += static_cast<double>(T[(16 * wv + (lane & 15)) * kKzTs + 4 * kq + (lane >> 4)]) *
+               static_cast<double>(xs[4 * kq + (lane >> 4)]);
+        d += __shfl_xor(d, 16);
+        d += __shfl_xor(d, 32);
+        dpart += d;
+      }
+      switch (wv) {
+        case 0: kz_gram_tiles<0>(op, acc); break;
+        case 1: kz_gram_tiles<1>(op, acc); break;
+        case 2: kz_gram_tiles<2>(op, acc); break;
+        default: kz_gram_tiles<3>(op, acc); break;
+      }
+    }
+    __syncthreads();  // T, Tp, xs and ured are rewritten by the next chunk
+  };
+  // chunks blockIdx.x + ci * gridDim.x, two per round of loads
+  for (int ci = 0; ci < p.chunks_per_wg; ci += 2) {
+    const int64_t ch0 = int64_t(blockIdx.x) + int64_t(ci) * gridDim.x;
+    const int64_t ch1 = ch0 + gridDim.x;
+    if (ch0 >= nchunks) break;
+    const bool two = ci + 1 < p.chunks_per_wg && ch1 < nchunks;
+    load_chunk(ch0, sp[0], sc[0], xv[0]);
+    if (two) load_chunk(ch1, sp[1], sc[1], xv[1]);
+    run_chunk(ch0, sp[0], sc[0], xv[0]);
+    if (two) run_chunk(ch1, sp[1], sc[1], xv[1]);
+  }
+  if (s == 0) return;
+
+  // ---- partials out; the last workgroup to arrive solves ----
+  const int wg = blockIdx.x;
+  if (lane < 16) p.part_d[int64_t(wg) * kKzMaxBlock + 16 * wv + lane] = dpart;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int b = wv + 4 * q;
+    if (b >= kKzTiles) break;
+    double* dst = p.part_G + (int64_t(wg) * kKzTiles + b) * kKzTileDoubles + lane;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[r * 64] = acc[q][r];
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = atomicAdd(p.ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+
+  // element t of every tile, and row t of A~x, summed over the workgroups
+  const int nwg = gridDim.x;
+  {
+    const double bt = static_cast<double>(p.bsrc.base[int64_t(cidx[t & (kKzMaxBlock - 1)]) * p.bsrc.rs]);
+    double g[kKzTiles][kKzMaxWG];
+    double dd[kKzMaxWG];
+    // unconditional loads from clamped indices (inside the workspace; values
+    // of workgroups past nwg and tiles past the block are discarded below)
+#pragma unroll
+    for (int b = 0; b < kKzTiles; ++b)
+#pragma unroll
+      for (int v = 0; v < kKzMaxWG; ++v)
+        g[b][v] = p.part_G[(int64_t(min(v, nwg - 1)) * kKzTiles + b) * kKzTileDoubles + t];
+#pragma unroll
+    for (int v = 0; v < kKzMaxWG; ++v)
+      dd[v] = p.part_d[int64_t(min(v, nwg - 1)) * kKzMaxBlock + (t & (kKzMaxBlock - 1))];
+    // accumulator layout: element t = 64 r + l is row l/16 + 4r, column l%16
+    const int row0 = (lane >> 4) + 4 * (t >> 6), col0 = lane & 15;
+#pragma unroll
+    for (int b = 0; b < kKzTiles; ++b) {
+      const int I = kz_ti(b), J = kz_tj(b);
+      if (J >= nb) continue;
+      double sum = 0.0;
+#pragma unroll
+      for (int v = 0; v < kKzMaxWG; ++v)
+        if (v < nwg) sum += g[b][v];
+      const int row = 16 * I + row0, col = 16 * J + col0;
+      Gs[row * kKzGs + col] = sum;
+      if (I != J) Gs[col * kKzGs + row] = sum;
+    }
+    if (t < kKzMaxBlock) {
+      double d = 0.0;
+#pragma unroll
+      for (int v = 0; v < kKzMaxWG; ++v)
+        if (v < nwg) d += dd[v];
+      // b~ - A~ x (kaczmarz.jl:305, mul!(vec_view, mat_view, x, -1.0, 1.0))
+      rv[t] = t < s ? bt - d : 0.0;
+    }
+  }
+  __syncthreads();
+  if (wv != 0) return;
+
+  // ---- one wavefront: lane i holds row i ----
+  // Rows s .. 16 nb - 1 of the Gram matrix are zero; their diagonal is set
+  // to 1, so the factor runs over whole 16-column blocks without branches
+  // (the padding factors to the identity and leaves rows < s unchanged).
+  const int i = lane;
+  const int np = 16 * nb;
+  if (i >= s && i < np) Gs[i * kKzGs + i] = 1.0;
+  const double gii = i < s ? Gs[i * kKzGs + i] : 0.0;
+  if (__ballot(gii != 0.0) == 0) {  // A~ == 0: no update
+    if (i < s) p.y_out[i] = 0.0;
+    if (i == 0) *p.ticket = 0u;
     return;
   }
-  // Cholesky G = L L' (lower, in place)
-  for (int j = 0; j < s; ++j) {
-    const double p = G[j][j];
-    if (!(p > fl[j]) || !(p > 0.0) || !isfinite(p)) {
-      if (t == 0) bad = j + 1;
-      break;
+  const double fl = 4.0 * s * 2.220446049250313e-16 * gii;  // pivot floor of row i
+  double z = rv[i];
+  double dme = 1.0, invme = 1.0;  // lane j: d_j and 1 / d_j
+  bool okme = true;               // lane j: pivot j above its floor
+  // Only entries strictly below the diagonal of L are read back (by the
+  // MFMA update and the back-solve); lanes above a pivot compute values
+  // that nothing reads.
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    if (kb < nb) {
+      const int r0 = 16 * kb;
+      // G(r0:, r0:r0+15) -= L(r0:, 0:r0-1) D L(r0:r0+15, 0:r0-1)'
+#pragma unroll
+      for (int I = kb; I < 4; ++I) {
+        if (kb > 0 && I < nb) {
+          kz_f64x4 acc4;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            acc4[r] = Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)];
+#pragma unroll
+          for (int k0 = 0; k0 < r0; k0 += 4) {
+            const int kk = k0 + (lane >> 4);
+            const double av_ = -Gs[(16 * I + (lane & 15)) * kKzGs + kk] * Dv[kk];
+            const double bv_ = Gs[(r0 + (lane & 15)) * kKzGs + kk];
+            acc4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av_, bv_, acc4, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)] = acc4[r];
+        }
+      }
+      kz_lane_order();
+      double a[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) a[c] = Gs[i * kKzGs + r0 + c];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int jj = r0 + j;
+        const double d = kz_bcast(a[j], jj);  // pivot d_jj
+        const bool ok = a[j] > fl && isfinite(a[j]);
+        // 1/d: reciprocal estimate refined by two Newton steps e = 1 - d r, r += r e
+        double inv = __builtin_amdgcn_rcp(d);
+        inv = fma(inv, fma(-d, inv, 1.0), inv);
+        inv = fma(inv, fma(-d, inv, 1.0), inv);
+        if (i == jj) {
+          okme = ok;
+          dme = d;
+          invme = inv;
+        }
+        const double u = a[j];  // L_ij d_j
+        const double l = u * inv;
+        a[j] = l;
+        // forward solve with unit L, right-looking
+        const double zj = kz_bcast(z, jj);
+        if (i > jj) z = fma(-l, zj, z);
+        // trailing columns of this block column: a_ic -= L_ij d_j L_cj
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c) a[c] = fma(-u, kz_bcast(l, r0 + c), a[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) Gs[i * kKzGs + r0 + c] = a[c];
+      if (i >= r0 && i < r0 + 16) Dv[i] = dme;
+      kz_lane_order();
     }
-    const double d = sqrt(p);
-    __syncthreads();
-    for (int i = j + t; i < s; i += blockDim.x) G[i][j] = i == j ? d : G[i][j] / d;
-    __syncthreads();
-    const int rem = s - j - 1;
-    for (int e = t; e < rem * rem; e += blockDim.x) {
-      const int i = j + 1 + e / rem, c = j + 1 + e % rem;
-      if (c <= i) G[i][c] -= G[i][j] * G[c][j];
-    }
-    __syncthreads();
   }
-  __syncthreads();
-  if (bad) {  // singular block: skipped, and recorded (info is sticky)
-    for (int i = t; i < s; i += blockDim.x) y[i] = 0.0;
-    if (t == 0) *info = bad;
+  const unsigned long long badm = __ballot(!okme && i < s);
+  if (badm) {  // singular block: skipped, and recorded (info is sticky)
+    if (i < s) p.y_out[i] = 0.0;
+    if (i == 0) {
+      *p.info = __ffsll(badm);  // 1-based first failing pivot
+      *p.ticket = 0u;
+    }
     return;
   }
-  // L z = r, then L' y = z (in r)
-  for (int l = 0; l < s; ++l) {
-    const double z = r[l] / G[l][l];
-    __syncthreads();
-    for (int i = l + 1 + t; i < s; i += blockDim.x) r[i] -= G[i][l] * z;
-    if (t == 0) r[l] = z;
-    __syncthreads();
+  // L' y = D^-1 z; column j of L' is row j of L (read from LDS by lanes below j)
+  z *= invme;
+#pragma unroll
+  for (int kb = 3; kb >= 0; --kb) {
+    if (kb < nb) {
+#pragma unroll
+      for (int j = 16 * kb + 15; j >= 16 * kb; --j) {
+        const double yj = kz_bcast(z, j);
+        const double lji = Gs[j * kKzGs + i];
+        if (i < j) z = fma(-lji, yj, z);
+      }
+    }
   }
-  for (int l = s - 1; l >= 0; --l) {
-    const double v = r[l] / G[l][l];
-    __syncthreads();
-    for (int i = t; i < l; i += blockDim.x) r[i] -= G[l][i] * v;
-    if (t == 0) r[l] = v;
-    __syncthreads();
-  }
-  for (int i = t; i < s; i += blockDim.x) y[i] = r[i];
+  if (i < s) p.y_out[i] = z;
+  if (i == 0) *p.ticket = 0u;
 }
 
 __global__ void set_int_kernel(int* p, int v) {
@@ -366,6 +633,47 @@ __global__ void set_int_kernel(int* p, int v) {
 
 // ---- FP64 residual ||b - A x|| (full_residual.jl:32-41) ----------------------
 
+constexpr int kResRowsPerWave = 4;
+
+// A row-major (row i at A + i*ld): one wavefront per row, FP64 sums of exact
+// FP32 products, a fixed shuffle tree; per-workgroup sums of r_i^2.
+template <bool VEC>
+__global__ __launch_bounds__(256) void residual_rows_kernel(int64_t m, int64_t n,
+                                                            const float* __restrict__ A,
+                                                            int64_t ld, const float* __restrict__ x,
+                                                            const float* __restrict__ b,
+                                                            double* __restrict__ part) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double r2 = 0.0;
+  for (int q = 0; q < kResRowsPerWave; ++q) {
+    const int64_t i = (int64_t(blockIdx.x) * 4 + wv) * kResRowsPerWave + q;
+    if (i >= m) break;
+    const float* row = A + i * ld;
+    double acc = 0.0;
+    if (VEC) {
+      for (int64_t c = int64_t(lane) * 4; c < n; c += 256) {
+        const float4 a = *reinterpret_cast<const float4*>(row + c);
+        const float4 xv = *reinterpret_cast<const float4*>(x + c);
+        acc += static_cast<double>(a.x) * xv.x;
+        acc += static_cast<double>(a.y) * xv.y;
+        acc += static_cast<double>(a.z) * xv.z;
+        acc += static_cast<double>(a.w) * xv.w;
+      }
+    } else {
+      for (int64_t c = lane; c < n; c += 64) acc += static_cast<double>(row[c]) * x[c];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    const double r = static_cast<double>(b[i]) - acc;
+    r2 += r * r;
+  }
+  if (lane == 0) red[wv] = r2;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// A with any strides: one thread per row.
 __global__ __launch_bounds__(256) void residual_part_kernel(int64_t m, int64_t n,
                                                             const float* __restrict__ A,
                                                             int64_t a_rs, int64_t a_cs,
@@ -392,12 +700,20 @@ __global__ __launch_bounds__(256) void residual_part_kernel(int64_t m, int64_t n
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void residual_final_kernel(int64_t nparts, const double* __restrict__ part,
-                                      double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// sqrt of the sum of the partials: strided per-thread sums, then a fixed tree
+__global__ __launch_bounds__(256) void residual_final_kernel(int64_t nparts,
+                                                             const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  __shared__ double red[256];
   double s = 0.0;
-  for (int64_t q = 0; q < nparts; ++q) s += part[q];
-  *out = sqrt(s);
+  for (int64_t q = threadIdx.x; q < nparts; q += 256) s += part[q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(red[0]);
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -477,44 +793,102 @@ static int check_sample(int64_t n, int64_t k, int replace) {
   return RLA_OK;
 }
 
+// partial sums the residual writes for m rows
+static int64_t residual_parts(int64_t m) { return ceil_div(m, 4 * kResRowsPerWave); }
+
 static int residual(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t a_rs,
                     int64_t a_cs, const float* x, const float* b, double* part, double* out) {
-  const int64_t nb = ceil_div(m, 256);
-  residual_part_kernel<<<dim3(unsigned(nb)), 256, 0, ctx->stream>>>(m, n, A, a_rs, a_cs, x, b,
-                                                                    part);
+  int64_t nb;
+  if (a_cs == 1 && a_rs >= n) {  // row-major: one wavefront per row
+    nb = ceil_div(m, 4 * kResRowsPerWave);
+    const bool vec = n % 4 == 0 && a_rs % 4 == 0 &&
+                     ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(x)) & 15) == 0;
+    if (vec)
+      residual_rows_kernel<true><<<dim3(unsigned(nb)), 256, 0, ctx->stream>>>(m, n, A, a_rs, x, b,
+                                                                             part);
+    else
+      residual_rows_kernel<false><<<dim3(unsigned(nb)), 256, 0, ctx->stream>>>(m, n, A, a_rs, x,
+                                                                              b, part);
+  } else {
+    nb = ceil_div(m, 256);
+    residual_part_kernel<<<dim3(unsigned(nb)), 256, 0, ctx->stream>>>(m, n, A, a_rs, a_cs, x, b,
+                                                                      part);
+  }
   RLA_LAUNCH_CHECK("residual");
-  residual_final_kernel<<<1, 64, 0, ctx->stream>>>(nb, part, out);
+  residual_final_kernel<<<1, 256, 0, ctx->stream>>>(nb, part, out);
   RLA_LAUNCH_CHECK("residual/final");
   return RLA_OK;
 }
 
 struct KzBufs {
-  int* sets;      // every x s
-  double* part_G; // W x s x s
-  double* part_d; // W x s
-  double* y;      // s
-  int* info;      // 1
-  double* res;    // residual partials + result
-  double* resv;
+  int* sets;          // every x s row indices
+  double* part_G;     // W x kKzTiles accumulator tiles
+  double* part_d;     // W x kKzMaxBlock
+  double* y;          // 2 x kKzMaxBlock (the pending block's and the new block's)
+  unsigned* ticket;   // arrival counter of kz_step_kernel
+  int* info;          // sticky breakdown flag
+  double* res;        // residual partials
+  double* resv;       // residual norm
   double *k1, *k2, *cdf;
   int *i1, *i2;
 };
+
+// workgroups of a step over n columns, and the 64-column chunks each sweeps
+static void kz_grid(int64_t n, int* wgs, int* chunks_per_wg) {
+  const int64_t chunks = ceil_div(n, kKzCols);
+  const int64_t per = ceil_div(chunks, kKzMaxWG);
+  *chunks_per_wg = static_cast<int>(per);
+  *wgs = static_cast<int>(ceil_div(chunks, per));
+}
 
 static size_t kz_layout(int64_t m, int64_t n, int64_t s, int64_t every, void* work, size_t bytes,
                         KzBufs* out) {
   Bump2 b{static_cast<char*>(work), bytes};
   KzBufs k{};
-  const int64_t W = ceil_div(n, kKzCols);
+  int W, per;
+  kz_grid(n, &W, &per);
   k.sets = b.take<int>(every * s);
-  k.part_G = b.take<double>(W * s * s);
-  k.part_d = b.take<double>(W * s);
-  k.y = b.take<double>(s);
+  k.part_G = b.take<double>(int64_t(W) * kKzTiles * kKzTileDoubles);
+  k.part_d = b.take<double>(int64_t(W) * kKzMaxBlock);
+  k.y = b.take<double>(2 * kKzMaxBlock);
+  k.ticket = b.take<unsigned>(1);
   k.info = b.take<int>(1);
-  k.res = b.take<double>(ceil_div(m, 256));
+  k.res = b.take<double>(std::max<int64_t>(residual_parts(m), ceil_div(m, 256)));
   k.resv = b.take<double>(1);
   sample_scratch(m, static_cast<int>(s), every, b, &k.k1, &k.i1, &k.k2, &k.i2, &k.cdf);
   if (out) *out = k;
   return b.need;
+}
+
+// one Kaczmarz step: the pending update (prev, y_prev) and the new block's
+// coefficients (cur, bsrc -> y_out)
+static int kz_step(rla_ctx* ctx, const KzBufs& k, int64_t n, const RowSrc& prev,
+                   const double* y_prev, float alpha, float* x, const RowSrc& cur,
+                   const RowSrc& bsrc, double* y_out) {
+  KzStep st;
+  int W;
+  kz_grid(n, &W, &st.chunks_per_wg);
+  st.n = n;
+  st.prev = prev;
+  st.y_prev = y_prev;
+  st.alpha = alpha;
+  st.x = x;
+  st.cur = cur;
+  st.bsrc = bsrc;
+  st.part_G = k.part_G;
+  st.part_d = k.part_d;
+  st.ticket = k.ticket;
+  st.y_out = y_out;
+  st.info = k.info;
+  auto vec_ok = [&](const RowSrc& r) {
+    return r.s == 0 || (r.cs == 1 && r.rs % 4 == 0 && (reinterpret_cast<uintptr_t>(r.base) & 15) == 0);
+  };
+  if (n % 4 == 0 && vec_ok(prev) && vec_ok(cur))
+    kz_step_kernel<true><<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(st);
+  else
+    kz_step_kernel<false><<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(st);
+  RLA_LAUNCH_CHECK("kz_step");
+  return RLA_OK;
 }
 
 }  // namespace rla
@@ -577,10 +951,11 @@ int rla_residual_norm_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, in
   int rc = use_device(ctx);
   if (rc) return rc;
   void* ws = nullptr;
-  rc = workspace(ctx, sizeof(double) * (ceil_div(m, 256) + 1), &ws);
+  const int64_t np = std::max<int64_t>(residual_parts(m), ceil_div(m, 256));
+  rc = workspace(ctx, sizeof(double) * (np + 1), &ws);
   if (rc) return rc;
   double* part = static_cast<double*>(ws);
-  double* res = part + ceil_div(m, 256);
+  double* res = part + np;
   rc = residual(ctx, m, n, A, a_rs, a_cs, x, b, part, res);
   if (rc) return rc;
   RLA_HIP(hipMemcpyAsync(out_host, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -609,22 +984,18 @@ int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const floa
   KzBufs k;
   if (kz_layout(s, n, s, 1, work, work_bytes, &k) > work_bytes)
     return fail(RLA_EARG, "Kaczmarz workspace too small");
-  const int64_t W = ceil_div(n, kKzCols);
   const int si = static_cast<int>(s);
-  RowSrc none{nullptr, 0, 0, nullptr, 0};
-  RowSrc cur{At, at_rs, at_cs, nullptr, si};
+  const RowSrc none{nullptr, 0, 0, nullptr, 0};
+  const RowSrc cur{At, at_rs, at_cs, nullptr, si};
+  const RowSrc bsrc{bt, bt_inc, 0, nullptr, si};
   set_int_kernel<<<1, 1, 0, ctx->stream>>>(k.info, 0);
   RLA_LAUNCH_CHECK("set_int");
-  kz_fused_kernel<<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(n, none, k.y, 0.f, x, cur,
-                                                              k.part_G, k.part_d);
-  RLA_LAUNCH_CHECK("kz_fused");
-  RowSrc bsrc{bt, bt_inc, 0, nullptr, si};
-  kz_solve_kernel<<<1, 256, 0, ctx->stream>>>(si, static_cast<int>(W), k.part_G, k.part_d, bsrc,
-                                              k.y, k.info);
-  RLA_LAUNCH_CHECK("kz_solve");
-  kz_fused_kernel<<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(
-      n, cur, k.y, static_cast<float>(alpha), x, none, k.part_G, k.part_d);
-  RLA_LAUNCH_CHECK("kz_update");
+  set_int_kernel<<<1, 1, 0, ctx->stream>>>(reinterpret_cast<int*>(k.ticket), 0);
+  RLA_LAUNCH_CHECK("set_int");
+  rc = kz_step(ctx, k, n, none, k.y, 0.f, x, cur, bsrc, k.y);
+  if (rc) return rc;
+  rc = kz_step(ctx, k, n, cur, k.y, static_cast<float>(alpha), x, none, none, nullptr);
+  if (rc) return rc;
   int info = 0;
   RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RLA_HIP(hipStreamSynchronize(ctx->stream));
@@ -635,9 +1006,10 @@ int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const floa
 // rsolve! of a Kaczmarz recipe with a Sampling compressor (kaczmarz.jl:313-368):
 // for i = 1..max_it: error and logger at i = 1, 1+every, ... and at max_it
 // (threshold_stop: converged when error < threshold; i == max_it always
-// stops), then a new row set and a block update.  A (column-major, lda) is read
-// for the FP64 residual, A_rm (row-major, ld_rm) for the blocks.  w: row
-// weights (NULL: uniform).  hist receives the evaluated errors.
+// stops), then a new row set and a block update.  A_rm (the row-major copy of
+// A, row stride ld_rm) is read for the blocks and the FP64 residual; A
+// (column-major, lda) is only checked.  w: row weights (NULL: uniform).  hist
+// receives the evaluated errors.
 int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t lda,
                            const float* A_rm, int64_t ld_rm, const float* b, float* x, int64_t s,
                            double alpha, const double* w, int replace, uint64_t seed,
@@ -656,12 +1028,15 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
   KzBufs k;
   if (kz_layout(m, n, s, every, work, work_bytes, &k) > work_bytes)
     return fail(RLA_EARG, "Kaczmarz workspace too small");
-  const int64_t W = ceil_div(n, kKzCols);
   const int si = static_cast<int>(s);
   const RowSrc none{nullptr, 0, 0, nullptr, 0};
+  const float al = static_cast<float>(alpha);
   RowSrc pending = none;
+  int yb = 0;  // k.y + yb * kKzMaxBlock holds the pending block's coefficients
   bool cdf_ready = false;
   set_int_kernel<<<1, 1, 0, ctx->stream>>>(k.info, 0);
+  RLA_LAUNCH_CHECK("set_int");
+  set_int_kernel<<<1, 1, 0, ctx->stream>>>(reinterpret_cast<int*>(k.ticket), 0);
   RLA_LAUNCH_CHECK("set_int");
   int64_t nh = 0, it = 0;
   int conv = 0;
@@ -670,12 +1045,11 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
     it = i;
     if ((i - 1) % every == 0 || i == max_it) {
       if (pending.s) {
-        kz_fused_kernel<<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(
-            n, pending, k.y, static_cast<float>(alpha), x, none, k.part_G, k.part_d);
-        RLA_LAUNCH_CHECK("kz_update");
+        rc = kz_step(ctx, k, n, pending, k.y + yb * kKzMaxBlock, al, x, none, none, nullptr);
+        if (rc) return rc;
         pending = none;
       }
-      rc = residual(ctx, m, n, A, 1, lda, x, b, k.res, k.resv);
+      rc = residual(ctx, m, n, A_rm, ld_rm, 1, x, b, k.res, k.resv);
       if (rc) return rc;
       double err = 0.0;
       int info = 0;
@@ -699,20 +1073,17 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
       cdf_ready = true;
       set_base = i;
     }
-    RowSrc cur{A_rm, ld_rm, 1, k.sets + (i - set_base) * s, si};
-    kz_fused_kernel<<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(
-        n, pending, k.y, static_cast<float>(alpha), x, cur, k.part_G, k.part_d);
-    RLA_LAUNCH_CHECK("kz_fused");
-    RowSrc bsrc{b, 1, 0, cur.idx, si};
-    kz_solve_kernel<<<1, 256, 0, ctx->stream>>>(si, static_cast<int>(W), k.part_G, k.part_d,
-                                                bsrc, k.y, k.info);
-    RLA_LAUNCH_CHECK("kz_solve");
+    const RowSrc cur{A_rm, ld_rm, 1, k.sets + (i - set_base) * s, si};
+    const RowSrc bsrc{b, 1, 0, cur.idx, si};
+    rc = kz_step(ctx, k, n, pending, k.y + yb * kKzMaxBlock, al, x, cur, bsrc,
+                 k.y + (yb ^ 1) * kKzMaxBlock);
+    if (rc) return rc;
+    yb ^= 1;
     pending = cur;
   }
   if (pending.s) {
-    kz_fused_kernel<<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(
-        n, pending, k.y, static_cast<float>(alpha), x, none, k.part_G, k.part_d);
-    RLA_LAUNCH_CHECK("kz_update");
+    rc = kz_step(ctx, k, n, pending, k.y + yb * kKzMaxBlock, al, x, none, none, nullptr);
+    if (rc) return rc;
   }
   int info = 0;
   RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

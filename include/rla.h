@@ -182,6 +182,46 @@ int rla_srht_plan_bytes(int elem_bytes, int64_t n_pad, int64_t s, size_t* bytes)
 int rla_srht_plan(rla_ctx* ctx, int elem_bytes, int64_t n_pad, int64_t s, const int32_t* op,
                   void* plan);
 
+/* ---- FJLT --------------------------------------------------------------
+ * K = sprandn(type, s, n_pad, q) (src/Compressors/fjlt.jl:183, 225): each
+ * entry independently nonzero with probability q (to within 2^-32), nonzero
+ * values N(0,1), as an s x n_pad CSC matrix with rows ascending per column.
+ * rla_fjlt_gen_count fills colptr[n_pad+1] and returns nnz to the host
+ * (synchronises); rla_fjlt_gen_fill then writes rowidx[nnz] and val[nnz].
+ * The index sets are a pure function of (seed, epoch) and identical to
+ * rla_host_fjlt_gen_f64's; values agree with it to rounding.  The recipe's
+ * signs come from rla_srht_gen with op = NULL. */
+int rla_fjlt_gen_count(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t s, int64_t n_pad,
+                       double q, int32_t* colptr, int64_t* nnz);
+int rla_fjlt_gen_fill_f32(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t s, int64_t n_pad,
+                          double q, const int32_t* colptr, int32_t* rowidx, float* val);
+int rla_fjlt_gen_fill_f64(rla_ctx* ctx, uint64_t seed, uint64_t epoch, int64_t s, int64_t n_pad,
+                          double q, const int32_t* colptr, int32_t* rowidx, double* val);
+int rla_host_fjlt_gen_f64(uint64_t seed, uint64_t epoch, int64_t s, int64_t n_pad, double q,
+                          int32_t* colptr_host, int32_t* rowidx_host, double* val_host,
+                          int64_t cap, int64_t* nnz);
+
+/* FJLT operator T = scale * K * H * D * pad (s x m; K above in CSC and its
+ * CSR twin from rla_sparse_transpose; H, D, pad as for the SRHT).
+ *   adjoint = 0:  Y = alpha * T * X + beta * Y      (X m x ncols, Y s x ncols)
+ *   adjoint = 1:  Y = alpha * T' * X + beta * Y     (X s x ncols, Y m x ncols)
+ * Left S*A is adjoint 0 on A; Right A*S is adjoint 0 on A' (Right recipes
+ * hold K' of the reference's n_pad x s K); Left A*S is adjoint 1 on A';
+ * Right S*A is adjoint 1 on A.  Replaces mul! at fjlt.jl:257-535; scale is the
+ * recipe's type(1/(sqrt(n_pad) sqrt(q) sqrt(s))) (fjlt.jl:184, 226). */
+int rla_fjlt_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* colptr, const int32_t* rowidx,
+                       const float* val, const int32_t* rowptr, const int32_t* colidx,
+                       const float* val_t, const uint64_t* sign_chunks, double scale,
+                       const float* X, int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs,
+                       int64_t y_cs, double alpha, double beta);
+int rla_fjlt_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* colptr, const int32_t* rowidx,
+                       const double* val, const int32_t* rowptr, const int32_t* colidx,
+                       const double* val_t, const uint64_t* sign_chunks, double scale,
+                       const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
+                       int64_t y_cs, double alpha, double beta);
+
 /* ---- dense products and conversions ------------------------------------
  * Column-major GEMM C = alpha*op(A)*op(B) + beta*C (vendor library: the plain
  * Float32/Float64 products of the Gaussian compressor, gaussian.jl:145-170,

@@ -15,7 +15,7 @@ would break (dense Hadamard / sparse products, self-inverse FWHT, Philox
 known-answer vectors, Floyd/Lemire distribution checks).
 
 Parity unpinned: the random streams (philox4x32_10, Stream, floyd_sorted and
-the *_gen recipe generators).  The reference draws with Julia's
+the *_gen recipe generators, fjlt_gen included).  The reference draws with Julia's
 Xoshiro/StatsBase (sparse_sign.jl:167,252; srht.jl:150-151; gaussian.jl:107),
 which cannot run here and which no reference test fixes; these functions
 define this build's counter-based stream, pinned only to Philox known-answer
@@ -290,6 +290,87 @@ def sparse_mul(M, X, alpha=1.0, beta=0.0, C=None, dtype=None):
     """C = alpha*M*X + beta*C for a dense-stored sparse M, SparseArrays order."""
     ptr, idx, val = csr_from_dense(M)
     return sparse_rowgather(ptr, idx, val, X, alpha, beta, C, dtype)
+
+
+# ---------------------------------------------------------------------------
+# FJLT: src/Compressors/fjlt.jl
+
+
+DOM_FJLT_ROWS, DOM_FJLT_VALS = 6, 7
+
+
+def _fjlt_block(seed, epoch, domain, item, b):
+    ctr = (b & MASK, item & MASK, ((item >> 32) ^ (domain << 24)) & MASK,
+           (epoch & MASK) ^ ((epoch >> 32) & MASK))
+    return philox4x32_10(ctr, (seed & MASK, (seed >> 32) & MASK))
+
+
+def fjlt_gen(seed, epoch, s, n_pad, q):
+    """This build's stream for K = sprandn(s, n_pad, q) (fjlt.jl:183): entry
+    (i, j) nonzero iff word i of stream (seed, epoch, 6, j) is below
+    floor(q*2^32) (all for q = 1); the r-th nonzero of column j takes
+    Box–Muller pair r//2 of stream (seed, epoch, 7, j).  Returns the CSC
+    arrays (colptr, rowidx, val) with rows ascending.  Parity unpinned (a
+    random stream; see the module header)."""
+    thr = (1 << 32) if q >= 1.0 else int(math.floor(q * 4294967296.0))
+    colptr, rowidx, val = [0], [], []
+    for j in range(n_pad):
+        r = 0
+        for i in range(s):
+            w = _fjlt_block(seed, epoch, DOM_FJLT_ROWS, j, i >> 2)[i & 3]
+            if w >= thr:
+                continue
+            v = _fjlt_block(seed, epoch, DOM_FJLT_VALS, j, r >> 1)
+            u1 = ((v[0] >> 5) * 67108864.0 + (v[1] >> 6) + 0.5) * (1.0 / 9007199254740992.0)
+            u2 = ((v[2] >> 5) * 67108864.0 + (v[3] >> 6) + 0.5) * (1.0 / 9007199254740992.0)
+            rad = math.sqrt(-2.0 * math.log(u1))
+            a = 6.283185307179586 * u2
+            rowidx.append(i)
+            val.append(rad * math.sin(a) if r & 1 else rad * math.cos(a))
+            r += 1
+        colptr.append(len(rowidx))
+    return (np.array(colptr, dtype=np.int64), np.array(rowidx, dtype=np.int64),
+            np.array(val, dtype=np.float64))
+
+
+def csc_dense(nrows, ncols, colptr, rowidx, val):
+    M = np.zeros((nrows, ncols))
+    for j in range(ncols):
+        M[np.asarray(rowidx[colptr[j]:colptr[j + 1]]), j] = val[colptr[j]:colptr[j + 1]]
+    return M
+
+
+def fjlt_left_mul(A, K, signs, scale, dtype, C=None, alpha=1.0, beta=0.0):
+    """mul!(C, S::FJLTRecipe{Left}, A, alpha, beta) — fjlt.jl:257-319: zero
+    padding, fwht!(col, signs; scaling=scale) per column, then
+    mul!(Cv, S.op, padding, alpha, beta) in the SparseArrays loop order.
+    K is the dense-stored s x n_pad factor."""
+    A = np.asarray(A)
+    m = A.shape[0]
+    k = A.shape[1] if A.ndim == 2 else 1
+    n_pad = np.asarray(signs).shape[0]
+    P = np.zeros((n_pad, k), dtype=dtype)
+    P[:m] = A.reshape(m, k)
+    P = fwht(P, signs, scale, dtype)
+    return sparse_mul(K, P, alpha, beta, C, dtype)
+
+
+def fjlt_adjoint_mul(X, K, signs, scale, m, dtype, Y=None, alpha=1.0, beta=0.0):
+    """Y = alpha*T'*X + beta*Y with T = scale*K*H*D*pad — the A*S orientation
+    of a Left recipe (fjlt.jl:321-393) on A' (and S*A of a Right recipe,
+    fjlt.jl:470-535): P = alpha*K'*X (mul!(pv', Av, S.op, alpha, 0)),
+    fwht!(col; scaling=scale) without signs, rows multiplied by +-1, first m
+    rows, axpby!(1, P, beta, Y)."""
+    X = np.asarray(X)
+    s = X.shape[0]
+    k = X.shape[1] if X.ndim == 2 else 1
+    P = sparse_mul(np.asarray(K).T, X.reshape(s, k), alpha, 0.0, None, dtype)
+    P = fwht(P, None, scale, dtype)
+    sg = np.where(np.asarray(signs, dtype=bool), 1, -1).astype(dtype)
+    P = P * sg[:, None]
+    Yd = np.zeros((m, k)) if Y is None else np.asarray(Y, dtype=np.float64).reshape(m, k)
+    out = P[:m].astype(np.float64) if beta == 0.0 else P[:m].astype(np.float64) + beta * Yd
+    return out.astype(dtype)
 
 
 # ---------------------------------------------------------------------------

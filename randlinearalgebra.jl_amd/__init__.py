@@ -11,8 +11,8 @@ The directory name contains a dot, so load it with
 from ._lib import (AdmissionError, ArgumentError, DeviceError, DimensionMismatch,  # noqa: F401
                    DomainError)
 from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F401
-                          CompressorRecipe, CountSketch, CountSketchRecipe, Gaussian,
-                          GaussianRecipe, Left, Right, SRHT,
+                          CompressorRecipe, CountSketch, CountSketchRecipe, FJLT, FJLTRecipe,
+                          Gaussian, GaussianRecipe, Left, Right, SRHT,
                           SRHTRecipe, SparseSign, SparseSignRecipe, Undef, View, adjoint,
                           as_view, complete_compressor, fwht, left_mul_dimcheck, mul, mul_,
                           pack_bits, padded_size, right_mul_dimcheck, seed, times,

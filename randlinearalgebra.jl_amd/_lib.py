@@ -77,6 +77,14 @@ _SIGS = {
                              i64, i64, f64, f64],
     "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
                              i64, i64, f64, f64],
+    "rla_fjlt_gen_count": [vp, u64, u64, i64, i64, f64, vp, vp],
+    "rla_fjlt_gen_fill_f32": [vp, u64, u64, i64, i64, f64, vp, vp, vp],
+    "rla_fjlt_gen_fill_f64": [vp, u64, u64, i64, i64, f64, vp, vp, vp],
+    "rla_host_fjlt_gen_f64": [u64, u64, i64, i64, f64, vp, vp, vp, i64, vp],
+    "rla_fjlt_apply_f32": [vp, i32, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, f64, vp, i64,
+                           i64, vp, i64, i64, f64, f64],
+    "rla_fjlt_apply_f64": [vp, i32, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, f64, vp, i64,
+                           i64, vp, i64, i64, f64, f64],
     "rla_fwht_f32": [vp, i64, i64, vp, f64, vp, i64],
     "rla_fwht_f64": [vp, i64, i64, vp, f64, vp, i64],
     "rla_srht_apply_f32": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, vp, i64, i64, vp, i64,

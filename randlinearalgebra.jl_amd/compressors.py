@@ -649,6 +649,122 @@ class SRHTRecipe(CompressorRecipe):
         self._apply(1 if self.cardinality == Left() else 0, A.T, C.T, alpha, beta)
 
 
+# ---------------------------------------------------------------------------
+# FJLT
+
+
+class FJLT(Compressor):
+    """FJLT(cardinality=Left(), compression_dim=2, block_size=10, sparsity=0.0,
+    type=Float64) — src/Compressors/fjlt.jl:65-107."""
+
+    def __init__(self, cardinality=None, compression_dim=2, block_size=10, sparsity=0.0,
+                 type=torch.float64):
+        cardinality = Left() if cardinality is None else cardinality
+        if compression_dim <= 0:
+            raise ArgumentError("Field `compression_dim` must be positive.")
+        if sparsity < 0 or sparsity > 1:
+            raise ArgumentError("Field `sparsity` must be between 0 and 1.")
+        if block_size <= 0:
+            raise ArgumentError("Field `block_size` must be positive.")
+        if cardinality == Undef():
+            raise ArgumentError("Cardinality must be of type `Left` or `Right`.")
+        self.cardinality = cardinality
+        self.compression_dim = int(compression_dim)
+        self.block_size = int(block_size)
+        self.sparsity = float(sparsity)
+        self.type = _dtype(type)
+
+    def _complete(self, *args):
+        A = args[0] if len(args) <= 2 else args[1]
+        return FJLTRecipe(self.compression_dim, self.block_size, self.cardinality,
+                          self.sparsity, A, self.type)
+
+
+class FJLTRecipe(CompressorRecipe):
+    """fjlt.jl:147-251.  Left: S = scale*K*H*D*pad is s x m with K =
+    sprandn(s, n_pad, q); Right: S = D*H*K*scale is n x s with K =
+    sprandn(n_pad, s, q).  Sparsity 0.0 selects q = 0.25*log(dim)^2/dim
+    (fjlt.jl:181, 223), scale = type(1/(sqrt(n_pad)*sqrt(q)*sqrt(s))).
+    `op` holds the s x n_pad factor M (M = K for Left, M = K' for Right; both
+    are sprandn laws), `signs` the BitVector chunks.  block_size is kept for
+    parity of the recipe fields: the kernels chunk columns by workspace size."""
+
+    def __init__(self, compression_dim, block_size, cardinality, sparsity, A, type_):
+        a_rows, a_cols = _size2(A)
+        self.cardinality = cardinality
+        self.block_size = block_size
+        if cardinality == Left():
+            self.n_rows, self.n_cols = compression_dim, a_rows
+            self.dim = a_rows
+        else:
+            self.n_rows, self.n_cols = a_cols, compression_dim
+            self.dim = a_cols
+        self.n_pad = padded_size(self.dim)
+        self.s = compression_dim
+        self.type = type_
+        self.sparsity = (0.25 * math.log(self.dim) ** 2 / self.dim) if sparsity == 0.0 \
+            else float(sparsity)
+        q = self.sparsity
+        inv = 1.0 / (math.sqrt(self.n_pad) * math.sqrt(q) * math.sqrt(compression_dim)) \
+            if q > 0 else math.inf
+        self.scale = float(torch.tensor(inv, dtype=type_))
+        self._dev = _device_of(A)
+        self.signs = torch.empty((self.n_pad + 63) // 64, dtype=torch.int64, device=self._dev)
+        self.op = None
+        self._gen()
+
+    def _gen(self):
+        sd, ep = _draw()
+        self.seed, self.epoch = sd, ep
+        dev = self._dev
+        ctx = _lib.ctx(dev)
+        colptr = torch.empty(self.n_pad + 1, dtype=torch.int32, device=dev)
+        nnz = ctypes.c_int64(0)
+        _lib.call("rla_fjlt_gen_count", ctx, sd, ep, self.s, self.n_pad, self.sparsity,
+                  _lib.ptr(colptr), ctypes.byref(nnz))
+        rowidx = torch.empty(max(1, nnz.value), dtype=torch.int32, device=dev)[: nnz.value]
+        val = torch.empty(max(1, nnz.value), dtype=self.type, device=dev)[: nnz.value]
+        _lib.call(f"rla_fjlt_gen_fill_{_lib.sfx(self.type)}", ctx, sd, ep, self.s, self.n_pad,
+                  self.sparsity, _lib.ptr(colptr), _lib.ptr(rowidx), _lib.ptr(val))
+        _lib.call("rla_srht_gen", ctx, sd, ep, self.n_pad, self.s, None, _lib.ptr(self.signs))
+        self.op = _SparseOp(self.s, self.n_pad, colptr, rowidx, val)
+
+    def _update(self, *args):
+        # update_compressor! (fjlt.jl:239-247): a new K with the same sparsity, new signs
+        self._gen()
+
+    def sign_vector(self):
+        """The signs as a host bool array of length n_pad (BitVector order)."""
+        import numpy as np
+        ch = self.signs.cpu().numpy().view(np.uint64)
+        bits = ((ch[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(bool)
+        return bits.reshape(-1)[: self.n_pad]
+
+    def K(self):
+        """K as the reference's recipe holds it, a dense host float64 array
+        (s x n_pad for Left, n_pad x s for Right)."""
+        M = self.op.dense()
+        return M if self.cardinality == Left() else M.T
+
+    def _apply(self, adjoint, X, Y, alpha, beta):
+        dt = X.dtype
+        op = self.op
+        val, val_t = op._vals_for(dt)
+        _lib.call(f"rla_fjlt_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(adjoint), self.dim,
+                  self.n_pad, self.s, X.cols, _lib.ptr(op.colptr), _lib.ptr(op.rowidx),
+                  _lib.ptr(val), _lib.ptr(op.rowptr), _lib.ptr(op.colidx), _lib.ptr(val_t),
+                  _lib.ptr(self.signs), self.scale, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t),
+                  Y.rs, Y.cs, alpha, beta)
+
+    def _left(self, C, A, alpha, beta):
+        # Left S*A: T*A (fjlt.jl:257-319); Right S*A: T'*A (fjlt.jl:470-535)
+        self._apply(0 if self.cardinality == Left() else 1, A, C, alpha, beta)
+
+    def _right(self, C, A, alpha, beta):
+        # Left A*S = (T'*A')' (fjlt.jl:321-393); Right A*S = (T*A')' (fjlt.jl:395-468)
+        self._apply(1 if self.cardinality == Left() else 0, A.T, C.T, alpha, beta)
+
+
 def fwht(x, signs=None, scaling=1.0):
     """fwht!(x, signs; scaling) / fwht!(x; scaling) on each column of a device
     matrix (or a vector) — src/Compressors/helpers/fwht.jl:21-118.  `signs` is

@@ -278,11 +278,105 @@ static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64
   return RLA_OK;
 }
 
+template <class T>
+static int sparse_apply_t(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
+                          const int32_t* colptr, const int32_t* rowidx, const T* val,
+                          const int32_t* rowptr, const int32_t* colidx, const T* val_t,
+                          int64_t n_rhs, const T* X, int64_t x_rs, int64_t x_cs, T* C,
+                          int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  if constexpr (sizeof(T) == 4)
+    return rla_sparse_apply_f32(ctx, trans_m, nrows, ncols, colptr, rowidx, val, rowptr, colidx,
+                                val_t, n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+  else
+    return rla_sparse_apply_f64(ctx, trans_m, nrows, ncols, colptr, rowidx, val, rowptr, colidx,
+                                val_t, n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+// FJLT (fjlt.jl:257-535) with T = scale * K * H * D * pad, an s x m operator
+// (K: s x n_pad, CSC (colptr, rowidx, val) and CSR (rowptr, colidx, val_t)):
+//   forward (adjoint = 0), Y = alpha * T * X + beta * Y: the FWHT passes (signs
+//     and scale on the first stage, as fwht!(x, signs; scaling)) into the
+//     padded workspace, then the CSR row-gather product with K
+//     (mul!(Cv, S.op, S.padding, alpha, beta), fjlt.jl:284-296);
+//   adjoint (adjoint = 1), Y = alpha * T' * X + beta * Y: W = alpha * K' * X
+//     (mul!(pv', Av, S.op, alpha, 0), fjlt.jl:341), the FWHT passes with the
+//     scale only, then the sign flip and Y = (D W)[0:m] + beta * Y
+//     (fjlt.jl:348-357).
+// Columns are processed in chunks that bound the padded workspace, in place
+// of the reference's block_size column blocks.
+template <class T>
+static int fjlt_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                      int64_t ncols, const int32_t* colptr, const int32_t* rowidx, const T* val,
+                      const int32_t* rowptr, const int32_t* colidx, const T* val_t,
+                      const uint64_t* signs, double scale, const T* X, int64_t x_rs,
+                      int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                      double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (m < 1 || m > n_pad || s < 1 || ncols < 0)
+    return fail(RLA_EARG, "FJLT sizes must satisfy 1 <= m <= n_pad and s >= 1");
+  if (ncols == 0) return RLA_OK;
+  if (!colptr || !rowidx || !val || !rowptr || !colidx || !val_t || !signs || !X || !Y)
+    return fail(RLA_EARG, "FJLT apply: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int L = ilog2(n_pad);
+  const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
+  void* ws = nullptr;
+  rc = workspace(ctx, static_cast<size_t>(n_pad * cc) * sizeof(T), &ws);
+  if (rc) return rc;
+  T* P = static_cast<T*>(ws);
+  for (int64_t k0 = 0; k0 < ncols; k0 += cc) {
+    const int64_t kc = std::min(cc, ncols - k0);
+    const T* Xk = X + k0 * x_cs;
+    T* Yk = Y + k0 * y_cs;
+    if (!adjoint) {
+      rc = fwht_passes<T>(ctx, Xk, x_rs, x_cs, m, P, n_pad, kc, L, signs, scale);
+      if (rc) return rc;
+      rc = sparse_apply_t<T>(ctx, 0, s, n_pad, colptr, rowidx, val, rowptr, colidx, val_t, kc,
+                             P, 1, n_pad, Yk, y_rs, y_cs, alpha, beta);
+      if (rc) return rc;
+    } else {
+      rc = sparse_apply_t<T>(ctx, 1, s, n_pad, colptr, rowidx, val, rowptr, colidx, val_t, kc,
+                             Xk, x_rs, x_cs, P, 1, n_pad, alpha, 0.0);
+      if (rc) return rc;
+      rc = fwht_passes<T>(ctx, nullptr, 0, 0, n_pad, P, n_pad, kc, L, nullptr, scale);
+      if (rc) return rc;
+      srht_adjoint_epilogue_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(m * kc, 256))), 256,
+                                        0, ctx->stream>>>(P, n_pad, m, kc, signs, Yk, y_rs, y_cs,
+                                                          beta);
+      RLA_LAUNCH_CHECK("fjlt_adjoint");
+    }
+  }
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
 
 extern "C" {
+
+int rla_fjlt_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* colptr, const int32_t* rowidx,
+                       const float* val, const int32_t* rowptr, const int32_t* colidx,
+                       const float* val_t, const uint64_t* sign_chunks, double scale,
+                       const float* X, int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs,
+                       int64_t y_cs, double alpha, double beta) {
+  return fjlt_apply<float>(ctx, adjoint, m, n_pad, s, ncols, colptr, rowidx, val, rowptr, colidx,
+                           val_t, sign_chunks, scale, X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_fjlt_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                       int64_t ncols, const int32_t* colptr, const int32_t* rowidx,
+                       const double* val, const int32_t* rowptr, const int32_t* colidx,
+                       const double* val_t, const uint64_t* sign_chunks, double scale,
+                       const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
+                       int64_t y_cs, double alpha, double beta) {
+  return fjlt_apply<double>(ctx, adjoint, m, n_pad, s, ncols, colptr, rowidx, val, rowptr,
+                            colidx, val_t, sign_chunks, scale, X, x_rs, x_cs, Y, y_rs, y_cs,
+                            alpha, beta);
+}
 
 int rla_fwht_f32(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_chunks,
                  double scaling, float* X, int64_t ldx) {

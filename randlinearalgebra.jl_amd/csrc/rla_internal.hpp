@@ -73,6 +73,9 @@ int use_device(rla_ctx* ctx);
 // Growth synchronises the context stream before freeing the old block.
 int workspace(rla_ctx* ctx, size_t bytes, void** out);
 
+// In-place inclusive prefix sum of a[0..n) on the context's stream.
+int scan_inclusive_i32(rla_ctx* ctx, int32_t* a, int64_t n);
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }

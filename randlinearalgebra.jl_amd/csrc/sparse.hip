@@ -70,6 +70,13 @@ __global__ __launch_bounds__(1024) void scan_inclusive_kernel(int32_t* a, int64_
   }
 }
 
+int scan_inclusive_i32(rla_ctx* ctx, int32_t* a, int64_t n) {
+  if (n <= 0) return RLA_OK;
+  scan_inclusive_kernel<<<1, 1024, 0, ctx->stream>>>(a, n);
+  RLA_LAUNCH_CHECK("scan_inclusive");
+  return RLA_OK;
+}
+
 template <class T>
 __global__ void fill_csr_kernel(int64_t ncols, const int32_t* __restrict__ colptr,
                                 const int32_t* __restrict__ rowidx, const T* __restrict__ val,

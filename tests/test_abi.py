@@ -2,6 +2,7 @@
 of the recipe generators match the Python restatement bit for bit; host-side
 validation mirrors the reference's errors.  No GPU needed."""
 import ctypes
+import math
 import os
 import re
 
@@ -149,3 +150,51 @@ def test_dimension_checks_mirror_reference_messages(rla):
 def test_mul_without_recipe_is_argument_error(rla):
     with pytest.raises(rla.ArgumentError):
         rla.mul(torch.zeros(2, 2), torch.zeros(2, 2), torch.zeros(2, 2))
+
+
+def _host_fjlt(lib, seed, epoch, s, n_pad, q):
+    cap = s * n_pad
+    cp = np.zeros(n_pad + 1, np.int32)
+    ri = np.zeros(cap, np.int32)
+    va = np.zeros(cap)
+    nnz = ctypes.c_int64(0)
+    assert lib.rla_host_fjlt_gen_f64(seed, epoch, s, n_pad, q, cp.ctypes.data, ri.ctypes.data,
+                                     va.ctypes.data, cap, ctypes.byref(nnz)) == 0
+    return cp, ri[: nnz.value], va[: nnz.value]
+
+
+@pytest.mark.parametrize("s,n_pad,q", [(12, 32, 0.3), (7, 8, 1.0), (40, 16, 0.05), (3, 4, 0.0)])
+def test_host_fjlt_gen_matches_restatement(rla, s, n_pad, q):
+    cp, ri, va = _host_fjlt(rla._lib.load(), 21, 8, s, n_pad, q)
+    rcp, rri, rva = R.fjlt_gen(21, 8, s, n_pad, q)
+    assert np.array_equal(cp, rcp) and np.array_equal(ri, rri)
+    assert np.allclose(va, rva, rtol=1e-15, atol=1e-15)
+
+
+def test_host_fjlt_gen_law(rla):
+    # entries independently nonzero with probability q, values N(0, 1)
+    s, n_pad, q = 256, 4096, 0.01
+    cp, ri, va = _host_fjlt(rla._lib.load(), 3, 1, s, n_pad, q)
+    N = s * n_pad
+    nnz = cp[-1]
+    assert abs(nnz - N * q) < 5 * math.sqrt(N * q * (1 - q))
+    assert abs(va.mean()) < 5 / math.sqrt(nnz) and abs(va.var() - 1) < 5 * math.sqrt(2 / nnz)
+    rows = np.bincount(ri, minlength=s)          # each ~ Binomial(n_pad, q)
+    z = (rows - n_pad * q) / math.sqrt(n_pad * q * (1 - q))
+    assert np.abs(z).max() < 5 and abs(z.mean()) < 5 / math.sqrt(s)
+    cols = np.diff(cp)                           # each ~ Binomial(s, q)
+    zc = (cols.mean() - s * q) / math.sqrt(s * q * (1 - q) / n_pad)
+    assert abs(zc) < 5
+
+
+def test_fjlt_constructor_errors(rla):
+    with pytest.raises(rla.ArgumentError, match="compression_dim"):
+        rla.FJLT(compression_dim=0)
+    with pytest.raises(rla.ArgumentError, match="sparsity"):
+        rla.FJLT(sparsity=1.5)
+    with pytest.raises(rla.ArgumentError, match="block_size"):
+        rla.FJLT(block_size=0)
+    with pytest.raises(rla.ArgumentError, match="Cardinality"):
+        rla.FJLT(cardinality=rla.Undef())
+    f = rla.FJLT()
+    assert (f.compression_dim, f.block_size, f.sparsity, f.type) == (2, 10, 0.0, torch.float64)

@@ -295,3 +295,118 @@ def test_mul_dimension_mismatch_on_device(rla, dev):
     S = rla.complete_compressor(rla.SparseSign(compression_dim=20), A)
     with pytest.raises(rla.DimensionMismatch, match="Matrix C has 19 rows while S has 20 rows."):
         rla.mul(torch.zeros(19, 3, dtype=F64, device=dev), S, A)
+
+
+# ---- FJLT ------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("s,n_pad,q", [(2048, 65536, 4.7e-4), (16, 64, 0.3), (5, 8, 1.0),
+                                       (300, 1024, 0.0)])
+def test_device_fjlt_gen_equals_host(rla, dev, s, n_pad, q):
+    import ctypes
+    lib = rla._lib
+    colptr = torch.empty(n_pad + 1, dtype=torch.int32, device=dev)
+    nnz = ctypes.c_int64(0)
+    lib.call("rla_fjlt_gen_count", lib.ctx(dev), 99, 4, s, n_pad, q, lib.ptr(colptr),
+             ctypes.byref(nnz))
+    k = nnz.value
+    rowidx = torch.empty(max(k, 1), dtype=torch.int32, device=dev)
+    val = torch.empty(max(k, 1), dtype=F64, device=dev)
+    lib.call("rla_fjlt_gen_fill_f64", lib.ctx(dev), 99, 4, s, n_pad, q, lib.ptr(colptr),
+             lib.ptr(rowidx), lib.ptr(val))
+    cap = max(k, 1)
+    hcp = np.zeros(n_pad + 1, np.int32)
+    hri = np.zeros(cap, np.int32)
+    hva = np.zeros(cap)
+    hn = ctypes.c_int64(0)
+    assert lib.load().rla_host_fjlt_gen_f64(99, 4, s, n_pad, q, hcp.ctypes.data, hri.ctypes.data,
+                                            hva.ctypes.data, cap, ctypes.byref(hn)) == 0
+    assert hn.value == k
+    assert np.array_equal(colptr.cpu().numpy(), hcp)
+    assert np.array_equal(rowidx.cpu().numpy()[:k], hri[:k])        # bit for bit
+    assert np.allclose(val.cpu().numpy()[:k], hva[:k], rtol=1e-14, atol=1e-14)
+    if q > 0:
+        assert abs(k - s * n_pad * q) < 6 * math.sqrt(s * n_pad * q)
+
+
+def _fjlt_T(S, m):
+    """The s x m operator T = scale*K*H*D*pad of a recipe (M = K or K'), float64."""
+    M = S.op.dense()
+    return R.fjlt_left_mul(np.eye(m), M, S.sign_vector(), S.scale, np.float64)
+
+
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+@pytest.mark.parametrize("m", [256, 300])
+def test_fjlt_all_orientations(rla, dev, dtype, tol, m):
+    rla.seed(31)
+    n, s = 5, 40
+    A = rnd((m, n), 1, NP[dtype])
+    SL = rla.complete_compressor(rla.FJLT(compression_dim=s, sparsity=0.1, type=dtype),
+                                 colmajor(A, dev))
+    assert SL.size() == (s, m) and SL.n_pad == R.padded_size(m)
+    T = _fjlt_T(SL, m)
+    M = SL.op.dense()
+    sg = SL.sign_vector()
+    C0 = rnd((s, n), 7, NP[dtype])
+    C = colmajor(C0, dev)
+    rla.mul(C, SL, colmajor(A, dev), 2.0, 0.5)                        # Left S*A
+    want = R.fjlt_left_mul(A, M, sg, SL.scale, NP[dtype], C0, 2.0, 0.5)
+    assert np.max(np.abs(C.cpu().numpy() - want)) <= tol * np.max(np.abs(want))
+    X = rnd((7, s), 3, NP[dtype])
+    C1 = torch.zeros(7, m, dtype=dtype, device=dev)
+    rla.mul(C1, colmajor(X, dev), SL)                                  # Left A*S
+    want1 = R.fjlt_adjoint_mul(X.T, M, sg, SL.scale, m, NP[dtype]).T
+    assert np.max(np.abs(C1.cpu().numpy() - want1)) <= tol * np.max(np.abs(want1))
+    assert np.max(np.abs(want1 - X @ T)) <= 1e-5 * np.max(np.abs(want1))
+    Y = rnd((s, 4), 4, NP[dtype])
+    C2 = torch.zeros(m, 4, dtype=dtype, device=dev)
+    rla.mul(C2, SL.T, colmajor(Y, dev))                                # S'*Y
+    assert np.max(np.abs(C2.cpu().numpy() - T.T @ Y)) <= tol * np.max(np.abs(T.T @ Y)) * 10
+    # Right recipe: S = D H K scale is m x s (K = M')
+    A2 = rnd((9, m), 5, NP[dtype])
+    SR = rla.complete_compressor(rla.FJLT(cardinality=rla.Right(), compression_dim=s,
+                                          sparsity=0.1, type=dtype), colmajor(A2, dev))
+    assert SR.size() == (m, s)
+    assert np.array_equal(SR.K(), SR.op.dense().T)
+    TR = _fjlt_T(SR, m)                                                 # S = TR'
+    C4 = torch.zeros(9, s, dtype=dtype, device=dev)
+    rla.mul(C4, colmajor(A2, dev), SR)                                  # Right A*S
+    want4 = R.fjlt_left_mul(A2.T, SR.op.dense(), SR.sign_vector(), SR.scale, NP[dtype]).T
+    assert np.max(np.abs(C4.cpu().numpy() - want4)) <= tol * np.max(np.abs(want4))
+    Z = rnd((s, 3), 6, NP[dtype])
+    C5 = torch.zeros(m, 3, dtype=dtype, device=dev)
+    rla.mul(C5, SR, colmajor(Z, dev))                                   # Right S*A
+    want5 = R.fjlt_adjoint_mul(Z, SR.op.dense(), SR.sign_vector(), SR.scale, m, NP[dtype])
+    assert np.max(np.abs(C5.cpu().numpy() - want5)) <= tol * np.max(np.abs(want5))
+    assert np.max(np.abs(want5 - TR.T @ Z)) <= 1e-5 * np.max(np.abs(want5))
+
+
+def test_fjlt_defaults_update_and_vector(rla, dev):
+    rla.seed(5)
+    m = 1000
+    A = colmajor(rnd((m, 3), 2), dev)
+    S = rla.complete_compressor(rla.FJLT(compression_dim=64), A)
+    q = 0.25 * math.log(m) ** 2 / m                                    # fjlt.jl:181
+    assert abs(S.sparsity - q) < 1e-15
+    assert S.scale == 1.0 / (math.sqrt(1024) * math.sqrt(q) * math.sqrt(64))
+    op0, sg0 = S.op.dense().copy(), S.signs.clone()
+    rla.update_compressor(S)
+    assert not np.array_equal(op0, S.op.dense()) and not torch.equal(sg0, S.signs)
+    assert abs(S.sparsity - q) < 1e-15
+    x = rnd(m, 3)
+    y = torch.zeros(64, dtype=F64, device=dev)
+    rla.mul(y, S, torch.as_tensor(x).to(dev))
+    want = R.fjlt_left_mul(x, S.op.dense(), S.sign_vector(), S.scale, np.float64).ravel()
+    assert np.allclose(y.cpu().numpy(), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_fjlt_preserves_norms_on_average(rla, dev):
+    # E||S x||^2 = ||x||^2: scale = 1/(sqrt(n_pad) sqrt(q) sqrt(s)) with N(0,1) K entries
+    rla.seed(17)
+    m, s, k = 4096, 512, 64
+    X = rnd((m, k), 8)
+    S = rla.complete_compressor(rla.FJLT(compression_dim=s), colmajor(X, dev))
+    C = torch.zeros(s, k, dtype=F64, device=dev)
+    rla.mul(C, S, colmajor(X, dev))
+    ratio = (C.cpu().numpy() ** 2).sum(0) / (X ** 2).sum(0)
+    assert abs(ratio.mean() - 1.0) < 0.1 and ratio.min() > 0.5

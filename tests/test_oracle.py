@@ -302,3 +302,62 @@ def test_successive_inclusion_closed_forms():
 def test_full_residual_closed_form():
     A = np.array([[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]])
     assert R.full_residual(A, [1.0, 1.0], [3.0, 7.0, 2.0]) == 1.0
+
+
+# ---------------------------------------------------------------------------
+# FJLT (fjlt.jl): pinned to the closed forms the reference's own tests use
+# (test/Compressors/fjlt.jl:343,371,402,428,460) with scipy's dense Hadamard
+
+
+def _fjlt_case(m, s, q, seed):
+    from scipy.linalg import hadamard
+    n_pad = R.padded_size(m)
+    cp, ri, va = R.fjlt_gen(seed, 1, s, n_pad, q)
+    K = R.csc_dense(s, n_pad, cp, ri, va)
+    rng = np.random.default_rng(seed)
+    sg = rng.random(n_pad) < 0.5
+    scale = 1.0 / (math.sqrt(n_pad) * math.sqrt(q) * math.sqrt(s))
+    return K, sg, scale, hadamard(n_pad).astype(np.float64), n_pad
+
+
+@pytest.mark.parametrize("m,s", [(8, 2), (10, 2), (37, 5), (64, 9)])
+def test_fjlt_left_matches_reference_test_formula(m, s):
+    K, sg, scale, H, n_pad = _fjlt_case(m, s, 0.4, 11 + m)
+    A = np.random.default_rng(m).standard_normal((m, 6))
+    C0 = np.random.default_rng(m + 1).standard_normal((s, 6))
+    pad = np.zeros((n_pad, 6))
+    pad[:m] = A
+    # SA = SAc + S.op * (H * (ifelse.(S.signs, 1, -1) .* pad) * 2 * S.scale)
+    want = C0 + K @ (H @ (np.where(sg, 1, -1)[:, None] * pad) * 2 * scale)
+    got = R.fjlt_left_mul(A, K, sg, scale, np.float64, C0, 2.0, 1.0)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("m,s", [(8, 2), (10, 3), (33, 4)])
+def test_fjlt_adjoint_matches_reference_test_formula(m, s):
+    K, sg, scale, H, n_pad = _fjlt_case(m, s, 0.5, 5 + m)
+    A = np.random.default_rng(m).standard_normal((7, s))
+    C0 = np.random.default_rng(m + 2).standard_normal((7, m))
+    # SA = SAc + 2 * (A * S.op * H * Diagonal(ifelse.(S.signs, 1, -1)) * S.scale)[:, 1:m]
+    want = C0 + 2 * (A @ K @ H @ np.diag(np.where(sg, 1.0, -1.0)) * scale)[:, :m]
+    got = R.fjlt_adjoint_mul(A.T, K, sg, scale, m, np.float64, C0.T, 2.0, 1.0).T
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    # and the adjoint is the transpose of the forward operator
+    T = R.fjlt_left_mul(np.eye(m), K, sg, scale, np.float64)
+    X = np.random.default_rng(3).standard_normal((s, 4))
+    assert np.allclose(R.fjlt_adjoint_mul(X, K, sg, scale, m, np.float64), T.T @ X,
+                       rtol=1e-12, atol=1e-13)
+
+
+def test_fjlt_gen_structure():
+    cp, ri, va = R.fjlt_gen(9, 4, 12, 32, 0.3)
+    assert cp[0] == 0 and np.all(np.diff(cp) >= 0) and cp[-1] == len(ri) == len(va)
+    for j in range(32):
+        rows = ri[cp[j]:cp[j + 1]]
+        assert np.all(np.diff(rows) > 0) and np.all((rows >= 0) & (rows < 12))
+    # q = 1: every entry; q = 0: none; a pure function of (seed, epoch)
+    cp1, ri1, _ = R.fjlt_gen(9, 4, 5, 8, 1.0)
+    assert cp1[-1] == 40 and np.array_equal(ri1, np.tile(np.arange(5), 8))
+    assert R.fjlt_gen(9, 4, 5, 8, 0.0)[0][-1] == 0
+    assert np.array_equal(R.fjlt_gen(9, 4, 12, 32, 0.3)[1], ri)
+    assert not np.array_equal(R.fjlt_gen(9, 5, 12, 32, 0.3)[1], ri)

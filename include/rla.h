@@ -350,8 +350,10 @@ int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const floa
  * evaluated at i = 1, 1+every, 1+2*every, ... and at max_it, recorded in hist
  * and tested against threshold (BasicLogger threshold_stop; i == max_it always
  * stops, basic_logger.jl:93-98); otherwise a new row set is drawn and x is
- * projected.  A: column-major (lda) for the residual; A_rm: the same matrix
- * row-major (ld_rm) for the block gathers.  x is updated in place.  s <= 64. */
+ * projected.  A_rm: the matrix row-major (ld_rm), read by the block gathers
+ * and the residual; A: the same matrix column-major (lda), only checked.
+ * One launch per iteration (the previous block's update fused with the new
+ * block's Gram matrix and solve).  x is updated in place.  s <= 64. */
 int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, int64_t lda,
                            const float* A_rm, int64_t ld_rm, const float* b, float* x, int64_t s,
                            double alpha, const double* w, int replace, uint64_t seed,

@@ -7,12 +7,15 @@
 // their MFMA accumulator registers for the whole factorisation (at most
 // 18 x 19 / 2 blocks, 25 per wavefront), so the k^3/3 flops of the trailing
 // updates touch no memory.  Block step s:
-//   1. the owner of the diagonal block G_ss copies it to LDS,
-//   2. the eighth wavefront factors it in registers (lane c holds column c;
-//      each row of R_ss is passed through LDS) and inverts R_ss,
-//   3. the owners of G_sj (j > s) form R_sj = inv(R_ss)' G_sj on the MFMA
+//   1. the eighth wavefront factors the diagonal block G_ss (in LDS) in
+//      registers (lane c holds column c; each row of R_ss is passed by
+//      register broadcast) and inverts R_ss, while the block wavefronts
+//      finish the trailing update of step s-1,
+//   2. the owners of G_sj (j > s) form R_sj = inv(R_ss)' G_sj on the MFMA
 //      and publish it in LDS,
-//   4. every wavefront subtracts R_si' R_sj from its trailing blocks (MFMA).
+//   3. the owner of G_{s+1,s+1} subtracts R_{s,s+1}' R_{s,s+1} from it first
+//      and copies it to LDS (look-ahead: step s+1's factor starts at once);
+//      the other trailing blocks are updated during that factor.
 // R is written column-major and row-major (the solve's operand layout), with
 // the inverses of the diagonal blocks for the solve.
 //
@@ -77,10 +80,17 @@ struct UpperBlocks {
   }
 };
 
-constexpr int kCholBlockWaves = kCholWaves - 1;  // the last wavefront factors
+// Workgroup shape: 512 threads, seven block wavefronts and the factoring one
+// (larger workgroups lower the VGPRs per wavefront and spill more).
+template <int NBLK>
+struct CholShape {
+  static constexpr int threads = 512;
+  static constexpr int block_waves = threads / 64 - 1;
+};
 
 // The trailing-matrix role of block wavefront W: its blocks are
-// W, W + 7, W + 14, ... of the column-by-column enumeration.  W is a template
+// W, W + BW, W + 2 BW, ... of the column-by-column enumeration (BW block
+// wavefronts).  W is a template
 // parameter so every block coordinate and LDS offset is a compile-time
 // constant (no address or index registers per block).
 template <int NBLK, int W>
@@ -90,7 +100,7 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
                                                 double (*P)[NBLK * 16 + 1], double (*Ds)[17],
                                                 double (*Dv)[17], volatile int* fail, int t) {
   constexpr int T = NBLK * (NBLK + 1) / 2;
-  constexpr int BW = kCholBlockWaves;
+  constexpr int BW = CholShape<NBLK>::block_waves;
   constexpr int MAXB = (T - W + BW - 1) / BW;  // blocks of this wavefront
   constexpr UpperBlocks<NBLK> tab{};
   const int lane = t & 63, lr = lane >> 4, lc = lane & 15;
@@ -116,25 +126,48 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
       if ((i >> 4) > (j >> 4)) R[i + j * ldr] = 0.0;
     }
   }
-  __syncthreads();
+  __syncthreads();  // B0: G read into registers, R zeroed
   double* pl = &P[lr][lc];  // this lane's element of a 4 x 16 slab (rows NBLK*16+1 apart)
   constexpr int SR = NBLK * 16 + 1;
+  // Barriers per step s: B1 (G_ss in LDS), B2 (R_ss and its inverse in LDS),
+  // B3 (panel row s in LDS).  Look-ahead: after B3 the owner of G_{s+1,s+1}
+  // updates that block first and publishes it (the next step's B1), so the
+  // factor wavefront works on it while the remaining trailing blocks are
+  // updated.
+  auto update = [&](int u, int bi, int bj) {
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = pl[4 * q * SR + bi * 16];
+      b[q] = pl[4 * q * SR + bj * 16];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[q], b[q], acc[u], 0, 0, 0);
+  };
+  // G_00 to LDS
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u)
+    if (tab.bi[W + BW * u] == 0 && tab.bj[W + BW * u] == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ds[lr + 4 * r][lc] = acc[u][r];
+    }
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
     if (js >= k) break;
-    // 1. the diagonal block to LDS
+    __syncthreads();  // B1: G_ss in LDS; the factor wavefront starts
+    if (s > 0) {  // the rest of step s-1's trailing update
 #pragma unroll
-    for (int u = 0; u < MAXB; ++u)
-      if (tab.bi[W + BW * u] == s && tab.bj[W + BW * u] == s) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Ds[lr + 4 * r][lc] = acc[u][r];
+      for (int u = 0; u < MAXB; ++u) {
+        const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+        if (bi > s - 1 && !(bi == s && bj == s)) update(u, bi, bj);
       }
-    __syncthreads();
-    __syncthreads();  // 2. factored by the last wavefront
+    }
+    __syncthreads();  // B2: factored by the last wavefront
     if (*fail) break;
-    // 3. panel row R_sj = inv(R_ss)' G_sj by its owners (MFMA); the
-    //    accumulator layout of G_sj is the operand layout the product needs.
-    //    The owner of G_ss stores R_ss and inv(R_ss).
+    // panel row R_sj = inv(R_ss)' G_sj by its owners (MFMA); the
+    // accumulator layout of G_sj is the operand layout the product needs.
+    // The owner of G_ss stores R_ss and inv(R_ss).
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
       const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
@@ -167,21 +200,15 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
         }
       }
     }
-    __syncthreads();
-    // 4. trailing update of the register-resident blocks
+    __syncthreads();  // B3: panel row s in LDS
+    // G_{s+1,s+1} first, to LDS for the next B1 (only diagonal blocks qualify)
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
       const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
-      if (bi > s) {
-        double a[4], b[4];
+      if (bi == bj && bi == s + 1) {
+        update(u, bi, bj);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          a[q] = pl[4 * q * SR + bi * 16];
-          b[q] = pl[4 * q * SR + bj * 16];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[q], b[q], acc[u], 0, 0, 0);
+        for (int r = 0; r < 4; ++r) Ds[lr + 4 * r][lc] = acc[u][r];
       }
     }
   }
@@ -189,28 +216,35 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
 
 template <int NBLK>
 __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds)[17],
-                                                 double (*Dv)[17], double* rowj, const double* fl,
+                                                 double (*Dv)[17], const double* fl, double* rdl,
                                                  volatile int* fail, int lane) {
-  __syncthreads();  // G read into registers, R zeroed
+  __syncthreads();  // B0: G read into registers, R zeroed
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
     if (js >= k) break;
     const int nb = min(16, k - js);
-    __syncthreads();  // 1. G_ss in LDS
+    __syncthreads();  // B1: G_ss in LDS
     // 2. factor in registers, lane c holding column c (lanes 16..63 repeat
-    //    lanes 0..15 and write nothing), then invert R_ss the same way
+    //    lanes 0..15 and write nothing).  Step j: the pivot is lane j's entry
+    //    j; row j of R is entry j of every lane, passed by register broadcast
+    //    (the lanes of one wavefront need no LDS round trip).  Past nb the
+    //    block is padded with the identity, so the fully unrolled loop has no
+    //    data-dependent exits.
     const int c = lane & 15;
-    // Lane c keeps column c in registers (lanes 16..63 repeat lanes 0..15 and
-    // write nothing).  Step j: the pivot comes from lane j, row j of R is
-    // published in LDS (rowj), and every lane updates its column with it.
-    // Past nb the block is padded with the identity, so the fully unrolled
-    // loop has no data-dependent exits.
     double col[16];
+    if (nb == 16) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const double g = Ds[i][c];
-      col[i] = (i < nb && c < nb) ? g : (i == c ? 1.0 : 0.0);
+      for (int i = 0; i < 16; ++i) col[i] = Ds[i][c];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const double g = Ds[i][c];
+        col[i] = (i < nb && c < nb) ? g : (i == c ? 1.0 : 0.0);
+      }
     }
+    // Entries below the diagonal (col[i], i > c) are updated like the others
+    // but never read: pivots and broadcasts use entries on or above it, and
+    // R_ss is written masked.
     int bad = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -219,41 +253,45 @@ __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds
       const bool ok = (dj > floor_j) && (dj > 0.0) && isfinite(dj);
       if (!bad && !ok && js + j < check_k) bad = js + j + 1;
       if (bad || !ok) dj = 1.0;  // unchecked columns past check_k continue
-      const double d = sqrt(dj), rd = 1.0 / d;
-      col[j] = c < j ? 0.0 : (c == j ? d : col[j] * rd);
-      if (lane < 16) rowj[c] = col[j];
-      lane_exchange();
+      // 1/sqrt(dj): reciprocal square-root estimate refined by two Newton
+      // steps r += r (1 - dj r^2) / 2; then d = dj r with one correction
+      double rd = __builtin_amdgcn_rsq(dj);
+      rd = fma(0.5 * rd, fma(-dj * rd, rd, 1.0), rd);
+      rd = fma(0.5 * rd, fma(-dj * rd, rd, 1.0), rd);
+      double d = dj * rd;
+      d = fma(0.5 * rd, fma(-d, d, dj), d);
+      if (lane == j) rdl[j] = rd;  // 1 / R_jj
+      col[j] = lane == j ? d : col[j] * rd;
+      // R_ic -= R_ji R_jc for i > j (R_ji is lane i's entry j)
 #pragma unroll
-      for (int i = j + 1; i < 16; ++i)
-        if (c >= i) col[i] -= rowj[i] * col[j];
-      lane_exchange();
+      for (int i = j + 1; i < 16; ++i) col[i] -= lane_bcast(col[j], i) * col[j];
     }
     if (bad) {
       if (lane == 0) *fail = bad;
     } else {
-      // R_ss to LDS, then column c of inv(R_ss):
-      // x_i = (e_c[i] - sum_{l>i} R_il x_l) / R_ii
+      // R_ss to LDS, then column c of inv(R_ss), right-looking:
+      // x_l = v_l / R_ll, then v_i -= R_il x_l for i < l (x_l = 0 for l > c
+      // without masking: v_l stays exactly 0 there)
       if (lane < 16) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) Ds[i][c] = i <= c ? col[i] : 0.0;
       }
       lane_exchange();
-      double x[16];
+      double v[16];
 #pragma unroll
-      for (int i = 15; i >= 0; --i) {
-        double v = i == c ? 1.0 : 0.0;
+      for (int i = 0; i < 16; ++i) v[i] = i == c ? 1.0 : 0.0;
+      const bool cin = c < nb;
 #pragma unroll
-        for (int l = i + 1; l < 16; ++l) v -= Ds[i][l] * x[l];
-        x[i] = i <= c ? v / Ds[i][i] : 0.0;
-      }
-      if (lane < 16) {
+      for (int l = 15; l >= 0; --l) {
+        const double xl = v[l] * rdl[l];
+        if (lane < 16) Dv[l][c] = (l < nb && cin) ? xl : 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) Dv[i][c] = (i < nb && c < nb) ? x[i] : 0.0;
+        for (int i = 0; i < l; ++i) v[i] -= Ds[i][l] * xl;
       }
     }
-    __syncthreads();  // 2. done
+    __syncthreads();  // B2: done
     if (*fail) break;
-    __syncthreads();  // 3. panel in LDS
+    __syncthreads();  // B3: panel in LDS
   }
 }
 
@@ -268,31 +306,39 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
   __shared__ double P[16][CP + 1];  // panel row R_s* of the current step
   __shared__ double Ds[16][17];     // diagonal block G_ss
   __shared__ double Dv[16][17];     // inv(R_ss)
-  __shared__ double rowj[16];       // row j of R_ss while factoring
   __shared__ double fl[CP];         // pivot floors k * eps * G_jj
+  __shared__ double rdl[16];        // 1 / R_jj of the current diagonal block
   __shared__ int fail;
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-  for (int j = t; j < CP; j += kCholThreads)
+  constexpr int BW = CholShape<NBLK>::block_waves;
+  for (int j = t; j < CP; j += CholShape<NBLK>::threads)
     fl[j] = j < k ? k * 2.220446049250313e-16 * G[j + j * ldg] : 0.0;
   if (t == 0) fail = 0;
   // The roles run separate loops with the same barriers (a workgroup barrier
   // counts arrivals, not program points).
-  switch (wave) {
+  if (wave == BW) {
+    chol_factor_role<NBLK>(k, check_k, Ds, Dv, fl, rdl, &fail, t & 63);
+  } else {
+    switch (wave) {
 #define RLA_ROLE(W)                                                                        \
   case W:                                                                                  \
-    chol_block_role<NBLK, W>(k, G, ldg, R, ldr, Rt, ldt, Dinv, P, Ds, Dv, &fail, t);       \
+    if constexpr (W < BW)                                                                  \
+      chol_block_role<NBLK, W>(k, G, ldg, R, ldr, Rt, ldt, Dinv, P, Ds, Dv, &fail, t);     \
     break;
-    RLA_ROLE(0) RLA_ROLE(1) RLA_ROLE(2) RLA_ROLE(3) RLA_ROLE(4) RLA_ROLE(5) RLA_ROLE(6)
+      RLA_ROLE(0) RLA_ROLE(1) RLA_ROLE(2) RLA_ROLE(3) RLA_ROLE(4) RLA_ROLE(5) RLA_ROLE(6)
+      RLA_ROLE(7) RLA_ROLE(8) RLA_ROLE(9) RLA_ROLE(10) RLA_ROLE(11) RLA_ROLE(12)
+      RLA_ROLE(13) RLA_ROLE(14)
 #undef RLA_ROLE
-    default:
-      chol_factor_role<NBLK>(k, check_k, Ds, Dv, rowj, fl, &fail, t & 63);
+      default:
+        break;
+    }
   }
   if (t == 0) *info = fail;
 }
 
 template <int NBLK>
-__global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
+__global__ __launch_bounds__(CholShape<NBLK>::threads) void chol_blocks_kernel(
     int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
     double* __restrict__ Dinv, int* __restrict__ info) {
   chol_blocks_body<NBLK>(k, k, G, ldg, R, ldr, Rt, ldt, Dinv, info);
@@ -300,7 +346,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_blocks_kernel(
 
 // Batched form: workgroup p factors matrix p of a descriptor array.
 template <int NBLK>
-__global__ __launch_bounds__(kCholThreads) void chol_blocks_batched_kernel(
+__global__ __launch_bounds__(CholShape<NBLK>::threads) void chol_blocks_batched_kernel(
     const CholBatchItem* __restrict__ items) {
   const CholBatchItem& it = items[blockIdx.x];
   chol_blocks_body<NBLK>(it.k, it.check_k, it.G, it.k, nullptr, 0, it.Rt, it.k, it.Dinv,
@@ -412,7 +458,7 @@ int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R
   const int ki = static_cast<int>(k);
 #define RLA_CHOL(NB)                                                                       \
   case NB:                                                                                 \
-    chol_blocks_kernel<NB><<<1, kCholThreads, 0, ctx->stream>>>(                           \
+    chol_blocks_kernel<NB><<<1, CholShape<NB>::threads, 0, ctx->stream>>>(                           \
         ki, G, static_cast<int>(ldg), R, static_cast<int>(ldr), Rt, static_cast<int>(ldt), \
         Dinv, info_dev);                                                                   \
     break;
@@ -426,7 +472,8 @@ int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R
 
 int chol_blocks_batched(hipStream_t stream, int64_t nitems, const CholBatchItem* items) {
   if (nitems <= 0) return RLA_OK;
-  chol_blocks_batched_kernel<17><<<dim3(unsigned(nitems)), kCholThreads, 0, stream>>>(items);
+  chol_blocks_batched_kernel<17><<<dim3(unsigned(nitems)), CholShape<17>::threads, 0, stream>>>(
+      items);
   RLA_LAUNCH_CHECK("chol_blocks_batched");
   return RLA_OK;
 }

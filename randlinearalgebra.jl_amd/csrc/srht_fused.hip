@@ -306,6 +306,116 @@ __global__ __launch_bounds__(1 << (LC - VB)) void srht_fused_kernel(
   }
 }
 
+// The same transform for NCH > 1 with the parked slot values in registers:
+// thread t handles slots t, t + NT, ..., t + (MS-1)*NT in every chunk of every
+// column, so the values chunks 0 .. NCH-2 leave for the cross-chunk stages are
+// only ever read back by the thread that parked them.  The slot positions are
+// read from the plan once per kernel, the output rows once per column.
+// Needs nslots <= MS * NT.
+template <class T, int LC, int NCH, int VB, int MS>
+__global__ __launch_bounds__(1 << (LC - VB)) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void srht_fused_reg_kernel(
+    const T* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
+    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ plan,
+    T* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta, int vec_ok,
+    int nslots_cap) {
+  using G = Geometry<LC, VB>;
+  constexpr int R = G::R;
+  constexpr int TB = G::TB;
+  constexpr int NT = 1 << TB;
+  constexpr int WL = G::WLAST;
+  static_assert(NCH > 1, "single-chunk columns need no parking");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int t = threadIdx.x;
+  const int nslots = static_cast<int>(plan[7]);
+  const int32_t* __restrict__ slot_pos =
+      reinterpret_cast<const int32_t*>(plan + kPlanHdr + 2 * ((1 << LC) >> 5));
+  const int32_t* __restrict__ rowmap = slot_pos + nslots_cap;
+  int spad[MS];  // padded LDS index of each of this thread's slots (-1: none)
+#pragma unroll
+  for (int j = 0; j < MS; ++j) {
+    const int sl = t + j * NT;
+    spad[j] = sl < nslots ? pad(slot_pos[sl]) : -1;
+  }
+  T park[NCH - 1][MS];
+  T x[R], nx[R];
+  const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t my_items = my_cols * NCH;
+  auto item_col = [&](int64_t k) { return blockIdx.x + (k / NCH) * int64_t(gridDim.x); };
+  if (my_items > 0) load_chunk<T, R, TB>(nx, X + item_col(0) * x_cs, 0, m, t, vec_ok);
+  for (int64_t k = 0; k < my_items; ++k) {
+    const int64_t col = item_col(k);
+    const int ch = static_cast<int>(k % NCH);
+    const int64_t base = int64_t(ch) << LC;
+#pragma unroll
+    for (int v = 0; v < R; ++v) x[v] = nx[v];
+    uint64_t sw;
+    if constexpr (R == 64) {
+      const uint32_t* sp = signs32 + (base >> 5) + 2 * t;
+      sw = uint64_t(sp[0]) | (uint64_t(sp[1]) << 32);
+    } else {
+      sw = signs32[(base >> 5) + t];
+    }
+    __syncthreads();  // the previous item's readers of lds are done
+    {
+      T* lb = lds + load_base<TB>(t);
+#pragma unroll
+      for (int vh = 0; vh < R / 4; ++vh)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lb[load_off<TB>(vh, q)] = x[4 * vh + q];
+    }
+    if (k + 1 < my_items)
+      load_chunk<T, R, TB>(nx, X + item_col(k + 1) * x_cs, int64_t((k + 1) % NCH) << LC, m, t,
+                           vec_ok);
+    __syncthreads();
+    Phases<T, LC, VB, 0>::run(x, lds, t, sw, scale);
+    __syncthreads();
+    {
+      T* lb = lds + lds_base<WL, VB>(t);
+#pragma unroll
+      for (int v = 0; v < R; ++v) lb[lds_off<WL, VB>(v)] = x[v];
+    }
+    __syncthreads();
+    if (ch < NCH - 1) {
+      // park this chunk's slot values (selects on the uniform chunk index)
+#pragma unroll
+      for (int j = 0; j < MS; ++j) {
+        const T v = spad[j] >= 0 ? lds[spad[j]] : T(0);
+#pragma unroll
+        for (int c = 0; c < NCH - 1; ++c)
+          if (c == ch) park[c][j] = v;
+      }
+    } else {
+      const int64_t ycol = col * y_cs;
+#pragma unroll
+      for (int j = 0; j < MS; ++j) {
+        if (spad[j] < 0) continue;
+        T a[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH - 1; ++c) a[c] = park[c][j];
+        a[NCH - 1] = lds[spad[j]];
+        // the stages of bits LC, LC+1, ... pair chunks c and c|h, in order
+#pragma unroll
+        for (int h = 1; h < NCH; h <<= 1)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c)
+            if (!(c & h)) {
+              const T z = a[c], y = a[c | h];
+              a[c] = z + y;
+              a[c | h] = z - y;
+            }
+        const int32_t* rmj = rowmap + int64_t(t + j * NT) * NCH;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          const int r = rmj[c];
+          if (r >= 0) emit<T>(Y, y_rs, ycol, r, double(a[c]), alpha, beta);
+        }
+      }
+    }
+  }
+}
+
 // ---- shape selection ---------------------------------------------------------
 
 struct FusedShape {
@@ -483,6 +593,33 @@ static int launch_fused(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64
   return RLA_OK;
 }
 
+template <class T, int LC, int NCH, int VB, int MS>
+static int launch_fused_reg(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, int64_t ncols,
+                            const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
+                            T* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  const int nt = 1 << (LC - VB);
+  const int64_t cap = std::min<int64_t>(s, int64_t(1) << LC);
+  const size_t lds = ((size_t(1) << LC) + (size_t(1) << (LC - 5))) * sizeof(T);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&srht_fused_reg_kernel<T, LC, NCH, VB, MS>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  RLA_HIP(attr_err);
+  const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / lds));
+  const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
+  const int vec_ok = (x_cs % (sizeof(T) == 4 ? 4 : 2) == 0) &&
+                     (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+  srht_fused_reg_kernel<T, LC, NCH, VB, MS><<<dim3(static_cast<unsigned>(grid)), nt, lds,
+                                              ctx->stream>>>(
+      X, x_cs, m, ncols, reinterpret_cast<const uint32_t*>(signs), scale, plan, Y, y_rs, y_cs,
+      alpha, beta, vec_ok, static_cast<int>(cap));
+  RLA_LAUNCH_CHECK("srht_fused_reg");
+  return RLA_OK;
+}
+
 template <class T>
 int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
                        const int32_t* op, const uint64_t* signs, double scale, const void* plan,
@@ -514,6 +651,18 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
                                              ((plan_bytes + 255) & ~size_t(255)))
                       : nullptr;
   *handled = true;
+  // Float32 with 2 or 4 chunks: slot values parked in registers, up to 4 slots
+  // per thread
+  if (sizeof(T) == 4 && f.NCH > 1 && nslots_cap(f, s) <= 4 * (int64_t(1) << (f.LC - 5))) {
+#define RLA_FUSED_REG(LCV, NCHV)                                                               \
+    if (f.LC == LCV && f.NCH == NCHV)                                                          \
+      return launch_fused_reg<T, LCV, NCHV, 5, 4>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, \
+                                                  Y, y_rs, y_cs, alpha, beta);
+    if constexpr (sizeof(T) == 4) {
+      RLA_FUSED_REG(14, 2) RLA_FUSED_REG(14, 4)
+    }
+#undef RLA_FUSED_REG
+  }
 #define RLA_FUSED(LCV, NCHV, SCRV, PFV)                                                       \
   if (f.LC == LCV && f.NCH == NCHV && lds_scr == SCRV)                                          \
     return launch_fused<T, LCV, NCHV, 5, PFV, SCRV>(ctx, X, x_cs, m, ncols, signs, scale, pl, s,   \

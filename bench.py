@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -92,12 +93,13 @@ def gpu_sparse_sign(rla, torch, steps, warmup):
             "frac_hbm_peak": byts / ms / 1e6 / HBM_PEAK_GBS, "rotating_copies": copies}
 
 
-def rangefinder_c3(rla, torch, reps=3):
+def rangefinder_c3(rla, torch, reps=3, batch=4):
     """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
     32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
     Haar U, V from QR of Gaussians (seed 1234); Gaussian sketch k = 256 + 10,
     2 orthogonalised power iterations.  Reports solves/s with A (and its BF16
-    copies) resident, and ||A - QQ'A||_F / ||A||_F in FP64."""
+    copies) resident, one solve at a time and `batch` solves in flight on
+    their own streams, and ||A - QQ'A||_F / ||A||_F in FP64."""
     import math
     dev = torch.device("cuda", 0)
     m, n, k, r = 32768, 8192, 266, 2560
@@ -133,8 +135,39 @@ def rangefinder_c3(rla, torch, reps=3):
         rla._dense.gemm(rla.as_view(R_), rla.as_view(Q), rla.as_view(B), -1.0, 1.0)
         err = float(torch.linalg.norm(R_)) / nA
         out[prec] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "rel_error": err}
-        del rec, Q, B, R_
+        del Q, B, R_
+        # a batch of solves in flight together: one host thread and HIP stream
+        # per solve, A and its BF16 copies shared, every solve its own sketch
+        recs = [rec]
+        for i in range(1, batch):
+            rla.seed(7 + i)
+            recs.append(rla.complete_approximator(rf, A32))
+            recs[-1]._bf16 = rec._bf16
+        streams = [torch.cuda.Stream(device=dev) for _ in recs]
+
+        def work(r, s, n_solves):
+            with torch.cuda.stream(s):
+                for _ in range(n_solves):
+                    rla.rapproximate_(r, A32)
+                s.synchronize()
+
+        for r, s in zip(recs, streams):
+            work(r, s, 1)
+        torch.cuda.synchronize()
+        threads = [threading.Thread(target=work, args=(r, s, reps)) for r, s in zip(recs, streams)]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+        bdt = time.perf_counter() - t0
+        out[prec]["batch"] = {"in_flight": batch, "solves": batch * reps,
+                              "solves_per_s": batch * reps / bdt}
+        del recs, rec
     out["speedup_solves_per_s"] = out["plain"]["seconds_per_solve"] / out["bf16"]["seconds_per_solve"]
+    out["batched_speedup_solves_per_s"] = (out["bf16"]["batch"]["solves_per_s"] /
+                                           out["plain"]["batch"]["solves_per_s"])
     out["error_ratio_bf16_over_plain"] = out["bf16"]["rel_error"] / out["plain"]["rel_error"]
     return out
 

@@ -11,7 +11,8 @@
 // If the Cholesky factorisation breaks down (Y numerically rank deficient in
 // FP64) the QR falls back to Householder (GEQRF + ORGQR) on the current Y,
 // which spans the same space.  Q is returned with the column signs of a
-// positive-diagonal R.
+// positive-diagonal R.  Inside the RangeFinder loop the intermediate iterates
+// take one CholeskyQR pass and the returned Q two (rf_run).
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -215,10 +216,13 @@ static int cholqr_pass(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ld
 // Y (m x k, ldy) <- Q; optionally R (k x k upper, ldr) with Y_in = Q R.
 // info_async == nullptr: the call checks each pass on the host and falls back
 // to Householder QR itself.  Otherwise (small k, R == nullptr) nothing is
-// synchronised: the two passes' breakdown flags land in info_async[0..1] and a
-// failed pass leaves Y as it was before that pass; the caller checks later.
+// synchronised: the passes' breakdown flags land in info_async[0..passes-1]
+// and a failed pass leaves Y as it was before that pass; the caller checks
+// later.  passes = 2 is CholeskyQR2; passes = 1 gives a basis of the same
+// range whose orthogonality error is about cond(Y)^2 * eps.
 int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R, int64_t ldr,
-           void* work, size_t work_bytes, bool* used_householder, int* info_async) {
+           void* work, size_t work_bytes, bool* used_householder, int* info_async,
+           int passes = 2) {
   if (used_householder) *used_householder = false;
   if (k == 0 || m == 0) return RLA_OK;
   if (m < k) return fail(RLA_EARG, "QR of a wide matrix (m < k) is not supported");
@@ -226,8 +230,9 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
   QrBufs q;
   if (qr_layout(m, k, work, work_bytes, &q) > work_bytes) return fail(RLA_EARG, "QR workspace too small");
   const bool small = cholqr_small_k(k);
+  if (passes != 1 && passes != 2) return fail(RLA_EARG, "CholeskyQR takes 1 or 2 passes");
   if (info_async && small && !R) {
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
       bool broke = false;
       int rc = cholqr_pass(ctx, m, k, Y, ldy, q, info_async + pass, &broke);
       if (rc) return rc;
@@ -241,7 +246,7 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
   const int mi = static_cast<int>(m), ki = static_cast<int>(k), li = static_cast<int>(ldy);
   int passes_done = 0;
   bool breakdown = false;
-  for (int pass = 0; pass < 2 && !breakdown; ++pass) {
+  for (int pass = 0; pass < passes && !breakdown; ++pass) {
     rc = cholqr_pass(ctx, m, k, Y, ldy, q, q.info + pass, &breakdown);
     if (rc) return rc;
     if (small) {
@@ -412,30 +417,45 @@ static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, 
   return RLA_OK;
 }
 
+// Breakdown flag slots of one solve: one per CholeskyQR pass (rf_run).
+static int64_t rf_info_slots(int orthogonalize, int64_t power_its) {
+  return orthogonalize ? 2 * power_its + 2 : 2;
+}
+
 // The iterations of power_its.jl.  info == nullptr: QRs check breakdown on
-// the host; otherwise QR number i records its flags in info[2i], info[2i+1].
+// the host; otherwise every CholeskyQR pass records its flag in the next
+// slot of info (rf_info_slots in all).
+//
+// rand_ortho_it orthonormalises after every product so that the next product
+// starts from a well-conditioned basis of the current range; the range of
+// Y R^-1 is that of Y whatever the accuracy of R, so the intermediate
+// iterates take one CholeskyQR pass (orthogonality error ~ cond(Y)^2 * eps;
+// a pivot below the breakdown floor reruns the solve with checked QRs) and
+// only the returned Q takes the second pass of CholeskyQR2.
 static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
                   int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
                   const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
                   double* Y, int64_t ldy, RfBufs& w, int* info) {
   int rc;
-  int64_t nq = 0;
-  auto qr = [&](int64_t rows, double* M, int64_t ld) {
-    int* slot = info ? info + 2 * nq++ : nullptr;
-    return qr_f64(ctx, rows, k, M, ld, nullptr, 0, w.qr, w.qr_bytes, nullptr, slot);
+  int64_t ns = 0;
+  auto qr = [&](int64_t rows, double* M, int64_t ld, bool last) {
+    const int passes = last ? 2 : 1;
+    int* slot = info ? info + ns : nullptr;
+    ns += passes;
+    return qr_f64(ctx, rows, k, M, ld, nullptr, 0, w.qr, w.qr_bytes, nullptr, slot, passes);
   };
   if (orthogonalize) {
     // rand_ortho_it: QR after every product (power_its.jl:53-80)
-    rc = qr(m, Y, ldy);
+    rc = qr(m, Y, ldy, power_its == 0);
     if (rc) return rc;
     for (int64_t it = 0; it < power_its; ++it) {
       rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
       if (rc) return rc;
-      rc = qr(n, w.Z64, n);
+      rc = qr(n, w.Z64, n, false);
       if (rc) return rc;
       rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
       if (rc) return rc;
-      rc = qr(m, Y, ldy);
+      rc = qr(m, Y, ldy, it + 1 == power_its);
       if (rc) return rc;
     }
     return RLA_OK;
@@ -447,7 +467,7 @@ static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int
     rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
     if (rc) return rc;
   }
-  return qr(m, Y, ldy);
+  return qr(m, Y, ldy, true);
 }
 
 }  // namespace rla
@@ -526,8 +546,8 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
   RfBufs w;
   if (rf_layout(precision, a_elem, m, n, k, work, work_bytes, &w) > work_bytes)
     return fail(RLA_EARG, "RangeFinder workspace too small");
-  const int64_t nqr = orthogonalize ? 1 + 2 * power_its : 1;
-  const bool async = cholqr_small_k(k) && 2 * nqr <= w.ninfo;
+  const int64_t nslots = rf_info_slots(orthogonalize, power_its);
+  const bool async = cholqr_small_k(k) && nslots <= w.ninfo;
   if (async) {
     // no host round trip inside the loop; a breakdown anywhere reruns the
     // whole solve with host-checked QRs (Householder fallback) from the copy
@@ -536,8 +556,8 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
     rc = rf_run(ctx, precision, orthogonalize, m, n, k, power_its, a_elem, A, lda, A_rm, ld_rm,
                 A_cm, ld_cm, Y, ldy, w, w.info);
     if (rc) return rc;
-    std::vector<int> h(2 * nqr);
-    RLA_HIP(hipMemcpyAsync(h.data(), w.info, sizeof(int) * 2 * nqr, hipMemcpyDeviceToHost,
+    std::vector<int> h(nslots);
+    RLA_HIP(hipMemcpyAsync(h.data(), w.info, sizeof(int) * nslots, hipMemcpyDeviceToHost,
                            ctx->stream));
     RLA_HIP(hipStreamSynchronize(ctx->stream));
     bool ok = true;

@@ -138,6 +138,36 @@ int rla_sparse_apply_f64(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols
                          int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
                          double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
 
+/* Row-interleaved copy of the gather structure of op(M) (its CSR for op = M,
+ * its CSC for op = M'; `rows` = rows of op(M), ptr/idx/val those arrays):
+ * entry q of row i at ell_idx/ell_val[q * rows + i], q < width, padded with
+ * index 0 and value 0 past the row length.  rla_sparse_ell_width returns the
+ * longest row (synchronises).  The layout is what the lanes of a wavefront,
+ * one output row each, read as consecutive words; built once per recipe
+ * (after complete_compressor / update_compressor!, sparse_sign.jl:229-336,
+ * count_sketch.jl:113-162). */
+int rla_sparse_ell_width(rla_ctx* ctx, int64_t rows, const int32_t* ptr, int32_t* width);
+int rla_sparse_ell_build_f32(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                             const int32_t* idx, const float* val, int32_t* ell_idx,
+                             float* ell_val);
+int rla_sparse_ell_build_f64(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                             const int32_t* idx, const double* val, int32_t* ell_idx,
+                             double* ell_val);
+
+/* C = alpha * op(M) * X + beta * C with op(M) given by its row lengths (ptr,
+ * rows + 1 entries) and the row-interleaved arrays above; X column-major
+ * (x_rs == 1), in_rows x n_rhs.  Same arithmetic and order as
+ * rla_sparse_apply, so results are bit-identical to it.  Replaces mul! at
+ * sparse_sign.jl:339-361 and count_sketch.jl:166-213 for column-major X. */
+int rla_sparse_apply_ell_f32(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, const int32_t* ptr,
+                             const int32_t* ell_idx, const float* ell_val, int64_t n_rhs,
+                             const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                             int64_t c_cs, double alpha, double beta);
+int rla_sparse_apply_ell_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, const int32_t* ptr,
+                             const int32_t* ell_idx, const double* ell_val, int64_t n_rhs,
+                             const double* X, int64_t x_rs, int64_t x_cs, double* C,
+                             int64_t c_rs, int64_t c_cs, double alpha, double beta);
+
 /* ---- Walsh–Hadamard / SRHT / FJLT -------------------------------------
  * In-place FWHT of each of ncols columns of X (length n, a power of two),
  * natural (Sylvester) order, radix-2 stages h = 1, 2, 4, ... exactly as

@@ -77,6 +77,13 @@ _SIGS = {
                              i64, i64, f64, f64],
     "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
                              i64, i64, f64, f64],
+    "rla_sparse_ell_width": [vp, i64, vp, vp],
+    "rla_sparse_ell_build_f32": [vp, i64, i32, vp, vp, vp, vp, vp],
+    "rla_sparse_ell_build_f64": [vp, i64, i32, vp, vp, vp, vp, vp],
+    "rla_sparse_apply_ell_f32": [vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64, f64,
+                                 f64],
+    "rla_sparse_apply_ell_f64": [vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64, f64,
+                                 f64],
     "rla_fjlt_gen_count": [vp, u64, u64, i64, i64, f64, vp, vp],
     "rla_fjlt_gen_fill_f32": [vp, u64, u64, i64, i64, f64, vp, vp, vp],
     "rla_fjlt_gen_fill_f64": [vp, u64, u64, i64, i64, f64, vp, vp, vp],

@@ -343,6 +343,7 @@ class _SparseOp:
         self.colptr, self.rowidx, self.val = colptr, rowidx, val
         self.nnz = rowidx.numel()
         self._vals = {}
+        self._ell = {}
         self.rebuild_csr()
 
     def rebuild_csr(self):
@@ -355,6 +356,7 @@ class _SparseOp:
                   _lib.ptr(self.val), _lib.ptr(self.rowptr), _lib.ptr(self.colidx),
                   _lib.ptr(self.val_t))
         self._vals = {}
+        self._ell = {}
 
     def _vals_for(self, dtype):
         if dtype == self.val.dtype:
@@ -363,9 +365,47 @@ class _SparseOp:
             self._vals[dtype] = (self.val.to(dtype), self.val_t.to(dtype))
         return self._vals[dtype]
 
+    def _ell_for(self, trans_m, dtype):
+        """Row-interleaved gather arrays of op(M) in `dtype` (built once per
+        operator and cached), or None when rows average under 4 entries (the
+        per-row reads are then few) or padding every row to the longest would
+        more than double the entries."""
+        key = (int(trans_m), dtype)
+        if key in self._ell:
+            return self._ell[key]
+        rows = self.ncols if trans_m else self.nrows
+        if self.nnz < 4 * rows:
+            self._ell[key] = None
+            return None
+        dev = self.rowidx.device
+        ctx = _lib.ctx(dev)
+        ptr, idx = (self.colptr, self.rowidx) if trans_m else (self.rowptr, self.colidx)
+        val, val_t = self._vals_for(dtype)
+        v = val if trans_m else val_t
+        w = ctypes.c_int32(0)
+        _lib.call("rla_sparse_ell_width", ctx, rows, _lib.ptr(ptr), ctypes.byref(w))
+        width = w.value
+        out = None
+        if rows > 0 and 0 < width and width * rows <= 2 * self.nnz + 64 * rows:
+            ei = torch.empty(width * rows, dtype=torch.int32, device=dev)
+            ev = torch.empty(width * rows, dtype=dtype, device=dev)
+            _lib.call(f"rla_sparse_ell_build_{_lib.sfx(dtype)}", ctx, rows, width, _lib.ptr(ptr),
+                      _lib.ptr(idx), _lib.ptr(v), _lib.ptr(ei), _lib.ptr(ev))
+            out = (ptr, ei, ev)
+        self._ell[key] = out
+        return out
+
     def apply(self, trans_m, X, C, alpha, beta):
         """C = alpha*op(M)*X + beta*C on strided views (op = M or M')."""
         dt = X.dtype
+        if X.rs == 1 and X.rows > 0 and X.cols > 0:
+            ell = self._ell_for(trans_m, dt)
+            if ell is not None:
+                ptr, ei, ev = ell
+                _lib.call(f"rla_sparse_apply_ell_{_lib.sfx(dt)}", _lib.ctx(X.t.device),
+                          C.rows, X.rows, _lib.ptr(ptr), _lib.ptr(ei), _lib.ptr(ev), X.cols,
+                          _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
+                return
         val, val_t = self._vals_for(dt)
         _lib.call(f"rla_sparse_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(trans_m),
                   self.nrows, self.ncols, _lib.ptr(self.colptr), _lib.ptr(self.rowidx),

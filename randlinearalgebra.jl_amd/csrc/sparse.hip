@@ -18,7 +18,11 @@
 //    block of rows of KT adjacent columns of X in LDS (coalesced reads,
 //    interleaved [row][KT]) and every thread gathers its output rows from
 //    LDS.  Rows of X beyond one LDS block are handled block by block, with
-//    the partial sums carried in C (the workgroup owns its C tile).
+//    the partial sums carried in C (the workgroup owns its C tile).  The
+//    index/value reads come either straight from the CSR/CSC arrays (each
+//    thread walks its own row: a wavefront's load touches up to 64 cache
+//    lines) or from a row-interleaved copy built once per recipe (entry q of
+//    row i at q * rows + i: a wavefront's load is 2-4 consecutive lines).
 //  * rowgather_direct: X row-major view (unit column stride).  One wavefront
 //    per output row, lanes over the right-hand sides, gathered rows read
 //    straight from global memory (each gathered row is contiguous).
@@ -153,21 +157,28 @@ static int sparse_transpose(rla_ctx* ctx, int64_t nrows, int64_t ncols, int64_t 
 
 // ---- gather kernels --------------------------------------------------------
 
-__device__ inline int32_t lower_bound_i32(const int32_t* __restrict__ a, int32_t lo, int32_t hi,
-                                          int64_t key) {
+// first q in [lo, hi) with a[q * es] >= key (a sorted at stride es)
+__device__ inline int32_t lower_bound_i32(const int32_t* __restrict__ a, int64_t es, int32_t lo,
+                                          int32_t hi, int64_t key) {
   while (lo < hi) {
     int32_t mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1; else hi = mid;
+    if (a[mid * es] < key) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-template <class T, int KT>
+// ELL = false: the entries of row i are idx/val[ptr[i] .. ptr[i+1]) (CSR or
+// CSC).  ELL = true: entry q of row i is at idx/val[q * ell_ld + i] (the
+// row-interleaved copy from rla_sparse_ell_build), so the lanes of a
+// wavefront, which own consecutive rows, read consecutive words; ptr still
+// gives the row lengths.
+template <class T, int KT, bool ELL>
 __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
     int64_t out_rows, int64_t in_rows, int64_t n_rhs, const int32_t* __restrict__ ptr,
-    const int32_t* __restrict__ idx, const T* __restrict__ val, const T* __restrict__ X,
-    int64_t x_cs, T* __restrict__ C, int64_t c_rs, int64_t c_cs, double alpha, double beta,
-    int64_t rb_rows, int nk_tiles, int rsplit, int64_t rows_per_part) {
+    const int32_t* __restrict__ idx, const T* __restrict__ val, int64_t ell_ld,
+    const T* __restrict__ X, int64_t x_cs, T* __restrict__ C, int64_t c_rs, int64_t c_cs,
+    double alpha, double beta, int64_t rb_rows, int nk_tiles, int rsplit,
+    int64_t rows_per_part) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   // Block -> (column tile, output-row part).  The parts of one column tile
@@ -207,10 +218,14 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
     }
     __syncthreads();
     for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
-      int32_t p = ptr[i], pe = ptr[i + 1];
+      const int32_t row_p = ptr[i], row_n = ptr[i + 1] - row_p;
+      const int64_t es = ELL ? ell_ld : 1;
+      const int32_t* __restrict__ ix = ELL ? idx + i : idx + row_p;
+      const T* __restrict__ vx = ELL ? val + i : val + row_p;
+      int32_t p = 0, pe = row_n;
       if (nblocks > 1) {
-        p = lower_bound_i32(idx, p, pe, j0);
-        pe = lower_bound_i32(idx, p, pe, j0 + jn);
+        p = lower_bound_i32(ix, es, 0, row_n, j0);
+        pe = lower_bound_i32(ix, es, p, row_n, j0 + jn);
       }
       T acc[KT];
 #pragma unroll
@@ -230,8 +245,8 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
         T vv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          jj[u] = idx[p + u];
-          vv[u] = val[p + u];
+          jj[u] = ix[(p + u) * es];
+          vv[u] = vx[(p + u) * es];
         }
         T xv[8][KT];
 #pragma unroll
@@ -245,8 +260,8 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_lds_kernel(
             acc[kk] = static_cast<T>(double(acc[kk]) + double(vv[u]) * (double(xv[u][kk]) * alpha));
       }
       for (; p < pe; ++p) {
-        const int64_t j = idx[p] - j0;
-        const T v = val[p];
+        const int64_t j = ix[p * es] - j0;
+        const T v = vx[p * es];
 #pragma unroll
         for (int kk = 0; kk < KT; ++kk)
           acc[kk] = static_cast<T>(double(acc[kk]) + double(v) * (double(lds[j * KT + kk]) * alpha));
@@ -289,7 +304,7 @@ __global__ __launch_bounds__(256) void rowgather_direct_kernel(
   }
 }
 
-template <class T, int KT>
+template <class T, int KT, bool ELL>
 static int launch_rowgather_lds(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, int64_t n_rhs,
                                 const int32_t* ptr, const int32_t* idx, const T* val,
                                 const T* X, int64_t x_cs, T* C, int64_t c_rs, int64_t c_cs,
@@ -305,12 +320,29 @@ static int launch_rowgather_lds(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
   int64_t rows_per_part = ceil_div(out_rows, rsplit);
   int64_t groups = ceil_div(nk, 8);
   int64_t blocks = groups * 8 * rsplit;
-  rowgather_lds_kernel<T, KT><<<dim3(static_cast<unsigned>(blocks)), kGatherThreads, lds,
-                                ctx->stream>>>(out_rows, in_rows, n_rhs, ptr, idx, val, X, x_cs,
-                                               C, c_rs, c_cs, alpha, beta, rb_rows, nk, rsplit,
-                                               rows_per_part);
+  rowgather_lds_kernel<T, KT, ELL><<<dim3(static_cast<unsigned>(blocks)), kGatherThreads, lds,
+                                     ctx->stream>>>(out_rows, in_rows, n_rhs, ptr, idx, val,
+                                                    out_rows, X, x_cs, C, c_rs, c_cs, alpha,
+                                                    beta, rb_rows, nk, rsplit, rows_per_part);
   RLA_LAUNCH_CHECK("rowgather_lds");
   return RLA_OK;
+}
+
+// X column-major: wider column tiles amortise the index reads; narrower ones
+// keep enough workgroups in flight.
+template <class T, bool ELL>
+static int rowgather_lds(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, int64_t n_rhs,
+                         const int32_t* ptr, const int32_t* idx, const T* v, const T* X,
+                         int64_t x_cs, T* C, int64_t c_rs, int64_t c_cs, double a, double bt) {
+  const int64_t kt_target = n_rhs / 512;
+  if (sizeof(T) == 4 && kt_target >= 4)
+    return launch_rowgather_lds<T, 4, ELL>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs,
+                                           C, c_rs, c_cs, a, bt);
+  if (kt_target >= 2)
+    return launch_rowgather_lds<T, 2, ELL>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs,
+                                           C, c_rs, c_cs, a, bt);
+  return launch_rowgather_lds<T, 1, ELL>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
+                                         c_rs, c_cs, a, bt);
 }
 
 template <class T>
@@ -334,19 +366,9 @@ static int sparse_apply(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
   int rc = use_device(ctx);
   if (rc) return rc;
   const double a = alpha, bt = beta;
-  if (x_rs == 1 && in_rows > 0) {
-    // Wider column tiles amortise the index reads; narrower ones keep enough
-    // workgroups in flight.
-    const int64_t kt_target = n_rhs / 512;
-    if (sizeof(T) == 4 && kt_target >= 4)
-      return launch_rowgather_lds<T, 4>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
-                                        c_rs, c_cs, a, bt);
-    if (kt_target >= 2)
-      return launch_rowgather_lds<T, 2>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
-                                        c_rs, c_cs, a, bt);
-    return launch_rowgather_lds<T, 1>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C,
-                                      c_rs, c_cs, a, bt);
-  }
+  if (x_rs == 1 && in_rows > 0)
+    return rowgather_lds<T, false>(ctx, out_rows, in_rows, n_rhs, ptr, idx, v, X, x_cs, C, c_rs,
+                                   c_cs, a, bt);
   int64_t gy = std::min<int64_t>(ceil_div(n_rhs, 64), 1024);
   rowgather_direct_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(out_rows, 4)),
                                     static_cast<unsigned>(gy)),
@@ -354,6 +376,96 @@ static int sparse_apply(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols,
                                                       x_cs, C, c_rs, c_cs, a, bt);
   RLA_LAUNCH_CHECK("rowgather_direct");
   return RLA_OK;
+}
+
+
+// ---- row-interleaved gather structure ----------------------------------------
+
+__global__ void ell_width_kernel(int64_t rows, const int32_t* __restrict__ ptr,
+                                 int32_t* __restrict__ width) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < rows) atomicMax(width, ptr[i + 1] - ptr[i]);
+}
+
+__global__ void set_i32_kernel(int32_t* p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ell[q * rows + i] = entry q of row i for q < its length, index 0 and value
+// 0 past it (the gathers stop at the row length and never read those).
+template <class T>
+__global__ void ell_build_kernel(int64_t rows, int32_t width, const int32_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                 int32_t* __restrict__ ell_idx, T* __restrict__ ell_val) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t p = ptr[i], n = ptr[i + 1] - p;
+  for (int32_t q = 0; q < width; ++q) {
+    ell_idx[int64_t(q) * rows + i] = q < n ? idx[p + q] : 0;
+    ell_val[int64_t(q) * rows + i] = q < n ? val[p + q] : T(0);
+  }
+}
+
+static int ell_width(rla_ctx* ctx, int64_t rows, const int32_t* ptr, int32_t* width) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0) return fail(RLA_EARG, "negative dimensions");
+  if (!width) return fail(RLA_EARG, "width is NULL");
+  *width = 0;
+  if (rows == 0) return RLA_OK;
+  if (!ptr) return fail(RLA_EARG, "ptr is NULL");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = workspace(ctx, sizeof(int32_t), &ws);
+  if (rc) return rc;
+  int32_t* w = static_cast<int32_t*>(ws);
+  set_i32_kernel<<<1, 64, 0, ctx->stream>>>(w, 0);
+  ell_width_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(rows, ptr, w);
+  RLA_LAUNCH_CHECK("ell_width");
+  RLA_HIP(hipMemcpyAsync(width, w, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  return RLA_OK;
+}
+
+template <class T>
+static int ell_build(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                     const int32_t* idx, const T* val, int32_t* ell_idx, T* ell_val) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || width < 0) return fail(RLA_EARG, "negative dimensions");
+  if (rows == 0 || width == 0) return RLA_OK;
+  if (!ptr || !idx || !val || !ell_idx || !ell_val) return fail(RLA_EARG, "ELL build: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  ell_build_kernel<T><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, width, ptr, idx, val, ell_idx, ell_val);
+  RLA_LAUNCH_CHECK("ell_build");
+  return RLA_OK;
+}
+
+template <class T>
+static int sparse_apply_ell(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, const int32_t* ptr,
+                            const int32_t* ell_idx, const T* ell_val, int64_t n_rhs, const T* X,
+                            int64_t x_rs, int64_t x_cs, T* C, int64_t c_rs, int64_t c_cs,
+                            double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (out_rows < 0 || in_rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  if (out_rows == 0 || n_rhs == 0) return RLA_OK;
+  if (!ptr || !ell_idx || !ell_val || !C || (in_rows > 0 && !X))
+    return fail(RLA_EARG, "interleaved sparse apply: NULL pointer");
+  if (x_rs != 1) return fail(RLA_EARG, "interleaved sparse apply needs X column-major (x_rs == 1)");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  if (in_rows == 0) {
+    // no nonzeros: C = beta * C
+    rowgather_direct_kernel<T><<<dim3(static_cast<unsigned>(ceil_div(out_rows, 4)),
+                                      static_cast<unsigned>(std::min<int64_t>(ceil_div(n_rhs, 64), 1024))),
+                                 256, 0, ctx->stream>>>(out_rows, n_rhs, ptr, ell_idx, ell_val, X,
+                                                        x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+    RLA_LAUNCH_CHECK("rowgather_direct");
+    return RLA_OK;
+  }
+  return rowgather_lds<T, true>(ctx, out_rows, in_rows, n_rhs, ptr, ell_idx, ell_val, X, x_cs, C,
+                                c_rs, c_cs, alpha, beta);
 }
 
 }  // namespace rla
@@ -392,6 +504,38 @@ int rla_sparse_apply_f64(rla_ctx* ctx, int trans_m, int64_t nrows, int64_t ncols
                          int64_t c_rs, int64_t c_cs, double alpha, double beta) {
   return sparse_apply<double>(ctx, trans_m, nrows, ncols, colptr, rowidx, val, rowptr, colidx,
                               val_t, n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_ell_width(rla_ctx* ctx, int64_t rows, const int32_t* ptr, int32_t* width) {
+  return ell_width(ctx, rows, ptr, width);
+}
+
+int rla_sparse_ell_build_f32(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                             const int32_t* idx, const float* val, int32_t* ell_idx,
+                             float* ell_val) {
+  return ell_build<float>(ctx, rows, width, ptr, idx, val, ell_idx, ell_val);
+}
+
+int rla_sparse_ell_build_f64(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                             const int32_t* idx, const double* val, int32_t* ell_idx,
+                             double* ell_val) {
+  return ell_build<double>(ctx, rows, width, ptr, idx, val, ell_idx, ell_val);
+}
+
+int rla_sparse_apply_ell_f32(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, const int32_t* ptr,
+                             const int32_t* ell_idx, const float* ell_val, int64_t n_rhs,
+                             const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                             int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_ell<float>(ctx, out_rows, in_rows, ptr, ell_idx, ell_val, n_rhs, X, x_rs,
+                                 x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_ell_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, const int32_t* ptr,
+                             const int32_t* ell_idx, const double* ell_val, int64_t n_rhs,
+                             const double* X, int64_t x_rs, int64_t x_cs, double* C,
+                             int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_ell<double>(ctx, out_rows, in_rows, ptr, ell_idx, ell_val, n_rhs, X, x_rs,
+                                  x_cs, C, c_rs, c_cs, alpha, beta);
 }
 
 }  // extern "C"

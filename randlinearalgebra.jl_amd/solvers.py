@@ -182,6 +182,7 @@ class SamplingRecipe(_SparseRecipe):
         op.colptr, op.rowidx, op.val = colptr, rowidx, val
         op.rowptr, op.colidx, op.val_t = rowptr, self.idx, ones
         op._vals = {}
+        op._ell = {}
         self.op = op
 
     def _update(self, *args):

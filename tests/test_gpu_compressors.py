@@ -278,6 +278,65 @@ def test_sparse_sign_config_c1_bit_exact(rla, dev):
     assert np.max(np.abs(got - dense)) <= 1e-12 * np.max(np.abs(dense))
 
 
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("trans_m", [0, 1])
+@pytest.mark.parametrize("m", [700, 20000])
+def test_sparse_interleaved_apply_equals_csr_apply(rla, dev, dtype, trans_m, m):
+    # the row-interleaved apply against the CSR/CSC apply, bit for bit: ragged
+    # rows (uneven SparseSign row counts, rows of M' with 3 entries), an empty
+    # row, beta != 0, and with m = 20000 more rows of X than one LDS block
+    # holds (blocked gathers with strided lower bounds)
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    s, nnz, ncols = 96, 3, 5
+    hi = np.zeros(m * nnz, np.int32)
+    hv = np.zeros(m * nnz)
+    lib.load().rla_host_sparse_sign_gen_f64(77, 1, m, s - 1, nnz, hi.ctypes.data, hv.ctypes.data)
+    colptr = torch.arange(0, m * nnz + 1, nnz, dtype=torch.int32, device=dev)
+    rowidx = torch.as_tensor(hi, device=dev)               # row s-1 of M stays empty
+    val = torch.as_tensor(hv, device=dev).to(dtype)
+    rowptr = torch.empty(s + 1, dtype=torch.int32, device=dev)
+    colidx = torch.empty(m * nnz, dtype=torch.int32, device=dev)
+    val_t = torch.empty(m * nnz, dtype=dtype, device=dev)
+    sfx = lib.sfx(dtype)
+    lib.call(f"rla_sparse_transpose_{sfx}", ctx, s, m, m * nnz, lib.ptr(colptr), lib.ptr(rowidx),
+             lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t))
+    rows, in_rows = (m, s) if trans_m else (s, m)
+    ptr, idx, v = (colptr, rowidx, val) if trans_m else (rowptr, colidx, val_t)
+    import ctypes
+    w = ctypes.c_int32(0)
+    lib.call("rla_sparse_ell_width", ctx, rows, lib.ptr(ptr), ctypes.byref(w))
+    lens = (ptr[1:] - ptr[:-1]).cpu().numpy()
+    assert w.value == lens.max() and (trans_m or lens[-1] == 0)
+    ei = torch.empty(w.value * rows, dtype=torch.int32, device=dev)
+    ev = torch.empty(w.value * rows, dtype=dtype, device=dev)
+    lib.call(f"rla_sparse_ell_build_{sfx}", ctx, rows, w.value, lib.ptr(ptr), lib.ptr(idx),
+             lib.ptr(v), lib.ptr(ei), lib.ptr(ev))
+    X = colmajor(rnd((in_rows, ncols), 3, NP[dtype]), dev)
+    C0 = colmajor(rnd((rows, ncols), 4, NP[dtype]), dev)
+    C1, C2 = C0.clone(), C0.clone()
+    lib.call(f"rla_sparse_apply_{sfx}", ctx, trans_m, s, m, lib.ptr(colptr), lib.ptr(rowidx),
+             lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t), ncols, lib.ptr(X), 1,
+             in_rows, lib.ptr(C1), 1, rows, 1.5, -0.5)
+    lib.call(f"rla_sparse_apply_ell_{sfx}", ctx, rows, in_rows, lib.ptr(ptr), lib.ptr(ei),
+             lib.ptr(ev), ncols, lib.ptr(X), 1, in_rows, lib.ptr(C2), 1, rows, 1.5, -0.5)
+    assert torch.equal(C1, C2)
+    want = R.sparse_mul(S_dense(colptr, rowidx, val, s, m, trans_m), X.cpu().numpy(), 1.5, -0.5,
+                        C0.cpu().numpy())
+    assert np.array_equal(C2.cpu().numpy(), want)
+    with pytest.raises(rla.ArgumentError):
+        lib.call(f"rla_sparse_apply_ell_{sfx}", ctx, rows, in_rows, lib.ptr(ptr), lib.ptr(ei),
+                 lib.ptr(ev), ncols, lib.ptr(X), ncols, 1, lib.ptr(C2), 1, rows, 1.0, 0.0)
+
+
+def S_dense(colptr, rowidx, val, s, m, trans_m):
+    M = np.zeros((s, m))
+    cp, ri, v = colptr.cpu().numpy(), rowidx.cpu().numpy(), val.double().cpu().numpy()
+    for j in range(m):
+        M[ri[cp[j]:cp[j + 1]], j] = v[cp[j]:cp[j + 1]]
+    return M.T if trans_m else M
+
+
 def test_sparse_update_changes_operator(rla, dev):
     A = colmajor(rnd((100, 3), 0), dev)
     S = rla.complete_compressor(rla.SparseSign(compression_dim=20), A)

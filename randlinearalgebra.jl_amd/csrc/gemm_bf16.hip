@@ -8,14 +8,18 @@
 // (columns of A contiguous: for A'*X), so every product streams A once with
 // coalesced 16-byte loads.
 //
-// Workgroup tile: 128 rows (4 wavefronts x 32 rows) x NB*32 columns.  Per
-// 64-wide K step the B tile (NB*32 columns x 64) is staged in LDS (padded
-// row stride, conflict-free 16-byte fragment reads) and every wavefront runs
-// 4 x NB v_mfma_f32_32x32x16_bf16 on its 32 rows.  The A fragments and the
-// next B tile are loaded while the current tile is multiplied.  Split-K
-// writes partial tiles that a second kernel sums in split order, so results
-// are deterministic.
+// N <= 288 (the RangeFinder's k columns) runs the pipelined kernel below
+// (gemm_bf16_pipe_kernel: all N columns in one workgroup tile, the wavefronts
+// split into two column groups, double-buffered B tile, A prefetched one step
+// ahead).  Wider N runs gemm_bf16_tn_kernel: workgroup tile 128 rows (4
+// wavefronts x 32 rows) x NB*32 columns; per 64-wide K step the B tile (NB*32
+// columns x 64) is staged in LDS (padded row stride, conflict-free 16-byte
+// fragment reads) and every wavefront runs 4 x NB v_mfma_f32_32x32x16_bf16 on
+// its 32 rows.  Both split K when the tiles alone do not fill the CUs; the
+// partial tiles are summed in split order by a second kernel, so results are
+// deterministic.
 #include <algorithm>
+#include <mutex>
 
 #include "rla_internal.hpp"
 
@@ -248,6 +252,259 @@ static int launch_gemm(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint
   return RLA_OK;
 }
 
+// ---- pipelined kernel for N <= 288 -------------------------------------------
+//
+// Workgroup tile 128 rows x BN = 32*(NB0+NB1) columns (all of N).  The
+// wavefronts form a (4/RT) x 2 grid: wavefront (wr, wc) computes rows
+// wr*32*RT .. +32*RT (RT 32-row MFMA tiles) times column tiles
+// [wc ? NB0 : 0, +NB0 or NB1), so each B fragment read from LDS feeds RT
+// MFMAs.  Per 64-wide K step:
+//   1. the A fragments of step s+1 are issued into the other half of a
+//      two-deep register ring (the loop is unrolled by two so both ring slots
+//      are static);
+//   2. the step's 4 x NB x RT MFMAs run from the ring slot of step s and the
+//      LDS buffer of step s, the B fragments of substep kk+1 read while
+//      substep kk multiplies (a scheduling barrier keeps the reads ahead);
+//   3. the B tile of step s+1, loaded into registers one step earlier, goes to
+//      the other LDS buffer;
+//   4. the B tile of step s+2 is issued; one barrier.
+// Rows past M and columns past N read clamped (valid) addresses and are never
+// stored; only a final partial K step zero-fills, in both operands.
+constexpr int kPM = 128;
+constexpr int kPipeRT = 1;  // row tiles per wavefront (1: 8 wavefronts, 2 per SIMD; 2: 4)
+
+template <int NB0, int NB1, int RT>
+struct PipeCfg {
+  static constexpr int NT = 256 * (2 / RT);  // threads: 4 or 8 wavefronts
+  static constexpr int WROWS = 4 / RT;       // wavefront rows of the grid
+  static constexpr int BN = (NB0 + NB1) * 32;
+  static constexpr int CHUNKS = BN * 8;  // 16-byte pieces of one K step of the B tile
+  static constexpr int PER_T = (CHUNKS + NT - 1) / NT;
+  static constexpr int BUF = BN * kBstride;  // bf16 per LDS buffer
+};
+
+template <int NB, int NB0, int NB1, int RT>
+__device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* __restrict__ A,
+                                         int64_t lda, const uint16_t* __restrict__ B,
+                                         int64_t ldb, float* __restrict__ C, int64_t c_rs,
+                                         int64_t c_cs, float alpha, float beta, int ksplit,
+                                         int split, int64_t kb, int64_t ke, int64_t tm, int wr,
+                                         int ct0, float* __restrict__ part, uint16_t* Bs) {
+  using P = PipeCfg<NB0, NB1, RT>;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int akq = (lane >> 5) * 8;
+
+  f32x16 acc[RT][NB];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[rt][nb][r] = 0.f;
+
+  const int64_t row_w = tm * kPM + wr * 32 * RT;  // first row of this wavefront
+  const uint16_t* ap[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+    ap[rt] = A + std::min<int64_t>(row_w + rt * 32 + (lane & 31), M - 1) * lda;
+  // piece u of this thread: column (tid >> 3) + u * NT/8 (clamped to N-1 in
+  // global memory), K offset (tid & 7) * 8; LDS offset boff0 + u * NT/8 * kBstride
+  const int bcol0 = tid >> 3;
+  const uint16_t* bbase = B + (tid & 7) * 8;
+  const int boff0 = bcol0 * kBstride + (tid & 7) * 8;
+  auto bsrc = [&](int u) {
+    return bbase + std::min<int64_t>(bcol0 + (P::NT / 8) * u, N - 1) * ldb;
+  };
+  auto has_piece = [&](int u) { return u + 1 < P::PER_T || tid + u * P::NT < P::CHUNKS; };
+
+  const int64_t nfull = (ke - kb) / kBK;
+  auto step_k = [&](int64_t s) { return kb + std::min<int64_t>(s, nfull - 1) * kBK; };
+  auto load_a = [&](int64_t k0, bf16x8(&af)[RT][4]) {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+        af[rt][kk] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u32x4*>(ap[rt] + k0 + kk * 16 + akq));
+  };
+  auto load_b = [&](int64_t k0, u32x4(&br)[P::PER_T]) {
+#pragma unroll
+    for (int u = 0; u < P::PER_T; ++u)
+      if (has_piece(u)) br[u] = *reinterpret_cast<const u32x4*>(bsrc(u) + k0);
+  };
+  auto store_b = [&](uint16_t* buf, const u32x4(&br)[P::PER_T]) {
+#pragma unroll
+    for (int u = 0; u < P::PER_T; ++u)
+      if (has_piece(u))
+        *reinterpret_cast<u32x4*>(buf + boff0 + u * (P::NT / 8) * kBstride) = br[u];
+  };
+  const int fbase = (ct0 * 32 + (lane & 31)) * kBstride + akq;
+  auto read_frags = [&](const uint16_t* buf, int kk, bf16x8(&bv)[NB]) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+      bv[nb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                              buf + fbase + nb * 32 * kBstride + kk * 16));
+  };
+  auto compute = [&](const bf16x8(&af)[RT][4], const uint16_t* buf) {
+    bf16x8 bv[2][NB];
+    read_frags(buf, 0, bv[0]);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      // keep the next substep's LDS reads ahead of this substep's MFMAs (the
+      // scheduler would otherwise sink them and serialise on LDS latency)
+      if (kk + 1 < 4) read_frags(buf, kk + 1, bv[(kk + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+          acc[rt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[rt][kk], bv[kk & 1][nb],
+                                                                acc[rt][nb], 0, 0, 0);
+    }
+  };
+
+  if (nfull > 0) {
+    bf16x8 r0[RT][4], r1[RT][4];
+    u32x4 br[P::PER_T];
+    load_b(step_k(0), br);
+    load_a(step_k(0), r0);
+    store_b(Bs, br);
+    load_b(step_k(1), br);
+    __syncthreads();
+    // step s: prefetch A(s+1) into `nxt`, multiply with `cur`, B(s+1) to LDS,
+    // prefetch B(s+2)
+    auto step = [&](int64_t s, const bf16x8(&cur)[RT][4], bf16x8(&nxt)[RT][4]) {
+      load_a(step_k(s + 1), nxt);
+      compute(cur, Bs + (s & 1) * P::BUF);
+      store_b(Bs + ((s + 1) & 1) * P::BUF, br);
+      load_b(step_k(s + 2), br);
+      __syncthreads();
+    };
+    for (int64_t s = 0; s < nfull; s += 2) {
+      step(s, r0, r1);
+      if (s + 1 >= nfull) break;
+      step(s + 1, r1, r0);
+    }
+  }
+  if (kb + nfull * kBK < ke) {
+    // partial last K step: guarded loads, zero past ke in both operands
+    const int64_t k0 = kb + nfull * kBK;
+    bf16x8 af[RT][4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int64_t gk = k0 + kk * 16 + akq;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        u32x4 v = {0u, 0u, 0u, 0u};
+        if (gk + 8 <= ke) v = *reinterpret_cast<const u32x4*>(ap[rt] + gk);
+        else if (gk < ke) v = load8_tail(ap[rt] + gk, ke - gk);
+        af[rt][kk] = __builtin_bit_cast(bf16x8, v);
+      }
+    }
+    u32x4 br[P::PER_T];
+#pragma unroll
+    for (int u = 0; u < P::PER_T; ++u) {
+      const int64_t gk = k0 + (tid & 7) * 8;
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if (gk + 8 <= ke) v = *reinterpret_cast<const u32x4*>(bsrc(u) + k0);
+      else if (gk < ke) v = load8_tail(bsrc(u) + k0, ke - gk);
+      br[u] = v;
+    }
+    uint16_t* buf = Bs + (nfull & 1) * P::BUF;
+    store_b(buf, br);
+    __syncthreads();
+    compute(af, buf);
+  }
+
+  // acc[rt][nb][r]: row row_w + rt*32 + (r/4)*8 + (lane/32)*4 + r%4,
+  // column (ct0 + nb)*32 + lane%32
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int64_t j = int64_t(ct0 + nb) * 32 + (lane & 31);
+    if (j >= N) continue;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = row_w + rt * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (i >= M) continue;
+        if (ksplit == 1) {
+          float* cp = C + i * c_rs + j * c_cs;
+          *cp = beta == 0.f ? alpha * acc[rt][nb][r] : alpha * acc[rt][nb][r] + beta * (*cp);
+        } else {
+          part[(int64_t(split) * N + j) * M + i] = acc[rt][nb][r];
+        }
+      }
+    }
+  }
+}
+
+template <int NB0, int NB1, int RT>
+__global__ __launch_bounds__(256 * (2 / RT))
+__attribute__((amdgpu_waves_per_eu(2 / RT, 2 / RT)))
+void gemm_bf16_pipe_kernel(int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A,
+                           int64_t lda, const uint16_t* __restrict__ B, int64_t ldb,
+                           float* __restrict__ C, int64_t c_rs, int64_t c_cs, float alpha,
+                           float beta, int ksplit, int64_t k_per_split,
+                           float* __restrict__ part) {
+  using P = PipeCfg<NB0, NB1, RT>;
+  extern __shared__ __align__(16) uint16_t pipe_lds[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int split = blockIdx.x % ksplit;
+  const int64_t tm = blockIdx.x / ksplit;
+  const int64_t kb = split * k_per_split;
+  const int64_t ke = std::min<int64_t>(K, kb + k_per_split);
+  if (kb >= ke) return;  // uniform over the workgroup
+  const int wr = wave % P::WROWS;
+  if (wave / P::WROWS)
+    pipe_run<NB1, NB0, NB1, RT>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
+                                kb, ke, tm, wr, NB0, part, pipe_lds);
+  else
+    pipe_run<NB0, NB0, NB1, RT>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
+                                kb, ke, tm, wr, 0, part, pipe_lds);
+}
+
+template <int NB0, int NB1, int RT>
+static int launch_pipe(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint16_t* A,
+                       int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t c_rs,
+                       int64_t c_cs, float alpha, float beta) {
+  using P = PipeCfg<NB0, NB1, RT>;
+  const size_t lds = size_t(2) * P::BUF * sizeof(uint16_t);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&gemm_bf16_pipe_kernel<NB0, NB1, RT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  RLA_HIP(attr_err);
+  const int64_t tiles = ceil_div(M, kPM);
+  static int cus = 0;
+  if (!cus) cus = num_cus_dev(ctx->device);
+  // split K until about one workgroup per CU, keeping >= 1024 K per split
+  int ksplit = 1;
+  while (tiles * ksplit * 2 <= cus && K / (ksplit * 2) >= 1024) ksplit *= 2;
+  const int64_t kps = ceil_div(ceil_div(K, ksplit), kBK) * kBK;
+  ksplit = static_cast<int>(ceil_div(K, kps));
+  float* part = nullptr;
+  if (ksplit > 1) {
+    void* ws = nullptr;
+    int rc = workspace(ctx, static_cast<size_t>(ksplit) * M * N * sizeof(float), &ws);
+    if (rc) return rc;
+    part = static_cast<float*>(ws);
+  }
+  gemm_bf16_pipe_kernel<NB0, NB1, RT><<<dim3(static_cast<unsigned>(tiles * ksplit)), P::NT, lds,
+                                        ctx->stream>>>(M, N, K, A, lda, B, ldb, C, c_rs, c_cs,
+                                                       alpha, beta, ksplit, kps, part);
+  RLA_LAUNCH_CHECK("gemm_bf16_pipe");
+  if (ksplit > 1) {
+    splitk_reduce_kernel<<<dim3(static_cast<unsigned>(ceil_div(M * N, 256))), 256, 0,
+                           ctx->stream>>>(M, N, ksplit, part, C, c_rs, c_cs, alpha, beta);
+    RLA_LAUNCH_CHECK("gemm_bf16_pipe/splitk");
+  }
+  return RLA_OK;
+}
+
 int gemm_bf16_tn(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, const uint16_t* A,
                  int64_t lda, const uint16_t* B, int64_t ldb, float beta, float* C, int64_t c_rs,
                  int64_t c_cs) {
@@ -256,8 +513,21 @@ int gemm_bf16_tn(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, con
   if ((lda % 8) || (ldb % 8) || (reinterpret_cast<uintptr_t>(A) % 16) ||
       (reinterpret_cast<uintptr_t>(B) % 16))
     return fail(RLA_EARG, "gemm_bf16: operands must be 16-byte aligned with ld a multiple of 8");
-  // widest N tile that covers N in one pass (up to 288 columns), else 256
+  // N <= 288: the pipelined 2 x 2 kernel, column tiles split between the two
+  // wavefront columns
   const int64_t nbk = ceil_div(N, 32);
+  switch (nbk) {
+    case 2: return launch_pipe<1, 1, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 3: return launch_pipe<2, 1, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 4: return launch_pipe<2, 2, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 5: return launch_pipe<3, 2, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 6: return launch_pipe<3, 3, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 7: return launch_pipe<4, 3, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 8: return launch_pipe<4, 4, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    case 9: return launch_pipe<5, 4, kPipeRT>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
+    default: break;
+  }
+  // otherwise the widest single-wavefront-column tile (256 columns per pass)
   if (nbk <= 4) return launch_gemm<4>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
   if (nbk <= 6) return launch_gemm<6>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);
   if (nbk <= 9) return launch_gemm<9>(ctx, M, N, K, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta);

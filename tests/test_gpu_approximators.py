@@ -50,7 +50,9 @@ def test_gaussian_all_orientations(rla, dev, dtype, tol, card):
 
 
 @pytest.mark.parametrize("M,N,K", [(256, 266, 8192), (100, 37, 300), (8192, 266, 4096),
-                                   (33, 300, 1000), (128, 64, 64)])
+                                   (33, 300, 1000), (128, 64, 64), (300, 266, 1000),
+                                   (1000, 288, 8200), (129, 100, 64), (5000, 33, 130),
+                                   (256, 266, 32768), (70, 200, 40), (4096, 160, 5000)])
 def test_gemm_bf16_matches_fp32_reference(rla, dev, M, N, K):
     # inputs rounded to bf16 on the host; products exact, FP32 accumulation
     A = rnd((M, K), 1).astype(np.float32)

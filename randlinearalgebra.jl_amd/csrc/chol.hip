@@ -94,11 +94,13 @@ struct CholShape {
 // parameter so every block coordinate and LDS offset is a compile-time
 // constant (no address or index registers per block).
 template <int NBLK, int W>
-__device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg, double* R,
-                                                int ldr, double* __restrict__ Rt, int ldt,
-                                                double* __restrict__ Dinv,
+__device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> G, int ldg,
+                                                global_ptr<double> R, int ldr,
+                                                global_ptr<double> __restrict__ Rt, int ldt,
+                                                global_ptr<double> __restrict__ Dinv,
                                                 double (*P)[NBLK * 16 + 1], double (*Ds)[17],
-                                                double (*Dv)[17], volatile int* fail, int t) {
+                                                double (*Dv)[17], lds_ptr<volatile int> fail,
+                                                int t) {
   constexpr int T = NBLK * (NBLK + 1) / 2;
   constexpr int BW = CholShape<NBLK>::block_waves;
   constexpr int MAXB = (T - W + BW - 1) / BW;  // blocks of this wavefront
@@ -217,7 +219,7 @@ __device__ __forceinline__ void chol_block_role(int k, const double* G, int ldg,
 template <int NBLK>
 __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds)[17],
                                                  double (*Dv)[17], const double* fl, double* rdl,
-                                                 volatile int* fail, int lane) {
+                                                 lds_ptr<volatile int> fail, int lane) {
   __syncthreads();  // B0: G read into registers, R zeroed
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
@@ -312,19 +314,23 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   constexpr int BW = CholShape<NBLK>::block_waves;
+  const global_ptr<const double> Gg = (global_ptr<const double>)G;
+  const lds_ptr<volatile int> failp = (lds_ptr<volatile int>)&fail;
   for (int j = t; j < CP; j += CholShape<NBLK>::threads)
-    fl[j] = j < k ? k * 2.220446049250313e-16 * G[j + j * ldg] : 0.0;
+    fl[j] = j < k ? k * 2.220446049250313e-16 * Gg[j + j * ldg] : 0.0;
   if (t == 0) fail = 0;
   // The roles run separate loops with the same barriers (a workgroup barrier
   // counts arrivals, not program points).
   if (wave == BW) {
-    chol_factor_role<NBLK>(k, check_k, Ds, Dv, fl, rdl, &fail, t & 63);
+    chol_factor_role<NBLK>(k, check_k, Ds, Dv, fl, rdl, failp, t & 63);
   } else {
     switch (wave) {
 #define RLA_ROLE(W)                                                                        \
   case W:                                                                                  \
     if constexpr (W < BW)                                                                  \
-      chol_block_role<NBLK, W>(k, G, ldg, R, ldr, Rt, ldt, Dinv, P, Ds, Dv, &fail, t);     \
+      chol_block_role<NBLK, W>(k, Gg, ldg, (global_ptr<double>)R, ldr,                  \
+                               (global_ptr<double>)Rt, ldt, (global_ptr<double>)Dinv, P, Ds, \
+                               Dv, failp, t);                                             \
     break;
       RLA_ROLE(0) RLA_ROLE(1) RLA_ROLE(2) RLA_ROLE(3) RLA_ROLE(4) RLA_ROLE(5) RLA_ROLE(6)
       RLA_ROLE(7) RLA_ROLE(8) RLA_ROLE(9) RLA_ROLE(10) RLA_ROLE(11) RLA_ROLE(12)
@@ -334,7 +340,7 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
         break;
     }
   }
-  if (t == 0) *info = fail;
+  if (t == 0) *(global_ptr<int>)info = fail;
 }
 
 template <int NBLK>

@@ -34,6 +34,17 @@ namespace rla {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Pointers read from a problem descriptor in memory are generic (flat) to the
+// compiler; flat loads also count as LDS traffic, so a wait for a descriptor
+// field would drain every load in flight.  Loads through these global-space
+// views are plain global loads.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p) {
+  return (gptr<T>)p;
+}
+
 constexpr int kLsThreads = 1024;
 constexpr int kLsMaxS = 1024;
 constexpr int kLsMaxN = 271;  // n + 1 <= 272 for the batched Cholesky
@@ -65,61 +76,130 @@ __global__ void ls_reset_kernel(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// Per tile, rows of [A b] stream through a double-buffered LDS chunk: the
+// global loads of chunk i+1 are in flight while chunk i is summed.  Each
+// thread owns one sketch row (bucket) r and a share of the CB columns, and
+// adds its bucket's members in ascending row order; the next KW member
+// indices and signs are loaded one chunk ahead into registers, so the
+// common case (at most KW members of a bucket per chunk) runs without a
+// dependent global load.
+constexpr int kLsWin = 8;
+
 template <int CB, int R>
 __global__ __launch_bounds__(kLsThreads) void ls_sketch_kernel(const LsDev* __restrict__ probs,
                                                                const SketchTile* __restrict__ tiles,
                                                                int ntiles, int* counter) {
-  __shared__ float T[R][CB + 1];
+  constexpr int PER = CB * R / kLsThreads;  // chunk entries loaded per thread
+  static_assert(PER * kLsThreads == CB * R, "chunk must split evenly over the threads");
+  __shared__ float T[2][R][CB + 1];
   __shared__ int tile_id;
+  const int tid = threadIdx.x;
   for (;;) {
-    if (threadIdx.x == 0) tile_id = atomicAdd(counter, 1);
+    if (tid == 0) tile_id = atomicAdd(counter, 1);
     __syncthreads();
     const int ti = tile_id;
     __syncthreads();  // every thread has read tile_id before it is reused
     if (ti >= ntiles) break;  // uniform: every wave leaves here once the queue is empty
     const SketchTile tl = tiles[ti];
     const LsDev& P = probs[tl.p];
-    const int s = P.s, k = P.n + 1;
+    const int s = P.s, n = P.n, k = P.n + 1;
+    const int64_t m = P.m, lda = P.lda;
+    const gptr<float> Ag = as_global(P.A);
+    const gptr<float> bg = as_global(P.b);
+    const gptr<int> rowptr = as_global(P.rowptr);
+    const gptr<int> colidx = as_global(P.colidx);
+    const gptr<float> val = as_global(P.val);
+    const global_ptr<float> SAb = (global_ptr<float>)P.SAb;
     int nsplit = 1;  // threads per sketch row
     while (nsplit < CB && nsplit * 2 * s <= kLsThreads) nsplit *= 2;
     const int cpt = CB / nsplit;
-    const int r = threadIdx.x / nsplit, g = threadIdx.x % nsplit;
+    const int r = tid / nsplit, g = tid % nsplit;
     const bool active = r < s;
-    int cur = active ? P.rowptr[r] : 0;
-    const int end = active ? P.rowptr[r + 1] : 0;
+    int cur = active ? rowptr[r] : 0;
+    const int end = active ? rowptr[r + 1] : 0;
     float acc[CB];
 #pragma unroll
     for (int q = 0; q < CB; ++q) acc[q] = 0.0f;
-    for (int64_t j0 = 0; j0 < P.m; j0 += R) {
-      const int rows = static_cast<int>(min<int64_t>(R, P.m - j0));
-      for (int e = threadIdx.x; e < CB * R; e += kLsThreads) {
-        const int c = e / R, jj = e % R;
-        const int col = tl.cb + c;
-        float v = 0.0f;
-        if (jj < rows && col < k)
-          v = col < P.n ? P.A[(j0 + jj) + int64_t(col) * P.lda] : P.b[j0 + jj];
-        T[jj][c] = v;
-      }
-      __syncthreads();
-      if (active) {
-        while (cur < end) {
-          const int j = P.colidx[cur];
-          if (j >= j0 + rows) break;
-          const float sg = P.val[cur];
-          const float* trow = T[j - j0];
+
+    // chunk entry i of this thread: column c = e / R, row jj = e % R with
+    // e = tid + i * kLsThreads.  Rows past m and columns past k read clamped
+    // (valid) addresses: those entries are never summed into a stored value.
+    float stage[PER];
+    auto load_chunk = [&](int64_t j0) {
 #pragma unroll
-          for (int q = 0; q < CB; ++q)
-            if (q < cpt) acc[q] = acc[q] + sg * trow[g + nsplit * q];
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * kLsThreads;
+        const int c = e / R, jj = e % R;
+        const int col = min(tl.cb + c, k - 1);
+        const int64_t row = min<int64_t>(j0 + jj, m - 1);
+        const gptr<float> src = col < n ? Ag + int64_t(col) * lda : bg;
+        stage[i] = src[row];
+      }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * kLsThreads;
+        T[buf][e % R][e / R] = stage[i];
+      }
+    };
+    // window: members cur .. cur+kLsWin-1 of the bucket (indices clamped to
+    // the bucket's last member; only those before `end` are used)
+    int wj[kLsWin];
+    float ws[kLsWin];
+    auto load_window = [&]() {
+#pragma unroll
+      for (int w = 0; w < kLsWin; ++w) {
+        const int pos = active ? min(cur + w, end - 1) : 0;
+        wj[w] = active && cur + w < end ? colidx[pos] : INT32_MAX;
+        ws[w] = active && cur + w < end ? val[pos] : 0.0f;
+      }
+    };
+    auto add_row = [&](const float* trow, float sg) {
+#pragma unroll
+      for (int q = 0; q < CB; ++q)
+        if (q < cpt) acc[q] = acc[q] + sg * trow[g + nsplit * q];
+    };
+
+    load_window();
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t j0 = 0; j0 < m; j0 += R) {
+      const bool more = j0 + R < m;
+      if (more) load_chunk(j0 + R);
+      const int64_t j1 = min<int64_t>(j0 + R, m);
+      int used = 0;
+#pragma unroll
+      for (int w = 0; w < kLsWin; ++w) {
+        if (wj[w] < j1 && used == w) {
+          add_row(T[buf][wj[w] - j0], ws[w]);
+          ++used;
+        }
+      }
+      cur += used;
+      if (used == kLsWin) {
+        // more than kLsWin members of this bucket in the chunk
+        while (cur < end) {
+          const int j = colidx[cur];
+          if (j >= j1) break;
+          add_row(T[buf][j - j0], val[cur]);
           ++cur;
         }
       }
+      if (more) {
+        load_window();
+        store_chunk(buf ^ 1);
+      }
       __syncthreads();
+      buf ^= 1;
     }
     if (active) {
 #pragma unroll
       for (int q = 0; q < CB; ++q) {
         const int c = tl.cb + g + nsplit * q;
-        if (q < cpt && c < k) P.SAb[r + int64_t(c) * s] = acc[q];
+        if (q < cpt && c < k) SAb[r + int64_t(c) * s] = acc[q];
       }
     }
   }
@@ -137,18 +217,33 @@ __global__ __launch_bounds__(256) void ls_gram_kernel(const LsDev* __restrict__ 
   const int s = P.s, k = P.n + 1;
   const int ca = tl.bi * 16 + (lane & 15), cb = tl.bj * 16 + (lane & 15);
   const bool oka = ca < k, okb = cb < k;
-  const float* pa = P.SAb + int64_t(oka ? ca : 0) * s;
-  const float* pb = P.SAb + int64_t(okb ? cb : 0) * s;
+  // a lane of a column past k reads column 0 instead: its products only reach
+  // output rows / columns past k, which are not stored
+  const gptr<float> pa = as_global(P.SAb) + int64_t(oka ? ca : 0) * s;
+  const gptr<float> pb = as_global(P.SAb) + int64_t(okb ? cb : 0) * s;
   f64x4 acc0 = f64x4{0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
   int r0 = 0;
+  // 32 rows per iteration: all 16 loads issued before the 8 MFMAs
+  for (; r0 + 32 <= s; r0 += 32) {
+    float a[8], b[8];
+    const int r = r0 + (lane >> 4);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      a[q] = pa[r + 4 * q];
+      b[q] = pb[r + 4 * q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a[q]), double(b[q]), acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a[q + 1]), double(b[q + 1]), acc1, 0, 0,
+                                                  0);
+    }
+  }
   for (; r0 + 8 <= s; r0 += 8) {
     const int r = r0 + (lane >> 4);
-    const double a0 = oka ? static_cast<double>(pa[r]) : 0.0;
-    const double b0 = okb ? static_cast<double>(pb[r]) : 0.0;
-    const double a1 = oka ? static_cast<double>(pa[r + 4]) : 0.0;
-    const double b1 = okb ? static_cast<double>(pb[r + 4]) : 0.0;
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(pa[r]), double(pb[r]), acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(pa[r + 4]), double(pb[r + 4]), acc1, 0,
+                                                0, 0);
   }
   for (; r0 < s; r0 += 4) {
     const int r = r0 + (lane >> 4);
@@ -160,7 +255,7 @@ __global__ __launch_bounds__(256) void ls_gram_kernel(const LsDev* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int i = tl.bi * 16 + (lane >> 4) + 4 * q, j = tl.bj * 16 + (lane & 15);
-    if (i < k && j < k) P.G[i + int64_t(j) * k] = acc0[q] + acc1[q];
+    if (i < k && j < k) ((global_ptr<double>)P.G)[i + int64_t(j) * k] = acc0[q] + acc1[q];
   }
 }
 
@@ -171,7 +266,9 @@ __global__ __launch_bounds__(256) void ls_backsolve_kernel(const LsDev* __restri
   __shared__ double tv[16];
   const LsDev& P = probs[blockIdx.x];
   const int n = P.n, k = n + 1, t = threadIdx.x;
-  if (*P.info) return;  // breakdown: x is left as it was
+  const gptr<double> Rt = as_global(P.Rt);
+  const gptr<double> Dinv = as_global(P.Dinv);
+  if (*as_global(P.info)) return;  // breakdown: x is left as it was
   const int nb = (n + 15) / 16;
   for (int b = nb - 1; b >= 0; --b) {
     const int rb = b * 16, w = min(16, n - rb);
@@ -179,14 +276,14 @@ __global__ __launch_bounds__(256) void ls_backsolve_kernel(const LsDev* __restri
       const int i = t >> 4, q = t & 15;
       double sum = 0.0;
       if (i < w)
-        for (int c = rb + 16 + q; c < n; c += 16) sum += P.Rt[int64_t(rb + i) * k + c] * xs[c];
+        for (int c = rb + 16 + q; c < n; c += 16) sum += Rt[int64_t(rb + i) * k + c] * xs[c];
       part[i][q] = sum;
     }
     __syncthreads();
     if (t < 16) {
       double v = 0.0;
       if (t < w) {
-        v = P.Rt[int64_t(rb + t) * k + n];
+        v = Rt[int64_t(rb + t) * k + n];
         for (int q = 0; q < 16; ++q) v -= part[t][q];
       }
       tv[t] = v;
@@ -194,12 +291,12 @@ __global__ __launch_bounds__(256) void ls_backsolve_kernel(const LsDev* __restri
     __syncthreads();
     if (t < w) {
       double v = 0.0;
-      for (int j = 0; j < w; ++j) v += P.Dinv[b * 256 + t * 16 + j] * tv[j];
+      for (int j = 0; j < w; ++j) v += Dinv[b * 256 + t * 16 + j] * tv[j];
       xs[rb + t] = v;
     }
     __syncthreads();
   }
-  for (int i = t; i < n; i += blockDim.x) P.x[i] = static_cast<float>(xs[i]);
+  for (int i = t; i < n; i += blockDim.x) ((global_ptr<float>)P.x)[i] = static_cast<float>(xs[i]);
 }
 
 }  // namespace rla

@@ -26,6 +26,16 @@ namespace rla {
 // one matrix of a batched factorisation (chol.hip): G (k x k, ld k) -> Rt
 // (row-major, ld k) and the inverses of its 16 x 16 diagonal blocks; pivots of
 // columns >= check_k are not tested
+// Address-space views for device code.  A pointer read from memory (a
+// descriptor) or a volatile pointer stays generic to the compiler, and generic
+// (flat) accesses are counted both as memory and as LDS traffic, so a wait
+// for one drains the other queue too.  Accesses through these views compile
+// to global_* / ds_* instructions.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
 struct CholBatchItem {
   int k, check_k;
   const double* G;

@@ -223,27 +223,30 @@ __global__ __launch_bounds__(256) void ls_gram_kernel(const LsDev* __restrict__ 
   const gptr<float> pb = as_global(P.SAb) + int64_t(okb ? cb : 0) * s;
   f64x4 acc0 = f64x4{0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
   int r0 = 0;
-  // 32 rows per iteration: all 16 loads issued before the 8 MFMAs
-  for (; r0 + 32 <= s; r0 += 32) {
-    float a[8], b[8];
-    const int r = r0 + (lane >> 4);
+  // 32 rows per iteration: lane (i, kq) loads rows 4kq .. 4kq+3 and
+  // 16+4kq .. 16+4kq+3 of its column as two 16-byte pieces (when s is a
+  // multiple of 4, as for s = 4n, so the pieces are aligned); MFMA q of a piece takes
+  // component q, i.e. rows {q, 4+q, 8+q, 12+q} of the 16: the same row
+  // permutation on both operands, so the sum is the Gram product.
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const int kq = lane >> 4;
+  for (; (s & 3) == 0 && r0 + 32 <= s; r0 += 32) {
+    const f32x4 a0 = *reinterpret_cast<gptr<f32x4>>(pa + r0 + 4 * kq);
+    const f32x4 a1 = *reinterpret_cast<gptr<f32x4>>(pa + r0 + 16 + 4 * kq);
+    const f32x4 b0 = *reinterpret_cast<gptr<f32x4>>(pb + r0 + 4 * kq);
+    const f32x4 b1 = *reinterpret_cast<gptr<f32x4>>(pb + r0 + 16 + 4 * kq);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      a[q] = pa[r + 4 * q];
-      b[q] = pb[r + 4 * q];
+    for (int q = 0; q < 4; q += 2) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a0[q]), double(b0[q]), acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a0[q + 1]), double(b0[q + 1]), acc1, 0,
+                                                  0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < 8; q += 2) {
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a[q]), double(b[q]), acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a[q + 1]), double(b[q + 1]), acc1, 0, 0,
-                                                  0);
+    for (int q = 0; q < 4; q += 2) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a1[q]), double(b1[q]), acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(a1[q + 1]), double(b1[q + 1]), acc1, 0,
+                                                  0, 0);
     }
-  }
-  for (; r0 + 8 <= s; r0 += 8) {
-    const int r = r0 + (lane >> 4);
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(pa[r]), double(pb[r]), acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(double(pa[r + 4]), double(pb[r + 4]), acc1, 0,
-                                                0, 0);
   }
   for (; r0 < s; r0 += 4) {
     const int r = r0 + (lane >> 4);

@@ -68,19 +68,26 @@ def test_batch_sketch_bit_exact_and_solution(rla, dev, cfg):
 
 
 def test_autotuned_plan_and_custom_sketch_sizes(rla, dev):
-    probs = [make_problem(m, n, 50 + i) for i, (m, n) in enumerate([(3000, 40), (800, 64)])]
+    # s = 123 is not a multiple of 4 (the Gram kernel's unaligned-column path)
+    probs = [make_problem(m, n, 50 + i)
+             for i, (m, n) in enumerate([(3000, 40), (800, 64), (2000, 33)])]
     As = [colmajor(A, dev) for A, _, _ in probs]
     bs = [torch.as_tensor(b).to(dev) for _, b, _ in probs]
     xs = [torch.zeros(A.shape[1], dtype=F32, device=dev) for A, _, _ in probs]
-    batch = rla.SketchSolveBatch(As, bs, xs, compression_dims=[120, 1024], autotune=1)
+    batch = rla.SketchSolveBatch(As, bs, xs, compression_dims=[120, 1024, 123], autotune=1)
     cfg = batch.config()
     assert cfg["cols_per_tile"] in (8, 16, 32) and min(cfg["tuned_ms"]) > 0
     batch.run()
-    assert batch.info() == [0, 0]
+    assert batch.info() == [0, 0, 0]
     for p, (A, b, xstar) in enumerate(probs):
         want = sketch_oracle(batch.recipes[p], A, b)
         assert want.shape == (batch.shapes[p][2], A.shape[1] + 1)
         assert np.array_equal(batch.sketch(p).cpu().numpy(), want)
+        # x solves the sketched problem (FP64 normal equations, cond(SA) ~ 1e3)
+        SA, Sb = want[:, :-1].astype(np.float64), want[:, -1].astype(np.float64)
+        xls = np.linalg.lstsq(SA, Sb, rcond=None)[0]
+        got = xs[p].double().cpu().numpy()
+        assert np.linalg.norm(got - xls) <= 1e-5 * np.linalg.norm(xls), p
 
 
 def test_rank_deficient_problem_reports_breakdown(rla, dev):

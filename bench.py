@@ -272,14 +272,15 @@ def lowrank(torch, m, n, r, decay, g, dev):
     return ((U * s) @ V.T / n ** 0.5).T.contiguous().T
 
 
-def serving_c5(rla, torch, qps=40.0, n_req=64):
+def serving_c5(rla, torch, qps=80.0, n_req=96):
     """Served mixed stream: Poisson arrivals at `qps` mixing RangeFinder
     requests (k = 266, q = 2, on resident low-rank matrices 4096x1024,
     8192x2048, 16384x4096, 32768x8192 with sigma_i = exp(-i/64)) and block
     Kaczmarz solves of a resident 100000x1000 system (64-row L2Norm blocks,
-    FP64 residual every 100, rtol 1e-6).  Run twice on the same arrival
-    trace: mixed precision with 4 slots (continuous batching), and the plain
-    FP32 path with one slot (a queue)."""
+    FP64 residual every 100, rtol 1e-6).  Run three times on the same
+    arrival trace: mixed precision with 4 slots (continuous batching), the
+    plain FP32 path with 4 slots (batching alone), and the plain path with one
+    slot (a queue).  At 80 QPS the single queue saturates."""
     import numpy as np
     dev = torch.device("cuda", 0)
     F32 = torch.float32
@@ -295,7 +296,9 @@ def serving_c5(rla, torch, qps=40.0, n_req=64):
     out = {"workload": f"Poisson {qps:g} QPS, {n_req} requests, RangeFinder 4096..32768 x "
                        "1024..8192 (k=266, q=2) and Kaczmarz 100000x1000 (s=64, rtol 1e-6)",
            "rangefinder_requests": int(n_req - kinds.sum()), "kaczmarz_requests": int(kinds.sum())}
-    for label, prec, slots in (("served_bf16_4slots", "bf16", 4), ("queue_plain_1slot", "plain", 1)):
+    for label, prec, slots in (("served_bf16_4slots", "bf16", 4),
+                               ("served_plain_4slots", "plain", 4),
+                               ("queue_plain_1slot", "plain", 1)):
         with rla.Server(device=dev, slots=slots) as srv:
             for name, A in mats.items():
                 srv.register(name, A, "rangefinder")

@@ -307,6 +307,13 @@ int rla_qr_workspace_bytes(int64_t m, int64_t k, size_t* bytes);
 int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R,
                int64_t ldr, void* work, size_t work_bytes, int* used_householder);
 
+/* x <- R^-1 x (trans = 0) or R'^-1 x (trans = 1) for an n x n upper
+ * triangular R (column-major, ldr); x a device vector.  The two solves of the
+ * IHS update, R'R u = g with R the triangular factor of S*A
+ * (src/Solvers/ihs.jl:243-246). */
+int rla_trsv_upper_f64(rla_ctx* ctx, int trans, int64_t n, const double* R, int64_t ldr,
+                       double* x);
+
 /* Cholesky factor and inverse of a k x k symmetric positive definite G
  * (k <= 288, else RLA_ENOTSUP; upper triangle used): R upper with G = R'R
  * (R may alias G), X = R^-1 (column-major, ldx; NULL to skip).  *info = 0,

@@ -448,6 +448,19 @@ def full_residual(A, x, b):
                                 np.asarray(x, np.float64)))
 
 
+def ihs_step(A, b, x, SA, alpha=1.0):
+    """One iterative-Hessian-sketch update (ihs.jl:223-252) for a given
+    sketched matrix SA (s x n): r = b - A x, g = A' r, R from qr(SA),
+    u = R^-1 R'^-1 g, returns x + alpha u.  FP64."""
+    A, SA = np.asarray(A, np.float64), np.asarray(SA, np.float64)
+    b, x = np.asarray(b, np.float64), np.asarray(x, np.float64)
+    g = A.T @ (b - A @ x)
+    Rf = np.linalg.qr(SA, mode="r")
+    y = np.linalg.solve(Rf.T, g)    # R' y = g
+    u = np.linalg.solve(Rf, y)      # R u = y
+    return x + alpha * u
+
+
 def successive_inclusion(w, k):
     """Inclusion probability of each item in a weighted sample of k without
     replacement drawn item by item (P(next = i) = w_i / remaining weight),

@@ -21,7 +21,8 @@ from .approximators import (Approximator, ApproximatorAdjoint, ApproximatorRecip
                             RandSVD, RandSVDRecipe, RangeFinder, RangeFinderRecipe,
                             complete_approximator, rapproximate, rapproximate_)
 from .solvers import (BasicLogger, BasicLoggerRecipe, DistributionRecipe,  # noqa: F401
-                      FullResidual, FullResidualRecipe, Kaczmarz, KaczmarzRecipe, L2Norm,
+                      FullResidual, FullResidualRecipe, IHS, IHSRecipe, Kaczmarz,
+                      KaczmarzRecipe, L2Norm,
                       LQSolver, LQSolverRecipe, Sampling, SamplingRecipe, Uniform,
                       complete_distribution, complete_logger, complete_solver, compute_error,
                       rsolve, rsolve_, threshold_stop, update_distribution)

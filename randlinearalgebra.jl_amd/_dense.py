@@ -32,9 +32,10 @@ def gemm(C, A, B, alpha=1.0, beta=0.0):
     if a is None or b is None:
         raise ArgumentError("operands must have a unit stride in one dimension")
     name = f"rla_gemm_{_lib.sfx(C.dtype)}"
+    ldc = C.cs if n > 1 else max(1, m, C.cs)  # a single column may carry any column stride
     _lib.call(name, _lib.ctx(C.t.device), a[1], b[1], m, n, k, float(alpha), _lib.ptr(a[0]),
               max(1, a[2]), _lib.ptr(b[0]), max(1, b[2]), float(beta), _lib.ptr(C.t),
-              max(1, C.cs))
+              max(1, ldc))
 
 
 def ld8(n):

@@ -112,6 +112,7 @@ _SIGS = {
     "rla_scale_cols_f64": [vp, i64, i64, vp, i64, vp, vp, i64],
     "rla_qr_workspace_bytes": [i64, i64, vp],
     "rla_qr_f64": [vp, i64, i64, vp, i64, vp, i64, vp, ctypes.c_size_t, vp],
+    "rla_trsv_upper_f64": [vp, i32, i64, vp, i64, vp],
     "rla_chol_inv_f64": [vp, i64, vp, i64, vp, i64, vp, i64, vp],
     "rla_row_norms2_f32": [vp, i64, i64, vp, i64, i64, vp],
     "rla_row_norms2_f64": [vp, i64, i64, vp, i64, i64, vp],

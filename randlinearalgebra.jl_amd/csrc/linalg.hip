@@ -515,6 +515,26 @@ int rla_qr_workspace_bytes(int64_t m, int64_t k, size_t* bytes) {
   return RLA_OK;
 }
 
+// x <- op(R)^-1 x for an n x n upper triangular R (vendor trsv)
+int rla_trsv_upper_f64(rla_ctx* ctx, int trans, int64_t n, const double* R, int64_t ldr,
+                       double* x) {
+  RLA_CHECK_CTX(ctx);
+  if (n < 0 || ldr < std::max<int64_t>(1, n)) return fail(RLA_EARG, "rla_trsv_upper_f64: bad sizes");
+  if (n == 0) return RLA_OK;
+  if (!R || !x) return fail(RLA_EARG, "rla_trsv_upper_f64: NULL pointer");
+  if (n > INT32_MAX || ldr > INT32_MAX) return fail(RLA_ENOTSUP, "dimensions exceed int32");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  rocblas_handle h;
+  rc = blas(ctx, &h);
+  if (rc) return rc;
+  RLA_BLAS(rocblas_dtrsv(h, rocblas_fill_upper,
+                         trans ? rocblas_operation_transpose : rocblas_operation_none,
+                         rocblas_diagonal_non_unit, static_cast<int>(n), R,
+                         static_cast<int>(ldr), x, 1));
+  return RLA_OK;
+}
+
 int rla_qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R,
                int64_t ldr, void* work, size_t work_bytes, int* used_householder) {
   RLA_CHECK_CTX(ctx);

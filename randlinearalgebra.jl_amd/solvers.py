@@ -334,7 +334,10 @@ def _check_float32(*ts):
 
 
 def complete_solver(ingredients, x, A, b):
-    """complete_solver(::Kaczmarz, x, A, b) — kaczmarz.jl:162-246."""
+    """complete_solver(::Kaczmarz, x, A, b) — kaczmarz.jl:162-246; for an IHS
+    solver, ihs.jl:122-201."""
+    if isinstance(ingredients, IHS):
+        return _complete_ihs(ingredients, x, A, b)
     if not isinstance(ingredients, Kaczmarz):
         raise ArgumentError(f"No method `complete_solver` defined for {type(ingredients)}.")
     compressor = complete_compressor(ingredients.compressor, x, A, b)
@@ -382,9 +385,9 @@ def _fast_path(recipe):
 
 
 def rsolve(solver, x, A, b):
-    """rsolve!(solver, x, A, b) -> (x, recipe) for a Kaczmarz solver
+    """rsolve!(solver, x, A, b) -> (x, recipe) for a Kaczmarz or IHS solver
     (Solvers.jl:145-149), or rsolve!(recipe, x, A, b) -> None
-    (kaczmarz.jl:313-368): x is updated in place."""
+    (kaczmarz.jl:313-368, ihs.jl:214-254): x is updated in place."""
     if isinstance(solver, Solver):
         recipe = complete_solver(solver, x, A, b)
         _rsolve(recipe, x, A, b)
@@ -397,6 +400,8 @@ rsolve_ = rsolve
 
 
 def _rsolve(recipe, x, A, b):
+    if isinstance(recipe, IHSRecipe):
+        return _rsolve_ihs(recipe, x, A, b)
     log = recipe.log
     log.reset()
     a_rows, a_cols = _size2(A)
@@ -480,3 +485,126 @@ def _rsolve_loop(recipe, x, A, b):
         _lib.call("rla_kaczmarz_block_update_f32", _lib.ctx(x.device), rows_s, n,
                   _lib.ptr(Mv.t), Mv.rs, Mv.cs, _lib.ptr(bt), bt.stride(0), _lib.ptr(x),
                   recipe.alpha, _lib.ptr(work), work.numel() * 8)
+
+
+# ---------------------------------------------------------------------------
+# IHS (src/Solvers/ihs.jl)
+
+
+class IHS(Solver):
+    """IHS(compressor=SparseSign(cardinality=Left()), log=BasicLogger(),
+    error=FullResidual(), alpha=1.0) — iterative Hessian sketch for
+    overdetermined least squares, ihs.jl:1-76."""
+
+    def __init__(self, compressor=None, log=None, error=None, alpha=1.0):
+        import warnings
+        self.compressor = SparseSign(cardinality=Left()) if compressor is None else compressor
+        if self.compressor.cardinality != Left():
+            warnings.warn("Compressor has cardinality `Right` but IHS compresses from the `Left`.")
+        if alpha < 0:
+            warnings.warn("Negative step size could lead to divergent iterates.")
+        self.log = BasicLogger() if log is None else log
+        self.error = FullResidual() if error is None else error
+        self.alpha = float(alpha)
+
+
+class IHSRecipe:
+    """ihs.jl:78-121: the compressor / logger / error recipes, alpha, the
+    compressed matrix buffer (s x n, Float32, column-major), the residual
+    b - A x, the gradient A'(b - A x), and on the device side the FP64 copy of
+    the compressed matrix, its triangular factor R and the QR workspace."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def _complete_ihs(ingredients, x, A, b):
+    """complete_solver(::IHS, x, A, b) — ihs.jl:122-201 (argument checks and
+    their messages as there)."""
+    compressor = complete_compressor(ingredients.compressor, x, A, b)
+    logger = complete_logger(ingredients.log)
+    if not isinstance(ingredients.error, FullResidual):
+        raise ArgumentError(
+            f"ErrorRecipe {type(ingredients.error).__name__} does not contain the field "
+            f"'residual' and is not valid for an IHS solver.")
+    error = FullResidualRecipe(ingredients.error, b)
+    sample_size = compressor.n_rows
+    rows_a, cols_a = _size2(A)
+    if cols_a > sample_size:
+        raise ArgumentError(
+            "Compression dimension not larger than column dimension this will lead to singular "
+            "QR decompositions, which cannot be inverted.")
+    if rows_a < sample_size:
+        raise ArgumentError("Compression dimension larger row dimension.")
+    if cols_a >= rows_a:
+        raise ArgumentError("Matrix must have more rows than columns.")
+    for t in (as_view(A).t, b, x):
+        if t.dtype != torch.float32:
+            raise ArgumentError(f"the GPU IHS solver runs on Float32 data (got {t.dtype})")
+    dev = as_view(A).t.device
+    F32, F64 = torch.float32, torch.float64
+    qr_bytes = _dense.query_bytes("rla_qr_workspace_bytes", sample_size, cols_a)
+    return IHSRecipe(
+        log=logger, compressor=compressor, error=error, alpha=ingredients.alpha,
+        compressed_mat=torch.zeros(cols_a, sample_size, dtype=F32, device=dev).T,
+        residual_vec=torch.zeros(rows_a, dtype=F32, device=dev),
+        gradient_vec=torch.zeros(cols_a, dtype=F32, device=dev),
+        solution_vec=x,
+        R=torch.zeros(cols_a, cols_a, dtype=F64, device=dev).T,
+        qr_work=_dense.workspace(qr_bytes, dev), qr_bytes=qr_bytes)
+
+
+def _col(v):
+    """(n x 1) View of a contiguous device vector (leading dimension n)."""
+    return as_view(v.reshape(-1))
+
+
+def _rsolve_ihs(recipe, x, A, b):
+    """rsolve!(::IHSRecipe, x, A, b) — ihs.jl:214-254.  Per iteration:
+    g = A'r; the error and logger; a new sketch S and SA = S*A (the sparse
+    and SRHT/Gaussian kernels); R with SA = QR (FP64 CholeskyQR2 of SA,
+    Householder if it breaks down); u = R^-1 R'^-1 g (FP64); x += alpha u and
+    r -= A u (as the reference: the residual update does not scale u by
+    alpha).  FP32 vectors and products, as the reference on Float32 data."""
+    log = recipe.log
+    log.reset()
+    a_rows, a_cols = _size2(A)
+    if x.shape[0] != a_cols:
+        raise DimensionMismatch(
+            f"Dimension of `x`, {x.shape[0]} is different from number of columns in `A`,"
+            f"{a_cols}.")
+    if b.shape[0] != a_rows:
+        raise DimensionMismatch(
+            f"Dimension of `b`, {b.shape[0]} is different from number of rows in `A`,"
+            f"{a_rows}.")
+    recipe.solution_vec = x
+    Av = as_view(A)
+    dev = Av.t.device
+    ctx = _lib.ctx(dev)
+    r, g = recipe.residual_vec, recipe.gradient_vec
+    one = torch.ones(1, 1, dtype=torch.float32, device=dev)
+    # r = b - A x  (ihs.jl:218-221)
+    r.copy_(b)
+    _dense.gemm(_col(r), Av, _col(x), -1.0, 1.0)
+    S = recipe.compressor
+    for i in range(1, log.max_it + 1):
+        _dense.gemm(_col(g), Av.T, _col(r), 1.0, 0.0)  # g = A'r
+        err = compute_error(recipe.error, recipe, A, b)
+        log.update(err, i)
+        if log.converged:
+            return
+        update_compressor(S, x, A, b)
+        rows_s = S.size(1)
+        SA = recipe.compressed_mat[:rows_s]
+        mul(SA, S, A)
+        SA64 = _dense.convert(SA, torch.float64)  # s x n, column-major
+        hh = ctypes.c_int(0)
+        _lib.call("rla_qr_f64", ctx, rows_s, a_cols, _lib.ptr(SA64), max(1, rows_s),
+                  _lib.ptr(recipe.R), a_cols, _lib.ptr(recipe.qr_work), recipe.qr_bytes,
+                  ctypes.byref(hh))
+        u = _dense.convert(g.view(-1, 1), torch.float64)[:, 0].contiguous()
+        _lib.call("rla_trsv_upper_f64", ctx, 1, a_cols, _lib.ptr(recipe.R), a_cols, _lib.ptr(u))
+        _lib.call("rla_trsv_upper_f64", ctx, 0, a_cols, _lib.ptr(recipe.R), a_cols, _lib.ptr(u))
+        u32 = _dense.convert(u.view(-1, 1), torch.float32)
+        _dense.gemm(_col(x), as_view(u32), as_view(one), recipe.alpha, 1.0)  # x += alpha u
+        _dense.gemm(_col(r), Av, as_view(u32), -1.0, 1.0)                    # r -= A u

@@ -198,3 +198,15 @@ def test_fjlt_constructor_errors(rla):
         rla.FJLT(cardinality=rla.Undef())
     f = rla.FJLT()
     assert (f.compression_dim, f.block_size, f.sparsity, f.type) == (2, 10, 0.0, torch.float64)
+
+
+def test_ihs_constructor_warnings_mirror_reference(rla):
+    # ihs.jl:42-57: a Right compressor and a negative step size only warn
+    import warnings
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        s = rla.IHS(compressor=rla.SparseSign(cardinality=rla.Right()), alpha=-1.0)
+    msgs = [str(x.message) for x in w]
+    assert any("cardinality `Right`" in m for m in msgs)
+    assert any("Negative step size" in m for m in msgs)
+    assert s.alpha == -1.0 and isinstance(s.log, rla.BasicLogger)

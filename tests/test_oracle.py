@@ -299,6 +299,24 @@ def test_successive_inclusion_closed_forms():
     assert R.successive_inclusion(w, 3)[4] == 0.0
 
 
+def test_ihs_step_with_exact_hessian_is_the_least_squares_solution():
+    # with S'S = I (S = I, or S with orthonormal columns) one IHS step from any
+    # x is the Newton step of ||Ax - b||^2, i.e. the least-squares solution
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((60, 7))
+    b = rng.standard_normal(60)
+    x0 = rng.standard_normal(7)
+    xls = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert np.allclose(R.ihs_step(A, b, x0, A), xls, rtol=1e-10, atol=1e-12)
+    Q, _ = np.linalg.qr(rng.standard_normal((60, 60)))
+    assert np.allclose(R.ihs_step(A, b, x0, Q[:, :60] @ A), xls, rtol=1e-10, atol=1e-12)
+    # a sketched Hessian moves x towards xls; alpha scales the step exactly
+    SA = rng.standard_normal((30, 60)) @ A / np.sqrt(30)
+    x1 = R.ihs_step(A, b, x0, SA)
+    assert np.linalg.norm(A @ (x1 - xls)) < np.linalg.norm(A @ (x0 - xls))
+    assert np.allclose(R.ihs_step(A, b, x0, SA, 0.5) - x0, 0.5 * (x1 - x0))
+
+
 def test_full_residual_closed_form():
     A = np.array([[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]])
     assert R.full_residual(A, [1.0, 1.0], [3.0, 7.0, 2.0]) == 1.0

@@ -416,6 +416,283 @@ void srht_fused_reg_kernel(
   }
 }
 
+// ---- wave-local Float32 transform ---------------------------------------------
+// Chunks of 2^14 Float32 entries, 8 wavefronts of 64 lanes x 32 registers.
+// Wavefront w owns entries w*2^11 .. w*2^11 + 2^11 - 1 of the chunk and runs
+// stages 0..10 on them without LDS:
+//   * registers r0, r1 hold entry bits e0, e1 (one float4 per load), r2..r4
+//     hold e6, e7, e8 (the load index);
+//   * lane bits 0..3 hold e2..e5 through a linear map chosen so that the
+//     partner of entry bit e(2+i) is lane ^ {1, 2, 7, 15}[i]: the DPP patterns
+//     quad_perm, row_half_mirror and row_mirror;
+//   * lane bits 4, 5 hold e9, e10 and are exchanged with r2, r3 by
+//     v_permlane16_swap / v_permlane32_swap before their stages.
+// One DPP stage is one v_fmac_f32 per value: w = c * w[partner] + w with c =
+// +1 on the lane that keeps z + y and -1 on the lane that keeps z - y, which
+// then holds -(z - y).  A lane's values therefore carry the sign
+// (-1)^popcount(e2..e5), the same for both entries of every later butterfly;
+// it is removed when the sampled rows are written.  fl(-a + b) = -fl(a - b),
+// so values are those of the reference order (an exact zero may change sign).
+// Stages 11..13 run in registers after one LDS exchange; the cross-chunk
+// stages and the parked slots are those of srht_fused_reg_kernel.
+
+// entry bits e2..e5 held by lane nibble l (inverse of the DPP lane map)
+__device__ __forceinline__ int wl_nibble(int l) {
+  const int l0 = l & 1, l1 = (l >> 1) & 1, l2 = (l >> 2) & 1, l3 = (l >> 3) & 1;
+  return (l0 ^ l2) | ((l1 ^ l2) << 1) | ((l2 ^ l3) << 2) | (l3 << 3);
+}
+
+// w[v] = c * w[v](partner lane) + w[v] for 8 registers.  The leading s_nop
+// gives the two wait states a DPP read needs after a VALU write of the same
+// register; inside the block every register is written long before it is
+// read again through DPP.
+#define RLA_DPP_FMAC8(CTRL, p, c)                                                       \
+  asm("s_nop 1\n\t"                                                                      \
+      "v_fmac_f32_dpp %0, %0, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %1, %1, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %2, %2, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %3, %3, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %4, %4, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %5, %5, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %6, %6, %8 " CTRL "\n\t"                                           \
+      "v_fmac_f32_dpp %7, %7, %8 " CTRL                                                  \
+      : "+v"(p[0]), "+v"(p[1]), "+v"(p[2]), "+v"(p[3]), "+v"(p[4]), "+v"(p[5]), "+v"(p[6]), \
+        "+v"(p[7])                                                                       \
+      : "v"(c))
+
+template <int I>
+__device__ __forceinline__ void wl_dpp_stage(float (&x)[32], float c) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    float* p = x + 8 * g;
+    if constexpr (I == 0) RLA_DPP_FMAC8("quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf", p, c);
+    if constexpr (I == 1) RLA_DPP_FMAC8("quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf", p, c);
+    if constexpr (I == 2) RLA_DPP_FMAC8("row_half_mirror row_mask:0xf bank_mask:0xf", p, c);
+    if constexpr (I == 3) RLA_DPP_FMAC8("row_mirror row_mask:0xf bank_mask:0xf", p, c);
+  }
+}
+#undef RLA_DPP_FMAC8
+
+// exchange lane bit 4 (ROW = 16) or 5 (ROW = 32) with register bit RB
+template <int RB, int ROW>
+__device__ __forceinline__ void wl_swap(float (&x)[32]) {
+#pragma unroll
+  for (int v = 0; v < 32; ++v) {
+    if (!(v & (1 << RB))) {
+      const uint32_t a = __float_as_uint(x[v]), b = __float_as_uint(x[v | (1 << RB)]);
+      if constexpr (ROW == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      }
+    }
+  }
+}
+
+// chunk-local entry offset of register 4k+q of thread t at load time is
+// wl_load_off(t) + 64k + q
+__device__ __forceinline__ int wl_load_off(int t) {
+  const int w = t >> 6, l = t & 63;
+  return (w << 11) | (((l >> 5) & 1) << 10) | (((l >> 4) & 1) << 9) | (wl_nibble(l & 15) << 2);
+}
+// padded LDS position of register v after stage 10: thread part + constant part
+__device__ __forceinline__ int wl_store_base(int t) {
+  const int w = t >> 6, l = t & 63;
+  return pad((w << 11) | (((l >> 5) & 1) << 7) | (((l >> 4) & 1) << 6) | (wl_nibble(l & 15) << 2));
+}
+__host__ __device__ constexpr int wl_store_off(int v) {
+  // r0 e0, r1 e1, r2 e9, r3 e10, r4 e8
+  return (v & 3) + (((v >> 2) & 1) << 9) + (((v >> 3) & 1) << 10) + (((v >> 4) & 1) << 8) +
+         ((((v >> 2) & 1) << 4) + (((v >> 3) & 1) << 5) + (((v >> 4) & 1) << 3));
+}
+
+// Loads of one chunk in the wave-local order.  MODE 0: the chunk lies inside
+// the matrix and columns are 16-byte aligned; MODE 1: aligned with m % 4 == 0,
+// float4 loads clamped into the column and zeroed past row m; MODE 2: scalar
+// loads, clamped and zeroed.  Every load stays inside the column.
+template <int MODE>
+__device__ __forceinline__ void wl_load(float (&x)[32], const float* __restrict__ xc,
+                                        int64_t base, int64_t m, int off) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int e0 = static_cast<int>(base) + off;
+  if constexpr (MODE == 0) {
+    const float* p = xc + e0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const v4f q4 = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p + 64 * k));
+      x[4 * k + 0] = q4.x;
+      x[4 * k + 1] = q4.y;
+      x[4 * k + 2] = q4.z;
+      x[4 * k + 3] = q4.w;
+    }
+  } else if constexpr (MODE == 1) {
+    const int mi = static_cast<int>(m);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = e0 + 64 * k;
+      const v4f q4 = __builtin_nontemporal_load(
+          reinterpret_cast<const v4f*>(xc + (e < mi ? e : mi - 4)));
+      const bool in = e < mi;
+      x[4 * k + 0] = in ? q4.x : 0.0f;
+      x[4 * k + 1] = in ? q4.y : 0.0f;
+      x[4 * k + 2] = in ? q4.z : 0.0f;
+      x[4 * k + 3] = in ? q4.w : 0.0f;
+    }
+  } else {
+    const int mi = static_cast<int>(m);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + 64 * k + q;
+        const float v = xc[e < mi ? e : mi - 1];
+        x[4 * k + q] = e < mi ? v : 0.0f;
+      }
+  }
+}
+
+template <int NCH, int MS, int MODE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void srht_wave_kernel(
+    const float* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
+    const uint32_t* __restrict__ signs32, double scale, const uint32_t* __restrict__ plan,
+    float* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha, double beta,
+    int nslots_cap) {
+  constexpr int LC = 14;
+  constexpr int NT = 512;
+  static_assert(NCH == 2 || NCH == 4, "columns of 2 or 4 chunks");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* lds = reinterpret_cast<float*>(smem_raw);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int nslots = static_cast<int>(plan[7]);
+  const int32_t* __restrict__ slot_pos =
+      reinterpret_cast<const int32_t*>(plan + kPlanHdr + 2 * ((1 << LC) >> 5));
+  const int32_t* __restrict__ rowmap = slot_pos + nslots_cap;
+  int spad[MS];      // padded LDS index of each of this thread's slots (-1: none)
+  bool sneg[MS];     // slot carries the lane sign (-1)^popcount(e2..e5)
+#pragma unroll
+  for (int j = 0; j < MS; ++j) {
+    const int sl = t + j * NT;
+    const int e = sl < nslots ? slot_pos[sl] : 0;
+    spad[j] = sl < nslots ? pad(e) : -1;
+    sneg[j] = __builtin_popcount((e >> 2) & 15) & 1;
+  }
+  // DPP coefficients: -1 where this lane keeps the difference
+  const int nib = wl_nibble(lane & 15);
+  const float c0 = (nib & 1) ? -1.0f : 1.0f, c1 = (nib & 2) ? -1.0f : 1.0f;
+  const float c2 = (nib & 4) ? -1.0f : 1.0f, c3 = (nib & 8) ? -1.0f : 1.0f;
+  const int off = wl_load_off(t);
+  // sign bits of this thread's 32 entries in each chunk: bit 4k+q <- entry off+64k+q
+  uint32_t swc[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    uint32_t sw = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      sw |= ((signs32[(ch << (LC - 5)) + (off >> 5) + 2 * k] >> (off & 31)) & 15u) << (4 * k);
+    swc[ch] = sw;
+  }
+  const int sbase = wl_store_base(t);
+  float* lb_b = lds + pad(t);  // phase B: registers hold entry bits e9..e13
+
+  float park[NCH - 1][MS];
+  float xa[32], xb[32];
+  const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  if (my_cols > 0) wl_load<MODE>(xa, X + int64_t(blockIdx.x) * x_cs, 0, m, off);
+
+  auto chunk = [&](auto chc, float(&x)[32], float(&nx)[32], int64_t col, bool more) {
+    constexpr int ch = decltype(chc)::value;
+    // next item: the next chunk of this column, or chunk 0 of the next column.
+    // The offset is opaque per chunk so that the per-load clamps and masks of
+    // the guarded modes are not hoisted out of the column loop.
+    int offo = off;
+    asm volatile("" : "+v"(offo));
+    if (ch + 1 < NCH)
+      wl_load<MODE>(nx, X + col * x_cs, int64_t(ch + 1) << LC, m, offo);
+    else if (more)
+      wl_load<MODE>(nx, X + (col + gridDim.x) * x_cs, 0, m, offo);
+    // opaque per chunk, so the 32 per-value sign masks are not hoisted out of
+    // the column loop (they would not fit in registers for every chunk)
+    uint32_t sw = swc[ch];
+    asm volatile("" : "+v"(sw));
+    stage0_signed<32>(x, sw, scale);
+    stage<1, 32>(x);
+    wl_dpp_stage<0>(x, c0);
+    wl_dpp_stage<1>(x, c1);
+    wl_dpp_stage<2>(x, c2);
+    wl_dpp_stage<3>(x, c3);
+    stage<2, 32>(x);
+    stage<3, 32>(x);
+    stage<4, 32>(x);
+    wl_swap<2, 16>(x);
+    stage<2, 32>(x);
+    wl_swap<3, 32>(x);
+    stage<3, 32>(x);
+    __syncthreads();  // the previous chunk's slot readers are done
+#pragma unroll
+    for (int v = 0; v < 32; ++v) lds[sbase + wl_store_off(v)] = x[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 32; ++v) x[v] = lb_b[v * 528];
+    stage<2, 32>(x);
+    stage<3, 32>(x);
+    stage<4, 32>(x);
+#pragma unroll
+    for (int v = 0; v < 32; ++v) lb_b[v * 528] = x[v];
+    __syncthreads();
+    if constexpr (ch < NCH - 1) {
+#pragma unroll
+      for (int j = 0; j < MS; ++j) park[ch][j] = spad[j] >= 0 ? lds[spad[j]] : 0.0f;
+    } else {
+      const int64_t ycol = col * y_cs;
+#pragma unroll
+      for (int j = 0; j < MS; ++j) {
+        if (spad[j] < 0) continue;
+        float a[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH - 1; ++c) a[c] = park[c][j];
+        a[NCH - 1] = lds[spad[j]];
+#pragma unroll
+        for (int h = 1; h < NCH; h <<= 1)
+#pragma unroll
+          for (int c = 0; c < NCH; ++c)
+            if (!(c & h)) {
+              const float z = a[c], y = a[c | h];
+              a[c] = z + y;
+              a[c | h] = z - y;
+            }
+        const double aj = sneg[j] ? -alpha : alpha;
+        // opaque base: keeps the 4*MS row-map addresses out of the column loop
+        const int32_t* rm = rowmap;
+        asm volatile("" : "+s"(rm));
+        const int32_t* rmj = rm + (t + j * NT) * NCH;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          const int r = rmj[c];
+          if (r >= 0) emit<float>(Y, y_rs, ycol, r, double(a[c]), aj, beta);
+        }
+      }
+    }
+  };
+
+  for (int64_t kc = 0; kc < my_cols; ++kc) {
+    const int64_t col = blockIdx.x + kc * int64_t(gridDim.x);
+    const bool more = kc + 1 < my_cols;
+    // even chunks use xa, odd chunks xb; each prefetches into the other set
+    chunk(std::integral_constant<int, 0>{}, xa, xb, col, more);
+    chunk(std::integral_constant<int, 1>{}, xb, xa, col, more);
+    if constexpr (NCH == 4) {
+      chunk(std::integral_constant<int, 2 % NCH>{}, xa, xb, col, more);
+      chunk(std::integral_constant<int, 3 % NCH>{}, xb, xa, col, more);
+    }
+  }
+}
+
 // ---- shape selection ---------------------------------------------------------
 
 struct FusedShape {
@@ -620,6 +897,44 @@ static int launch_fused_reg(rla_ctx* ctx, const T* X, int64_t x_cs, int64_t m, i
   return RLA_OK;
 }
 
+template <int NCH, int MODE>
+static int launch_wave(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t ncols,
+                       const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
+                       float* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  const int64_t cap = std::min<int64_t>(s, int64_t(1) << 14);
+  const size_t lds = ((size_t(1) << 14) + (size_t(1) << 9)) * sizeof(float);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&srht_wave_kernel<NCH, 4, MODE>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  RLA_HIP(attr_err);
+  const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / lds));
+  const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
+  srht_wave_kernel<NCH, 4, MODE><<<dim3(static_cast<unsigned>(grid)), 512, lds, ctx->stream>>>(
+      X, x_cs, m, ncols, reinterpret_cast<const uint32_t*>(signs), scale, plan, Y, y_rs, y_cs,
+      alpha, beta, static_cast<int>(cap));
+  RLA_LAUNCH_CHECK("srht_wave");
+  return RLA_OK;
+}
+
+// n_pad = NCH * 2^14 rows: picks the load mode from the column alignment and m
+template <int NCH>
+static int launch_wave_any(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t ncols,
+                           const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
+                           float* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  const bool aligned = (x_cs % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+  if (aligned && m == (int64_t(NCH) << 14))
+    return launch_wave<NCH, 0>(ctx, X, x_cs, m, ncols, signs, scale, plan, s, Y, y_rs, y_cs,
+                               alpha, beta);
+  if (aligned && m % 4 == 0)
+    return launch_wave<NCH, 1>(ctx, X, x_cs, m, ncols, signs, scale, plan, s, Y, y_rs, y_cs,
+                               alpha, beta);
+  return launch_wave<NCH, 2>(ctx, X, x_cs, m, ncols, signs, scale, plan, s, Y, y_rs, y_cs, alpha,
+                             beta);
+}
+
 template <class T>
 int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
                        const int32_t* op, const uint64_t* signs, double scale, const void* plan,
@@ -651,8 +966,15 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
                                              ((plan_bytes + 255) & ~size_t(255)))
                       : nullptr;
   *handled = true;
-  // Float32 with 2 or 4 chunks: slot values parked in registers, up to 4 slots
-  // per thread
+  // Float32 with 2 or 4 chunks of 2^14: the wave-local transform, slot values
+  // parked in registers, up to 4 slots per thread
+  if constexpr (sizeof(T) == 4) {
+    if (f.LC == 14 && (f.NCH == 2 || f.NCH == 4) && nslots_cap(f, s) <= 4 * 512)
+      return f.NCH == 2 ? launch_wave_any<2>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, Y,
+                                             y_rs, y_cs, alpha, beta)
+                        : launch_wave_any<4>(ctx, X, x_cs, m, ncols, signs, scale, pl, s, Y,
+                                             y_rs, y_cs, alpha, beta);
+  }
   if (sizeof(T) == 4 && f.NCH > 1 && nslots_cap(f, s) <= 4 * (int64_t(1) << (f.LC - 5))) {
 #define RLA_FUSED_REG(LCV, NCHV)                                                               \
     if (f.LC == LCV && f.NCH == NCHV)                                                          \

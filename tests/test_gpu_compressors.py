@@ -103,7 +103,8 @@ def test_fwht_rejects_non_power_of_two(rla, dev):
 @pytest.mark.parametrize("m,n,s", [(8, 3, 4), (1000, 7, 64), (4096, 33, 512), (5000, 4, 100),
                                    (32768, 6, 1024), (65536, 16, 2048), (100000, 2, 300),
                                    (32768, 300, 64), (65536, 300, 256), (2000, 3, 2048),
-                                   (16384, 5, 700)])
+                                   (16384, 5, 700), (40000, 5, 2048), (40001, 3, 1000),
+                                   (20000, 4, 512), (50000, 1, 4000)])
 def test_srht_left_bit_exact(rla, dev, dtype, m, n, s):
     rla.seed(m + n)
     A = rnd((m, n), m, NP[dtype])
@@ -115,6 +116,24 @@ def test_srht_left_bit_exact(rla, dev, dtype, m, n, s):
     rla.mul(Cd, S, Ad, 1.5, 0.25)
     want = R.srht_left_mul(A, S.op.cpu().numpy(), S.sign_vector(), S.scale, NP[dtype], C0, 1.5,
                            0.25)
+    assert np.array_equal(Cd.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("m,lda", [(65536, 65537), (65536, 65540), (30000, 30001)])
+def test_srht_left_strided_columns_bit_exact(rla, dev, m, lda):
+    # column strides that are not a multiple of 4 (or not 16-byte aligned
+    # column starts) take the scalar-load path of the fused transform
+    rla.seed(lda)
+    n, s = 3, 1024
+    A = rnd((m, n), m, np.float32)
+    buf = torch.zeros((n, lda), dtype=F32, device=dev)
+    buf[:, :m] = torch.as_tensor(A.T)
+    Ad = buf.T[:m]
+    assert Ad.stride() == (1, lda)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=F32), Ad)
+    Cd = torch.zeros((n, s), dtype=F32, device=dev).T
+    rla.mul(Cd, S, Ad)
+    want = R.srht_left_mul(A, S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float32)
     assert np.array_equal(Cd.cpu().numpy(), want)
 
 

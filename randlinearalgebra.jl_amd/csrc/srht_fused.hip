@@ -419,22 +419,28 @@ void srht_fused_reg_kernel(
 // ---- wave-local Float32 transform ---------------------------------------------
 // Chunks of 2^14 Float32 entries, 8 wavefronts of 64 lanes x 32 registers.
 // Wavefront w owns entries w*2^11 .. w*2^11 + 2^11 - 1 of the chunk and runs
-// stages 0..10 on them without LDS:
+// stages 0..8 on them without LDS:
 //   * registers r0, r1 hold entry bits e0, e1 (one float4 per load), r2..r4
 //     hold e6, e7, e8 (the load index);
 //   * lane bits 0..3 hold e2..e5 through a linear map chosen so that the
 //     partner of entry bit e(2+i) is lane ^ {1, 2, 7, 15}[i]: the DPP patterns
 //     quad_perm, row_half_mirror and row_mirror;
-//   * lane bits 4, 5 hold e9, e10 and are exchanged with r2, r3 by
-//     v_permlane16_swap / v_permlane32_swap before their stages.
+//   * lane bits 4, 5 hold e9, e10 and the wavefront index e11..e13.
 // One DPP stage is one v_fmac_f32 per value: w = c * w[partner] + w with c =
 // +1 on the lane that keeps z + y and -1 on the lane that keeps z - y, which
 // then holds -(z - y).  A lane's values therefore carry the sign
 // (-1)^popcount(e2..e5), the same for both entries of every later butterfly;
 // it is removed when the sampled rows are written.  fl(-a + b) = -fl(a - b),
 // so values are those of the reference order (an exact zero may change sign).
-// Stages 11..13 run in registers after one LDS exchange; the cross-chunk
-// stages and the parked slots are those of srht_fused_reg_kernel.
+// Stages 9..13 run in registers after one LDS exchange (each thread then
+// holds e9..e13); the cross-chunk stages and the parked slots are those of
+// srht_fused_reg_kernel.
+//
+// LDS layout: entry e at e + (e >> 5) + 2 * (e >> 9).  The extra two words per
+// 512 entries put e9 = 1 sixteen-plus-two banks away, so the exchange's
+// stores (lanes spread over e2..e5 and e9) and loads (lanes over e0..e4) are
+// free of bank conflicts; the map is linear in the entry bits, so every access
+// is a thread base plus a compile-time offset.
 
 // entry bits e2..e5 held by lane nibble l (inverse of the DPP lane map)
 __device__ __forceinline__ int wl_nibble(int l) {
@@ -473,42 +479,15 @@ __device__ __forceinline__ void wl_dpp_stage(float (&x)[32], float c) {
 }
 #undef RLA_DPP_FMAC8
 
-// exchange lane bit 4 (ROW = 16) or 5 (ROW = 32) with register bit RB
-template <int RB, int ROW>
-__device__ __forceinline__ void wl_swap(float (&x)[32]) {
-#pragma unroll
-  for (int v = 0; v < 32; ++v) {
-    if (!(v & (1 << RB))) {
-      const uint32_t a = __float_as_uint(x[v]), b = __float_as_uint(x[v | (1 << RB)]);
-      if constexpr (ROW == 16) {
-        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
-        x[v] = __uint_as_float(r[0]);
-        x[v | (1 << RB)] = __uint_as_float(r[1]);
-      } else {
-        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
-        x[v] = __uint_as_float(r[0]);
-        x[v | (1 << RB)] = __uint_as_float(r[1]);
-      }
-    }
-  }
-}
-
 // chunk-local entry offset of register 4k+q of thread t at load time is
 // wl_load_off(t) + 64k + q
 __device__ __forceinline__ int wl_load_off(int t) {
   const int w = t >> 6, l = t & 63;
   return (w << 11) | (((l >> 5) & 1) << 10) | (((l >> 4) & 1) << 9) | (wl_nibble(l & 15) << 2);
 }
-// padded LDS position of register v after stage 10: thread part + constant part
-__device__ __forceinline__ int wl_store_base(int t) {
-  const int w = t >> 6, l = t & 63;
-  return pad((w << 11) | (((l >> 5) & 1) << 7) | (((l >> 4) & 1) << 6) | (wl_nibble(l & 15) << 2));
-}
-__host__ __device__ constexpr int wl_store_off(int v) {
-  // r0 e0, r1 e1, r2 e9, r3 e10, r4 e8
-  return (v & 3) + (((v >> 2) & 1) << 9) + (((v >> 3) & 1) << 10) + (((v >> 4) & 1) << 8) +
-         ((((v >> 2) & 1) << 4) + (((v >> 3) & 1) << 5) + (((v >> 4) & 1) << 3));
-}
+__device__ __forceinline__ int wl_pad(int e) { return e + (e >> 5) + 2 * (e >> 9); }
+// LDS words of a chunk in the wave-local layout
+constexpr int kWaveLdsWords = 16384 + 512 + 64;
 
 // Loads of one chunk in the wave-local order.  MODE 0: the chunk lies inside
 // the matrix and columns are 16-byte aligned; MODE 1: aligned with m % 4 == 0,
@@ -579,7 +558,7 @@ void srht_wave_kernel(
   for (int j = 0; j < MS; ++j) {
     const int sl = t + j * NT;
     const int e = sl < nslots ? slot_pos[sl] : 0;
-    spad[j] = sl < nslots ? pad(e) : -1;
+    spad[j] = sl < nslots ? wl_pad(e) : -1;
     sneg[j] = __builtin_popcount((e >> 2) & 15) & 1;
   }
   // DPP coefficients: -1 where this lane keeps the difference
@@ -597,8 +576,8 @@ void srht_wave_kernel(
       sw |= ((signs32[(ch << (LC - 5)) + (off >> 5) + 2 * k] >> (off & 31)) & 15u) << (4 * k);
     swc[ch] = sw;
   }
-  const int sbase = wl_store_base(t);
-  float* lb_b = lds + pad(t);  // phase B: registers hold entry bits e9..e13
+  float* lb_a = lds + wl_pad(off);  // exchange stores: register v at + (v & 3) + 66 (v >> 2)
+  float* lb_b = lds + wl_pad(t);    // exchange loads: register v holds entry t + (v << 9)
 
   float park[NCH - 1][MS];
   float xa[32], xb[32];
@@ -629,21 +608,15 @@ void srht_wave_kernel(
     stage<2, 32>(x);
     stage<3, 32>(x);
     stage<4, 32>(x);
-    wl_swap<2, 16>(x);
-    stage<2, 32>(x);
-    wl_swap<3, 32>(x);
-    stage<3, 32>(x);
     __syncthreads();  // the previous chunk's slot readers are done
 #pragma unroll
-    for (int v = 0; v < 32; ++v) lds[sbase + wl_store_off(v)] = x[v];
+    for (int v = 0; v < 32; ++v) lb_a[(v & 3) + 66 * (v >> 2)] = x[v];
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < 32; ++v) x[v] = lb_b[v * 528];
-    stage<2, 32>(x);
-    stage<3, 32>(x);
-    stage<4, 32>(x);
+    for (int v = 0; v < 32; ++v) x[v] = lb_b[530 * v];
+    stage_range<0, 5, 32>(x);
 #pragma unroll
-    for (int v = 0; v < 32; ++v) lb_b[v * 528] = x[v];
+    for (int v = 0; v < 32; ++v) lb_b[530 * v] = x[v];
     __syncthreads();
     if constexpr (ch < NCH - 1) {
 #pragma unroll
@@ -902,7 +875,7 @@ static int launch_wave(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, in
                        const uint64_t* signs, double scale, const uint32_t* plan, int64_t s,
                        float* Y, int64_t y_rs, int64_t y_cs, double alpha, double beta) {
   const int64_t cap = std::min<int64_t>(s, int64_t(1) << 14);
-  const size_t lds = ((size_t(1) << 14) + (size_t(1) << 9)) * sizeof(float);
+  const size_t lds = size_t(kWaveLdsWords) * sizeof(float);
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {

@@ -461,6 +461,24 @@ def ihs_step(A, b, x, SA, alpha=1.0):
     return x + alpha * u
 
 
+def colproj_step(A, b, x, S, alpha=1.0):
+    """One column-projection update (col_projection.jl:265-310, 356-364) for a
+    given compression S (n x k): r = b - A x, v = argmin ||(A S) v - r|| (the
+    minimum-norm least-squares solution, as the QR sub-solver's ldiv!),
+    returns x + alpha S v.  FP64."""
+    A, S = np.asarray(A, np.float64), np.asarray(S, np.float64).reshape(A.shape[1], -1)
+    b, x = np.asarray(b, np.float64), np.asarray(x, np.float64)
+    r = b - A @ x
+    v = np.linalg.lstsq(A @ S, r, rcond=None)[0]
+    return x + alpha * (S @ v)
+
+
+def ls_gradient(A, x, b):
+    """compute_error of ls_gradient.jl:39-51: ||-A'(b - A x)||_2 (FP64)."""
+    A = np.asarray(A, np.float64)
+    return float(np.linalg.norm(A.T @ (np.asarray(b, np.float64) - A @ np.asarray(x, np.float64))))
+
+
 def successive_inclusion(w, k):
     """Inclusion probability of each item in a weighted sample of k without
     replacement drawn item by item (P(next = i) = w_i / remaining weight),

@@ -20,8 +20,10 @@ from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F
 from .approximators import (Approximator, ApproximatorAdjoint, ApproximatorRecipe,  # noqa: F401
                             RandSVD, RandSVDRecipe, RangeFinder, RangeFinderRecipe,
                             complete_approximator, rapproximate, rapproximate_)
-from .solvers import (BasicLogger, BasicLoggerRecipe, DistributionRecipe,  # noqa: F401
+from .solvers import (BasicLogger, BasicLoggerRecipe, ColumnProjection,  # noqa: F401
+                      ColumnProjectionRecipe, DistributionRecipe,
                       FullResidual, FullResidualRecipe, IHS, IHSRecipe, Kaczmarz,
+                      LSGradient, LSGradientRecipe,
                       KaczmarzRecipe, L2Norm,
                       LQSolver, LQSolverRecipe, Sampling, SamplingRecipe, Uniform,
                       complete_distribution, complete_logger, complete_solver, compute_error,

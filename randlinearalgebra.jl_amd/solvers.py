@@ -275,8 +275,34 @@ class FullResidualRecipe:
         self.residual = None  # the norm is reduced on the device
 
 
+class LSGradient:
+    """LSGradient() — ls_gradient.jl:1-11: the error is the norm of the
+    least-squares gradient -A'(b - A x)."""
+
+
+class LSGradientRecipe:
+    """ls_gradient.jl:13-36: holds the gradient (Float32, length n)."""
+
+    def __init__(self, gradient):
+        self.gradient = gradient
+
+
+def _norm2(v):
+    """||v||_2 of a contiguous Float32 device vector, accumulated in FP64 by
+    the residual kernel (||v - 0 x|| with a matrix of no columns)."""
+    out = ctypes.c_double(0.0)
+    _lib.call("rla_residual_norm_f32", _lib.ctx(v.device), v.shape[0], 0, _lib.ptr(v), 1, 1,
+              _lib.ptr(v), _lib.ptr(v), ctypes.byref(out))
+    return out.value
+
+
 def compute_error(err, solver, A, b):
-    """compute_error (full_residual.jl:32-41) in FP64."""
+    """compute_error: FullResidual (full_residual.jl:32-41) in FP64;
+    LSGradient (ls_gradient.jl:39-51): gradient = -A' r with the solver's
+    residual r (Float32 product), norm accumulated in FP64."""
+    if isinstance(err, LSGradientRecipe):
+        _dense.gemm(_col(err.gradient), as_view(A).T, _col(solver.residual_vec), -1.0, 0.0)
+        return _norm2(err.gradient)
     Av = as_view(A)
     out = ctypes.c_double(0.0)
     _lib.call("rla_residual_norm_f32", _lib.ctx(Av.t.device), Av.rows, Av.cols, _lib.ptr(Av.t),
@@ -335,9 +361,12 @@ def _check_float32(*ts):
 
 def complete_solver(ingredients, x, A, b):
     """complete_solver(::Kaczmarz, x, A, b) — kaczmarz.jl:162-246; for an IHS
-    solver, ihs.jl:122-201."""
+    solver, ihs.jl:122-201; for a ColumnProjection solver,
+    col_projection.jl:175-247."""
     if isinstance(ingredients, IHS):
         return _complete_ihs(ingredients, x, A, b)
+    if isinstance(ingredients, ColumnProjection):
+        return _complete_colproj(ingredients, x, A, b)
     if not isinstance(ingredients, Kaczmarz):
         raise ArgumentError(f"No method `complete_solver` defined for {type(ingredients)}.")
     compressor = complete_compressor(ingredients.compressor, x, A, b)
@@ -385,9 +414,10 @@ def _fast_path(recipe):
 
 
 def rsolve(solver, x, A, b):
-    """rsolve!(solver, x, A, b) -> (x, recipe) for a Kaczmarz or IHS solver
-    (Solvers.jl:145-149), or rsolve!(recipe, x, A, b) -> None
-    (kaczmarz.jl:313-368, ihs.jl:214-254): x is updated in place."""
+    """rsolve!(solver, x, A, b) -> (x, recipe) for a Kaczmarz, IHS or
+    ColumnProjection solver (Solvers.jl:145-149), or rsolve!(recipe, x, A, b)
+    -> None (kaczmarz.jl:313-368, ihs.jl:214-254, col_projection.jl:322-372):
+    x is updated in place."""
     if isinstance(solver, Solver):
         recipe = complete_solver(solver, x, A, b)
         _rsolve(recipe, x, A, b)
@@ -402,6 +432,8 @@ rsolve_ = rsolve
 def _rsolve(recipe, x, A, b):
     if isinstance(recipe, IHSRecipe):
         return _rsolve_ihs(recipe, x, A, b)
+    if isinstance(recipe, ColumnProjectionRecipe):
+        return _rsolve_colproj(recipe, x, A, b)
     log = recipe.log
     log.reset()
     a_rows, a_cols = _size2(A)
@@ -608,3 +640,126 @@ def _rsolve_ihs(recipe, x, A, b):
         u32 = _dense.convert(u.view(-1, 1), torch.float32)
         _dense.gemm(_col(x), as_view(u32), as_view(one), recipe.alpha, 1.0)  # x += alpha u
         _dense.gemm(_col(r), Av, as_view(u32), -1.0, 1.0)                    # r -= A u
+
+
+# ---------------------------------------------------------------------------
+# ColumnProjection (src/Solvers/col_projection.jl)
+
+
+class ColumnProjection(Solver):
+    """ColumnProjection(alpha=1.0, compressor=SparseSign(cardinality=Right()),
+    error=LSGradient(), log=BasicLogger(), sub_solver=QRSolver()) — block
+    coordinate descent for least squares, col_projection.jl:1-121."""
+
+    def __init__(self, alpha=1.0, compressor=None, error=None, log=None, sub_solver=None):
+        import warnings
+
+        from .batched import QRSolver
+        self.compressor = SparseSign(cardinality=Right()) if compressor is None else compressor
+        if self.compressor.cardinality != Right():
+            warnings.warn("Compressor has cardinality `Left` but ColumnProjection compresses "
+                          "from the `Right`. This may cause an inefficiency.")
+        self.alpha = float(alpha)
+        self.error = LSGradient() if error is None else error
+        self.log = BasicLogger() if log is None else log
+        self.sub_solver = QRSolver() if sub_solver is None else sub_solver
+
+
+class ColumnProjectionRecipe:
+    """col_projection.jl:123-173: the compressor / logger / error recipes,
+    alpha, the compressed matrix buffer A*S (m x k, Float32, column-major),
+    the solution, the update v, the residual b - A x, and on the device side
+    the triangular factor R of A*S (FP64) and the QR workspace."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def _complete_colproj(ingredients, x, A, b):
+    """complete_solver(::ColumnProjection, x, A, b) — col_projection.jl:175-247
+    (the error-recipe check and its message as there)."""
+    from .batched import QRSolver
+    compressor = complete_compressor(ingredients.compressor, A, b)
+    logger = complete_logger(ingredients.log)
+    if not isinstance(ingredients.error, LSGradient):
+        raise ArgumentError(
+            f"ErrorRecipe {type(ingredients.error).__name__} does not contain the field "
+            f"'gradient' and is not valid for a `ColumnProjection` solver.")
+    if not isinstance(ingredients.sub_solver, QRSolver):
+        raise ArgumentError("the GPU ColumnProjection solver projects with QRSolver() only")
+    for t in (as_view(A).t, b, x):
+        if t.dtype != torch.float32:
+            raise ArgumentError(
+                f"the GPU ColumnProjection solver runs on Float32 data (got {t.dtype})")
+    rows_a, cols_a = _size2(A)
+    sample_size = compressor.n_cols
+    if sample_size > rows_a:
+        raise ArgumentError("Compression dimension larger than the row dimension of A.")
+    dev = as_view(A).t.device
+    F32, F64 = torch.float32, torch.float64
+    qr_bytes = _dense.query_bytes("rla_qr_workspace_bytes", rows_a, sample_size)
+    return ColumnProjectionRecipe(
+        compressor=compressor, log=logger,
+        error=LSGradientRecipe(torch.zeros(cols_a, dtype=F32, device=dev)),
+        sub_solver=ingredients.sub_solver, alpha=ingredients.alpha,
+        compressed_mat=torch.zeros(sample_size, rows_a, dtype=F32, device=dev).T,
+        solution_vec=x, update_vec=torch.zeros(sample_size, dtype=F32, device=dev),
+        residual_vec=torch.zeros(rows_a, dtype=F32, device=dev),
+        R=torch.zeros(sample_size, sample_size, dtype=F64, device=dev).T,
+        qr_work=_dense.workspace(qr_bytes, dev), qr_bytes=qr_bytes)
+
+
+def _rsolve_colproj(recipe, x, A, b):
+    """rsolve!(::ColumnProjectionRecipe, x, A, b) — col_projection.jl:322-372.
+    Per iteration: the LSGradient error and logger (iteration i - 1); a new
+    sketch S (n x k) and AS = A*S (the compressor kernels); v minimises
+    ||AS v - r||: R from FP64 CholeskyQR2 of AS (Householder if it breaks
+    down), v = R^-1 R'^-1 (AS)'r (FP64 solves); x += alpha S v (the
+    compressor applied to v) and r -= alpha AS v.  One column (k = 1) is the
+    same formula, v = a'r / a'a (colproj_update!, :265-275).  The error after
+    the last iteration is logged at max_it (:368-371)."""
+    log = recipe.log
+    log.reset()
+    a_rows, a_cols = _size2(A)
+    if x.shape[0] != a_cols:
+        raise DimensionMismatch(
+            f"Dimension of `x`, {x.shape[0]} is different from number of columns in `A`,"
+            f"{a_cols}.")
+    if b.shape[0] != a_rows:
+        raise DimensionMismatch(
+            f"Dimension of `b`, {b.shape[0]} is different from number of rows in `A`,"
+            f"{a_rows}.")
+    recipe.solution_vec = x
+    Av = as_view(A)
+    ctx = _lib.ctx(Av.t.device)
+    r = recipe.residual_vec
+    # r = b - A x  (col_projection.jl:329-333)
+    r.copy_(b)
+    _dense.gemm(_col(r), Av, _col(x), -1.0, 1.0)
+    S = recipe.compressor
+    for i in range(1, log.max_it + 1):
+        err = compute_error(recipe.error, recipe, A, b)
+        log.update(err, i - 1)
+        if log.converged:
+            return
+        update_compressor(S, x, A, b)
+        cols_s = S.size(2)
+        AS = recipe.compressed_mat[:, :cols_s]
+        mul(AS, A, S)
+        AS64 = _dense.convert(AS, torch.float64)  # m x k, column-major
+        hh = ctypes.c_int(0)
+        _lib.call("rla_qr_f64", ctx, a_rows, cols_s, _lib.ptr(AS64), max(1, a_rows),
+                  _lib.ptr(recipe.R), cols_s, _lib.ptr(recipe.qr_work), recipe.qr_bytes,
+                  ctypes.byref(hh))
+        v = recipe.update_vec[:cols_s]
+        _dense.gemm(_col(v), as_view(AS).T, _col(r), 1.0, 0.0)  # v = (AS)'r
+        v64 = _dense.convert(v.view(-1, 1), torch.float64)[:, 0].contiguous()
+        _lib.call("rla_trsv_upper_f64", ctx, 1, cols_s, _lib.ptr(recipe.R), cols_s,
+                  _lib.ptr(v64))
+        _lib.call("rla_trsv_upper_f64", ctx, 0, cols_s, _lib.ptr(recipe.R), cols_s,
+                  _lib.ptr(v64))
+        v.copy_(_dense.convert(v64.view(-1, 1), torch.float32)[:, 0])
+        mul(x, S, v, recipe.alpha, 1.0)                                  # x += alpha S v
+        _dense.gemm(_col(r), as_view(AS), _col(v), -recipe.alpha, 1.0)  # r -= alpha AS v
+    err = compute_error(recipe.error, recipe, A, b)
+    log.update(err, log.max_it)

@@ -210,3 +210,16 @@ def test_ihs_constructor_warnings_mirror_reference(rla):
     assert any("cardinality `Right`" in m for m in msgs)
     assert any("Negative step size" in m for m in msgs)
     assert s.alpha == -1.0 and isinstance(s.log, rla.BasicLogger)
+
+
+def test_column_projection_constructor_mirrors_reference(rla):
+    # col_projection.jl:90-121: defaults, and a Left compressor only warns
+    import warnings
+    s = rla.ColumnProjection()
+    assert s.alpha == 1.0 and s.compressor.cardinality == rla.Right()
+    assert isinstance(s.error, rla.LSGradient) and isinstance(s.log, rla.BasicLogger)
+    assert isinstance(s.sub_solver, rla.QRSolver)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        rla.ColumnProjection(compressor=rla.SparseSign(cardinality=rla.Left()), alpha=1.5)
+    assert any("cardinality `Left`" in str(x.message) for x in w)

@@ -317,6 +317,37 @@ def test_ihs_step_with_exact_hessian_is_the_least_squares_solution():
     assert np.allclose(R.ihs_step(A, b, x0, SA, 0.5) - x0, 0.5 * (x1 - x0))
 
 
+def test_colproj_step_closed_forms():
+    # S square and invertible: the projection onto the whole column space is
+    # the least-squares solution from any x (alpha = 1)
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((50, 6))
+    b = rng.standard_normal(50)
+    x0 = rng.standard_normal(6)
+    xls = np.linalg.lstsq(A, b, rcond=None)[0]
+    Q, _ = np.linalg.qr(rng.standard_normal((6, 6)))
+    assert np.allclose(R.colproj_step(A, b, x0, Q), xls, rtol=1e-10, atol=1e-12)
+    assert np.allclose(R.colproj_step(A, b, x0, 3.0 * np.eye(6)), xls, rtol=1e-10, atol=1e-12)
+    # one column e_j: exact coordinate minimisation, x_j += a_j'r / a_j'a_j, and
+    # the new residual is orthogonal to a_j
+    e = np.zeros(6)
+    e[2] = 1.0
+    x1 = R.colproj_step(A, b, x0, e)
+    r0 = b - A @ x0
+    want = x0.copy()
+    want[2] += A[:, 2] @ r0 / (A[:, 2] @ A[:, 2])
+    assert np.allclose(x1, want, rtol=1e-12)
+    assert abs(A[:, 2] @ (b - A @ x1)) < 1e-10
+    # a block of columns: the residual is orthogonal to A S; alpha scales the step
+    S = rng.standard_normal((6, 3))
+    x2 = R.colproj_step(A, b, x0, S)
+    assert np.allclose((A @ S).T @ (b - A @ x2), 0.0, atol=1e-10)
+    assert np.allclose(R.colproj_step(A, b, x0, S, 0.5) - x0, 0.5 * (x2 - x0))
+    # the LSGradient error vanishes at the least-squares solution
+    assert R.ls_gradient(A, xls, b) < 1e-10
+    assert np.isclose(R.ls_gradient(A, x0, b), np.linalg.norm(A.T @ r0))
+
+
 def test_full_residual_closed_form():
     A = np.array([[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]])
     assert R.full_residual(A, [1.0, 1.0], [3.0, 7.0, 2.0]) == 1.0

@@ -168,6 +168,35 @@ int rla_sparse_apply_ell_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, co
                              const double* X, int64_t x_rs, int64_t x_cs, double* C,
                              int64_t c_rs, int64_t c_cs, double alpha, double beta);
 
+/* Sign-packed row-interleaved copy, for operators whose nonzeros share one
+ * magnitude (SparseSign +-1/sqrt(nnz), sparse_sign.jl:248-252; CountSketch
+ * +-1, count_sketch.jl:124-127): entry q of row i is the 32-bit word
+ * column | (value < 0) << 31 at ell_sidx[q * rows + i], 0 past the row length;
+ * *magnitude receives the common |value| (host pointer; synchronises).
+ * RLA_EDOMAIN if two nonzeros differ in magnitude. */
+int rla_sparse_ell_sign_build_f32(rla_ctx* ctx, int64_t rows, int32_t width,
+                                  const int32_t* ptr, const int32_t* idx, const float* val,
+                                  uint32_t* ell_sidx, double* magnitude);
+int rla_sparse_ell_sign_build_f64(rla_ctx* ctx, int64_t rows, int32_t width,
+                                  const int32_t* ptr, const int32_t* idx, const double* val,
+                                  uint32_t* ell_sidx, double* magnitude);
+
+/* The same product as rla_sparse_apply_ell (and as mul! at
+ * sparse_sign.jl:339-361, count_sketch.jl:166-213) from the sign-packed copy:
+ * one workgroup per column of X stages the whole column in LDS, so
+ * 1 <= in_rows <= 160 KiB / element size (else RLA_ENOTSUP).  Same arithmetic
+ * and order: bit-identical to rla_sparse_apply. */
+int rla_sparse_apply_ell_sign_f32(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
+                                  const int32_t* ptr, const uint32_t* ell_sidx, double magnitude,
+                                  int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs,
+                                  float* C, int64_t c_rs, int64_t c_cs, double alpha,
+                                  double beta);
+int rla_sparse_apply_ell_sign_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
+                                  const int32_t* ptr, const uint32_t* ell_sidx, double magnitude,
+                                  int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
+                                  double* C, int64_t c_rs, int64_t c_cs, double alpha,
+                                  double beta);
+
 /* ---- Walsh–Hadamard / SRHT / FJLT -------------------------------------
  * In-place FWHT of each of ncols columns of X (length n, a power of two),
  * natural (Sylvester) order, radix-2 stages h = 1, 2, 4, ... exactly as

@@ -78,6 +78,12 @@ _SIGS = {
     "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
                              i64, i64, f64, f64],
     "rla_sparse_ell_width": [vp, i64, vp, vp],
+    "rla_sparse_ell_sign_build_f32": [vp, i64, i32, vp, vp, vp, vp, vp],
+    "rla_sparse_ell_sign_build_f64": [vp, i64, i32, vp, vp, vp, vp, vp],
+    "rla_sparse_apply_ell_sign_f32": [vp, i64, i64, vp, vp, f64, i64, vp, i64, i64, vp, i64, i64,
+                                      f64, f64],
+    "rla_sparse_apply_ell_sign_f64": [vp, i64, i64, vp, vp, f64, i64, vp, i64, i64, vp, i64, i64,
+                                      f64, f64],
     "rla_sparse_ell_build_f32": [vp, i64, i32, vp, vp, vp, vp, vp],
     "rla_sparse_ell_build_f64": [vp, i64, i32, vp, vp, vp, vp, vp],
     "rla_sparse_apply_ell_f32": [vp, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64, f64,
@@ -214,6 +220,11 @@ def call(name, *args):
     """Calls an entry point and raises the mapped exception on failure."""
     rc = getattr(load(), name)(*args)
     check(rc, name)
+
+
+def call_rc(name, *args):
+    """Calls an entry point and returns its status without raising."""
+    return int(getattr(load(), name)(*args))
 
 
 def ptr(t):

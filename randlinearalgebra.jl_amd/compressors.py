@@ -336,7 +336,11 @@ def _device_of(*xs):
 
 
 class _SparseOp:
-    """An nrows x ncols CSC matrix M on the device and its CSR twin."""
+    """An nrows x ncols CSC matrix M on the device and its CSR twin.
+    `_uniform`: all nonzeros share one magnitude (the sign-packed gather is
+    tried first; its builder verifies it)."""
+
+    _uniform = False
 
     def __init__(self, nrows, ncols, colptr, rowidx, val):
         self.nrows, self.ncols = nrows, ncols
@@ -387,11 +391,23 @@ class _SparseOp:
         width = w.value
         out = None
         if rows > 0 and 0 < width and width * rows <= 2 * self.nnz + 64 * rows:
-            ei = torch.empty(width * rows, dtype=torch.int32, device=dev)
-            ev = torch.empty(width * rows, dtype=dtype, device=dev)
-            _lib.call(f"rla_sparse_ell_build_{_lib.sfx(dtype)}", ctx, rows, width, _lib.ptr(ptr),
-                      _lib.ptr(idx), _lib.ptr(v), _lib.ptr(ei), _lib.ptr(ev))
-            out = (ptr, ei, ev)
+            in_rows = self.nrows if trans_m else self.ncols
+            if self._uniform and in_rows * torch.finfo(dtype).bits // 8 <= 160 * 1024:
+                # one magnitude: signs packed into the index words, whole X
+                # columns staged in LDS
+                es = torch.empty(width * rows, dtype=torch.int32, device=dev)
+                mag = ctypes.c_double(0.0)
+                rc = _lib.call_rc(f"rla_sparse_ell_sign_build_{_lib.sfx(dtype)}", ctx, rows,
+                                  width, _lib.ptr(ptr), _lib.ptr(idx), _lib.ptr(v), _lib.ptr(es),
+                                  ctypes.byref(mag))
+                if rc == 0:
+                    out = ("sign", ptr, es, mag.value)
+            if out is None:
+                ei = torch.empty(width * rows, dtype=torch.int32, device=dev)
+                ev = torch.empty(width * rows, dtype=dtype, device=dev)
+                _lib.call(f"rla_sparse_ell_build_{_lib.sfx(dtype)}", ctx, rows, width,
+                          _lib.ptr(ptr), _lib.ptr(idx), _lib.ptr(v), _lib.ptr(ei), _lib.ptr(ev))
+                out = ("ell", ptr, ei, ev)
         self._ell[key] = out
         return out
 
@@ -400,8 +416,14 @@ class _SparseOp:
         dt = X.dtype
         if X.rs == 1 and X.rows > 0 and X.cols > 0:
             ell = self._ell_for(trans_m, dt)
+            if ell is not None and ell[0] == "sign":
+                _, ptr, es, mag = ell
+                _lib.call(f"rla_sparse_apply_ell_sign_{_lib.sfx(dt)}", _lib.ctx(X.t.device),
+                          C.rows, X.rows, _lib.ptr(ptr), _lib.ptr(es), mag, X.cols,
+                          _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
+                return
             if ell is not None:
-                ptr, ei, ev = ell
+                _, ptr, ei, ev = ell
                 _lib.call(f"rla_sparse_apply_ell_{_lib.sfx(dt)}", _lib.ctx(X.t.device),
                           C.rows, X.rows, _lib.ptr(ptr), _lib.ptr(ei), _lib.ptr(ev), X.cols,
                           _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
@@ -506,6 +528,7 @@ class SparseSignRecipe(_SparseRecipe):
         val = torch.empty(initial * nnz, dtype=type_, device=dev)
         self._gen(rowidx, val, compression_dim, initial)
         self.op = _SparseOp(compression_dim, initial, colptr, rowidx, val)
+        self.op._uniform = True  # values +-1/sqrt(nnz)
 
     def _gen(self, rowidx, val, s, initial):
         sd, ep = _draw()
@@ -559,6 +582,7 @@ class CountSketchRecipe(_SparseRecipe):
         val = torch.empty(initial, dtype=type_, device=dev)
         self._gen(rowidx, val, compression_dim, initial)
         self.op = _SparseOp(compression_dim, initial, colptr, rowidx, val)
+        self.op._uniform = True  # values +-1
 
     @property
     def mat(self):

@@ -30,6 +30,8 @@
 
 #include <algorithm>
 
+#include <mutex>
+
 #include "rla_internal.hpp"
 
 namespace rla {
@@ -468,6 +470,163 @@ static int sparse_apply_ell(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, con
                                 c_rs, c_cs, alpha, beta);
 }
 
+// ---- sign-packed interleaved gather --------------------------------------------
+// For operators whose nonzeros all share one magnitude (SparseSign +-1/sqrt(nnz),
+// CountSketch +-1, Sampling 1): entry q of row i is one 32-bit word,
+// column | (value < 0) << 31, at sidx[q * rows + i], and the magnitude is a
+// scalar.  A third of the metadata of the index + value arrays.
+
+template <class T>
+__global__ void ell_sign_build_kernel(int64_t rows, int32_t width,
+                                      const int32_t* __restrict__ ptr,
+                                      const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                      uint32_t* __restrict__ sidx, int32_t* __restrict__ mixed) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  const T mag = val[0] < T(0) ? -val[0] : val[0];
+  const int32_t p = ptr[i], n = ptr[i + 1] - p;
+  bool bad = false;
+  for (int32_t q = 0; q < width; ++q) {
+    uint32_t e = 0;
+    if (q < n) {
+      const T v = val[p + q];
+      bad |= !(v == mag || v == -mag) || idx[p + q] < 0;
+      e = static_cast<uint32_t>(idx[p + q]) | (v < T(0) ? 0x80000000u : 0u);
+    }
+    sidx[int64_t(q) * rows + i] = e;
+  }
+  if (bad) atomicOr(mixed, 1);
+}
+
+constexpr int kSignThreads = 1024;
+constexpr size_t kSignLdsMax = 160 * 1024;
+
+// One workgroup per column of X: the column is staged in LDS, then each
+// thread sums its output rows' entries in ascending column order with the
+// products of the index/value kernel (bit-identical results).
+template <class T>
+__global__ __launch_bounds__(kSignThreads) void rowgather_sign_kernel(
+    int64_t out_rows, int64_t in_rows, const int32_t* __restrict__ ptr,
+    const uint32_t* __restrict__ sidx, T mag, const T* __restrict__ X, int64_t x_cs,
+    T* __restrict__ C, int64_t c_rs, int64_t c_cs, double alpha, double beta, int vec16) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int64_t k = blockIdx.x;
+  const T* xc = X + k * x_cs;
+  if (vec16) {
+    typedef T v16 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int VW = 16 / sizeof(T);
+    const int64_t nv = in_rows / VW;
+    for (int64_t vb = 0; vb < nv; vb += 4 * int64_t(kSignThreads)) {
+      v16 tmp[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t v = vb + u * int64_t(kSignThreads) + threadIdx.x;
+        if (v < nv) tmp[u] = *reinterpret_cast<const v16*>(xc + v * VW);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t v = vb + u * int64_t(kSignThreads) + threadIdx.x;
+        if (v < nv) *reinterpret_cast<v16*>(lds + v * VW) = tmp[u];
+      }
+    }
+    for (int64_t j = nv * VW + threadIdx.x; j < in_rows; j += kSignThreads) lds[j] = xc[j];
+  } else {
+    for (int64_t j = threadIdx.x; j < in_rows; j += kSignThreads) lds[j] = xc[j];
+  }
+  __syncthreads();
+  const double mpos = double(mag), mneg = -double(mag);
+  for (int64_t i = threadIdx.x; i < out_rows; i += kSignThreads) {
+    const int32_t n = ptr[i + 1] - ptr[i];
+    T* cp = C + i * c_rs + k * c_cs;
+    T acc = (beta == 0.0) ? T(0) : static_cast<T>(beta * double(*cp));
+    const uint32_t* sx = sidx + i;
+    int32_t p = 0;
+    for (; p + 8 <= n; p += 8) {
+      uint32_t e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) e[u] = sx[int64_t(p + u) * out_rows];
+      T xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = lds[e[u] & 0x7fffffffu];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        acc = static_cast<T>(double(acc) +
+                             ((e[u] >> 31) ? mneg : mpos) * (double(xv[u]) * alpha));
+    }
+    for (; p < n; ++p) {
+      const uint32_t e = sx[int64_t(p) * out_rows];
+      acc = static_cast<T>(double(acc) +
+                           ((e >> 31) ? mneg : mpos) * (double(lds[e & 0x7fffffffu]) * alpha));
+    }
+    *cp = acc;
+  }
+}
+
+template <class T>
+static int ell_sign_build(rla_ctx* ctx, int64_t rows, int32_t width, const int32_t* ptr,
+                          const int32_t* idx, const T* val, uint32_t* sidx, double* magnitude) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || width < 0) return fail(RLA_EARG, "negative dimensions");
+  if (!magnitude) return fail(RLA_EARG, "magnitude is NULL");
+  *magnitude = 0.0;
+  if (rows == 0 || width == 0) return RLA_OK;
+  if (!ptr || !idx || !val || !sidx) return fail(RLA_EARG, "sign-packed ELL build: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = workspace(ctx, 16, &ws);
+  if (rc) return rc;
+  int32_t* mixed = static_cast<int32_t*>(ws);
+  set_i32_kernel<<<1, 64, 0, ctx->stream>>>(mixed, 0);
+  ell_sign_build_kernel<T><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, width, ptr, idx, val, sidx, mixed);
+  RLA_LAUNCH_CHECK("ell_sign_build");
+  int32_t h_mixed = 0;
+  T h_v0 = T(0);
+  RLA_HIP(hipMemcpyAsync(&h_mixed, mixed, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipMemcpyAsync(&h_v0, val, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (h_mixed)
+    return fail(RLA_EDOMAIN, "sign-packed ELL: the nonzeros do not share one magnitude");
+  *magnitude = double(h_v0 < T(0) ? -h_v0 : h_v0);
+  return RLA_OK;
+}
+
+template <class T>
+static int sparse_apply_ell_sign(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
+                                 const int32_t* ptr, const uint32_t* sidx, double magnitude,
+                                 int64_t n_rhs, const T* X, int64_t x_rs, int64_t x_cs, T* C,
+                                 int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (out_rows < 0 || in_rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  if (out_rows == 0 || n_rhs == 0) return RLA_OK;
+  if (!ptr || !sidx || !C || (in_rows > 0 && !X))
+    return fail(RLA_EARG, "sign-packed sparse apply: NULL pointer");
+  if (x_rs != 1) return fail(RLA_EARG, "sign-packed sparse apply needs X column-major (x_rs == 1)");
+  if (in_rows < 1 || size_t(in_rows) * sizeof(T) > kSignLdsMax)
+    return fail(RLA_ENOTSUP, "sign-packed sparse apply stages a whole column of X: "
+                             "1 <= in_rows <= 160 KiB / element size");
+  if (n_rhs > INT32_MAX) return fail(RLA_ENOTSUP, "sign-packed sparse apply: too many columns");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const size_t lds = size_t(in_rows) * sizeof(T);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&rowgather_sign_kernel<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(kSignLdsMax));
+  });
+  RLA_HIP(attr_err);
+  const int vw = 16 / static_cast<int>(sizeof(T));
+  const int vec16 = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (x_cs % vw == 0);
+  rowgather_sign_kernel<T><<<dim3(static_cast<unsigned>(n_rhs)), kSignThreads, lds,
+                             ctx->stream>>>(out_rows, in_rows, ptr, sidx, static_cast<T>(magnitude),
+                                            X, x_cs, C, c_rs, c_cs, alpha, beta, vec16);
+  RLA_LAUNCH_CHECK("rowgather_sign");
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -536,6 +695,33 @@ int rla_sparse_apply_ell_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows, co
                              int64_t c_rs, int64_t c_cs, double alpha, double beta) {
   return sparse_apply_ell<double>(ctx, out_rows, in_rows, ptr, ell_idx, ell_val, n_rhs, X, x_rs,
                                   x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_ell_sign_build_f32(rla_ctx* ctx, int64_t rows, int32_t width,
+                                  const int32_t* ptr, const int32_t* idx, const float* val,
+                                  uint32_t* ell_sidx, double* magnitude) {
+  return ell_sign_build<float>(ctx, rows, width, ptr, idx, val, ell_sidx, magnitude);
+}
+int rla_sparse_ell_sign_build_f64(rla_ctx* ctx, int64_t rows, int32_t width,
+                                  const int32_t* ptr, const int32_t* idx, const double* val,
+                                  uint32_t* ell_sidx, double* magnitude) {
+  return ell_sign_build<double>(ctx, rows, width, ptr, idx, val, ell_sidx, magnitude);
+}
+int rla_sparse_apply_ell_sign_f32(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
+                                  const int32_t* ptr, const uint32_t* ell_sidx, double magnitude,
+                                  int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs,
+                                  float* C, int64_t c_rs, int64_t c_cs, double alpha,
+                                  double beta) {
+  return sparse_apply_ell_sign<float>(ctx, out_rows, in_rows, ptr, ell_sidx, magnitude, n_rhs, X,
+                                      x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+int rla_sparse_apply_ell_sign_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_rows,
+                                  const int32_t* ptr, const uint32_t* ell_sidx, double magnitude,
+                                  int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
+                                  double* C, int64_t c_rs, int64_t c_cs, double alpha,
+                                  double beta) {
+  return sparse_apply_ell_sign<double>(ctx, out_rows, in_rows, ptr, ell_sidx, magnitude, n_rhs,
+                                       X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
 }
 
 }  // extern "C"

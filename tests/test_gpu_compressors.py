@@ -348,6 +348,78 @@ def test_sparse_interleaved_apply_equals_csr_apply(rla, dev, dtype, trans_m, m):
                  lib.ptr(ev), ncols, lib.ptr(X), ncols, 1, lib.ptr(C2), 1, rows, 1.0, 0.0)
 
 
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("trans_m", [0, 1])
+@pytest.mark.parametrize("m", [700, 8192])
+def test_sparse_sign_packed_apply_equals_csr_apply(rla, dev, dtype, trans_m, m):
+    # the sign-packed gather (column | sign << 31, one magnitude) against the
+    # CSR/CSC apply, bit for bit: ragged rows, an empty row, beta != 0, a
+    # column count that is not a multiple of the vector width, and an odd X
+    # column stride (scalar staging); a mixed-magnitude operator is refused
+    import ctypes
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    s, nnz, ncols = 96, 3, 5
+    hi = np.zeros(m * nnz, np.int32)
+    hv = np.zeros(m * nnz)
+    lib.load().rla_host_sparse_sign_gen_f64(77, 1, m, s - 1, nnz, hi.ctypes.data, hv.ctypes.data)
+    colptr = torch.arange(0, m * nnz + 1, nnz, dtype=torch.int32, device=dev)
+    rowidx = torch.as_tensor(hi, device=dev)               # row s-1 of M stays empty
+    val = torch.as_tensor(hv, device=dev).to(dtype)
+    rowptr = torch.empty(s + 1, dtype=torch.int32, device=dev)
+    colidx = torch.empty(m * nnz, dtype=torch.int32, device=dev)
+    val_t = torch.empty(m * nnz, dtype=dtype, device=dev)
+    sfx = lib.sfx(dtype)
+    lib.call(f"rla_sparse_transpose_{sfx}", ctx, s, m, m * nnz, lib.ptr(colptr), lib.ptr(rowidx),
+             lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t))
+    rows, in_rows = (m, s) if trans_m else (s, m)
+    ptr, idx, v = (colptr, rowidx, val) if trans_m else (rowptr, colidx, val_t)
+    w = ctypes.c_int32(0)
+    lib.call("rla_sparse_ell_width", ctx, rows, lib.ptr(ptr), ctypes.byref(w))
+    es = torch.empty(w.value * rows, dtype=torch.int32, device=dev)
+    mag = ctypes.c_double(0.0)
+    lib.call(f"rla_sparse_ell_sign_build_{sfx}", ctx, rows, w.value, lib.ptr(ptr), lib.ptr(idx),
+             lib.ptr(v), lib.ptr(es), ctypes.byref(mag))
+    assert mag.value == float(abs(val[0]))
+    for ld in (in_rows, in_rows + 1):
+        Xbuf = colmajor(rnd((ld, ncols), 3, NP[dtype]), dev)
+        X = Xbuf[:in_rows]
+        C0 = colmajor(rnd((rows, ncols), 4, NP[dtype]), dev)
+        C1, C2 = C0.clone(), C0.clone()
+        lib.call(f"rla_sparse_apply_{sfx}", ctx, trans_m, s, m, lib.ptr(colptr), lib.ptr(rowidx),
+                 lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t), ncols,
+                 lib.ptr(X), 1, ld, lib.ptr(C1), 1, rows, 1.5, -0.5)
+        lib.call(f"rla_sparse_apply_ell_sign_{sfx}", ctx, rows, in_rows, lib.ptr(ptr), lib.ptr(es),
+                 mag.value, ncols, lib.ptr(X), 1, ld, lib.ptr(C2), 1, rows, 1.5, -0.5)
+        assert torch.equal(C1, C2), ld
+    want = R.sparse_mul(S_dense(colptr, rowidx, val, s, m, trans_m),
+                        Xbuf[:in_rows].cpu().numpy(), 1.5, -0.5, C0.cpu().numpy())
+    assert np.array_equal(C2.cpu().numpy(), want)
+    # two magnitudes: refused
+    val2 = val.clone()
+    val2[0] = 2 * val2[0]
+    v2 = val2 if trans_m else torch.empty_like(val_t)
+    if not trans_m:
+        lib.call(f"rla_sparse_transpose_{sfx}", ctx, s, m, m * nnz, lib.ptr(colptr),
+                 lib.ptr(rowidx), lib.ptr(val2), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(v2))
+    with pytest.raises(rla.DomainError):
+        lib.call(f"rla_sparse_ell_sign_build_{sfx}", ctx, rows, w.value, lib.ptr(ptr),
+                 lib.ptr(idx), lib.ptr(v2), lib.ptr(es), ctypes.byref(mag))
+
+
+def test_sparse_sign_recipe_uses_packed_gather_bit_exact(rla, dev):
+    # C1 shape: the recipe's mul! takes the sign-packed path and equals the oracle
+    rla.seed(1234)
+    A = rnd((8192, 16), 1234)
+    Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=1024, nnz=8), Ad)
+    C = torch.zeros(16, 1024, dtype=F64, device=dev).T
+    rla.mul(C, S, Ad)
+    assert S.op._ell[(0, F64)][0] == "sign"
+    want = R.sparse_mul(S.op.dense(), A)
+    assert np.array_equal(C.cpu().numpy(), want)
+
+
 def S_dense(colptr, rowidx, val, s, m, trans_m):
     M = np.zeros((s, m))
     cp, ri, v = colptr.cpu().numpy(), rowidx.cpu().numpy(), val.double().cpu().numpy()

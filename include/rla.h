@@ -197,6 +197,19 @@ int rla_sparse_apply_ell_sign_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_row
                                   double* C, int64_t c_rs, int64_t c_cs, double alpha,
                                   double beta);
 
+/* Agmon maximal-residual selection, the top-k step of sample_distribution!
+ * for the Agmon distribution (src/Compressors/Distributions/agmon.jl:354-416):
+ * out[t] (t < k) = the candidate with the t-th largest |v[c]|, ties to the
+ * smaller index, over the candidates cand[0..n_cand) (device int32, 0-based;
+ * NULL = 0..n_items-1 with n_cand = n_items).  v (length n_items) is the
+ * residual A x - b (Left) or A'(A x - b) (Right).  A repeated candidate may be
+ * chosen once per occurrence; a candidate outside 0..n_items-1 is never chosen
+ * (out[t] = -1 when none is left).  1 <= k <= n_cand <= 2^20. */
+int rla_agmon_select_f32(rla_ctx* ctx, int64_t n_items, const int32_t* cand, int64_t n_cand,
+                         const float* v, int64_t k, int32_t* out);
+int rla_agmon_select_f64(rla_ctx* ctx, int64_t n_items, const int32_t* cand, int64_t n_cand,
+                         const double* v, int64_t k, int32_t* out);
+
 /* ---- Walsh–Hadamard / SRHT / FJLT -------------------------------------
  * In-place FWHT of each of ncols columns of X (length n, a power of two),
  * natural (Sylvester) order, radix-2 stages h = 1, 2, 4, ... exactly as

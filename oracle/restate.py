@@ -479,6 +479,16 @@ def ls_gradient(A, x, b):
     return float(np.linalg.norm(A.T @ (np.asarray(b, np.float64) - A @ np.asarray(x, np.float64))))
 
 
+def agmon_select(v, cand, k):
+    """The top-k step of sample_distribution!(::AgmonRecipe)
+    (agmon.jl:400-414): partialsortperm of the candidates by
+    (-|v[c]|, c); returns the chosen candidates (0-based) in that order."""
+    v = np.abs(np.asarray(v, np.float64))
+    cand = np.asarray(cand, np.int64)
+    order = sorted(range(len(cand)), key=lambda i: (-v[cand[i]], cand[i]))
+    return [int(cand[i]) for i in order[:k]]
+
+
 def successive_inclusion(w, k):
     """Inclusion probability of each item in a weighted sample of k without
     replacement drawn item by item (P(next = i) = w_i / remaining weight),

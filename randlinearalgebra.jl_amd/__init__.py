@@ -20,7 +20,8 @@ from .compressors import (Cardinality, Compressor, CompressorAdjoint,  # noqa: F
 from .approximators import (Approximator, ApproximatorAdjoint, ApproximatorRecipe,  # noqa: F401
                             RandSVD, RandSVDRecipe, RangeFinder, RangeFinderRecipe,
                             complete_approximator, rapproximate, rapproximate_)
-from .solvers import (BasicLogger, BasicLoggerRecipe, ColumnProjection,  # noqa: F401
+from .solvers import (Agmon, AgmonRecipe, BasicLogger, BasicLoggerRecipe,  # noqa: F401
+                      ColumnProjection, sample_distribution,
                       ColumnProjectionRecipe, DistributionRecipe,
                       FullResidual, FullResidualRecipe, IHS, IHSRecipe, Kaczmarz,
                       LSGradient, LSGradientRecipe,

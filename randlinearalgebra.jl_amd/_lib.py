@@ -78,6 +78,8 @@ _SIGS = {
     "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
                              i64, i64, f64, f64],
     "rla_sparse_ell_width": [vp, i64, vp, vp],
+    "rla_agmon_select_f32": [vp, i64, vp, i64, vp, i64, vp],
+    "rla_agmon_select_f64": [vp, i64, vp, i64, vp, i64, vp],
     "rla_sparse_ell_sign_build_f32": [vp, i64, i32, vp, vp, vp, vp, vp],
     "rla_sparse_ell_sign_build_f64": [vp, i64, i32, vp, vp, vp, vp, vp],
     "rla_sparse_apply_ell_sign_f32": [vp, i64, i64, vp, vp, f64, i64, vp, i64, i64, vp, i64, i64,

@@ -94,16 +94,125 @@ class DistributionRecipe:
         return int((self.weights > 0).sum())
 
 
-def complete_distribution(dist, A):
+class Agmon(Distribution):
+    """Agmon(cardinality=Undef(), replace=False, beta=1) — maximal-residual
+    selection among beta random candidates, agmon.jl:1-70."""
+
+    def __init__(self, cardinality=None, replace=False, beta=1):
+        if beta < 1:
+            raise ArgumentError(f"`Agmon` beta must be >= 1, got beta={beta}")
+        self.cardinality = Undef() if cardinality is None else cardinality
+        self.replace = bool(replace)
+        self.beta = int(beta)
+
+
+class AgmonRecipe:
+    """agmon.jl:72-133: cardinality, replace, beta, the size of the state
+    space (indices 0..d-1), the candidate buffer (device int32, beta entries,
+    allocated on first use), references to A, b, x and the optional stored
+    residual r (A x - b, maintained by the caller) or None."""
+
+    def __init__(self, dist, x, A, b):
+        self.cardinality = dist.cardinality
+        self.replace = dist.replace
+        self.beta = dist.beta
+        self.sample_buffer = None
+        self.r = None
+        self._set(x, A, b, "complete_distribution")
+
+    def _set(self, x, A, b, name):
+        """The checks of complete_distribution / update_distribution!
+        (agmon.jl:156-205, 235-301), with their messages."""
+        n_rows, n_cols = _size2(A)
+        if self.cardinality == Left():
+            d, x_len, b_len = n_rows, n_cols, n_rows
+        elif self.cardinality == Right():
+            d, x_len, b_len = n_cols, n_cols, n_rows
+        else:
+            raise ArgumentError("`Agmon` cardinality must be specified as `Left()` or `Right()`."
+                                f"`Undef()` is not allowed in `{name}`.")
+        if b.shape[0] != b_len:
+            raise DimensionMismatch(
+                f"Vector b has length {b.shape[0]}, expected {b_len} to match rows of A")
+        if x.shape[0] != x_len:
+            raise DimensionMismatch(
+                f"Vector x has length {x.shape[0]}, expected {x_len} to match columns of A")
+        if self.beta > d:
+            dim_name = "rows" if self.cardinality == Left() else "columns"
+            raise ArgumentError(
+                f"`Agmon` beta must be <= number of {dim_name} ({d}), got beta={self.beta}")
+        self.state_space = d
+        self.A, self.b, self.x = A, b, x
+
+
+def complete_distribution(dist, *args):
+    """complete_distribution(dist, A) for Uniform / L2Norm; (dist, x, A, b)
+    for Agmon (agmon.jl:156-205)."""
+    if isinstance(dist, Agmon):
+        if len(args) != 3:
+            raise ArgumentError("`Agmon` is completed with complete_distribution(distribution, "
+                                "x, A, b).")
+        return AgmonRecipe(dist, *args)
+    A = args[0] if len(args) == 1 else args[1]
     return DistributionRecipe(dist, A)
 
 
-def update_distribution(recipe, *args):
+def update_distribution(recipe, *args, r=None):
     """update_distribution!(recipe, A) or (recipe, x, A, b): L2Norm recomputes
     its weights from A (strohmer_vershynin.jl:115-134); they are cached while
-    A is unchanged."""
+    A is unchanged.  Agmon (agmon.jl:235-301) stores x, A, b and the optional
+    residual r = A x - b kept by the caller (None: computed when sampling)."""
+    if isinstance(recipe, AgmonRecipe):
+        x, A, b = args
+        recipe._set(x, A, b, "update_distribution!")
+        recipe.r = r
+        return
     A = args[0] if len(args) == 1 else args[1]
     recipe._compute(A)
+
+
+def sample_distribution(indices, recipe):
+    """sample_distribution!(indices, ::AgmonRecipe) — agmon.jl:354-416:
+    indices (device int32, 0-based) receive the len(indices) candidates with
+    the largest |residual|, descending, ties to the smaller index.  Candidates:
+    one uniform draw when beta = 1, all indices when beta >= d, else beta
+    uniform draws (with or without replacement; at most 1024).  Residuals:
+    rows |A x - b|, columns |A'(A x - b)|, with the stored r when given."""
+    if not isinstance(recipe, AgmonRecipe):
+        raise ArgumentError(f"No method `sample_distribution!` defined for {type(recipe)}.")
+    k = indices.shape[0]
+    if k > recipe.beta:
+        raise ArgumentError(f"`Agmon` cannot output {k} indices when beta={recipe.beta}.")
+    d = recipe.state_space
+    dev = indices.device
+    ctx = _lib.ctx(dev)
+    sd, ep = _draw()
+    if recipe.beta == 1:
+        _lib.call("rla_wsample", ctx, sd, ep, d, None, 1, 0, 1, _lib.ptr(indices))
+        return
+    cand, n_cand = None, d
+    if recipe.beta < d:
+        if recipe.beta > 1024:
+            raise ArgumentError("the device sampler draws at most 1024 `Agmon` candidates "
+                                "(beta < d)")
+        if recipe.sample_buffer is None or recipe.sample_buffer.shape[0] != recipe.beta:
+            recipe.sample_buffer = torch.empty(recipe.beta, dtype=torch.int32, device=dev)
+        cand, n_cand = recipe.sample_buffer, recipe.beta
+        _lib.call("rla_wsample", ctx, sd, ep, d, None, recipe.beta, 1 if recipe.replace else 0,
+                  1, _lib.ptr(cand))
+    Av = as_view(recipe.A)
+    if recipe.r is not None:
+        res = recipe.r
+    else:
+        res = recipe.b.clone()  # r = A x - b
+        _dense.gemm(_col(res), Av, _col(recipe.x), 1.0, -1.0)
+    if recipe.cardinality == Left():
+        v = res
+    else:
+        v = torch.empty(Av.cols, dtype=res.dtype, device=dev)
+        _dense.gemm(_col(v), Av.T, _col(res), 1.0, 0.0)
+    _lib.call(f"rla_agmon_select_{_lib.sfx(v.dtype)}", ctx, d, _lib.ptr(cand), n_cand,
+              _lib.ptr(v.contiguous()), k, _lib.ptr(indices))
 
 
 # ---------------------------------------------------------------------------

@@ -223,3 +223,31 @@ def test_column_projection_constructor_mirrors_reference(rla):
         warnings.simplefilter("always")
         rla.ColumnProjection(compressor=rla.SparseSign(cardinality=rla.Left()), alpha=1.5)
     assert any("cardinality `Left`" in str(x.message) for x in w)
+
+
+def test_agmon_checks_mirror_reference(rla):
+    # agmon.jl:58-66, 156-205, 235-301 (shape checks only: no device work)
+    with pytest.raises(rla.ArgumentError, match="beta must be >= 1, got beta=0"):
+        rla.Agmon(beta=0)
+    A, b, x = torch.zeros(10, 3), torch.zeros(10), torch.zeros(3)
+    with pytest.raises(rla.ArgumentError, match="`Undef\\(\\)` is not allowed in `complete_distribution`"):
+        rla.complete_distribution(rla.Agmon(), x, A, b)
+    with pytest.raises(rla.ArgumentError, match=r"beta must be <= number of rows \(10\), got beta=11"):
+        rla.complete_distribution(rla.Agmon(cardinality=rla.Left(), beta=11), x, A, b)
+    with pytest.raises(rla.ArgumentError, match=r"number of columns \(3\), got beta=4"):
+        rla.complete_distribution(rla.Agmon(cardinality=rla.Right(), beta=4), x, A, b)
+    with pytest.raises(rla.DimensionMismatch, match="Vector b has length 9, expected 10"):
+        rla.complete_distribution(rla.Agmon(cardinality=rla.Left()), x, A, torch.zeros(9))
+    with pytest.raises(rla.DimensionMismatch, match="Vector x has length 4, expected 3"):
+        rla.complete_distribution(rla.Agmon(cardinality=rla.Right()), torch.zeros(4), A, b)
+    mr = rla.complete_distribution(rla.Agmon(cardinality=rla.Left(), beta=5), x, A, b)
+    assert mr.state_space == 10 and mr.r is None
+    with pytest.raises(rla.ArgumentError, match=r"number of rows \(4\), got beta=5"):
+        rla.update_distribution(mr, x, torch.zeros(4, 3), torch.zeros(4))
+    r = torch.ones(10)
+    rla.update_distribution(mr, x, A, b, r=r)
+    assert mr.r is r
+    rla.update_distribution(mr, x, A, b)
+    assert mr.r is None
+    with pytest.raises(rla.ArgumentError, match="cannot output 6 indices when beta=5"):
+        rla.sample_distribution(torch.zeros(6, dtype=torch.int32), mr)

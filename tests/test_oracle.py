@@ -348,6 +348,38 @@ def test_colproj_step_closed_forms():
     assert np.isclose(R.ls_gradient(A, x0, b), np.linalg.norm(A.T @ r0))
 
 
+def test_agmon_select_known_answers_and_brute_force():
+    # the reference's own known answers (test/Compressors/Distributions/agmon.jl:
+    # 301-315, 382-398, 400-424, 426-443, 445-477), 0-based here
+    def left(A, x, b):
+        return np.asarray(A, float) @ np.asarray(x, float) - np.asarray(b, float)
+    r = left([[1, 0], [0, 1], [0, 0]], [0, 0], [1, 5, 0.1])
+    assert R.agmon_select(r, range(3), 1) == [1]
+    A = np.array([[1.0, 0.0], [0.0, 1.0], [1.0, 1.0]])
+    g = A.T @ left(A, [1, 2], [0, 0, 0])              # |A'(Ax - b)| = [4, 5]
+    assert R.agmon_select(g, range(2), 1) == [1]
+    r = left([[2, 1], [1, 3], [0.5, 0.5], [1, 1]], [0.5, 0.5], [1, 2, 3, 0.5])
+    assert R.agmon_select(r, range(4), 1) == [int(np.argmax(np.abs(r)))] == [2]
+    A = np.array([[1.0, 0.0], [0.0, 1.0]])
+    assert R.agmon_select(left(A, [0, 0], [2, 3]), range(2), 1) == [1]
+    assert R.agmon_select(left(A, [10, 0], [2, 3]), range(2), 1) == [0]
+    r = left([[1, 0], [0, 1], [1, 0], [0, 1]], [0, 0], [2, 2, 2, 1])
+    assert R.agmon_select(r, range(4), 2) == [0, 1]     # tie at the cut: smaller index
+    assert R.agmon_select([0.1, 0.2, 9.9, 0.3], range(4), 1) == [2]
+    # brute force: every subset order on small random cases with ties
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        v = rng.integers(-3, 4, 9).astype(float)
+        cand = rng.choice(9, 6, replace=False)
+        k = int(rng.integers(1, 7))
+        got = R.agmon_select(v, cand, k)
+        for a, b in zip(got, got[1:]):
+            assert abs(v[a]) > abs(v[b]) or (abs(v[a]) == abs(v[b]) and a < b)
+        rest = [c for c in cand if c not in got]
+        assert all(abs(v[got[-1]]) > abs(v[c]) or
+                   (abs(v[got[-1]]) == abs(v[c]) and got[-1] < c) for c in rest)
+
+
 def test_full_residual_closed_form():
     A = np.array([[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]])
     assert R.full_residual(A, [1.0, 1.0], [3.0, 7.0, 2.0]) == 1.0

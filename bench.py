@@ -128,6 +128,44 @@ def gpu_sparse_sign(rla, torch, steps, warmup):
             "graph_replays_equal_eager": graph_exact}
 
 
+def fwht_fjlt_rows(rla, torch, reps=10):
+    """The FWHT alone (fwht!, helpers/fwht.jl) on 65536x512 f32 in place, and
+    the FJLT left apply (fjlt.jl, default sparsity 0.25 log^2(n)/n) of the same
+    A to 2048 rows: per-call time and the algorithmic GB/s (one read of X or A,
+    one write of X or C)."""
+    dev = torch.device("cuda", 0)
+    m, n, s = 65536, 512, 2048
+    g = torch.Generator(device=dev).manual_seed(1234)
+    X = torch.randn(n, m, generator=g, device=dev, dtype=torch.float32).T
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rla.fwht(X)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        rla.fwht(X)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fwht = e0.elapsed_time(e1) / reps
+    A = torch.randn(n, m, generator=g, device=dev, dtype=torch.float32).T
+    C = torch.zeros(n, s, device=dev, dtype=torch.float32).T
+    rla.seed(1234)
+    S = rla.complete_compressor(rla.FJLT(compression_dim=s, type=torch.float32), A)
+    rla.mul(C, S, A)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        rla.mul(C, S, A)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fjlt = e0.elapsed_time(e1) / reps
+    fb, jb = 2 * m * n * 4, (m * n + s * n) * 4
+    return {"fwht": {"workload": "fwht! 65536x512 f32 in place", "ms_per_call": ms_fwht,
+                     "GB_s": fb / ms_fwht / 1e6, "frac_hbm_peak": fb / ms_fwht / 1e6 / HBM_PEAK_GBS},
+            "fjlt": {"workload": f"FJLT left apply 65536x512 f32 -> 2048x512, q = {S.sparsity:.3g}",
+                     "ms_per_apply": ms_fjlt, "GB_s": jb / ms_fjlt / 1e6,
+                     "frac_hbm_peak": jb / ms_fjlt / 1e6 / HBM_PEAK_GBS}}
+
+
 def rangefinder_c3(rla, torch, reps=3, batch=4):
     """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
     32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
@@ -200,6 +238,25 @@ def rangefinder_c3(rla, torch, reps=3, batch=4):
         out[prec]["batch"] = {"in_flight": batch, "solves": batch * reps,
                               "solves_per_s": batch * reps / bdt}
         del recs, rec
+    # RandSVD (randsvd.jl:174-189) on the same A, BF16 products: the range, then
+    # B = Q'A and its FP64 SVD; singular values against the planted ones
+    rla.seed(7)
+    sv = rla.RandSVD(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=k,
+                                             type=torch.float32),
+                     power_its=2, orthogonalize=True, precision="bf16")
+    srec = rla.complete_approximator(sv, A32)
+    rla.rapproximate_(srec, A32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rla.rapproximate_(srec, A32)
+    torch.cuda.synchronize()
+    sdt = (time.perf_counter() - t0) / reps
+    top = 128
+    sv_err = float((srec.S[:top] / full[:top] - 1.0).abs().max())
+    out["randsvd_bf16"] = {"seconds_per_solve": sdt, "solves_per_s": 1.0 / sdt,
+                           f"max_rel_error_top{top}_singular_values": sv_err}
+    del srec
     out["speedup_solves_per_s"] = out["plain"]["seconds_per_solve"] / out["bf16"]["seconds_per_solve"]
     out["batched_speedup_solves_per_s"] = (out["bf16"]["batch"]["solves_per_s"] /
                                            out["plain"]["batch"]["solves_per_s"])
@@ -421,6 +478,7 @@ def main():
     ms, wall, exact, S, A0 = gpu_srht(rla, torch, args.steps, args.warmup)
     gbs = srht_bytes() / ms / 1e6
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
+    fwht_fjlt = None if args.skip_sparse else fwht_fjlt_rows(rla, torch)
     rf = None if args.skip_rangefinder else rangefinder_c3(rla, torch)
     ls = None if args.skip_ls else ls_batched_c4(rla, torch)
     kz = None if args.skip_kaczmarz else kaczmarz_c5(rla, torch)
@@ -461,6 +519,7 @@ def main():
                      "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": srht_bytes()},
         "sparse_sign_c1": extra,
+        "fwht_fjlt": fwht_fjlt,
         "rangefinder_c3": rf,
         "ls_batched_c4": ls,
         "kaczmarz_c5": kz,

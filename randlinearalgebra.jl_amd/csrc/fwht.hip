@@ -117,6 +117,65 @@ __global__ __launch_bounds__(kPassThreads) void fwht_pass_kernel(
   }
 }
 
+// Stages 14 .. L-1 of Float32 columns of length 2^L = NCH * 2^14, after the
+// chunk kernel: thread (e, column) holds entries e + c * 2^14 (c < NCH; four
+// consecutive e per thread when VEC) and runs the remaining butterflies in
+// registers, in the reference's order (chunk bit 0 first).
+template <int NCH, bool VEC>
+__global__ __launch_bounds__(256) void fwht_cross_kernel(float* P, int64_t ldp) {
+  constexpr int W = VEC ? 4 : 1;
+  const int64_t e = (int64_t(blockIdx.x) * 256 + threadIdx.x) * W;
+  float* pc = P + int64_t(blockIdx.y) * ldp + e;
+  float a[NCH][W];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    if constexpr (VEC) {
+      const float4 q = *reinterpret_cast<const float4*>(pc + (int64_t(c) << 14));
+      a[c][0] = q.x;
+      a[c][1] = q.y;
+      a[c][2] = q.z;
+      a[c][3] = q.w;
+    } else {
+      a[c][0] = pc[int64_t(c) << 14];
+    }
+  }
+#pragma unroll
+  for (int h = 1; h < NCH; h <<= 1)
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+      if (!(c & h))
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const float z = a[c][w], y = a[c | h][w];
+          a[c][w] = z + y;
+          a[c | h][w] = z - y;
+        }
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    if constexpr (VEC) {
+      *reinterpret_cast<float4*>(pc + (int64_t(c) << 14)) =
+          make_float4(a[c][0], a[c][1], a[c][2], a[c][3]);
+    } else {
+      pc[int64_t(c) << 14] = a[c][0];
+    }
+  }
+}
+
+template <int NCH>
+static int launch_fwht_cross(rla_ctx* ctx, float* P, int64_t ldp, int64_t ncols) {
+  const bool vec = (reinterpret_cast<uintptr_t>(P) % 16 == 0) && (ldp % 4 == 0);
+  const unsigned bx = vec ? (1u << 14) / (256 * 4) : (1u << 14) / 256;
+  for (int64_t c0 = 0; c0 < ncols; c0 += 65535) {
+    const unsigned by = static_cast<unsigned>(std::min<int64_t>(65535, ncols - c0));
+    if (vec)
+      fwht_cross_kernel<NCH, true><<<dim3(bx, by), 256, 0, ctx->stream>>>(P + c0 * ldp, ldp);
+    else
+      fwht_cross_kernel<NCH, false><<<dim3(bx, by), 256, 0, ctx->stream>>>(P + c0 * ldp, ldp);
+    RLA_LAUNCH_CHECK("fwht_cross");
+  }
+  return RLA_OK;
+}
+
 // Y[r, k] = beta*Y[r, k] + alpha*P[op[r], k] in double (the reference's
 // broadcast `beta .* Cv .+ alpha .* padding[op, :]`, srht.jl:259); beta == 0
 // does not read Y.
@@ -175,6 +234,25 @@ static int fwht_passes(rla_ctx* ctx, const T* src, int64_t src_rs, int64_t src_c
                        const uint64_t* signs, double scaling) {
   const int cap_lg = ilog2(static_cast<int64_t>(kPassLds / sizeof(T)));  // 14 f32, 13 f64
   int lo = 0;
+  if constexpr (sizeof(T) == 4) {
+    // Float32 columns of 2^14 or more with unit row stride: stages 0..13 in
+    // the wave-local chunk kernel, the rest in the passes below
+    if (L >= 14 && (!src || src_rs == 1)) {
+      int rc = fwht_chunks_f32(ctx, src, src_cs, src ? m_valid : (int64_t(1) << L), P, ldp,
+                               ncols, L, signs, scaling);
+      if (rc) return rc;
+      lo = 14;
+      // up to 16 chunks: the remaining stages in one register pass
+      switch (L - 14) {
+        case 0: return RLA_OK;
+        case 1: return launch_fwht_cross<2>(ctx, reinterpret_cast<float*>(P), ldp, ncols);
+        case 2: return launch_fwht_cross<4>(ctx, reinterpret_cast<float*>(P), ldp, ncols);
+        case 3: return launch_fwht_cross<8>(ctx, reinterpret_cast<float*>(P), ldp, ncols);
+        case 4: return launch_fwht_cross<16>(ctx, reinterpret_cast<float*>(P), ldp, ncols);
+        default: break;
+      }
+    }
+  }
   while (lo < L) {
     int nb, lg_gpw;
     if (lo == 0) {

@@ -96,4 +96,8 @@ inline int ilog2(int64_t n) {
   return l;
 }
 
+// srht_fused.hip: stages 0..13 of Float32 FWHTs of length 2^L, L >= 14
+int fwht_chunks_f32(rla_ctx* ctx, const float* src, int64_t src_cs, int64_t m_valid, float* P,
+                    int64_t ldp, int64_t ncols, int L, const uint64_t* signs, double scale);
+
 }  // namespace rla

@@ -666,6 +666,116 @@ void srht_wave_kernel(
   }
 }
 
+// Full transform of 2^14-entry chunks (stages 0..13, all outputs) for the
+// generic FWHT: item i is chunk i % nch of column i / nch.  The same
+// wave-local stages and exchange as srht_wave_kernel; after stage 13 each
+// thread holds entries t + 512 v of the chunk and stores them, lane sign
+// removed, to P in natural order (a wavefront stores 64 consecutive floats
+// per instruction).  The stages across chunks (14, 15, ...) are later passes.
+// src == nullptr transforms P in place: every chunk is read whole before it
+// is written, by its own workgroup.
+template <int MODE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void fwht_chunk_wave_kernel(const float* __restrict__ src, int64_t src_cs, int64_t m_valid,
+                            float* P, int64_t ldp, int64_t ncols, int64_t nch,
+                            const uint32_t* __restrict__ signs32, double scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* lds = reinterpret_cast<float*>(smem_raw);
+  const int t = threadIdx.x;
+  const int nib = wl_nibble(t & 15);
+  const float c0 = (nib & 1) ? -1.0f : 1.0f, c1 = (nib & 2) ? -1.0f : 1.0f;
+  const float c2 = (nib & 4) ? -1.0f : 1.0f, c3 = (nib & 8) ? -1.0f : 1.0f;
+  const int off = wl_load_off(t);
+  const bool neg_b = __builtin_popcount((t >> 2) & 15) & 1;  // sign carried by entries t + 512 v
+  float* lb_a = lds + wl_pad(off);
+  float* lb_b = lds + wl_pad(t);
+  const int64_t items = ncols * nch;
+  auto col_of = [&](int64_t it) { return it / nch; };
+  auto base_of = [&](int64_t it) { return (it % nch) << 14; };
+  auto xc_of = [&](int64_t it) -> const float* {
+    return src ? src + col_of(it) * src_cs : P + col_of(it) * ldp;
+  };
+  float xa[32], xb[32];
+  if (blockIdx.x < items) wl_load<MODE>(xa, xc_of(blockIdx.x), base_of(blockIdx.x), m_valid, off);
+
+  auto body = [&](float(&x)[32], float(&nx)[32], int64_t it) {
+    const int64_t nit = it + gridDim.x;
+    int offo = off;
+    asm volatile("" : "+v"(offo));
+    if (nit < items) wl_load<MODE>(nx, xc_of(nit), base_of(nit), m_valid, offo);
+    const int64_t cb = base_of(it);
+    uint32_t sw = 0xffffffffu;  // no signs: every entry scaled by +scale
+    if (signs32) {
+      sw = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        sw |= ((signs32[(cb >> 5) + (off >> 5) + 2 * k] >> (off & 31)) & 15u) << (4 * k);
+    }
+    asm volatile("" : "+v"(sw));
+    stage0_signed<32>(x, sw, scale);
+    stage<1, 32>(x);
+    wl_dpp_stage<0>(x, c0);
+    wl_dpp_stage<1>(x, c1);
+    wl_dpp_stage<2>(x, c2);
+    wl_dpp_stage<3>(x, c3);
+    stage<2, 32>(x);
+    stage<3, 32>(x);
+    stage<4, 32>(x);
+    __syncthreads();  // the previous item's readers of LDS are done
+#pragma unroll
+    for (int v = 0; v < 32; ++v) lb_a[(v & 3) + 66 * (v >> 2)] = x[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 32; ++v) x[v] = lb_b[530 * v];
+    stage_range<0, 5, 32>(x);
+    float* pc = P + col_of(it) * ldp + cb + t;
+#pragma unroll
+    for (int v = 0; v < 32; ++v) pc[v << 9] = neg_b ? -x[v] : x[v];
+  };
+  for (int64_t it = blockIdx.x; it < items; it += 2 * int64_t(gridDim.x)) {
+    body(xa, xb, it);
+    if (it + gridDim.x < items) body(xb, xa, it + gridDim.x);
+  }
+}
+
+static int num_cus(int device);
+
+template <int MODE>
+static int launch_fwht_chunks(rla_ctx* ctx, const float* src, int64_t src_cs, int64_t m_valid,
+                              float* P, int64_t ldp, int64_t ncols, int64_t nch,
+                              const uint64_t* signs, double scale) {
+  const size_t lds = size_t(kWaveLdsWords) * sizeof(float);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&fwht_chunk_wave_kernel<MODE>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  RLA_HIP(attr_err);
+  const int64_t items = ncols * nch;
+  const int64_t grid = std::min<int64_t>(items, int64_t(num_cus(ctx->device)) * 2);
+  fwht_chunk_wave_kernel<MODE><<<dim3(static_cast<unsigned>(grid)), 512, lds, ctx->stream>>>(
+      src, src_cs, m_valid, P, ldp, ncols, nch, reinterpret_cast<const uint32_t*>(signs), scale);
+  RLA_LAUNCH_CHECK("fwht_chunk_wave");
+  return RLA_OK;
+}
+
+// Stages 0..13 of the FWHT of ncols columns of length n_pad = 2^L (L >= 14),
+// Float32: read from src (column stride src_cs, unit row stride, the first
+// m_valid rows, zeros past them) or in place from P; written to P (ldp).
+int fwht_chunks_f32(rla_ctx* ctx, const float* src, int64_t src_cs, int64_t m_valid, float* P,
+                    int64_t ldp, int64_t ncols, int L, const uint64_t* signs, double scale) {
+  const int64_t nch = int64_t(1) << (L - 14);
+  const float* base = src ? src : P;
+  const int64_t cs = src ? src_cs : ldp;
+  const bool aligned = (reinterpret_cast<uintptr_t>(base) % 16 == 0) && (cs % 4 == 0);
+  if (aligned && m_valid == (int64_t(1) << L))
+    return launch_fwht_chunks<0>(ctx, src, src_cs, m_valid, P, ldp, ncols, nch, signs, scale);
+  if (aligned && m_valid % 4 == 0)
+    return launch_fwht_chunks<1>(ctx, src, src_cs, m_valid, P, ldp, ncols, nch, signs, scale);
+  return launch_fwht_chunks<2>(ctx, src, src_cs, m_valid, P, ldp, ncols, nch, signs, scale);
+}
+
 // ---- shape selection ---------------------------------------------------------
 
 struct FusedShape {

@@ -294,6 +294,21 @@ int rla_fjlt_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int6
                        const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
                        int64_t y_cs, double alpha, double beta);
 
+/* The forward FJLT (adjoint = 0 above) with K's rows as the row-interleaved
+ * copy from rla_sparse_ell_build (ptr = K's CSR row pointers): the gather's
+ * lanes read consecutive words of K.  Bit-identical to rla_fjlt_apply with
+ * adjoint = 0; replaces mul! at fjlt.jl:257-319 (and 395-468 for Right A*S). */
+int rla_fjlt_apply_ell_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           const int32_t* ptr, const int32_t* ell_idx, const float* ell_val,
+                           const uint64_t* sign_chunks, double scale, const float* X,
+                           int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs,
+                           double alpha, double beta);
+int rla_fjlt_apply_ell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           const int32_t* ptr, const int32_t* ell_idx, const double* ell_val,
+                           const uint64_t* sign_chunks, double scale, const double* X,
+                           int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs,
+                           double alpha, double beta);
+
 /* ---- dense products and conversions ------------------------------------
  * Column-major GEMM C = alpha*op(A)*op(B) + beta*C (vendor library: the plain
  * Float32/Float64 products of the Gaussian compressor, gaussian.jl:145-170,

@@ -79,6 +79,10 @@ _SIGS = {
                              i64, i64, f64, f64],
     "rla_sparse_ell_width": [vp, i64, vp, vp],
     "rla_agmon_select_f32": [vp, i64, vp, i64, vp, i64, vp],
+    "rla_fjlt_apply_ell_f32": [vp, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp, i64,
+                               i64, f64, f64],
+    "rla_fjlt_apply_ell_f64": [vp, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp, i64,
+                               i64, f64, f64],
     "rla_agmon_select_f64": [vp, i64, vp, i64, vp, i64, vp],
     "rla_sparse_ell_sign_build_f32": [vp, i64, i32, vp, vp, vp, vp, vp],
     "rla_sparse_ell_sign_build_f64": [vp, i64, i32, vp, vp, vp, vp, vp],

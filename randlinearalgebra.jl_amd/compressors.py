@@ -813,6 +813,15 @@ class FJLTRecipe(CompressorRecipe):
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
         op = self.op
+        ell = op._ell_for(0, dt) if not adjoint else None
+        if ell is not None and ell[0] == "ell":
+            # forward: the gather reads K's rows from the row-interleaved copy
+            _, ptr, ei, ev = ell
+            _lib.call(f"rla_fjlt_apply_ell_{_lib.sfx(dt)}", _lib.ctx(X.t.device), self.dim,
+                      self.n_pad, self.s, X.cols, _lib.ptr(ptr), _lib.ptr(ei), _lib.ptr(ev),
+                      _lib.ptr(self.signs), self.scale, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t),
+                      Y.rs, Y.cs, alpha, beta)
+            return
         val, val_t = op._vals_for(dt)
         _lib.call(f"rla_fjlt_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(adjoint), self.dim,
                   self.n_pad, self.s, X.cols, _lib.ptr(op.colptr), _lib.ptr(op.rowidx),

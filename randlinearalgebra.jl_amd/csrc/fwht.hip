@@ -429,6 +429,46 @@ static int fjlt_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64
   return RLA_OK;
 }
 
+// The forward FJLT with K's rows given as the row-interleaved copy (ptr of
+// K's CSR, ell_idx/ell_val from rla_sparse_ell_build): the same FWHT passes,
+// then the interleaved gather, whose lanes read consecutive words of K.
+// Bit-identical to fjlt_apply with adjoint = 0.
+template <class T>
+static int fjlt_apply_ell(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                          const int32_t* ptr, const int32_t* ell_idx, const T* ell_val,
+                          const uint64_t* signs, double scale, const T* X, int64_t x_rs,
+                          int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                          double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (m < 1 || m > n_pad || s < 1 || ncols < 0)
+    return fail(RLA_EARG, "FJLT sizes must satisfy 1 <= m <= n_pad and s >= 1");
+  if (ncols == 0) return RLA_OK;
+  if (!ptr || !ell_idx || !ell_val || !signs || !X || !Y)
+    return fail(RLA_EARG, "FJLT apply: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int L = ilog2(n_pad);
+  const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
+  void* ws = nullptr;
+  rc = workspace(ctx, static_cast<size_t>(n_pad * cc) * sizeof(T), &ws);
+  if (rc) return rc;
+  T* P = static_cast<T*>(ws);
+  for (int64_t k0 = 0; k0 < ncols; k0 += cc) {
+    const int64_t kc = std::min(cc, ncols - k0);
+    rc = fwht_passes<T>(ctx, X + k0 * x_cs, x_rs, x_cs, m, P, n_pad, kc, L, signs, scale);
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4)
+      rc = rla_sparse_apply_ell_f32(ctx, s, n_pad, ptr, ell_idx, ell_val, kc, P, 1, n_pad,
+                                    Y + k0 * y_cs, y_rs, y_cs, alpha, beta);
+    else
+      rc = rla_sparse_apply_ell_f64(ctx, s, n_pad, ptr, ell_idx, ell_val, kc, P, 1, n_pad,
+                                    Y + k0 * y_cs, y_rs, y_cs, alpha, beta);
+    if (rc) return rc;
+  }
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -482,6 +522,24 @@ int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int6
                        double beta) {
   return srht_apply<double>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
                             x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_fjlt_apply_ell_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           const int32_t* ptr, const int32_t* ell_idx, const float* ell_val,
+                           const uint64_t* sign_chunks, double scale, const float* X,
+                           int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs,
+                           double alpha, double beta) {
+  return fjlt_apply_ell<float>(ctx, m, n_pad, s, ncols, ptr, ell_idx, ell_val, sign_chunks, scale,
+                               X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_fjlt_apply_ell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           const int32_t* ptr, const int32_t* ell_idx, const double* ell_val,
+                           const uint64_t* sign_chunks, double scale, const double* X,
+                           int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs,
+                           double alpha, double beta) {
+  return fjlt_apply_ell<double>(ctx, m, n_pad, s, ncols, ptr, ell_idx, ell_val, sign_chunks,
+                                scale, X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
 }
 
 }  // extern "C"

@@ -560,3 +560,27 @@ def test_fjlt_preserves_norms_on_average(rla, dev):
     rla.mul(C, S, colmajor(X, dev))
     ratio = (C.cpu().numpy() ** 2).sum(0) / (X ** 2).sum(0)
     assert abs(ratio.mean() - 1.0) < 0.1 and ratio.min() > 0.5
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
+    # the forward FJLT through the row-interleaved copy of K (the recipe's
+    # default) and through K's CSR give the same bits, on a strided input
+    lib = rla._lib
+    m, n, s = 40000, 3, 512
+    rla.seed(11)
+    A = rnd((m + 1, n), 5, NP[dtype])
+    Ad = colmajor(A, dev)[:m]
+    S = rla.complete_compressor(rla.FJLT(compression_dim=s, sparsity=0.02, type=dtype), Ad)
+    C1 = torch.zeros(n, s, dtype=dtype, device=dev).T
+    rla.mul(C1, S, Ad, 1.25, 0.0)
+    assert S.op._ell[(0, dtype)][0] == "ell"
+    op = S.op
+    val, val_t = op._vals_for(dtype)
+    C2 = torch.zeros(n, s, dtype=dtype, device=dev).T
+    Av = rla.as_view(Ad)
+    lib.call(f"rla_fjlt_apply_{lib.sfx(dtype)}", lib.ctx(dev), 0, S.dim, S.n_pad, S.s, n,
+             lib.ptr(op.colptr), lib.ptr(op.rowidx), lib.ptr(val), lib.ptr(op.rowptr),
+             lib.ptr(op.colidx), lib.ptr(val_t), lib.ptr(S.signs), S.scale, lib.ptr(Av.t),
+             Av.rs, Av.cs, lib.ptr(C2), 1, s, 1.25, 0.0)
+    assert torch.equal(C1, C2)

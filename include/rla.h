@@ -197,6 +197,40 @@ int rla_sparse_apply_ell_sign_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_row
                                   double* C, int64_t c_rs, int64_t c_cs, double alpha,
                                   double beta);
 
+/* Block-interleaved copy of op(M) for X taller than one LDS block: the rows of
+ * X are cut into blocks of rb rows; entry s (in column order) of row i among
+ * its columns in block b is at bidx/bval[rows * (w_0 + ... + w_{b-1}) + s * rows
+ * + i], w_b being the most entries any row has in block b, and bidx = -1 past
+ * a row's entries.  Every load of the gather below then reads consecutive
+ * words.  No counterpart in the reference; it serves the same mul! as
+ * rla_sparse_apply (fjlt.jl:257-319 through K, sparse_sign.jl:339-361).
+ * rla_sparse_bell_widths writes w (ceil(in_rows / rb) device int32) and
+ * *total_slots = sum of w (host pointer; synchronises); the copy holds
+ * total_slots * rows entries.  RLA_EDOMAIN if a row's columns are outside
+ * [0, in_rows) or not strictly ascending; RLA_ENOTSUP beyond 2^31 entries. */
+int rla_sparse_bell_widths(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                           const int32_t* ptr, const int32_t* idx, int32_t* widths,
+                           int64_t* total_slots);
+int rla_sparse_bell_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* ptr, const int32_t* idx, const float* val,
+                              const int32_t* widths, int32_t* bidx, float* bval);
+int rla_sparse_bell_build_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* ptr, const int32_t* idx, const double* val,
+                              const int32_t* widths, int32_t* bidx, double* bval);
+
+/* C = alpha * op(M) * X + beta * C from the block-interleaved copy; X
+ * column-major (x_rs == 1), in_rows x n_rhs; rb * element size <= 64 KiB
+ * (else RLA_ENOTSUP).  Same arithmetic and order as rla_sparse_apply
+ * (sparse_sign.jl:339-361), so results are bit-identical to it. */
+int rla_sparse_apply_bell_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* widths, const int32_t* bidx, const float* bval,
+                              int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                              int64_t c_rs, int64_t c_cs, double alpha, double beta);
+int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* widths, const int32_t* bidx, const double* bval,
+                              int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
+                              double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
+
 /* Agmon maximal-residual selection, the top-k step of sample_distribution!
  * for the Agmon distribution (src/Compressors/Distributions/agmon.jl:354-416):
  * out[t] (t < k) = the candidate with the t-th largest |v[c]|, ties to the
@@ -308,6 +342,20 @@ int rla_fjlt_apply_ell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, in
                            const uint64_t* sign_chunks, double scale, const double* X,
                            int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs,
                            double alpha, double beta);
+
+/* The forward FJLT product (T * X, fjlt.jl:257-319) with K (s x n_pad) given
+ * by its block-interleaved copy (rla_sparse_bell_*, in_rows = n_pad, blocks of
+ * rb rows).  Bit-identical to rla_fjlt_apply with adjoint = 0. */
+int rla_fjlt_apply_bell_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rb, const int32_t* widths, const int32_t* bidx,
+                            const float* bval, const uint64_t* sign_chunks, double scale,
+                            const float* X, int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs,
+                            int64_t y_cs, double alpha, double beta);
+int rla_fjlt_apply_bell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rb, const int32_t* widths, const int32_t* bidx,
+                            const double* bval, const uint64_t* sign_chunks, double scale,
+                            const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
+                            int64_t y_cs, double alpha, double beta);
 
 /* ---- dense products and conversions ------------------------------------
  * Column-major GEMM C = alpha*op(A)*op(B) + beta*C (vendor library: the plain

@@ -78,6 +78,17 @@ _SIGS = {
     "rla_sparse_apply_f64": [vp, i32, i64, i64, vp, vp, vp, vp, vp, vp, i64, vp, i64, i64, vp,
                              i64, i64, f64, f64],
     "rla_sparse_ell_width": [vp, i64, vp, vp],
+    "rla_sparse_bell_widths": [vp, i64, i64, i64, vp, vp, vp, vp],
+    "rla_sparse_bell_build_f32": [vp, i64, i64, i64, vp, vp, vp, vp, vp, vp],
+    "rla_sparse_bell_build_f64": [vp, i64, i64, i64, vp, vp, vp, vp, vp, vp],
+    "rla_sparse_apply_bell_f32": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64,
+                                  f64, f64],
+    "rla_sparse_apply_bell_f64": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64,
+                                  f64, f64],
+    "rla_fjlt_apply_bell_f32": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,
+                                i64, i64, f64, f64],
+    "rla_fjlt_apply_bell_f64": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,
+                                i64, i64, f64, f64],
     "rla_agmon_select_f32": [vp, i64, vp, i64, vp, i64, vp],
     "rla_fjlt_apply_ell_f32": [vp, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp, i64,
                                i64, f64, f64],

@@ -411,6 +411,37 @@ class _SparseOp:
         self._ell[key] = out
         return out
 
+    def _bell_for(self, dtype):
+        """Block-interleaved gather arrays of M (X blocks of 64 KiB) in `dtype`,
+        built once per operator and cached: (rb, widths, bidx, bval), or None
+        when X fits one block or the copy would hold more than four times the
+        entries plus one slot per row and block."""
+        key = ("bell", dtype)
+        if key in self._ell:
+            return self._ell[key]
+        rows, in_rows = self.nrows, self.ncols
+        rb = 65536 // (torch.finfo(dtype).bits // 8)
+        out = None
+        if rows > 0 and in_rows > rb and self.nnz > 0:
+            dev = self.rowidx.device
+            ctx = _lib.ctx(dev)
+            nb = (in_rows + rb - 1) // rb
+            widths = torch.empty(nb, dtype=torch.int32, device=dev)
+            total = ctypes.c_int64(0)
+            rc = _lib.call_rc("rla_sparse_bell_widths", ctx, rows, in_rows, rb,
+                              _lib.ptr(self.rowptr), _lib.ptr(self.colidx), _lib.ptr(widths),
+                              ctypes.byref(total))
+            if rc == 0 and total.value * rows <= 4 * self.nnz + nb * rows:
+                _, val_t = self._vals_for(dtype)
+                bidx = torch.empty(total.value * rows, dtype=torch.int32, device=dev)
+                bval = torch.empty(total.value * rows, dtype=dtype, device=dev)
+                _lib.call(f"rla_sparse_bell_build_{_lib.sfx(dtype)}", ctx, rows, in_rows, rb,
+                          _lib.ptr(self.rowptr), _lib.ptr(self.colidx), _lib.ptr(val_t),
+                          _lib.ptr(widths), _lib.ptr(bidx), _lib.ptr(bval))
+                out = (rb, widths, bidx, bval)
+        self._ell[key] = out
+        return out
+
     def apply(self, trans_m, X, C, alpha, beta):
         """C = alpha*op(M)*X + beta*C on strided views (op = M or M')."""
         dt = X.dtype
@@ -813,6 +844,15 @@ class FJLTRecipe(CompressorRecipe):
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
         op = self.op
+        bell = op._bell_for(dt) if not adjoint else None
+        if bell is not None:
+            # forward: the gather reads K's rows from the block-interleaved copy
+            rb, widths, bidx, bval = bell
+            _lib.call(f"rla_fjlt_apply_bell_{_lib.sfx(dt)}", _lib.ctx(X.t.device), self.dim,
+                      self.n_pad, self.s, X.cols, rb, _lib.ptr(widths), _lib.ptr(bidx),
+                      _lib.ptr(bval), _lib.ptr(self.signs), self.scale, _lib.ptr(X.t), X.rs,
+                      X.cs, _lib.ptr(Y.t), Y.rs, Y.cs, alpha, beta)
+            return
         ell = op._ell_for(0, dt) if not adjoint else None
         if ell is not None and ell[0] == "ell":
             # forward: the gather reads K's rows from the row-interleaved copy

@@ -469,6 +469,45 @@ static int fjlt_apply_ell(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int
   return RLA_OK;
 }
 
+// The forward product through the block-interleaved copy of K (rows s, X
+// blocks of rb rows): the same transform, then rla_sparse_apply_bell, whose
+// sums run in the same order as the CSR gather's.
+template <class T>
+static int fjlt_apply_bell(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           int64_t rb, const int32_t* widths, const int32_t* bidx, const T* bval,
+                           const uint64_t* signs, double scale, const T* X, int64_t x_rs,
+                           int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                           double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (m < 1 || m > n_pad || s < 1 || ncols < 0)
+    return fail(RLA_EARG, "FJLT sizes must satisfy 1 <= m <= n_pad and s >= 1");
+  if (ncols == 0) return RLA_OK;
+  if (!widths || !bidx || !bval || !signs || !X || !Y)
+    return fail(RLA_EARG, "FJLT apply: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int L = ilog2(n_pad);
+  const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
+  void* ws = nullptr;
+  rc = workspace(ctx, static_cast<size_t>(n_pad * cc) * sizeof(T), &ws);
+  if (rc) return rc;
+  T* P = static_cast<T*>(ws);
+  for (int64_t k0 = 0; k0 < ncols; k0 += cc) {
+    const int64_t kc = std::min(cc, ncols - k0);
+    rc = fwht_passes<T>(ctx, X + k0 * x_cs, x_rs, x_cs, m, P, n_pad, kc, L, signs, scale);
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4)
+      rc = rla_sparse_apply_bell_f32(ctx, s, n_pad, rb, widths, bidx, bval, kc, P, 1, n_pad,
+                                     Y + k0 * y_cs, y_rs, y_cs, alpha, beta);
+    else
+      rc = rla_sparse_apply_bell_f64(ctx, s, n_pad, rb, widths, bidx, bval, kc, P, 1, n_pad,
+                                     Y + k0 * y_cs, y_rs, y_cs, alpha, beta);
+    if (rc) return rc;
+  }
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -540,6 +579,24 @@ int rla_fjlt_apply_ell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, in
                            double alpha, double beta) {
   return fjlt_apply_ell<double>(ctx, m, n_pad, s, ncols, ptr, ell_idx, ell_val, sign_chunks,
                                 scale, X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_fjlt_apply_bell_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rb, const int32_t* widths, const int32_t* bidx,
+                            const float* bval, const uint64_t* sign_chunks, double scale,
+                            const float* X, int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs,
+                            int64_t y_cs, double alpha, double beta) {
+  return fjlt_apply_bell<float>(ctx, m, n_pad, s, ncols, rb, widths, bidx, bval, sign_chunks,
+                                scale, X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+}
+
+int rla_fjlt_apply_bell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rb, const int32_t* widths, const int32_t* bidx,
+                            const double* bval, const uint64_t* sign_chunks, double scale,
+                            const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
+                            int64_t y_cs, double alpha, double beta) {
+  return fjlt_apply_bell<double>(ctx, m, n_pad, s, ncols, rb, widths, bidx, bval, sign_chunks,
+                                 scale, X, x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
 }
 
 }  // extern "C"

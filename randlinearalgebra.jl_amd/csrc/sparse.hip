@@ -31,6 +31,7 @@
 #include <algorithm>
 
 #include <mutex>
+#include <vector>
 
 #include "rla_internal.hpp"
 
@@ -627,6 +628,318 @@ static int sparse_apply_ell_sign(rla_ctx* ctx, int64_t out_rows, int64_t in_rows
   return RLA_OK;
 }
 
+
+// ---- block-interleaved gather -------------------------------------------------
+// For X taller than one LDS block: the rows of X are cut into blocks of rb,
+// and for each block b the entries of op(M) whose column falls in b are kept
+// slot-major, entry s of row i (within the block) at
+// bidx/bval[off_b + s * rows + i], with off_b = rows * (w_0 + ... + w_{b-1})
+// and w_b the most entries any row has in block b; -1 marks the slots past a
+// row's entries.  The lanes of a wavefront own consecutive rows and read the
+// same slot of every block, so each load reads consecutive words.  (In the
+// row-interleaved copy the lanes' positions drift apart after the first block,
+// and one load then touches up to 64 cache lines.)
+
+constexpr int kBellRoundSlots = 4;  // slots per row and round
+
+// w_b by atomic maxima; *bad set if a row's columns are out of range or not
+// strictly ascending.
+__global__ void bell_count_kernel(int64_t rows, int64_t in_rows, int64_t rb,
+                                  const int32_t* __restrict__ ptr,
+                                  const int32_t* __restrict__ idx, int32_t* __restrict__ widths,
+                                  int32_t* __restrict__ bad) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  int64_t cur = -1, prev = -1;
+  int32_t cnt = 0;
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+    const int64_t c = idx[p];
+    if (c <= prev || c >= in_rows) {
+      atomicOr(bad, 1);
+      return;
+    }
+    prev = c;
+    const int64_t b = c / rb;
+    if (b != cur) {
+      if (cur >= 0) atomicMax(&widths[cur], cnt);
+      cur = b;
+      cnt = 0;
+    }
+    ++cnt;
+  }
+  if (cur >= 0) atomicMax(&widths[cur], cnt);
+}
+
+template <class T>
+__global__ void bell_fill_kernel(int64_t rows, int64_t rb, int64_t nblocks,
+                                 const int32_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                 const int32_t* __restrict__ widths, int32_t* __restrict__ bidx,
+                                 T* __restrict__ bval) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  int32_t p = ptr[i];
+  const int32_t pe = ptr[i + 1];
+  int64_t off = i;
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const int32_t w = widths[b];
+    int32_t sl = 0;
+    for (; p < pe && idx[p] / rb == b && sl < w; ++p, ++sl) {
+      bidx[off + int64_t(sl) * rows] = idx[p];
+      bval[off + int64_t(sl) * rows] = val[p];
+    }
+    for (; sl < w; ++sl) {
+      bidx[off + int64_t(sl) * rows] = -1;
+      bval[off + int64_t(sl) * rows] = T(0);
+    }
+    off += int64_t(w) * rows;
+  }
+}
+
+// One column of X per workgroup (and a part of the output rows), RP rows per
+// thread with their sums in registers for the whole launch; per block, stage
+// the block of X in LDS, then rounds of kBellRoundSlots slots of all the
+// thread's rows, issued together.  The entries of each row are added in
+// ascending column order, block after block, as rla_sparse_apply does, so
+// the results are identical to it.
+template <class T, int RP>
+__global__ __launch_bounds__(kGatherThreads) void rowgather_bell_kernel(
+    int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,
+    const int32_t* __restrict__ widths, const int32_t* __restrict__ bidx,
+    const T* __restrict__ bval, const T* __restrict__ X, int64_t x_cs, T* __restrict__ C,
+    int64_t c_rs, int64_t c_cs, double alpha, double beta, int rsplit, int64_t rows_per_part,
+    int vec_ok) {
+  constexpr int B = kBellRoundSlots;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  // Block -> (column, part); the parts of one column are 8 block ids apart,
+  // so they land on the same XCD and share its L2 for the column of X.
+  const int bid = blockIdx.x;
+  const int grp = bid / (8 * rsplit), within = bid % (8 * rsplit);
+  const int64_t k = int64_t(grp) * 8 + (within % 8);
+  if (k >= n_rhs) return;
+  const int64_t r_begin = int64_t(within / 8) * rows_per_part;
+  const int64_t r_end = r_begin + rows_per_part < rows ? r_begin + rows_per_part : rows;
+  const uint32_t rows32 = static_cast<uint32_t>(rows);
+  bool valid[RP];
+  T acc[RP];
+#pragma unroll
+  for (int q = 0; q < RP; ++q) {
+    const int64_t i = r_begin + threadIdx.x + int64_t(q) * blockDim.x;
+    valid[q] = i < r_end;
+    acc[q] = (!valid[q] || beta == 0.0) ? T(0)
+                                        : static_cast<T>(beta * double(C[i * c_rs + k * c_cs]));
+  }
+  const T* xc = X + k * x_cs;
+  const int64_t nblocks = (in_rows + rb - 1) / rb;
+  uint32_t off = static_cast<uint32_t>(r_begin + threadIdx.x);  // element of slot 0, row q = 0
+  for (int64_t blk = 0; blk < nblocks; ++blk) {
+    const int64_t j0 = blk * rb;
+    const int64_t jn = in_rows - j0 < rb ? in_rows - j0 : rb;
+    const int32_t w = widths[blk];
+    __syncthreads();
+    if (vec_ok) {
+      constexpr int VW = 16 / sizeof(T);
+      const int64_t nv = jn / VW;
+      for (int64_t vb = 0; vb < nv; vb += 8 * int64_t(blockDim.x)) {
+        uint4 tmp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
+          tmp[u] = v < nv ? reinterpret_cast<const uint4*>(xc + j0)[v] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
+          if (v < nv) reinterpret_cast<uint4*>(lds)[v] = tmp[u];
+        }
+      }
+      for (int64_t j = nv * VW + threadIdx.x; j < jn; j += blockDim.x) lds[j] = xc[j0 + j];
+    } else {
+      for (int64_t jb = 0; jb < jn; jb += 8 * int64_t(blockDim.x)) {
+        T tmp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t j = jb + u * int64_t(blockDim.x) + threadIdx.x;
+          tmp[u] = j < jn ? xc[j0 + j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t j = jb + u * int64_t(blockDim.x) + threadIdx.x;
+          if (j < jn) lds[j] = tmp[u];
+        }
+      }
+    }
+    __syncthreads();
+    bool act[RP];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < RP; ++q) {
+      act[q] = valid[q] && w > 0;
+      any = any || act[q];
+    }
+    for (int32_t s0 = 0; any; s0 += B) {
+      int32_t jj[RP][B];
+      T vv[RP][B];
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const bool in = act[q] && s0 + u < w;
+          const uint32_t e = off + static_cast<uint32_t>(q) * blockDim.x +
+                             static_cast<uint32_t>(s0 + u) * rows32;
+          jj[q][u] = in ? bidx[e] : -1;
+          vv[q][u] = in ? bval[e] : T(0);
+        }
+      T xv[RP][B];
+#pragma unroll
+      for (int q = 0; q < RP; ++q)
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const int64_t d = int64_t(jj[q][u]) - j0;
+          xv[q][u] = lds[(d >= 0 && d < jn) ? d : 0];
+        }
+      any = false;
+#pragma unroll
+      for (int q = 0; q < RP; ++q) {
+        bool more = act[q];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          if (more && jj[q][u] >= 0) {
+            acc[q] = static_cast<T>(double(acc[q]) + double(vv[q][u]) * (double(xv[q][u]) * alpha));
+          } else {
+            more = false;  // first empty slot (or the block's width): the row is done
+          }
+        }
+        act[q] = more && s0 + B < w;
+        any = any || act[q];
+      }
+    }
+    off += static_cast<uint32_t>(w) * rows32;
+  }
+#pragma unroll
+  for (int q = 0; q < RP; ++q) {
+    const int64_t i = r_begin + threadIdx.x + int64_t(q) * blockDim.x;
+    if (valid[q]) C[i * c_rs + k * c_cs] = acc[q];
+  }
+}
+
+static int bell_widths(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                       const int32_t* ptr, const int32_t* idx, int32_t* widths,
+                       int64_t* total) {
+  RLA_CHECK_CTX(ctx);
+  if (!total) return fail(RLA_EARG, "block-interleaved widths: total is NULL");
+  *total = 0;
+  if (rows < 0 || in_rows < 1 || rb < 1)
+    return fail(RLA_EARG, "block-interleaved widths: need rows >= 0, in_rows >= 1, rb >= 1");
+  if (rows > INT32_MAX) return fail(RLA_ENOTSUP, "block-interleaved copy: too many rows");
+  if (!ptr || !widths || (rows > 0 && !idx))
+    return fail(RLA_EARG, "block-interleaved widths: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int64_t nblocks = ceil_div(in_rows, rb);
+  void* ws = nullptr;
+  rc = workspace(ctx, 16, &ws);
+  if (rc) return rc;
+  int32_t* bad = static_cast<int32_t*>(ws);
+  set_i32_kernel<<<1, 64, 0, ctx->stream>>>(bad, 0);
+  zero_i32_kernel<<<dim3(unsigned(ceil_div(nblocks, 256))), 256, 0, ctx->stream>>>(widths,
+                                                                                nblocks);
+  if (rows > 0)
+    bell_count_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+        rows, in_rows, rb, ptr, idx, widths, bad);
+  RLA_LAUNCH_CHECK("bell_count");
+  std::vector<int32_t> hw(static_cast<size_t>(nblocks));
+  int32_t h_bad = 0;
+  RLA_HIP(hipMemcpyAsync(hw.data(), widths, hw.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  RLA_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (h_bad)
+    return fail(RLA_EDOMAIN, "block-interleaved copy: each row's columns must lie in "
+                             "[0, in_rows) and be strictly ascending");
+  int64_t t = 0;
+  for (int32_t v : hw) t += v;
+  if (t * rows >= (int64_t(1) << 31))
+    return fail(RLA_ENOTSUP, "block-interleaved copy: more than 2^31 slots");
+  *total = t;
+  return RLA_OK;
+}
+
+template <class T>
+static int bell_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb, const int32_t* ptr,
+                      const int32_t* idx, const T* val, const int32_t* widths, int32_t* bidx,
+                      T* bval) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || in_rows < 1 || rb < 1)
+    return fail(RLA_EARG, "block-interleaved build: need rows >= 0, in_rows >= 1, rb >= 1");
+  if (rows == 0) return RLA_OK;
+  if (!ptr || !idx || !val || !widths || !bidx || !bval)
+    return fail(RLA_EARG, "block-interleaved build: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  bell_fill_kernel<T><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, rb, ceil_div(in_rows, rb), ptr, idx, val, widths, bidx, bval);
+  RLA_LAUNCH_CHECK("bell_fill");
+  return RLA_OK;
+}
+
+template <class T, int RP>
+static void launch_bell(rla_ctx* ctx, dim3 grid, size_t lds, int64_t rows, int64_t in_rows,
+                        int64_t rb, int64_t n_rhs, const int32_t* widths, const int32_t* bidx,
+                        const T* bval, const T* X, int64_t x_cs, T* C, int64_t c_rs,
+                        int64_t c_cs, double alpha, double beta, int rsplit,
+                        int64_t rows_per_part, int vec_ok) {
+  rowgather_bell_kernel<T, RP><<<grid, kGatherThreads, lds, ctx->stream>>>(
+      rows, in_rows, rb, n_rhs, widths, bidx, bval, X, x_cs, C, c_rs, c_cs, alpha, beta, rsplit,
+      rows_per_part, vec_ok);
+}
+
+template <class T>
+static int sparse_apply_bell(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                             const int32_t* widths, const int32_t* bidx, const T* bval,
+                             int64_t n_rhs, const T* X, int64_t x_rs, int64_t x_cs, T* C,
+                             int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  if (in_rows < 1 || rb < 1)
+    return fail(RLA_EARG, "block-interleaved apply: need in_rows >= 1 and rb >= 1");
+  if (rows == 0 || n_rhs == 0) return RLA_OK;
+  if (!widths || !bidx || !bval || !X || !C)
+    return fail(RLA_EARG, "block-interleaved apply: NULL pointer");
+  if (x_rs != 1)
+    return fail(RLA_EARG, "block-interleaved apply needs X column-major (x_rs == 1)");
+  if (size_t(rb) * sizeof(T) > kGatherLds)
+    return fail(RLA_ENOTSUP, "block-interleaved apply: rb * element size must be <= 64 KiB");
+  if (rows > INT32_MAX) return fail(RLA_ENOTSUP, "block-interleaved apply: too many rows");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  // Parts of at most 4 * kGatherThreads rows, and about two workgroups per CU.
+  const int64_t min_parts = ceil_div(rows, 4 * int64_t(kGatherThreads));
+  const int64_t max_parts = ceil_div(rows, kGatherThreads);
+  const int64_t want = ceil_div(512, n_rhs);
+  const int64_t rsplit = std::max(min_parts, std::min(max_parts, want));
+  const int64_t rows_per_part = ceil_div(rows, rsplit);
+  const int64_t blocks = ceil_div(n_rhs, 8) * 8 * rsplit;
+  if (blocks > INT32_MAX) return fail(RLA_ENOTSUP, "block-interleaved apply: grid too large");
+  const size_t lds = size_t(std::min(rb, in_rows)) * sizeof(T);
+  const int vec_ok = reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
+                     (x_cs * int64_t(sizeof(T))) % 16 == 0 && (rb * int64_t(sizeof(T))) % 16 == 0;
+  const dim3 grid(static_cast<unsigned>(blocks));
+  const int rs = static_cast<int>(rsplit);
+  if (rows_per_part <= kGatherThreads)
+    launch_bell<T, 1>(ctx, grid, lds, rows, in_rows, rb, n_rhs, widths, bidx, bval, X, x_cs, C,
+                      c_rs, c_cs, alpha, beta, rs, rows_per_part, vec_ok);
+  else if (rows_per_part <= 2 * kGatherThreads)
+    launch_bell<T, 2>(ctx, grid, lds, rows, in_rows, rb, n_rhs, widths, bidx, bval, X, x_cs, C,
+                      c_rs, c_cs, alpha, beta, rs, rows_per_part, vec_ok);
+  else
+    launch_bell<T, 4>(ctx, grid, lds, rows, in_rows, rb, n_rhs, widths, bidx, bval, X, x_cs, C,
+                      c_rs, c_cs, alpha, beta, rs, rows_per_part, vec_ok);
+  RLA_LAUNCH_CHECK("rowgather_bell");
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -722,6 +1035,40 @@ int rla_sparse_apply_ell_sign_f64(rla_ctx* ctx, int64_t out_rows, int64_t in_row
                                   double beta) {
   return sparse_apply_ell_sign<double>(ctx, out_rows, in_rows, ptr, ell_sidx, magnitude, n_rhs,
                                        X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_bell_widths(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                           const int32_t* ptr, const int32_t* idx, int32_t* widths,
+                           int64_t* total_slots) {
+  return bell_widths(ctx, rows, in_rows, rb, ptr, idx, widths, total_slots);
+}
+
+int rla_sparse_bell_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* ptr, const int32_t* idx, const float* val,
+                              const int32_t* widths, int32_t* bidx, float* bval) {
+  return bell_build<float>(ctx, rows, in_rows, rb, ptr, idx, val, widths, bidx, bval);
+}
+
+int rla_sparse_bell_build_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* ptr, const int32_t* idx, const double* val,
+                              const int32_t* widths, int32_t* bidx, double* bval) {
+  return bell_build<double>(ctx, rows, in_rows, rb, ptr, idx, val, widths, bidx, bval);
+}
+
+int rla_sparse_apply_bell_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* widths, const int32_t* bidx, const float* bval,
+                              int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                              int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_bell<float>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X, x_rs,
+                                  x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                              const int32_t* widths, const int32_t* bidx, const double* bval,
+                              int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
+                              double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_bell<double>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X, x_rs,
+                                   x_cs, C, c_rs, c_cs, alpha, beta);
 }
 
 }  // extern "C"

@@ -564,8 +564,9 @@ def test_fjlt_preserves_norms_on_average(rla, dev):
 
 @pytest.mark.parametrize("dtype", [F32, F64])
 def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
-    # the forward FJLT through the row-interleaved copy of K (the recipe's
-    # default) and through K's CSR give the same bits, on a strided input
+    # the forward FJLT through the block-interleaved copy of K (the recipe's
+    # default), the row-interleaved copy and K's CSR give the same bits, on a
+    # strided input with n_pad = 65536 (several X blocks)
     lib = rla._lib
     m, n, s = 40000, 3, 512
     rla.seed(11)
@@ -574,8 +575,8 @@ def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
     S = rla.complete_compressor(rla.FJLT(compression_dim=s, sparsity=0.02, type=dtype), Ad)
     C1 = torch.zeros(n, s, dtype=dtype, device=dev).T
     rla.mul(C1, S, Ad, 1.25, 0.0)
-    assert S.op._ell[(0, dtype)][0] == "ell"
     op = S.op
+    assert op._ell[("bell", dtype)] is not None
     val, val_t = op._vals_for(dtype)
     C2 = torch.zeros(n, s, dtype=dtype, device=dev).T
     Av = rla.as_view(Ad)
@@ -584,3 +585,109 @@ def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
              lib.ptr(op.colidx), lib.ptr(val_t), lib.ptr(S.signs), S.scale, lib.ptr(Av.t),
              Av.rs, Av.cs, lib.ptr(C2), 1, s, 1.25, 0.0)
     assert torch.equal(C1, C2)
+    ell = op._ell_for(0, dtype)
+    assert ell[0] == "ell"
+    C3 = torch.zeros(n, s, dtype=dtype, device=dev).T
+    lib.call(f"rla_fjlt_apply_ell_{lib.sfx(dtype)}", lib.ctx(dev), S.dim, S.n_pad, S.s, n,
+             lib.ptr(ell[1]), lib.ptr(ell[2]), lib.ptr(ell[3]), lib.ptr(S.signs), S.scale,
+             lib.ptr(Av.t), Av.rs, Av.cs, lib.ptr(C3), 1, s, 1.25, 0.0)
+    assert torch.equal(C1, C3)
+
+
+def _bell_layout(rowptr, colidx, vals, rows, in_rows, rb):
+    """The block-interleaved arrays built on the host from the format's
+    definition: per block b, slot s of row i at rows * (w_0+...+w_{b-1}) + s *
+    rows + i, -1 past a row's entries."""
+    nb = -(-in_rows // rb)
+    per = [[[] for _ in range(nb)] for _ in range(rows)]
+    for i in range(rows):
+        for p in range(rowptr[i], rowptr[i + 1]):
+            per[i][colidx[p] // rb].append((colidx[p], vals[p]))
+    w = [max([len(per[i][b]) for i in range(rows)] + [0]) for b in range(nb)]
+    bidx = np.full(sum(w) * rows, -1, np.int32)
+    bval = np.zeros(sum(w) * rows, vals.dtype)
+    off = 0
+    for b in range(nb):
+        for i in range(rows):
+            for sl, (c, v) in enumerate(per[i][b]):
+                bidx[off + sl * rows + i] = c
+                bval[off + sl * rows + i] = v
+        off += w[b] * rows
+    return np.array(w, np.int32), bidx, bval
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("rows,in_rows,ncols,rb", [(96, 20000, 5, None), (3000, 40001, 2, None),
+                                                  (700, 33000, 40, 1000), (5, 9, 3, 4),
+                                                  (2049, 17000, 1, None), (1000, 17000, 512, None),
+                                                  (2048, 17000, 512, None)])
+def test_sparse_block_interleaved_apply_equals_csr_apply(rla, dev, dtype, rows, in_rows, ncols,
+                                                         rb):
+    # the block-interleaved layout against its definition (built on the host),
+    # and its apply against the CSR apply and the oracle, bit for bit: ragged
+    # rows, an empty row, blocks of uneven width, a last block shorter than
+    # rb, beta != 0, 1 to 4 output rows per thread and parts of rows
+    import ctypes
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    rb = rb or 65536 // NP[dtype]().itemsize
+    g = np.random.default_rng(rows + in_rows)
+    nnz_col = 2
+    # two distinct rows per column, never the last row (it stays empty); rows
+    # get uneven counts
+    a = g.integers(0, rows - 1, in_rows)
+    b = (a + 1 + g.integers(0, rows - 2, in_rows)) % (rows - 1)
+    hi = np.sort(np.stack([a, b], axis=1), axis=1)
+    hv = g.standard_normal(hi.size)
+    colptr = torch.arange(0, hi.size + 1, nnz_col, dtype=torch.int32, device=dev)
+    rowidx = torch.as_tensor(hi.reshape(-1).astype(np.int32), device=dev)
+    val = torch.as_tensor(hv, device=dev).to(dtype)
+    rowptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
+    colidx = torch.empty(hi.size, dtype=torch.int32, device=dev)
+    val_t = torch.empty(hi.size, dtype=dtype, device=dev)
+    sfx = lib.sfx(dtype)
+    lib.call(f"rla_sparse_transpose_{sfx}", ctx, rows, in_rows, hi.size, lib.ptr(colptr),
+             lib.ptr(rowidx), lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t))
+    nb = -(-in_rows // rb)
+    widths = torch.empty(nb, dtype=torch.int32, device=dev)
+    total = ctypes.c_int64(0)
+    lib.call("rla_sparse_bell_widths", ctx, rows, in_rows, rb, lib.ptr(rowptr), lib.ptr(colidx),
+             lib.ptr(widths), ctypes.byref(total))
+    bidx = torch.empty(total.value * rows, dtype=torch.int32, device=dev)
+    bval = torch.empty(total.value * rows, dtype=dtype, device=dev)
+    lib.call(f"rla_sparse_bell_build_{sfx}", ctx, rows, in_rows, rb, lib.ptr(rowptr),
+             lib.ptr(colidx), lib.ptr(val_t), lib.ptr(widths), lib.ptr(bidx), lib.ptr(bval))
+    w_h, bidx_h, bval_h = _bell_layout(rowptr.cpu().numpy(), colidx.cpu().numpy(),
+                                       val_t.cpu().numpy(), rows, in_rows, rb)
+    assert total.value == w_h.sum()
+    assert np.array_equal(widths.cpu().numpy(), w_h)
+    assert np.array_equal(bidx.cpu().numpy(), bidx_h)
+    assert np.array_equal(bval.cpu().numpy(), bval_h)
+    X = colmajor(rnd((in_rows, ncols), 3, NP[dtype]), dev)
+    C0 = colmajor(rnd((rows + 3, ncols), 4, NP[dtype]), dev)[:rows]
+    C1, C2 = C0.clone(), C0.clone()
+    lib.call(f"rla_sparse_apply_{sfx}", ctx, 0, rows, in_rows, lib.ptr(colptr), lib.ptr(rowidx),
+             lib.ptr(val), lib.ptr(rowptr), lib.ptr(colidx), lib.ptr(val_t), ncols, lib.ptr(X), 1,
+             in_rows, lib.ptr(C1), C1.stride(0), C1.stride(1), 1.5, -0.5)
+    lib.call(f"rla_sparse_apply_bell_{sfx}", ctx, rows, in_rows, rb, lib.ptr(widths),
+             lib.ptr(bidx), lib.ptr(bval), ncols, lib.ptr(X), 1, in_rows, lib.ptr(C2),
+             C2.stride(0), C2.stride(1), 1.5, -0.5)
+    assert torch.equal(C1, C2)
+    if rows * in_rows <= 10**7:   # the oracle builds M densely
+        want = R.sparse_mul(S_dense(colptr, rowidx, val, rows, in_rows, 0), X.cpu().numpy(), 1.5,
+                            -0.5, C0.cpu().numpy())
+        assert np.array_equal(C2.cpu().numpy(), want)
+
+
+def test_sparse_block_interleaved_rejects_unsorted_rows(rla, dev):
+    import ctypes
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    rowptr = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)
+    for cols in ([3, 1, 0], [1, 1, 0], [0, 9, 2]):       # descending, repeated, out of range
+        colidx = torch.tensor(cols, dtype=torch.int32, device=dev)
+        widths = torch.empty(3, dtype=torch.int32, device=dev)
+        total = ctypes.c_int64(-1)
+        rc = lib.call_rc("rla_sparse_bell_widths", ctx, 2, 9, 4, lib.ptr(rowptr),
+                         lib.ptr(colidx), lib.ptr(widths), ctypes.byref(total))
+        assert rc == lib.RLA_EDOMAIN and total.value == 0

@@ -411,33 +411,35 @@ class _SparseOp:
         self._ell[key] = out
         return out
 
-    def _bell_for(self, dtype):
-        """Block-interleaved gather arrays of M (X blocks of 64 KiB) in `dtype`,
-        built once per operator and cached: (rb, widths, bidx, bval), or None
-        when X fits one block or the copy would hold more than four times the
-        entries plus one slot per row and block."""
-        key = ("bell", dtype)
+    def _bell_for(self, trans_m, dtype):
+        """Block-interleaved gather arrays of op(M) (X blocks of 64 KiB) in
+        `dtype`, built once per operator and cached: (rb, widths, bidx, bval),
+        or None when X fits one block, rows average under 4 entries, or the
+        copy would hold more than four times the entries plus one slot per row
+        and block."""
+        key = ("bell", int(trans_m), dtype)
         if key in self._ell:
             return self._ell[key]
-        rows, in_rows = self.nrows, self.ncols
+        rows, in_rows = (self.ncols, self.nrows) if trans_m else (self.nrows, self.ncols)
         rb = 65536 // (torch.finfo(dtype).bits // 8)
         out = None
-        if rows > 0 and in_rows > rb and self.nnz > 0:
+        if rows > 0 and in_rows > rb and self.nnz >= 4 * rows:
             dev = self.rowidx.device
             ctx = _lib.ctx(dev)
+            ptr, idx = (self.colptr, self.rowidx) if trans_m else (self.rowptr, self.colidx)
             nb = (in_rows + rb - 1) // rb
             widths = torch.empty(nb, dtype=torch.int32, device=dev)
             total = ctypes.c_int64(0)
-            rc = _lib.call_rc("rla_sparse_bell_widths", ctx, rows, in_rows, rb,
-                              _lib.ptr(self.rowptr), _lib.ptr(self.colidx), _lib.ptr(widths),
-                              ctypes.byref(total))
+            rc = _lib.call_rc("rla_sparse_bell_widths", ctx, rows, in_rows, rb, _lib.ptr(ptr),
+                              _lib.ptr(idx), _lib.ptr(widths), ctypes.byref(total))
             if rc == 0 and total.value * rows <= 4 * self.nnz + nb * rows:
-                _, val_t = self._vals_for(dtype)
+                val, val_t = self._vals_for(dtype)
+                v = val if trans_m else val_t
                 bidx = torch.empty(total.value * rows, dtype=torch.int32, device=dev)
                 bval = torch.empty(total.value * rows, dtype=dtype, device=dev)
                 _lib.call(f"rla_sparse_bell_build_{_lib.sfx(dtype)}", ctx, rows, in_rows, rb,
-                          _lib.ptr(self.rowptr), _lib.ptr(self.colidx), _lib.ptr(val_t),
-                          _lib.ptr(widths), _lib.ptr(bidx), _lib.ptr(bval))
+                          _lib.ptr(ptr), _lib.ptr(idx), _lib.ptr(v), _lib.ptr(widths),
+                          _lib.ptr(bidx), _lib.ptr(bval))
                 out = (rb, widths, bidx, bval)
         self._ell[key] = out
         return out
@@ -446,6 +448,18 @@ class _SparseOp:
         """C = alpha*op(M)*X + beta*C on strided views (op = M or M')."""
         dt = X.dtype
         if X.rs == 1 and X.rows > 0 and X.cols > 0:
+            col_bytes = X.rows * (torch.finfo(dt).bits // 8)
+            if col_bytes > 64 * 1024 and not (self._uniform and col_bytes <= 160 * 1024):
+                # X taller than one LDS block (and no whole-column sign path):
+                # the block-interleaved copy keeps every metadata load coalesced
+                bell = self._bell_for(trans_m, dt)
+                if bell is not None:
+                    rb, widths, bidx, bval = bell
+                    _lib.call(f"rla_sparse_apply_bell_{_lib.sfx(dt)}", _lib.ctx(X.t.device),
+                              C.rows, X.rows, rb, _lib.ptr(widths), _lib.ptr(bidx),
+                              _lib.ptr(bval), X.cols, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t),
+                              C.rs, C.cs, alpha, beta)
+                    return
             ell = self._ell_for(trans_m, dt)
             if ell is not None and ell[0] == "sign":
                 _, ptr, es, mag = ell
@@ -844,7 +858,7 @@ class FJLTRecipe(CompressorRecipe):
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
         op = self.op
-        bell = op._bell_for(dt) if not adjoint else None
+        bell = op._bell_for(0, dt) if not adjoint else None
         if bell is not None:
             # forward: the gather reads K's rows from the block-interleaved copy
             rb, widths, bidx, bval = bell

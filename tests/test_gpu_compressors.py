@@ -281,6 +281,27 @@ def test_sparse_vector_and_beta(rla, dev):
     assert out.shape == (30,)
 
 
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("kind", ["sparse_sign", "count_sketch"])
+def test_tall_left_apply_block_interleaved_bit_exact(rla, dev, dtype, kind):
+    # columns of A above 160 KiB (no whole-column sign path): the left apply
+    # runs the block-interleaved gather over 3 (Float32) or 7 (Float64) X
+    # blocks, bit for bit against the oracle, with beta != 0
+    m, n, s = 50000, 3, 64
+    rla.seed(5)
+    A = rnd((m, n), 7, NP[dtype])
+    Ad = colmajor(A, dev)
+    tech = (rla.SparseSign(compression_dim=s, nnz=4, type=dtype) if kind == "sparse_sign"
+            else rla.CountSketch(compression_dim=s, type=dtype))
+    S = rla.complete_compressor(tech, Ad)
+    C0 = rnd((s, n), 8, NP[dtype])
+    C = colmajor(C0, dev)
+    rla.mul(C, S, Ad, 0.75, -2.0)
+    assert S.op._ell[("bell", 0, dtype)] is not None
+    want = R.sparse_mul(S.op.dense(), A, 0.75, -2.0, C0)
+    assert np.array_equal(C.cpu().numpy(), want)
+
+
 def test_sparse_sign_config_c1_bit_exact(rla, dev):
     # 8192 x 256 Float64 N(0,1) (seed 1234), s = 1024, nnz = 8
     rla.seed(1234)
@@ -576,7 +597,7 @@ def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
     C1 = torch.zeros(n, s, dtype=dtype, device=dev).T
     rla.mul(C1, S, Ad, 1.25, 0.0)
     op = S.op
-    assert op._ell[("bell", dtype)] is not None
+    assert op._ell[("bell", 0, dtype)] is not None
     val, val_t = op._vals_for(dtype)
     C2 = torch.zeros(n, s, dtype=dtype, device=dev).T
     Av = rla.as_view(Ad)

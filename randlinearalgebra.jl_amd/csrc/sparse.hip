@@ -914,9 +914,11 @@ static int sparse_apply_bell(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_
   if (rows > INT32_MAX) return fail(RLA_ENOTSUP, "block-interleaved apply: too many rows");
   int rc = use_device(ctx);
   if (rc) return rc;
-  // Parts of at most 4 * kGatherThreads rows, and about two workgroups per CU.
+  // Parts of at most 4 * kGatherThreads rows and at least one wavefront's
+  // worth, and about two workgroups per CU.  The parts of a column read it
+  // from the L2 of their XCD.
   const int64_t min_parts = ceil_div(rows, 4 * int64_t(kGatherThreads));
-  const int64_t max_parts = ceil_div(rows, kGatherThreads);
+  const int64_t max_parts = std::max(min_parts, ceil_div(rows, 64));
   const int64_t want = ceil_div(512, n_rhs);
   const int64_t rsplit = std::max(min_parts, std::min(max_parts, want));
   const int64_t rows_per_part = ceil_div(rows, rsplit);

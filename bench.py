@@ -166,6 +166,33 @@ def fwht_fjlt_rows(rla, torch, reps=10):
                      "frac_hbm_peak": jb / ms_fjlt / 1e6 / HBM_PEAK_GBS}}
 
 
+def tall_sparse_sign_row(rla, torch, reps=10):
+    """SparseSign left apply (sparse_sign.jl:339-361) of a tall A, 1000000x64
+    f32 to 1024 rows, nnz = 8: the columns of A exceed one workgroup's LDS, so
+    the gather walks A's 64 KiB row blocks through the block-interleaved copy
+    of the operator.  Per-apply time and the algorithmic GB/s (one read of A,
+    one write of C)."""
+    dev = torch.device("cuda", 0)
+    m, n, s = 1_000_000, 64, 1024
+    g = torch.Generator(device=dev).manual_seed(1234)
+    A = torch.randn(n, m, generator=g, device=dev, dtype=torch.float32).T
+    C = torch.zeros(n, s, device=dev, dtype=torch.float32).T
+    rla.seed(1234)
+    S = rla.complete_compressor(rla.SparseSign(compression_dim=s, nnz=8, type=torch.float32), A)
+    rla.mul(C, S, A)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        rla.mul(C, S, A)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    b = (m * n + s * n) * 4
+    return {"workload": "SparseSign left apply 1000000x64 f32 -> 1024x64, nnz 8",
+            "ms_per_apply": ms, "GB_s": b / ms / 1e6, "frac_hbm_peak": b / ms / 1e6 / HBM_PEAK_GBS}
+
+
 def rangefinder_c3(rla, torch, reps=3, batch=4):
     """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
     32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
@@ -479,6 +506,7 @@ def main():
     gbs = srht_bytes() / ms / 1e6
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)
     fwht_fjlt = None if args.skip_sparse else fwht_fjlt_rows(rla, torch)
+    tall_sparse = None if args.skip_sparse else tall_sparse_sign_row(rla, torch)
     rf = None if args.skip_rangefinder else rangefinder_c3(rla, torch)
     ls = None if args.skip_ls else ls_batched_c4(rla, torch)
     kz = None if args.skip_kaczmarz else kaczmarz_c5(rla, torch)
@@ -520,6 +548,7 @@ def main():
                      "algorithmic_bytes_per_launch": srht_bytes()},
         "sparse_sign_c1": extra,
         "fwht_fjlt": fwht_fjlt,
+        "tall_sparse_sign": tall_sparse,
         "rangefinder_c3": rf,
         "ls_batched_c4": ls,
         "kaczmarz_c5": kz,

@@ -35,7 +35,7 @@ enum {
   RLA_EDIM = 2,     /* DimensionMismatch (messages mirror src/helpers/mul_dim_checks.jl:41-159) */
   RLA_EDOMAIN = 3,  /* DomainError */
   RLA_ENOMEM = 4,   /* device allocation failed */
-  RLA_EADMIT = 5,   /* request over the context's HBM budget */
+  RLA_EADMIT = 5,   /* request over the device's HBM budget (rla_hbm_*) */
   RLA_EHIP = 6,     /* HIP runtime error */
   RLA_ENOTSUP = 7   /* size outside what the kernels support */
 };
@@ -49,7 +49,29 @@ const char* rla_last_error(void);
  * default stream is). */
 int rla_ctx_create(int device, void* hip_stream, rla_ctx** out);
 int rla_ctx_destroy(rla_ctx* ctx);
+/* Later calls go to hip_stream.  Work the context enqueued on its previous
+ * stream is ordered before them (an event on the old stream that the new one
+ * waits for), since it may still use the context's workspace. */
 int rla_ctx_set_stream(rla_ctx* ctx, void* hip_stream);
+
+/* ---- HBM admission ----------------------------------------------------
+ * One account per device, shared by every context of the process.  No
+ * counterpart in the reference (its Julia arrays are host memory); it is the
+ * admission-by-HBM-budget of a serving loop, moved below the C boundary so
+ * that a caller of the library gets it without the Python host layer.
+ *   budget 0 (the default) means unlimited.
+ *   rla_hbm_admit charges `bytes` for a request (its resident operands and
+ *   working memory) and fails with RLA_EADMIT, charging nothing, when the
+ *   account would exceed the budget; rla_hbm_release returns them.
+ *   The library's own device allocations (context workspaces, batched
+ *   least-squares plans) are charged to the same account: an entry point
+ *   whose workspace would have to grow past the budget fails with
+ *   RLA_EADMIT before it allocates or launches anything.
+ * These calls touch no device and are safe from any thread. */
+int rla_hbm_set_budget(int device, size_t bytes);
+int rla_hbm_admit(int device, size_t bytes);
+int rla_hbm_release(int device, size_t bytes);
+int rla_hbm_query(int device, size_t* used, size_t* budget);
 int rla_ctx_sync(rla_ctx* ctx);
 /* Bytes of device workspace currently held by the context. */
 int rla_ctx_workspace_bytes(rla_ctx* ctx, size_t* out);

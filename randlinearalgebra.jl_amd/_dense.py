@@ -38,6 +38,34 @@ def gemm(C, A, B, alpha=1.0, beta=0.0):
               max(1, ldc))
 
 
+class SourceKey:
+    """Identity of a caller's tensor, for caches of data derived from it.
+
+    The key keeps a reference to the tensor, so its memory cannot be freed and
+    handed to another allocation while the key lives: an equal data pointer
+    then means the same memory, and the version counter, which every view of
+    the storage shares, catches in-place changes.  Derived data must be keyed
+    on the caller's tensor, never on a temporary made from it (a temporary's
+    counter starts at zero and its address is reused by the next one)."""
+
+    __slots__ = ("ref", "sig")
+
+    def __init__(self, t):
+        self.ref = t
+        self.sig = self._sig(t)
+
+    @staticmethod
+    def _sig(t):
+        return (t.data_ptr(), t._version, tuple(t.shape), tuple(t.stride()), t.dtype, t.device)
+
+    def matches(self, t):
+        return self._sig(t) == self.sig
+
+
+def key_matches(key, t):
+    return key is not None and key.matches(t)
+
+
 def ld8(n):
     return (n + 7) & ~7
 

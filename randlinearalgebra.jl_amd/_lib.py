@@ -57,6 +57,10 @@ _SIGS = {
     "rla_ctx_destroy": [vp],
     "rla_ctx_set_stream": [vp, vp],
     "rla_ctx_sync": [vp],
+    "rla_hbm_set_budget": [i32, ctypes.c_size_t],
+    "rla_hbm_admit": [i32, ctypes.c_size_t],
+    "rla_hbm_release": [i32, ctypes.c_size_t],
+    "rla_hbm_query": [i32, vp, vp],
     "rla_ctx_workspace_bytes": [vp, vp],
     "rla_philox4x32_10": [vp, vp, vp],
     "rla_index_from_julia": [vp, vp, i64, i64, vp],
@@ -215,6 +219,22 @@ class _Ctx:
 
 
 _ctxs = {}
+_ctxs_lock = threading.Lock()
+
+
+def _destroy(keys):
+    lib = load()
+    for key in keys:
+        c = _ctxs.pop(key, None)
+        if c is not None and c.handle:
+            lib.rla_ctx_destroy(c.handle)  # waits for its stream, frees its workspace
+
+
+def drop_thread_contexts():
+    """Destroys the calling thread's contexts (a worker thread about to exit)."""
+    me = threading.get_ident()
+    with _ctxs_lock:
+        _destroy([k for k in _ctxs if k[0] == me])
 
 
 def ctx(device=None):
@@ -228,8 +248,12 @@ def ctx(device=None):
     key = (threading.get_ident(), device)
     c = _ctxs.get(key)
     if c is None:
-        c = _Ctx(device)
-        _ctxs[key] = c
+        with _ctxs_lock:
+            # contexts of threads that have ended hold workspaces nobody uses
+            alive = {t.ident for t in threading.enumerate()}
+            _destroy([k for k in _ctxs if k[0] not in alive])
+            c = _Ctx(device)
+            _ctxs[key] = c
     return c.bind()
 
 

@@ -86,12 +86,12 @@ class _Bf16Copies:
         self.rm = self.cm = None
 
     def get(self, A):
-        key = (A.data_ptr(), A._version, tuple(A.shape), tuple(A.stride()), A.dtype)
-        if key != self.key:
+        """A is the caller's matrix (any strides), not a copy of it."""
+        if not _dense.key_matches(self.key, A):
             v = as_view(A)
             self.rm = _dense.bf16_rows(v)      # m x ld8(n)
             self.cm = _dense.bf16_rows(v.T)    # n x ld8(m)
-            self.key = key
+            self.key = _dense.SourceKey(A)
         return self.rm, self.cm
 
 
@@ -110,7 +110,7 @@ def _range(recipe, A):
     prec = recipe.precision
     Y = torch.empty(k, m, dtype=torch.float64, device=dev).T
     if prec == "bf16":
-        rm, cm = recipe._bf16.get(A_)
+        rm, cm = recipe._bf16.get(A)
         if isinstance(S, GaussianRecipe):
             om = _dense.bf16_rows(as_view(S.op).T)    # columns of the sketch, K-contiguous
             Y32 = torch.empty(k, m, dtype=torch.float32, device=dev).T

@@ -1,5 +1,6 @@
 // Context, error reporting, workspace and index conversion for
 // librla_mi355x.so.
+#include <mutex>
 #include <string>
 
 #include "philox.hpp"
@@ -16,6 +17,41 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// ---- HBM admission account, one per device, process-wide -----------------
+// Admitted request bytes and the library's own device allocations (context
+// workspaces, batched least-squares plans) are charged to the same account;
+// a charge that would take `used` past a non-zero budget fails with
+// RLA_EADMIT and changes nothing.
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_hbm_mu;
+size_t g_hbm_budget[kMaxDevices];
+size_t g_hbm_used[kMaxDevices];
+}  // namespace
+
+static int hbm_device_ok(int device, const char* what) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(RLA_EARG, std::string(what) + ": device " + std::to_string(device) +
+                              " out of range");
+  return RLA_OK;
+}
+
+int hbm_resize(int device, size_t old_bytes, size_t new_bytes, const char* what) {
+  int rc = hbm_device_ok(device, what);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_hbm_mu);
+  size_t& used = g_hbm_used[device];
+  const size_t base = used >= old_bytes ? used - old_bytes : 0;
+  const size_t budget = g_hbm_budget[device];
+  if (budget && new_bytes > old_bytes && (new_bytes > budget || base > budget - new_bytes))
+    return fail(RLA_EADMIT, std::string(what) + ": " + std::to_string(new_bytes) +
+                                " bytes exceed the HBM budget of device " +
+                                std::to_string(device) + " (" + std::to_string(base) + " of " +
+                                std::to_string(budget) + " bytes in use)");
+  used = base + new_bytes;
+  return RLA_OK;
+}
+
 int use_device(rla_ctx* ctx) {
   RLA_HIP(hipSetDevice(ctx->device));
   return RLA_OK;
@@ -28,6 +64,8 @@ int workspace(rla_ctx* ctx, size_t bytes, void** out) {
   }
   size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
   want = (want + 255) & ~size_t(255);
+  int rc = hbm_resize(ctx->device, ctx->ws_bytes, want, "device workspace");
+  if (rc) return rc;  // the old workspace stays
   if (ctx->ws) {
     RLA_HIP(hipStreamSynchronize(ctx->stream));
     RLA_HIP(hipFree(ctx->ws));
@@ -37,6 +75,7 @@ int workspace(rla_ctx* ctx, size_t bytes, void** out) {
   void* p = nullptr;
   if (hipMalloc(&p, want) != hipSuccess) {
     (void)hipGetLastError();
+    (void)hbm_resize(ctx->device, want, 0, "device workspace");
     return fail(RLA_ENOMEM, "device workspace allocation of " + std::to_string(want) +
                                 " bytes failed");
   }
@@ -95,7 +134,9 @@ int rla_ctx_destroy(rla_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  (void)hbm_resize(ctx->device, ctx->ws_bytes, 0, "rla_ctx_destroy");
   destroy_blas(ctx);
+  if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RLA_OK;
@@ -103,12 +144,56 @@ int rla_ctx_destroy(rla_ctx* ctx) {
 
 int rla_ctx_set_stream(rla_ctx* ctx, void* hip_stream) {
   RLA_CHECK_CTX(ctx);
+  hipStream_t next = static_cast<hipStream_t>(hip_stream);
+  if (next == ctx->stream) return RLA_OK;
+  int rc = use_device(ctx);
+  if (rc) return rc;
   if (ctx->own_stream) {
-    (void)hipStreamSynchronize(ctx->stream);
+    RLA_HIP(hipStreamSynchronize(ctx->stream));
     (void)hipStreamDestroy(ctx->stream);
     ctx->own_stream = false;
+  } else {
+    // Work already enqueued on the old stream may still use the context's
+    // workspace and scratch: the new stream waits for it before anything
+    // enqueued from now on.
+    if (!ctx->order_ev) RLA_HIP(hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming));
+    RLA_HIP(hipEventRecord(ctx->order_ev, ctx->stream));
+    RLA_HIP(hipStreamWaitEvent(next, ctx->order_ev, 0));
   }
-  ctx->stream = static_cast<hipStream_t>(hip_stream);
+  ctx->stream = next;
+  return RLA_OK;
+}
+
+int rla_hbm_set_budget(int device, size_t bytes) {
+  int rc = hbm_device_ok(device, "rla_hbm_set_budget");
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_hbm_mu);
+  g_hbm_budget[device] = bytes;
+  return RLA_OK;
+}
+
+int rla_hbm_admit(int device, size_t bytes) {
+  return hbm_resize(device, 0, bytes, "rla_hbm_admit");
+}
+
+int rla_hbm_release(int device, size_t bytes) {
+  int rc = hbm_device_ok(device, "rla_hbm_release");
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_hbm_mu);
+  if (bytes > g_hbm_used[device])
+    return fail(RLA_EARG, "rla_hbm_release: " + std::to_string(bytes) +
+                              " bytes is more than the " + std::to_string(g_hbm_used[device]) +
+                              " bytes in use on device " + std::to_string(device));
+  g_hbm_used[device] -= bytes;
+  return RLA_OK;
+}
+
+int rla_hbm_query(int device, size_t* used, size_t* budget) {
+  int rc = hbm_device_ok(device, "rla_hbm_query");
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_hbm_mu);
+  if (used) *used = g_hbm_used[device];
+  if (budget) *budget = g_hbm_budget[device];
   return RLA_OK;
 }
 

@@ -285,7 +285,9 @@ __device__ __forceinline__ void kz_lane_order() { __builtin_amdgcn_wave_barrier(
 //     the MFMA, then a right-looking factor of its 16 columns with lane i
 //     holding row i, the forward solve L z = r folded in), then solves
 //     L' y = D^-1 z.
-// A pivot d_j not above 4 s eps G_jj (a numerically dependent row) gives
+// A pivot d_j not above 4 s eps G_jj (a numerically dependent row, e.g. a
+// row drawn twice with replacement) gets a zero column of L and y_j = 0: the
+// correction projects onto the independent rows.  A non-finite pivot gives
 // y = 0 and sets *info to its 1-based position; an all-zero block gives
 // y = 0 (kaczmarz.jl:300 skips it).
 // Gram tiles of wavefront W: b = W, W + 4, W + 8 (tile row, tile column from
@@ -574,18 +576,24 @@ __global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int jj = r0 + j;
-        const double d = kz_bcast(a[j], jj);  // pivot d_jj
-        const bool ok = a[j] > fl && isfinite(a[j]);
+        const double d = kz_bcast(a[j], jj);     // pivot d_jj
+        const double flj = kz_bcast(fl, jj);     // its floor
+        // A pivot not above its floor is a row dependent on the rows before
+        // it (a row drawn twice, or a combination of others): its column of
+        // L and its coefficient are zero, so the projection is onto the span
+        // of the independent rows (exact for a consistent block).
+        const bool dep = !(d > flj);
         // 1/d: reciprocal estimate refined by two Newton steps e = 1 - d r, r += r e
         double inv = __builtin_amdgcn_rcp(d);
         inv = fma(inv, fma(-d, inv, 1.0), inv);
         inv = fma(inv, fma(-d, inv, 1.0), inv);
+        if (dep) inv = 0.0;
         if (i == jj) {
-          okme = ok;
+          okme = isfinite(d);
           dme = d;
           invme = inv;
         }
-        const double u = a[j];  // L_ij d_j
+        const double u = dep ? 0.0 : a[j];  // L_ij d_j
         const double l = u * inv;
         a[j] = l;
         // forward solve with unit L, right-looking
@@ -602,7 +610,7 @@ __global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
     }
   }
   const unsigned long long badm = __ballot(!okme && i < s);
-  if (badm) {  // singular block: skipped, and recorded (info is sticky)
+  if (badm) {  // non-finite block: skipped, and recorded (info is sticky)
     if (i < s) p.y_out[i] = 0.0;
     if (i == 0) {
       *p.info = __ffsll(badm);  // 1-based first failing pivot
@@ -999,7 +1007,7 @@ int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const floa
   int info = 0;
   RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RLA_HIP(hipStreamSynchronize(ctx->stream));
-  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is singular (rank-deficient block)");
+  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
   return RLA_OK;
 }
 
@@ -1057,7 +1065,7 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
       RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
       RLA_HIP(hipStreamSynchronize(ctx->stream));
       if (info > 0)
-        return fail(RLA_EDOMAIN, "block Gram matrix is singular (rank-deficient block)");
+        return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
       if (hist && nh < hist_cap) hist[nh] = err;
       ++nh;
       conv = (i < max_it) ? (err < threshold) : 1;  // update_logger!, basic_logger.jl:93-98
@@ -1088,7 +1096,7 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
   int info = 0;
   RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   RLA_HIP(hipStreamSynchronize(ctx->stream));
-  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is singular (rank-deficient block)");
+  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
   if (n_hist) *n_hist = nh;
   if (iterations) *iterations = it;
   if (converged) *converged = conv;

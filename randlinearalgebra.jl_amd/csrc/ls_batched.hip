@@ -310,6 +310,7 @@ struct rla_ls_plan {
   rla_ctx* ctx;
   int64_t nprob;
   void* mem;  // one device allocation for descriptors, tiles and per-problem state
+  size_t bytes;  // its size, charged to the device's HBM account
   LsDev* probs;
   SketchTile* stiles;
   int nstiles;
@@ -433,12 +434,16 @@ int rla_ls_plan_create(rla_ctx* ctx, int64_t nprob, const rla_ls_problem* probs,
     o_dinv[p] = take(sizeof(double) * ceil_div(k, 16) * 256);
   }
   void* mem = nullptr;
+  rc = hbm_resize(ctx->device, 0, off, "rla_ls_plan_create");
+  if (rc) return rc;
   if (hipMalloc(&mem, off) != hipSuccess) {
     (void)hipGetLastError();
+    (void)hbm_resize(ctx->device, off, 0, "rla_ls_plan_create");
     return fail(RLA_ENOMEM, "rla_ls_plan_create: device allocation failed");
   }
   auto* plan = new rla_ls_plan{};
   plan->ctx = ctx;
+  plan->bytes = off;
   plan->nprob = nprob;
   plan->mem = mem;
   char* base = static_cast<char*>(mem);
@@ -474,6 +479,7 @@ int rla_ls_plan_create(rla_ctx* ctx, int64_t nprob, const rla_ls_problem* probs,
   }
   auto cleanup = [&](int code) {
     (void)hipFree(mem);
+    (void)hbm_resize(ctx->device, off, 0, "rla_ls_plan_create");
     delete plan;
     return code;
   };
@@ -592,6 +598,7 @@ int rla_ls_plan_destroy(rla_ls_plan* plan) {
   if (plan->graph) (void)hipGraphDestroy(plan->graph);
   if (plan->cap) (void)hipStreamDestroy(plan->cap);
   (void)hipFree(plan->mem);
+  (void)hbm_resize(plan->ctx->device, plan->bytes, 0, "rla_ls_plan_destroy");
   delete plan;
   return RLA_OK;
 }

@@ -19,6 +19,7 @@ struct rla_ctx {
   int* flag = nullptr;      // small device word for kernel-side checks
   int* flag_host = nullptr; // pinned mirror of flag
   void* blas = nullptr;     // rocblas_handle, created on first use
+  hipEvent_t order_ev = nullptr;  // orders work across a change of stream
 };
 
 namespace rla {
@@ -75,6 +76,11 @@ int fail(int code, const std::string& msg);
 
 // Releases the context's rocBLAS handle (linalg.hip).
 void destroy_blas(rla_ctx* ctx);
+
+// Moves a device allocation of the library from old_bytes to new_bytes on the
+// device's HBM account (0 -> n charges, n -> 0 releases).  Growth past a
+// non-zero budget fails with RLA_EADMIT and leaves the account unchanged.
+int hbm_resize(int device, size_t old_bytes, size_t new_bytes, const char* what);
 
 // Makes ctx->device current for the calling thread.
 int use_device(rla_ctx* ctx);

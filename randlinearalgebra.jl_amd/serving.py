@@ -14,6 +14,7 @@ Latency is measured from a request's arrival time (given, or the submit
 time) to the completion of its last kernel.
 """
 import collections
+import ctypes
 import threading
 import time
 from concurrent.futures import Future
@@ -21,7 +22,7 @@ from concurrent.futures import Future
 import numpy as np
 import torch
 
-from . import _dense
+from . import _dense, _lib
 from ._lib import AdmissionError, ArgumentError
 from .approximators import RangeFinder, _Bf16Copies, complete_approximator, rapproximate_
 from .compressors import Gaussian, Right, as_view, seed
@@ -29,6 +30,8 @@ from .solvers import (BasicLogger, FullResidual, Kaczmarz, L2Norm, Sampling, com
                       rsolve)
 
 _setup_lock = threading.Lock()  # recipe construction draws from the global stream
+# per request: growth of the slot context's own library workspace
+_SLOT_SCRATCH = 8 << 20
 
 
 class RangeFinderRequest:
@@ -64,6 +67,7 @@ class _Resident:
         self.bf16 = None
         self.A_rm = None
         self.weights = None
+        self.key = None
         self.bytes = A.numel() * A.element_size()
 
 
@@ -80,8 +84,18 @@ class Server:
         self.slots = int(slots)
         if self.slots < 1:
             raise ArgumentError("slots must be positive")
+        # Admission runs on the library's HBM account of the device (rla_hbm_*),
+        # which also carries the library's own workspaces: the account's budget
+        # is what the library already holds plus this server's budget, so the
+        # growth of workspaces while serving counts against the budget.  One
+        # server per device at a time.
+        self._dev = self.device.index if self.device.index is not None else 0
+        used0, prev = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        _lib.call("rla_hbm_query", self._dev, ctypes.byref(used0), ctypes.byref(prev))
+        self._prev_budget = prev.value
+        _lib.call("rla_hbm_set_budget", self._dev, used0.value + self.budget)
         self.resident = {}
-        self.used = 0
+        self._resident_bytes = 0
         self.in_flight = 0
         self.max_in_flight = 0
         self._cv = threading.Condition()
@@ -114,16 +128,17 @@ class Server:
                 r.A_rm = torch.empty(v.rows, v.cols, dtype=A.dtype, device=A.device)
                 r.A_rm.copy_(A)
                 w = torch.empty(v.rows, dtype=torch.float64, device=A.device)
-                from . import _lib
                 _lib.call("rla_row_norms2_f32", _lib.ctx(A.device), v.rows, v.cols,
                           _lib.ptr(A), 1, v.cs, _lib.ptr(w))
                 r.weights = w
+                r.key = _dense.SourceKey(A)  # what A_rm and the weights were made from
                 r.bytes += r.A_rm.numel() * 4 + w.numel() * 8
             torch.cuda.current_stream(self.device).synchronize()
         with self._cv:
-            if self.used + r.bytes > self.budget:
-                raise AdmissionError(f"matrix {name!r} ({r.bytes} B) exceeds the HBM budget")
-            self.used += r.bytes
+            if _lib.call_rc("rla_hbm_admit", self._dev, r.bytes) != _lib.RLA_OK:
+                raise AdmissionError(f"matrix {name!r} ({r.bytes} B) exceeds the HBM budget: "
+                                     + _lib.last_error())
+            self._resident_bytes += r.bytes
             self.resident[name] = r
 
     # -- requests ------------------------------------------------------------------
@@ -134,9 +149,9 @@ class Server:
         if req.kind == "rangefinder":
             p = 1 if req.precision == "bf16" else 0
             ws = _dense.query_bytes("rla_rangefinder_workspace_bytes", p, 4, m, n, req.k)
-            return ws + 8 * m * req.k * 3 + 4 * n * req.k * 2 + (1 << 20)
+            return ws + 8 * m * req.k * 3 + 4 * n * req.k * 2 + _SLOT_SCRATCH
         ws = _dense.query_bytes("rla_kaczmarz_workspace_bytes", m, n, req.block, req.every)
-        return ws + 4 * n * 2 + (1 << 20)
+        return ws + 4 * n * 2 + _SLOT_SCRATCH
 
     def submit(self, req):
         """Queue a request; returns a Future resolving to its result."""
@@ -145,7 +160,7 @@ class Server:
         if self.resident[req.matrix].kind != req.kind:
             raise ArgumentError(f"matrix {req.matrix!r} is not registered for {req.kind}")
         need = self._need(req)
-        resident = sum(r.bytes for r in self.resident.values())
+        resident = self._resident_bytes
         if need > self.budget - resident:
             raise AdmissionError(f"request needs {need} B, more than the budget leaves "
                                  f"({self.budget - resident} B)")
@@ -164,13 +179,21 @@ class Server:
             with self._cv:
                 while True:
                     if self._closed and not self._queue:
+                        _lib.drop_thread_contexts()
                         return
-                    if self._queue and self.used + self._queue[0][1] <= self.budget:
-                        req, need, fut = self._queue.popleft()
-                        self.used += need
-                        self.in_flight += 1
-                        self.max_in_flight = max(self.max_in_flight, self.in_flight)
-                        break
+                    if self._queue:
+                        if _lib.call_rc("rla_hbm_admit", self._dev,
+                                        self._queue[0][1]) == _lib.RLA_OK:
+                            req, need, fut = self._queue.popleft()
+                            self.in_flight += 1
+                            self.max_in_flight = max(self.max_in_flight, self.in_flight)
+                            break
+                        if self.in_flight == 0:
+                            # nothing in flight will free memory: the request
+                            # can never be admitted
+                            _, _, dead = self._queue.popleft()
+                            dead.set_exception(AdmissionError(_lib.last_error()))
+                            continue
                     self._cv.wait()
             t_admit = time.perf_counter()
             try:
@@ -186,7 +209,7 @@ class Server:
                 fut.set_exception(e)
             finally:
                 with self._cv:
-                    self.used -= need
+                    _lib.call("rla_hbm_release", self._dev, need)
                     self.in_flight -= 1
                     self._cv.notify_all()
 
@@ -216,9 +239,8 @@ class Server:
         rec.rng = (req.seed, 0)
         rec.A_rm, rec._rm_key = r.A_rm, None
         d = rec.compressor.distribution_recipe
-        v = as_view(A)
-        d.weights, d._key = r.weights, (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
-        rec._rm_key = (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
+        d.weights, d._key = r.weights, r.key
+        rec._rm_key = r.key
         rsolve(rec, x, A, b)
         return x, rec.log
 
@@ -245,6 +267,13 @@ class Server:
             self._cv.notify_all()
         for w in self._workers:
             w.join()
+        with self._cv:
+            if self._resident_bytes:
+                _lib.call("rla_hbm_release", self._dev, self._resident_bytes)
+                self._resident_bytes = 0
+            if self._prev_budget is not None:
+                _lib.call("rla_hbm_set_budget", self._dev, self._prev_budget)
+                self._prev_budget = None
 
     def __enter__(self):
         return self

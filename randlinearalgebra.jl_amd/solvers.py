@@ -77,8 +77,7 @@ class DistributionRecipe:
             self.weights = None
             return
         v = as_view(A)
-        key = (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
-        if key == self._key:
+        if _dense.key_matches(self._key, v.t):
             return  # A unchanged since the weights were computed
         w = torch.empty(self.n, dtype=torch.float64, device=v.t.device)
         rs, cs = (v.rs, v.cs) if self.cardinality == Left() else (v.cs, v.rs)
@@ -86,7 +85,7 @@ class DistributionRecipe:
         _lib.call(f"rla_row_norms2_{_lib.sfx(v.dtype)}", _lib.ctx(v.t.device), self.n, cols,
                   _lib.ptr(v.t), rs, cs, _lib.ptr(w))
         self.weights = w
-        self._key = key
+        self._key = _dense.SourceKey(v.t)
 
     def positive(self):
         if self.weights is None:
@@ -506,11 +505,10 @@ def _row_major(recipe, A):
     v = as_view(A)
     if v.cs == 1 and v.rs >= v.cols:
         return v.t, v.rs
-    key = (v.t.data_ptr(), v.t._version, v.rows, v.cols, v.rs, v.cs)
-    if recipe._rm_key != key:
+    if not _dense.key_matches(recipe._rm_key, v.t):
         recipe.A_rm = torch.empty(v.rows, v.cols, dtype=v.dtype, device=v.t.device)
         recipe.A_rm.copy_(A)
-        recipe._rm_key = key
+        recipe._rm_key = _dense.SourceKey(v.t)
     return recipe.A_rm, v.cols
 
 

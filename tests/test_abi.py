@@ -251,3 +251,32 @@ def test_agmon_checks_mirror_reference(rla):
     assert mr.r is None
     with pytest.raises(rla.ArgumentError, match="cannot output 6 indices when beta=5"):
         rla.sample_distribution(torch.zeros(6, dtype=torch.int32), mr)
+
+
+def test_hbm_admission_account(rla):
+    # the per-device HBM account behind RLA_EADMIT: host bookkeeping, no device
+    lib = rla._lib.load()
+    dev = 63  # an account no other test touches
+    used, budget = ctypes.c_size_t(), ctypes.c_size_t()
+    assert lib.rla_hbm_query(dev, ctypes.byref(used), ctypes.byref(budget)) == 0
+    assert used.value == 0 and budget.value == 0
+    # unlimited by default
+    assert lib.rla_hbm_admit(dev, 1 << 40) == 0
+    assert lib.rla_hbm_release(dev, 1 << 40) == 0
+    assert lib.rla_hbm_set_budget(dev, 1000) == 0
+    assert lib.rla_hbm_admit(dev, 600) == 0
+    assert lib.rla_hbm_admit(dev, 400) == 0          # exactly at the budget
+    assert lib.rla_hbm_admit(dev, 1) == rla._lib.RLA_EADMIT
+    assert b"exceed the HBM budget" in lib.rla_last_error()
+    with pytest.raises(rla.AdmissionError):
+        rla._lib.check(lib.rla_hbm_admit(dev, 5))
+    lib.rla_hbm_query(dev, ctypes.byref(used), ctypes.byref(budget))
+    assert used.value == 1000 and budget.value == 1000  # a refused charge changes nothing
+    assert lib.rla_hbm_release(dev, 400) == 0
+    assert lib.rla_hbm_admit(dev, 2 ** 64 - 1) == rla._lib.RLA_EADMIT  # no wrap-around
+    assert lib.rla_hbm_admit(dev, 400) == 0
+    assert lib.rla_hbm_release(dev, 1001) == rla._lib.RLA_EARG  # more than in use
+    assert lib.rla_hbm_release(dev, 1000) == 0
+    assert lib.rla_hbm_set_budget(dev, 0) == 0
+    assert lib.rla_hbm_admit(-1, 1) == rla._lib.RLA_EARG
+    assert lib.rla_hbm_set_budget(64, 1) == rla._lib.RLA_EARG

@@ -249,6 +249,41 @@ def test_rangefinder_bf16_error_close_to_plain(rla, dev):
     assert errs["bf16"] <= 1.1 * errs["plain"], errs
 
 
+def test_rangefinder_bf16_recipe_sees_changed_and_reallocated_matrix(rla, dev):
+    # the bf16 copies of A are cached per recipe; a row-major A (a fresh
+    # column-major temporary would be made from it on every call), changed in
+    # place or replaced by another matrix of the same shape, must not reuse them
+    def rowmajor(a):
+        return torch.as_tensor(a).to(dev).contiguous()
+
+    def proj_err(Q, A):
+        return np.linalg.norm(A - Q @ (Q.T @ A)) / np.linalg.norm(A)
+
+    rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=12),
+                         power_its=1, orthogonalize=True, precision="bf16")
+    A1 = R._q(rnd((500, 10), 1)) @ rnd((10, 200), 2)
+    Ad = rowmajor(A1)
+    rla.seed(4)
+    rec = rla.rapproximate(rf, Ad)
+    assert proj_err(rec.range.cpu().numpy(), A1) < 2e-2
+    A2 = R._q(rnd((500, 10), 3)) @ rnd((10, 200), 4)
+    Ad.copy_(torch.as_tensor(A2))              # in place
+    rla.rapproximate_(rec, Ad)
+    assert proj_err(rec.range.cpu().numpy(), A2) < 2e-2
+    del Ad
+    torch.cuda.synchronize()
+    A3 = R._q(rnd((500, 10), 5)) @ rnd((10, 200), 6)
+    Ad3 = rowmajor(A3)                         # reallocated, same shape
+    rla.rapproximate_(rec, Ad3)
+    assert proj_err(rec.range.cpu().numpy(), A3) < 2e-2
+    # a fresh recipe with the same compressor gives the same range
+    rec2 = rla.complete_approximator(rf, Ad3)
+    rec2.compressor = rec.compressor
+    rla.rapproximate_(rec2, Ad3)
+    Q, Q2 = rec.range.cpu().numpy(), rec2.range.cpu().numpy()
+    assert np.linalg.norm(Q @ Q.T - Q2 @ Q2.T) < 1e-12
+
+
 @pytest.mark.parametrize("prec", ["plain", "bf16"])
 @pytest.mark.parametrize("ortho", [True, False])
 def test_rangefinder_rank_deficient_reruns_with_householder(rla, dev, prec, ortho):

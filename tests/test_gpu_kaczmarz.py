@@ -160,14 +160,31 @@ def test_block_update_matches_oracle(rla, dev, s, n, alpha):
         assert np.allclose(got, R.kaczmarz_update(x, At[0], bt, alpha), rtol=1e-5, atol=1e-5)
 
 
-def test_block_update_skips_zero_block_and_rejects_singular(rla, dev):
+def test_block_update_skips_zero_block_and_dependent_rows(rla, dev):
     x = rnd(20, 1).astype(np.float32)
     got = block_update(rla, dev, np.zeros((4, 20), np.float32), np.ones(4, np.float32), x, 1.0)
     assert np.array_equal(got, x.astype(np.float64))
+    # a row drawn twice (sampling with replacement) of a consistent system:
+    # the pseudoinverse projection, which is the projection onto the other rows
     At = rnd((4, 20), 2).astype(np.float32)
     At[3] = At[1]
-    with pytest.raises(rla._lib.DomainError):
-        block_update(rla, dev, At, np.ones(4, np.float32), x, 1.0)
+    bt = rnd(4, 3).astype(np.float32)
+    bt[3] = bt[1]
+    got = block_update(rla, dev, At, bt, x, 1.0)
+    want = R.kaczmarz_update_block(x, At, bt, 1.0)
+    assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+    want3 = R.kaczmarz_update_block(x, At[:3], bt[:3], 1.0)
+    assert np.linalg.norm(got - want3) <= 1e-5 * np.linalg.norm(want3)
+    # a row that is a combination of two others
+    At[3] = 0.5 * At[0] - 2.0 * At[2]
+    bt[3] = 0.5 * bt[0] - 2.0 * bt[2]
+    got = block_update(rla, dev, At, bt, x, 1.0)
+    want3 = R.kaczmarz_update_block(x, At[:3], bt[:3], 1.0)
+    assert np.linalg.norm(got - want3) <= 1e-5 * np.linalg.norm(want3)
+    # a non-finite block is rejected
+    At[2, 5] = np.nan
+    with pytest.raises(rla._lib.DomainError, match="not finite"):
+        block_update(rla, dev, At, bt, x, 1.0)
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (1000, 37), (100000, 1000)])
@@ -214,6 +231,50 @@ def test_kaczmarz_device_loop_converges(rla, dev, dist):
     # the recorded history decreases (geometric convergence)
     h = rec.log.hist[:rec.log.record_location]
     assert h[0] > h[-1]
+
+
+def test_kaczmarz_device_loop_with_replacement_converges(rla, dev):
+    # Uniform(replace=True) draws a row twice in some blocks (here about one
+    # block in three); those blocks project onto their distinct rows
+    A, xs, Ad, b = consistent_system(2000, 100, 12, dev)
+    bn = float(torch.linalg.vector_norm(b.double()))
+    solver = rla.Kaczmarz(
+        compressor=rla.Sampling(compression_dim=32, distribution=rla.Uniform(replace=True)),
+        log=rla.BasicLogger(max_it=4000, threshold=1e-6 * bn), error=rla.FullResidual(every=25))
+    x = torch.zeros(100, dtype=F32, device=dev)
+    rla.seed(3)
+    x, rec = rla.rsolve(solver, x, Ad, b)
+    assert rec.log.converged and rec.log.iteration < 4000
+    assert np.linalg.norm(x.cpu().numpy().astype(np.float64) - xs) <= 1e-5 * np.linalg.norm(xs)
+
+
+def test_kaczmarz_recipe_sees_changed_and_reallocated_matrix(rla, dev):
+    # the recipe caches a row-major copy of A and the L2Norm weights; an A
+    # changed in place, or a new A of the same shape, must not reuse them
+    m, n = 1500, 80
+    solver = rla.Kaczmarz(compressor=rla.Sampling(compression_dim=16, distribution=rla.L2Norm()),
+                          log=rla.BasicLogger(max_it=3000, threshold=0.0),
+                          error=rla.FullResidual(every=50))
+    A1, xs1, Ad, b1 = consistent_system(m, n, 40, dev)
+    x = torch.zeros(n, dtype=F32, device=dev)
+    rla.seed(5)
+    rec = rla.complete_solver(solver, x, Ad, b1)
+    rec.log.threshold_info = 1e-6 * float(torch.linalg.vector_norm(b1.double()))
+    rla.rsolve(rec, x, Ad, b1)
+    assert np.linalg.norm(x.cpu().numpy() - xs1) <= 1e-4 * np.linalg.norm(xs1)
+    # in place: A <- 2 A with the same b, so the solution halves
+    Ad.mul_(2.0)
+    x2 = torch.zeros(n, dtype=F32, device=dev)
+    rla.rsolve(rec, x2, Ad, b1)
+    assert np.linalg.norm(x2.cpu().numpy() - xs1 / 2) <= 1e-4 * np.linalg.norm(xs1)
+    # reallocated: drop A and make another matrix of the same shape
+    del Ad
+    torch.cuda.synchronize()
+    A3, xs3, Ad3, b3 = consistent_system(m, n, 41, dev)
+    x3 = torch.zeros(n, dtype=F32, device=dev)
+    rec.log.threshold_info = 1e-6 * float(torch.linalg.vector_norm(b3.double()))
+    rla.rsolve(rec, x3, Ad3, b3)
+    assert np.linalg.norm(x3.cpu().numpy() - xs3) <= 1e-4 * np.linalg.norm(xs3)
 
 
 def test_kaczmarz_device_loop_stops_at_max_it(rla, dev):

@@ -96,3 +96,42 @@ def test_admission_by_hbm_budget(rla, dev, system):
     with pytest.raises(rla._lib.AdmissionError):
         with rla.Server(slots=1, hbm_budget=1000) as srv:
             srv.register("kz", A2, "kaczmarz")
+
+
+def test_entry_point_workspace_honours_hbm_budget(rla, dev):
+    # an entry point whose context workspace would grow past the device's HBM
+    # budget fails with RLA_EADMIT before it launches anything
+    import ctypes
+    import threading
+    lib = rla._lib.load()
+    used, budget = ctypes.c_size_t(), ctypes.c_size_t()
+    rla._lib.check(lib.rla_hbm_query(0, ctypes.byref(used), ctypes.byref(budget)))
+    out = {}
+
+    def work():
+        idx1 = torch.arange(1, 11, dtype=torch.int64, device=dev)
+        idx0 = torch.full((10,), -1, dtype=torch.int32, device=dev)
+        c = rla._lib.ctx(dev)  # this thread's new context holds no workspace yet
+        try:
+            lib.rla_hbm_set_budget(0, used.value + 4096)
+            out["rc"] = lib.rla_index_from_julia(c, rla._lib.ptr(idx1), 10, 100,
+                                                 rla._lib.ptr(idx0))
+            out["msg"] = lib.rla_last_error()
+            out["untouched"] = idx0.cpu().clone()
+            lib.rla_hbm_set_budget(0, budget.value)
+            out["rc2"] = lib.rla_index_from_julia(c, rla._lib.ptr(idx1), 10, 100,
+                                                  rla._lib.ptr(idx0))
+            out["idx0"] = idx0.cpu()
+        finally:
+            lib.rla_hbm_set_budget(0, budget.value)
+            rla._lib.drop_thread_contexts()
+
+    t = threading.Thread(target=work)
+    t.start()
+    t.join()
+    assert out["rc"] == rla._lib.RLA_EADMIT and b"HBM budget" in out["msg"]
+    assert (out["untouched"] == -1).all()
+    assert out["rc2"] == 0 and torch.equal(out["idx0"], torch.arange(10, dtype=torch.int32))
+    after = ctypes.c_size_t()
+    lib.rla_hbm_query(0, ctypes.byref(after), None)
+    assert after.value == used.value  # the dropped context released its workspace

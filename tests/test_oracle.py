@@ -140,6 +140,34 @@ def test_srht_adjoint_is_transpose_of_forward():
     assert np.max(np.abs(got - 2.0 * T.T @ X)) < 1e-12
 
 
+@pytest.mark.parametrize("m,ncols,s", [(4096, 5, 256), (1000, 3, 64), (70000, 2, 2048)])
+def test_srht_left_c_restatement_equals_numpy(coracle, m, ncols, s):
+    # oracle_srht_left_f32 (the CPU-baseline leg) against the numpy restatement,
+    # which test_srht_left_matches_reference_test_formula pins to the reference
+    # test formula: same operation order, so the bits must agree, including
+    # padded rows (m not a power of two), alpha and beta
+    rng = np.random.default_rng(m + s)
+    A = np.asfortranarray(rng.standard_normal((m, ncols)).astype(np.float32))
+    n_pad = R.padded_size(m)
+    op = rng.choice(n_pad, s, replace=False).astype(np.int32)
+    sg = _signs(n_pad, m + 3)
+    scale = float(np.float32(math.sqrt(s) / math.sqrt(n_pad)))
+    C0 = np.asfortranarray(rng.standard_normal((s, ncols)).astype(np.float32))
+    chunks, neg_chunks = _pack(sg), _pack(~sg)  # kept alive across the foreign calls
+    for alpha, beta in ((1.0, 0.0), (1.5, 0.5)):
+        want = R.srht_left_mul(A, op, sg, scale, np.float32, C0.copy(), alpha, beta)
+        C = C0.copy(order="F")
+        coracle.oracle_srht_left_f32(A.ctypes.data, m, ncols, m, n_pad, s, op.ctypes.data,
+                                     chunks.ctypes.data, scale, C.ctypes.data, s, alpha,
+                                     beta, 2)
+        assert np.array_equal(C, want)
+    # a planted mistake (the sign vector negated) must be seen
+    C = C0.copy(order="F")
+    coracle.oracle_srht_left_f32(A.ctypes.data, m, ncols, m, n_pad, s, op.ctypes.data,
+                                 neg_chunks.ctypes.data, scale, C.ctypes.data, s, 1.0, 0.0, 2)
+    assert not np.array_equal(C, R.srht_left_mul(A, op, sg, scale, np.float32))
+
+
 @pytest.mark.parametrize("m,s,nnz", [(10, 6, 3), (300, 40, 8), (64, 64, 64)])
 def test_sparse_rowgather_matches_dense_product(m, s, nnz):
     idx, val = R.sparse_sign_gen(99, 0, m, s, nnz)
@@ -187,6 +215,34 @@ def test_sparse_sign_gen_structure():
     counts = np.bincount(idx, minlength=32)
     chi2 = ((counts - 125.0) ** 2 / 125.0).sum()
     assert chi2 < 70  # 31 dof, p ~ 1e-4
+
+
+def test_count_sketch_gen_distribution():
+    # count_sketch.jl:124-127: each column has one entry, at a row uniform on
+    # 1:s, with a Rademacher sign; rows and signs independent of each other
+    s, n = 64, 12800
+    idx, val = R.count_sketch_gen(1234, 0, n, s)
+    assert idx.shape == (n,) and val.shape == (n,)
+    assert idx.min() >= 0 and idx.max() < s
+    assert set(np.unique(val)) == {-1.0, 1.0}
+    # uniform rows: chi-square over s bins (63 dof; 120 is p ~ 1e-5)
+    counts = np.bincount(idx, minlength=s)
+    e = n / s
+    assert ((counts - e) ** 2 / e).sum() < 120
+    # sign balance: |#(+1) - n/2| within 4.5 sigma
+    assert abs((val > 0).sum() - n / 2) < 4.5 * math.sqrt(n / 4)
+    # independence of row and sign: 2 x s contingency chi-square (63 dof)
+    pos = np.bincount(idx[val > 0], minlength=s)
+    neg = counts - pos
+    exp_pos = counts * (val > 0).mean()
+    exp_neg = counts - exp_pos
+    chi2 = ((pos - exp_pos) ** 2 / exp_pos).sum() + ((neg - exp_neg) ** 2 / exp_neg).sum()
+    assert chi2 < 120
+    # successive columns are not correlated (a stream reused per column would be)
+    assert abs(np.corrcoef(idx[:-1], idx[1:])[0, 1]) < 0.05
+    # another epoch draws another operator
+    idx2, _ = R.count_sketch_gen(1234, 1, n, s)
+    assert np.mean(idx2 == idx) < 0.1
 
 
 def test_floyd_is_uniform_over_subsets():

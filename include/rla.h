@@ -397,6 +397,37 @@ int rla_gemm_f64_mfma(rla_ctx* ctx, int trans_a, int trans_b, int64_t m, int64_t
                       double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                       double beta, double* C, int64_t ldc);
 
+/* Dense product of strided views on the FP32 / FP64 matrix cores
+ * (v_mfma_f32_16x16x4_f32, v_mfma_f64_16x16x4_f64), products and sums in the
+ * data's own precision: C = alpha * A * B + beta * C with A (m x k) at
+ * A[i*a_rs + l*a_cs], B (k x n) at B[l*b_rs + j*b_cs], C at C[i*c_rs + j*c_cs].
+ * A Julia column-major matrix is (p, 1, ld), its adjoint (p, ld, 1).  beta = 0
+ * does not read C.  Deterministic: long-K products split K over workgroups and
+ * add the partial sums in a fixed order.  Replaces the BLAS gemm!/gemv! calls of
+ * the reference's dense paths (gaussian.jl:145-170, rangefinder.jl:151-191,
+ * randsvd.jl:198-293, ihs.jl:220-249, col_projection.jl:282-334). */
+int rla_gemm_strided_f32(rla_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                         const float* A, int64_t a_rs, int64_t a_cs, const float* B, int64_t b_rs,
+                         int64_t b_cs, double beta, float* C, int64_t c_rs, int64_t c_cs);
+int rla_gemm_strided_f64(rla_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* A, int64_t a_rs, int64_t a_cs, const double* B,
+                         int64_t b_rs, int64_t b_cs, double beta, double* C, int64_t c_rs,
+                         int64_t c_cs);
+
+/* Gaussian compressor apply, gaussian.jl:145-170: side 0 (Left)
+ * C = alpha * S * A + beta * C, side 1 (Right) C = alpha * A * S + beta * C,
+ * with S = recipe.op (s_rows x s_cols, strided: pass the adjoint's strides for
+ * S') and A (a_rows x a_cols, strided).  RLA_EDIM when the inner dimensions
+ * differ.  The same kernels as rla_gemm_strided_*. */
+int rla_gaussian_apply_f32(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols,
+                           const float* S, int64_t s_rs, int64_t s_cs, int64_t a_rows,
+                           int64_t a_cols, const float* A, int64_t a_rs, int64_t a_cs,
+                           double alpha, double beta, float* C, int64_t c_rs, int64_t c_cs);
+int rla_gaussian_apply_f64(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols,
+                           const double* S, int64_t s_rs, int64_t s_cs, int64_t a_rows,
+                           int64_t a_cols, const double* A, int64_t a_rs, int64_t a_cs,
+                           double alpha, double beta, double* C, int64_t c_rs, int64_t c_cs);
+
 /* BF16 copies: dst[l + i*ldd] = bf16(src[i, l]) (round to nearest even) for
  * the rows x cols strided view src (rows become contiguous K-runs). */
 int rla_to_bf16_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t s_rs,
@@ -464,8 +495,20 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
                     const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
                     double* Y, int64_t ldy, void* work, size_t work_bytes);
 
+/* FP64 SVD of a k x k matrix, k <= 288 (the `svd(QA)` of randsvd.jl:186 once
+ * QA is reduced to its k x k triangular factor): X = U diag(S) V' (X' when
+ * transpose != 0), S descending, U and V k x k.  One-sided Jacobi on the
+ * device (one cooperative launch); `sweeps` (host, may be NULL, then nothing
+ * blocks) receives the number of Jacobi sweeps and makes the call wait for
+ * the result.  A zero singular value gets a zero column of U. */
+int rla_svd_small_workspace_bytes(int64_t k, size_t* bytes);
+int rla_svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int transpose,
+                      double* U, int64_t ldu, double* S, double* V, int64_t ldv, void* work,
+                      size_t work_bytes, int* sweeps);
+
 /* RandSVD finish (randsvd.jl:174-189) from the range Q (m x k): B' = A'Q,
- * B' = Qb Rb (CholeskyQR2), Rb' = Ur S Vr' (FP64 SVD), U = Q Ur, V = Qb Vr.
+ * B' = Qb Rb (CholeskyQR2), Rb' = Ur S Vr' (rla_svd_small_f64, k <= 288),
+ * U = Q Ur, V = Qb Vr.
  * Outputs U (m x k), S (k), V (n x k), FP64. */
 int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,
                                 size_t* bytes);

@@ -20,22 +20,14 @@ def _colmajor(v):
 
 
 def gemm(C, A, B, alpha=1.0, beta=0.0):
-    """C = alpha*A*B + beta*C on strided Views (Float32 / Float64)."""
-    if C.rs != 1:
-        if C.cs == 1:
-            return gemm(C.T, B.T, A.T, alpha, beta)
-        raise ArgumentError("output must have a unit stride in one dimension")
+    """C = alpha*A*B + beta*C on strided Views (Float32 / Float64), on the
+    library's FP32 / FP64 MFMA GEMM (any strides)."""
     m, n, k = C.rows, C.cols, A.cols
     if m == 0 or n == 0:
         return
-    a, b = _colmajor(A), _colmajor(B)
-    if a is None or b is None:
-        raise ArgumentError("operands must have a unit stride in one dimension")
-    name = f"rla_gemm_{_lib.sfx(C.dtype)}"
-    ldc = C.cs if n > 1 else max(1, m, C.cs)  # a single column may carry any column stride
-    _lib.call(name, _lib.ctx(C.t.device), a[1], b[1], m, n, k, float(alpha), _lib.ptr(a[0]),
-              max(1, a[2]), _lib.ptr(b[0]), max(1, b[2]), float(beta), _lib.ptr(C.t),
-              max(1, ldc))
+    _lib.call(f"rla_gemm_strided_{_lib.sfx(C.dtype)}", _lib.ctx(C.t.device), m, n, k,
+              float(alpha), _lib.ptr(A.t), A.rs, A.cs, _lib.ptr(B.t), B.rs, B.cs, float(beta),
+              _lib.ptr(C.t), C.rs, C.cs)
 
 
 class SourceKey:

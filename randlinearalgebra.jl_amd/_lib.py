@@ -130,6 +130,14 @@ _SIGS = {
     "rla_gemm_f32": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_gemm_f64": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_gemm_f64_mfma": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
+    "rla_gemm_strided_f32": [vp, i64, i64, i64, f64, vp, i64, i64, vp, i64, i64, f64, vp, i64,
+                             i64],
+    "rla_gemm_strided_f64": [vp, i64, i64, i64, f64, vp, i64, i64, vp, i64, i64, f64, vp, i64,
+                             i64],
+    "rla_gaussian_apply_f32": [vp, i32, i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, f64, f64,
+                               vp, i64, i64],
+    "rla_gaussian_apply_f64": [vp, i32, i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, f64, f64,
+                               vp, i64, i64],
     "rla_to_bf16_f32": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_to_bf16_f64": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_gemm_bf16": [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64, ctypes.c_float, vp,
@@ -154,6 +162,8 @@ _SIGS = {
     "rla_rangefinder_workspace_bytes": [i32, i32, i64, i64, i64, vp],
     "rla_rangefinder": [vp, i32, i32, i64, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp,
                         i64, vp, ctypes.c_size_t],
+    "rla_svd_small_workspace_bytes": [i64, vp],
+    "rla_svd_small_f64": [vp, i64, vp, i64, i32, vp, i64, vp, vp, i64, vp, ctypes.c_size_t, vp],
     "rla_randsvd_workspace_bytes": [i32, i32, i64, i64, i64, vp],
     "rla_randsvd_finish": [vp, i32, i64, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, i64, vp,
                            vp, i64, vp, ctypes.c_size_t],

@@ -929,21 +929,32 @@ def pack_bits(bits):
 # Gaussian
 
 
+GAUSSIAN_PRECISIONS = ("plain", "bf16")
+
+
 class Gaussian(Compressor):
     """Gaussian(cardinality=Left(), compression_dim=2, type=Float64) —
-    src/Compressors/gaussian.jl:49-61."""
+    src/Compressors/gaussian.jl:49-61.  An extra `precision` keyword picks the
+    arithmetic of mul!: "plain" (the data's own type on the FP32 / FP64
+    matrix cores, as the reference's BLAS gemm) or "bf16" (operands rounded to
+    BF16, FP32 accumulation; Float32 data only)."""
 
-    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64):
+    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64,
+                 precision="plain"):
         cardinality = Left() if cardinality is None else cardinality
         if compression_dim <= 0:
             raise ArgumentError("Field 'compression_dim' must be positive.")
+        if precision not in GAUSSIAN_PRECISIONS:
+            raise ArgumentError(f"precision must be one of {GAUSSIAN_PRECISIONS}")
         self.cardinality = cardinality
         self.compression_dim = int(compression_dim)
         self.type = _dtype(type)
+        self.precision = precision
 
     def _complete(self, *args):
         A = args[0] if len(args) <= 2 else args[1]
-        return GaussianRecipe(self.cardinality, self.compression_dim, self.type, A)
+        return GaussianRecipe(self.cardinality, self.compression_dim, self.type, A,
+                              getattr(self, "precision", "plain"))
 
 
 class GaussianRecipe(CompressorRecipe):
@@ -951,10 +962,12 @@ class GaussianRecipe(CompressorRecipe):
     scale = type(1/sqrt(compression_dim)); Left: s x size(A, 1), Right:
     size(A, 2) x s.  `op` is the dense operator S itself (column-major)."""
 
-    def __init__(self, cardinality, compression_dim, type_, A):
+    def __init__(self, cardinality, compression_dim, type_, A, precision="plain"):
         a_rows, a_cols = _size2(A)
         self.cardinality = cardinality
         self.compression_dim = compression_dim
+        self.precision = precision
+        self._bf16_op = None  # (key of op, BF16 copy of op, its orientation)
         if cardinality == Right():
             self.n_rows, self.n_cols = a_cols, compression_dim
         else:
@@ -968,6 +981,7 @@ class GaussianRecipe(CompressorRecipe):
     def _gen(self):
         sd, ep = _draw()
         self.seed, self.epoch = sd, ep
+        self._bf16_op = None  # op is rewritten in place below
         _lib.call(f"rla_gaussian_gen_{_lib.sfx(self.type)}", _lib.ctx(self.op.device), sd, ep,
                   self.n_rows, self.n_cols, self.scale, _lib.ptr(self.op), max(1, self.n_rows))
 
@@ -977,10 +991,37 @@ class GaussianRecipe(CompressorRecipe):
     def _op_as(self, dtype):
         return self.op if dtype == self.op.dtype else self.op.to(dtype)
 
+    def _apply(self, side, C, A, alpha, beta):
+        """side 0: C = alpha*S*A + beta*C; side 1: C = alpha*A*S + beta*C
+        (gaussian.jl:145-170), on the library's MFMA kernels."""
+        if self.precision == "bf16":
+            return self._apply_bf16(side, C, A, alpha, beta)
+        S = as_view(self._op_as(A.dtype))
+        _lib.call(f"rla_gaussian_apply_{_lib.sfx(A.dtype)}", _lib.ctx(C.t.device), side, S.rows,
+                  S.cols, _lib.ptr(S.t), S.rs, S.cs, A.rows, A.cols, _lib.ptr(A.t), A.rs, A.cs,
+                  alpha, beta, _lib.ptr(C.t), C.rs, C.cs)
+
+    def _apply_bf16(self, side, C, A, alpha, beta):
+        from . import _dense
+        if C.dtype != torch.float32 or A.dtype != torch.float32:
+            raise ArgumentError("the bf16 Gaussian apply takes Float32 data")
+        S = as_view(self.op)
+        # the BF16 GEMM reads both operands K-contiguous: S with its rows
+        # contiguous for S*A, its columns for A*S; kept while op is unchanged
+        c = self._bf16_op
+        if c is None or c[2] != side or not _dense.key_matches(c[0], self.op):
+            ob = _dense.bf16_rows(S if side == 0 else S.T)
+            self._bf16_op = c = (_dense.SourceKey(self.op), ob, side)
+        ob = c[1]
+        if side == 0:   # C (s x n) = S (s x m) * A (m x n)
+            Ab = _dense.bf16_rows(A.T)
+            _dense.gemm_bf16(C.rows, C.cols, A.rows, ob, Ab, C, alpha, beta)
+        else:           # C (m x s) = A (m x n) * S (n x s)
+            Ab = _dense.bf16_rows(A)
+            _dense.gemm_bf16(C.rows, C.cols, A.cols, Ab, ob, C, alpha, beta)
+
     def _left(self, C, A, alpha, beta):
-        from ._dense import gemm
-        gemm(C, as_view(self._op_as(A.dtype)), A, alpha, beta)
+        self._apply(0, C, A, alpha, beta)
 
     def _right(self, C, A, alpha, beta):
-        from ._dense import gemm
-        gemm(C, A, as_view(self._op_as(A.dtype)), alpha, beta)
+        self._apply(1, C, A, alpha, beta)

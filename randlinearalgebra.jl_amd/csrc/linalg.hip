@@ -85,6 +85,16 @@ struct Bump {
   }
 };
 
+// dst = src * diag(d)
+__global__ void scale_cols_kernel(int64_t rows, int64_t cols, const double* __restrict__ src,
+                                  int64_t lds, const double* __restrict__ d,
+                                  double* __restrict__ dst, int64_t ldd) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int64_t i = e % rows, j = e / rows;
+  dst[i + j * ldd] = src[i + j * lds] * d[j];
+}
+
 template <class S, class D>
 __global__ void convert_kernel(int64_t rows, int64_t cols, const S* __restrict__ src, int64_t lds,
                                D* __restrict__ dst, int64_t ldd) {
@@ -599,9 +609,8 @@ int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n,
   Bump b{nullptr, 0};
   b.take<double>(k * k);  // Rb
   b.take<double>(k * k);  // Ur
-  b.take<double>(k * k);  // Vr'
-  b.take<double>(1);      // Jacobi residual
-  b.take<int>(2);         // sweeps, info
+  b.take<double>(k * k);  // Vr
+  b.take<char>(svd_small_work_bytes(k));
   *bytes = b.need + rf_layout(precision, a_elem, m, n, k, nullptr, 0, nullptr);
   return RLA_OK;
 }
@@ -614,47 +623,33 @@ int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_
   int rc = check_rf(precision, a_elem, m, n, k, A, A_cm, A_cm);
   if (rc) return rc;
   if (k > n) return fail(RLA_EARG, "RandSVD needs k <= n");
+  if (k > 288) return fail(RLA_ENOTSUP, "RandSVD: the small SVD takes k <= 288");
   rc = use_device(ctx);
   if (rc) return rc;
   Bump b{static_cast<char*>(work), work_bytes};
   double* Rb = b.take<double>(k * k);
   double* Ur = b.take<double>(k * k);
-  double* Vt = b.take<double>(k * k);
-  double* jres = b.take<double>(1);
-  int* sweeps = b.take<int>(2);
-  int* info = sweeps + 1;
+  double* Vr = b.take<double>(k * k);
+  const size_t svd_bytes = svd_small_work_bytes(k);
+  void* svd_work = b.take<char>(svd_bytes);
   RfBufs w;
-  if (!info || rf_layout(precision, a_elem, m, n, k, b.p, b.left, &w) > b.left)
+  if (!svd_work || rf_layout(precision, a_elem, m, n, k, b.p, b.left, &w) > b.left)
     return fail(RLA_EARG, "RandSVD workspace too small");
-  // B' = A'Q -> w.Z64 (n x k), then V <- B' and QR in place
+  // B' = A'Q -> w.Z64 (n x k), then V <- B' and QR in place: B' = Qb Rb
   rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Q, ldq, w);
   if (rc) return rc;
   rc = convert<double, double>(ctx, n, k, w.Z64, n, V, ldv);
   if (rc) return rc;
   rc = qr_f64(ctx, n, k, V, ldv, Rb, k, w.qr, w.qr_bytes, nullptr, nullptr);
   if (rc) return rc;
-  // SVD of Rb' (k x k): transpose Rb into Ur's slot first
-  rocblas_handle h;
-  rc = blas(ctx, &h);
+  // Rb' = Ur S Vr' (one-sided Jacobi, FP64), so B = Rb' Qb' = Ur S (Qb Vr)'
+  rc = svd_small_f64(ctx, k, Rb, k, 1, Ur, k, S, Vr, k, svd_work, svd_bytes, nullptr);
   if (rc) return rc;
-  const double one = 1.0, zero = 0.0;
-  // Ur <- Rb' via geam-free route: gemm with identity would cost k^3; use a kernel
-  RLA_BLAS(rocblas_dgeam(h, rocblas_operation_transpose, rocblas_operation_none, int(k), int(k),
-                         &one, Rb, int(k), &zero, Rb, int(k), Ur, int(k)));
-  // Ur holds Rb'; the SVD overwrites its input, so move it to Rb
-  RLA_HIP(hipMemcpyAsync(Rb, Ur, sizeof(double) * k * k, hipMemcpyDeviceToDevice, ctx->stream));
-  // Jacobi SVD (eigenvectors of Rb Rb' by parallel Jacobi rotations, then a
-  // QR): for k = 266 the whole finish takes 31 ms against 200 ms with the
-  // bidiagonal QR iteration of gesvd; tolerance = machine precision
-  RLA_BLAS(rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, int(k), int(k),
-                             Rb, int(k), 0.0, jres, 100, sweeps, S, Ur, int(k), Vt, int(k),
-                             info));
-  // U = Q Ur (m x k), V = Qb Vr = V * Vt'
-  RLA_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, int(m), int(k),
-                         int(k), &one, Q, int(ldq), Ur, int(k), &zero, U, int(ldu)));
-  // V currently holds Qb (n x k); result into w.Z64 then copy back
-  RLA_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, int(n), int(k),
-                         int(k), &one, V, int(ldv), Vt, int(k), &zero, w.Z64, int(n)));
+  // U = Q Ur (m x k); V = Qb Vr, through w.Z64 since V holds Qb
+  rc = gemm_mfma<double>(ctx, m, k, k, 1.0, Q, 1, ldq, Ur, 1, k, 0.0, U, 1, ldu);
+  if (rc) return rc;
+  rc = gemm_mfma<double>(ctx, n, k, k, 1.0, V, 1, ldv, Vr, 1, k, 0.0, w.Z64, 1, n);
+  if (rc) return rc;
   return convert<double, double>(ctx, n, k, w.Z64, n, V, ldv);
 }
 
@@ -681,13 +676,13 @@ int rla_convert_f64_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const double* 
 int rla_scale_cols_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src, int64_t lds,
                        const double* d, double* dst, int64_t ldd) {
   RLA_CHECK_CTX(ctx);
+  if (rows <= 0 || cols <= 0) return RLA_OK;
   int rc = use_device(ctx);
   if (rc) return rc;
-  rocblas_handle h;
-  rc = rla::blas(ctx, &h);
-  if (rc) return rc;
-  RLA_BLAS(rocblas_ddgmm(h, rocblas_side_right, int(rows), int(cols), src, int(lds), d, 1, dst,
-                         int(ldd)));
+  const int64_t n = rows * cols;
+  rla::scale_cols_kernel<<<dim3(unsigned(rla::ceil_div(n, 256))), 256, 0, ctx->stream>>>(
+      rows, cols, src, lds, d, dst, ldd);
+  RLA_LAUNCH_CHECK("scale_cols");
   return RLA_OK;
 }
 

@@ -102,6 +102,18 @@ inline int ilog2(int64_t n) {
   return l;
 }
 
+// gemm_mfma.hip: C = alpha A B + beta C on strided views, FP32 / FP64 MFMA
+template <class T>
+int gemm_mfma(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const T* A,
+              int64_t a_rs, int64_t a_cs, const T* B, int64_t b_rs, int64_t b_cs, double beta,
+              T* C, int64_t c_rs, int64_t c_cs);
+
+// svd_jacobi.hip: X (or X') = U diag(S) V' for k <= 288, one-sided Jacobi
+size_t svd_small_work_bytes(int64_t k);
+int svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int transpose,
+                  double* U, int64_t ldu, double* S, double* V, int64_t ldv, void* work,
+                  size_t work_bytes, int* sweeps_out);
+
 // srht_fused.hip: stages 0..13 of Float32 FWHTs of length 2^L, L >= 14
 int fwht_chunks_f32(rla_ctx* ctx, const float* src, int64_t src_cs, int64_t m_valid, float* P,
                     int64_t ldp, int64_t ncols, int L, const uint64_t* signs, double scale);

@@ -49,6 +49,106 @@ def test_gaussian_all_orientations(rla, dev, dtype, tol, card):
     assert np.max(np.abs(C3.double().cpu().numpy() - op.T @ Z)) < tol * np.max(np.abs(op.T @ Z)) * 4
 
 
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+@pytest.mark.parametrize("card", ["left", "right"])
+def test_gaussian_all_orientations_c3_size(rla, dev, dtype, tol, card):
+    # the Gaussian apply at the C3 sketch shape (8192 x 266 operator) on the
+    # MFMA GEMM, all four orientations, against FP64 numpy products
+    rla.seed(6)
+    cardinality = rla.Left() if card == "left" else rla.Right()
+    m, n, s = (8192, 300, 266) if card == "left" else (700, 8192, 266)
+    A = colmajor(rnd((m, n), 1), dev, dtype)
+    S = rla.complete_compressor(rla.Gaussian(cardinality=cardinality, compression_dim=s,
+                                             type=dtype), A)
+    op = S.op.double().cpu().numpy()
+    r, c = op.shape
+
+    def check(got, want):
+        err = np.linalg.norm(got.double().cpu().numpy() - want) / np.linalg.norm(want)
+        assert err <= tol, err
+
+    X = rnd((c, 33), 2)
+    C = torch.full((r, 33), 3.0, dtype=dtype, device=dev)
+    rla.mul(C, S, colmajor(X, dev, dtype), 1.5, 0.5)
+    check(C, 1.5 * op @ X + 1.5)
+    Y = rnd((31, r), 3)
+    C2 = torch.zeros(31, c, dtype=dtype, device=dev)
+    rla.mul(C2, colmajor(Y, dev, dtype), S)
+    check(C2, Y @ op)
+    Z = rnd((r, 5), 4)
+    C3 = torch.zeros(c, 5, dtype=dtype, device=dev)
+    rla.mul(C3, S.T, colmajor(Z, dev, dtype))
+    check(C3, op.T @ Z)
+    W = rnd((7, c), 5)
+    C4 = torch.zeros(7, r, dtype=dtype, device=dev)
+    rla.mul(C4, colmajor(W, dev, dtype), S.T)
+    check(C4, W @ op.T)
+    # a row-major A and a row-major C (Julia adjoint views) give the same result
+    Xr = torch.as_tensor(X).to(dtype).to(dev).contiguous()
+    Cr = torch.zeros(r, 33, dtype=dtype, device=dev)
+    rla.mul(Cr, S, Xr)
+    check(Cr, op @ X)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (64, 64, 16), (65, 63, 17), (300, 7, 20000),
+                                   (5, 3000, 2), (1000, 266, 8192), (129, 1, 4096)])
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-13), (F32, 2e-6)])
+@pytest.mark.parametrize("layout", ["nn", "tn", "nt", "tt"])
+def test_gemm_strided_matches_numpy(rla, dev, M, N, K, dtype, tol, layout):
+    # C = alpha A B + beta C through rla_gemm_strided_*: transposed operands
+    # are stride swaps; long K splits over workgroups (deterministic sums)
+    A = rnd((M, K), M + K)
+    B = rnd((K, N), N)
+    C0 = rnd((M, N), 7)
+    At = colmajor(A, dev, dtype) if layout[0] == "n" else colmajor(A.T, dev, dtype).T
+    Bt = colmajor(B, dev, dtype) if layout[1] == "n" else colmajor(B.T, dev, dtype).T
+    for beta in (0.0, -0.5):
+        C = colmajor(C0, dev, dtype)
+        rla._dense.gemm(rla.as_view(C), rla.as_view(At), rla.as_view(Bt), 1.25, beta)
+        want = 1.25 * A @ B + beta * C0
+        err = np.abs(C.double().cpu().numpy() - want).max() / max(1.0, np.abs(want).max())
+        assert err <= tol * max(1.0, np.sqrt(K)), err
+        # same inputs, same bits
+        C2 = colmajor(C0, dev, dtype)
+        rla._dense.gemm(rla.as_view(C2), rla.as_view(At), rla.as_view(Bt), 1.25, beta)
+        assert torch.equal(C, C2)
+
+
+def test_gaussian_bf16_apply(rla, dev):
+    # precision="bf16": operands rounded to BF16, FP32 accumulation; equals the
+    # FP64 product of the BF16-rounded operands to FP32 accumulation error
+    rla.seed(8)
+    A = colmajor(rnd((4000, 300), 1), dev, F32)
+    for card in (rla.Left(), rla.Right()):
+        S = rla.complete_compressor(rla.Gaussian(cardinality=card, compression_dim=266,
+                                                 type=F32, precision="bf16"), A)
+        opb = S.op.to(torch.bfloat16).double().cpu().numpy()
+        r, c = opb.shape
+        X = torch.as_tensor(rnd((c, 40), 2)).float()
+        C = torch.zeros(r, 40, dtype=F32, device=dev)
+        rla.mul(C, S, X.to(dev))
+        want = opb @ X.to(torch.bfloat16).double().numpy()
+        err = np.linalg.norm(C.double().cpu().numpy() - want) / np.linalg.norm(want)
+        assert err < 1e-5, err
+        Y = torch.as_tensor(rnd((25, r), 3)).float()
+        C2 = torch.zeros(25, c, dtype=F32, device=dev)
+        rla.mul(C2, Y.to(dev), S)
+        want2 = Y.to(torch.bfloat16).double().numpy() @ opb
+        err2 = np.linalg.norm(C2.double().cpu().numpy() - want2) / np.linalg.norm(want2)
+        assert err2 < 1e-5, err2
+        # and it is the bf16 product, not the plain one (BF16 rounding is visible)
+        plain = S.op.double().cpu().numpy() @ X.double().numpy()
+        assert np.linalg.norm(C.double().cpu().numpy() - plain) / np.linalg.norm(plain) > 1e-4
+        # update_compressor! regenerates op; the cached BF16 copy follows it
+        rla.update_compressor(S)
+        opb2 = S.op.to(torch.bfloat16).double().cpu().numpy()
+        rla.mul(C, S, X.to(dev))
+        want3 = opb2 @ X.to(torch.bfloat16).double().numpy()
+        assert np.linalg.norm(C.double().cpu().numpy() - want3) < 1e-5 * np.linalg.norm(want3)
+    with pytest.raises(rla._lib.ArgumentError):
+        rla.Gaussian(precision="fp8")
+
+
 @pytest.mark.parametrize("M,N,K", [(256, 266, 8192), (100, 37, 300), (8192, 266, 4096),
                                    (33, 300, 1000), (128, 64, 64), (300, 266, 1000),
                                    (1000, 288, 8200), (129, 100, 64), (5000, 33, 130),
@@ -299,6 +399,65 @@ def test_rangefinder_rank_deficient_reruns_with_householder(rla, dev, prec, orth
     assert np.linalg.norm(Q.T @ Q - np.eye(30)) < 1e-10
     tol = 1e-10 if prec == "plain" else 5e-2   # BF16 rounding of A is full rank
     assert np.linalg.norm(A - Q @ (Q.T @ A)) < tol * np.linalg.norm(A)
+
+
+def small_svd(rla, dev, X, transpose=False):
+    import ctypes
+    k = X.shape[0]
+    Xd = colmajor(X, dev, F64)
+    U = torch.zeros(k, k, dtype=F64, device=dev).T
+    V = torch.zeros(k, k, dtype=F64, device=dev).T
+    S = torch.zeros(k, dtype=F64, device=dev)
+    nb = rla._dense.query_bytes("rla_svd_small_workspace_bytes", k)
+    work = rla._dense.workspace(nb, dev)
+    sweeps = ctypes.c_int(0)
+    rla._lib.call("rla_svd_small_f64", rla._lib.ctx(dev), k, rla._lib.ptr(Xd), k,
+                  1 if transpose else 0, rla._lib.ptr(U), k, rla._lib.ptr(S), rla._lib.ptr(V), k,
+                  rla._lib.ptr(work), work.numel() * 8, ctypes.byref(sweeps))
+    return U.cpu().numpy(), S.cpu().numpy(), V.cpu().numpy(), sweeps.value
+
+
+@pytest.mark.parametrize("k,decay", [(1, 1.0), (2, 1.0), (17, 4.0), (40, 8.0), (266, 64.0),
+                                     (288, 20.0), (100, 1e9)])
+def test_small_svd_one_sided_jacobi(rla, dev, k, decay):
+    # against LAPACK (numpy) on X = P diag(s) W' with known s: singular values
+    # within 1e-12 * s_max (forming X in FP64 already moves them by ~eps s_max;
+    # relative 1e-12 where s >= 1e-4), U and V orthogonal, X = U S V'
+    rng = np.random.default_rng(k)
+    P = np.linalg.qr(rng.standard_normal((k, k)))[0]
+    W = np.linalg.qr(rng.standard_normal((k, k)))[0]
+    s = np.exp(-np.arange(k) / decay)
+    X = (P * s) @ W.T
+    for tr in (False, True):
+        U, S, V, sweeps = small_svd(rla, dev, X.T if tr else X, tr)
+        assert 0 < sweeps < 60
+        assert np.all(np.diff(S) <= 0)
+        tol = 1e-12 * max(1.0, k / 40)
+        big = s >= 1e-4
+        assert np.max(np.abs(S - s)) < tol
+        assert np.max(np.abs(S - s)[big] / s[big]) < tol
+        assert np.max(np.abs(S - np.linalg.svd(X, compute_uv=False))) < tol
+        assert np.linalg.norm(U.T @ U - np.eye(k)) < 1e-12 * k
+        assert np.linalg.norm(V.T @ V - np.eye(k)) < 1e-12 * k
+        assert np.linalg.norm((U * S) @ V.T - X) < 1e-13 * k * np.linalg.norm(X)
+
+
+def test_small_svd_triangular_and_rank_deficient(rla, dev):
+    # the RandSVD case: the transposed triangular factor; and a matrix of rank
+    # 5 (zero singular values come out as zeros, the rest exact)
+    rng = np.random.default_rng(3)
+    R = np.triu(rng.standard_normal((120, 120))) + 10 * np.eye(120)
+    U, S, V, _ = small_svd(rla, dev, R, True)
+    ref = np.linalg.svd(R, compute_uv=False)
+    assert np.max(np.abs(S - ref) / ref) < 1e-12
+    assert np.linalg.norm((U * S) @ V.T - R.T) < 1e-12 * np.linalg.norm(R)
+    X = rng.standard_normal((60, 5)) @ rng.standard_normal((5, 60))
+    U, S, V, _ = small_svd(rla, dev, X)
+    ref = np.linalg.svd(X, compute_uv=False)
+    assert np.max(np.abs(S[:5] - ref[:5]) / ref[:5]) < 1e-12
+    assert np.max(np.abs(S[5:])) < 1e-12 * ref[0]
+    assert np.linalg.norm(V.T @ V - np.eye(60)) < 1e-12 * 60
+    assert np.linalg.norm((U * S) @ V.T - X) < 1e-12 * np.linalg.norm(X)
 
 
 @pytest.mark.parametrize("prec", ["plain", "bf16"])

@@ -287,15 +287,24 @@ int rla_fwht_f64(rla_ctx* ctx, int64_t n, int64_t ncols, const uint64_t* sign_ch
  * Replaces mul! at src/Compressors/srht.jl:220-504.  scale is the recipe's
  * stored Float64 (reference: type(sqrt(s)/sqrt(n_pad)), srht.jl:152).
  * plan: from rla_srht_plan for this op (NULL: built in the workspace on each
- * call).  adjoint = 0 with unit row stride and 2^11 <= n_pad <= 2^16 (Float32)
- * or 2^15 (Float64) runs as one fused kernel; other shapes use the multi-pass
- * path.  beta == 0 does not read Y.  Bit-identical to the reference loops for
- * adjoint = 0. */
+ * call).  beta == 0 does not read Y.
+ * rla_srht_apply_f64 and rla_srht_apply_exact_f32 are bit-identical to the
+ * reference loops for adjoint = 0: with unit row stride and 2^11 <= n_pad <=
+ * 2^17 (Float32) or 2^16 (Float64) as one fused kernel, else multi-pass.
+ * rla_srht_apply_f32 is the Float32 product within rounding of the reference
+ * (1e-5 relative, the Float32 bar): adjoint = 0 with unit row stride,
+ * 2^14 <= n_pad <= 2^24 and s <= 4096 runs the pruned kernel (only the
+ * sampled rows of H D pad(X) are formed), other shapes as the exact entry. */
 int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
                        double scale, const void* plan, const float* X, int64_t x_rs,
                        int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
                        double beta);
+int rla_srht_apply_exact_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                             int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                             double scale, const void* plan, const float* X, int64_t x_rs,
+                             int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                             double beta);
 int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
                        double scale, const void* plan, const double* X, int64_t x_rs,
@@ -308,7 +317,7 @@ int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int6
  * kernel (then pass plan = NULL). */
 int rla_srht_plan_bytes(int elem_bytes, int64_t n_pad, int64_t s, size_t* bytes);
 int rla_srht_plan(rla_ctx* ctx, int elem_bytes, int64_t n_pad, int64_t s, const int32_t* op,
-                  void* plan);
+                  const uint64_t* sign_chunks, void* plan);
 
 /* ---- FJLT --------------------------------------------------------------
  * K = sprandn(type, s, n_pad, q) (src/Compressors/fjlt.jl:183, 225): each

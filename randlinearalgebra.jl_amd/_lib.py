@@ -126,7 +126,9 @@ _SIGS = {
     "rla_srht_apply_f64": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, vp, i64, i64, vp, i64,
                            i64, f64, f64],
     "rla_srht_plan_bytes": [i32, i64, i64, vp],
-    "rla_srht_plan": [vp, i32, i64, i64, vp, vp],
+    "rla_srht_plan": [vp, i32, i64, i64, vp, vp, vp],
+    "rla_srht_apply_exact_f32": [vp, i32, i64, i64, i64, i64, vp, vp, f64, vp, vp, i64, i64, vp,
+                                 i64, i64, f64, f64],
     "rla_gemm_f32": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_gemm_f64": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],
     "rla_gemm_f64_mfma": [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64],

@@ -658,9 +658,13 @@ def padded_size(n):
 
 class SRHT(Compressor):
     """SRHT(cardinality=Left(), compression_dim=2, block_size=10, type=Float64)
-    — src/Compressors/srht.jl:60-91."""
+    — src/Compressors/srht.jl:60-91.  An extra `exact` keyword (default False)
+    keeps Float32 applies in the reference's operation order, bit for bit;
+    otherwise Float32 forward applies of 2^14..2^24 rows may take the pruned
+    kernel, within Float32 rounding of the reference (1e-5 relative)."""
 
-    def __init__(self, cardinality=None, compression_dim=2, block_size=10, type=torch.float64):
+    def __init__(self, cardinality=None, compression_dim=2, block_size=10, type=torch.float64,
+                 exact=False):
         cardinality = Left() if cardinality is None else cardinality
         if compression_dim <= 0:
             raise ArgumentError("Field `compression_dim` must be positive.")
@@ -672,10 +676,13 @@ class SRHT(Compressor):
         self.compression_dim = int(compression_dim)
         self.block_size = int(block_size)
         self.type = _dtype(type)
+        self.exact = bool(exact)
 
     def _complete(self, *args):
         A = args[0] if len(args) <= 2 else args[1]
-        return SRHTRecipe(self.compression_dim, self.block_size, self.cardinality, A, self.type)
+        rec = SRHTRecipe(self.compression_dim, self.block_size, self.cardinality, A, self.type)
+        rec.exact = getattr(self, "exact", False)
+        return rec
 
 
 class SRHTRecipe(CompressorRecipe):
@@ -730,7 +737,7 @@ class SRHTRecipe(CompressorRecipe):
             if nb.value:
                 pl = torch.empty((nb.value + 3) // 4, dtype=torch.int32, device=self.op.device)
                 _lib.call("rla_srht_plan", _lib.ctx(self.op.device), elem, self.n_pad, self.s,
-                          _lib.ptr(self.op), _lib.ptr(pl))
+                          _lib.ptr(self.op), _lib.ptr(self.signs), _lib.ptr(pl))
             self._plans[dtype] = pl
         return self._plans[dtype]
 
@@ -744,7 +751,10 @@ class SRHTRecipe(CompressorRecipe):
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
         plan = self.plan(dt) if not adjoint else None
-        _lib.call(f"rla_srht_apply_{_lib.sfx(dt)}", _lib.ctx(X.t.device), int(adjoint), self.dim,
+        name = f"rla_srht_apply_{_lib.sfx(dt)}"
+        if getattr(self, "exact", False) and dt == torch.float32:
+            name = "rla_srht_apply_exact_f32"
+        _lib.call(name, _lib.ctx(X.t.device), int(adjoint), self.dim,
                   self.n_pad, self.s, X.cols, _lib.ptr(self.op), _lib.ptr(self.signs),
                   self.scale, _lib.ptr(plan), _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(Y.t), Y.rs, Y.cs,
                   alpha, beta)

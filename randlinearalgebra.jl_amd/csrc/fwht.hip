@@ -303,13 +303,13 @@ template <class T>
 int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
                        const int32_t* op, const uint64_t* signs, double scale, const void* plan,
                        const T* X, int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs,
-                       double alpha, double beta, bool* handled);
+                       double alpha, double beta, bool exact, bool* handled);
 
 template <class T>
 static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                       int64_t ncols, const int32_t* op, const uint64_t* signs, double scale,
                       const void* plan, const T* X, int64_t x_rs, int64_t x_cs, T* Y,
-                      int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+                      int64_t y_rs, int64_t y_cs, double alpha, double beta, bool exact) {
   RLA_CHECK_CTX(ctx);
   if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
   if (m < 1 || m > n_pad || s < 1 || s > n_pad || ncols < 0)
@@ -322,7 +322,7 @@ static int srht_apply(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64
   if (!adjoint) {
     bool handled = false;
     rc = srht_fused_forward<T>(ctx, m, n_pad, s, ncols, op, signs, scale, plan, X, x_rs, x_cs, Y,
-                               y_rs, y_cs, alpha, beta, &handled);
+                               y_rs, y_cs, alpha, beta, exact, &handled);
     if (rc || handled) return rc;
   }
   const int64_t cc = srht_chunk_cols<T>(n_pad, ncols);
@@ -551,7 +551,16 @@ int rla_srht_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int6
                        int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
                        double beta) {
   return srht_apply<float>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
-                           x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+                           x_rs, x_cs, Y, y_rs, y_cs, alpha, beta, false);
+}
+
+int rla_srht_apply_exact_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
+                             int64_t ncols, const int32_t* op, const uint64_t* sign_chunks,
+                             double scale, const void* plan, const float* X, int64_t x_rs,
+                             int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                             double beta) {
+  return srht_apply<float>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
+                           x_rs, x_cs, Y, y_rs, y_cs, alpha, beta, true);
 }
 
 int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
@@ -560,7 +569,7 @@ int rla_srht_apply_f64(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int6
                        int64_t x_cs, double* Y, int64_t y_rs, int64_t y_cs, double alpha,
                        double beta) {
   return srht_apply<double>(ctx, adjoint, m, n_pad, s, ncols, op, sign_chunks, scale, plan, X,
-                            x_rs, x_cs, Y, y_rs, y_cs, alpha, beta);
+                            x_rs, x_cs, Y, y_rs, y_cs, alpha, beta, true);
 }
 
 int rla_fjlt_apply_ell_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,

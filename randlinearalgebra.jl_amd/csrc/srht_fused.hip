@@ -666,6 +666,196 @@ void srht_wave_kernel(
   }
 }
 
+// ---- pruned Float32 SRHT: the tolerance path --------------------------------
+// Y = alpha * (scale * H * D * pad(X))[op, :] + beta * Y for Float32 columns of
+// n_pad = 2^L >= 2^14 entries, within the Float32 tolerance of the reference
+// (the stages run in another order and the sampled outputs are summed
+// differently, so bits may differ).  Only the s sampled rows of H D x are
+// formed: with entry e = (h, l), h = e >> 11 its 2048-entry block and
+// l = e & 2047, and a sampled row rho = (rho_h, rho_l),
+//   (H D x)[rho] = sum_h (-1)^popcount(rho_h & h) * T_h[rho_l],
+//   T_h = H_2048 D_h x_h  (the transform of block h alone).
+// A wavefront transforms one block completely in registers (11 stages):
+// registers hold entry bits e0, e1 (one float4 load) and e6..e8 (the load
+// index), lanes hold e2..e5 through the DPP lane map of srht_wave_kernel and
+// e9, e10; the signs are applied with one v_cndmask per value from 64-bit lane
+// masks (scalar loads from the plan); e2..e5 are DPP stages, and e9, e10 are
+// exchanged with the finished register bits e8, e7 by v_permlane16_swap /
+// v_permlane32_swap before their stages.  The 8 wavefronts of a workgroup
+// store blocks 8i .. 8i+7 of a column to LDS (round i), and each thread adds
+// the 8-term signed sums for its sampled rows into its accumulators.  A
+// column takes 2^(L-14) rounds; each wavefront's next block (or the first
+// block of its next column) is loaded during the current one.  The scale,
+// alpha and beta are applied in Float64 when the rows are written.
+// Per 2048 entries a wavefront issues about 416 VALU instructions for the
+// transform and one ds_write_b128 per 4 values, against ~700 in
+// srht_wave_kernel's bit-exact order.
+
+constexpr int kPrRegion = 2048 + 256;  // LDS words per block: 4 pad words per 32 entries
+
+__device__ __forceinline__ int pr_pad(int e) { return e + ((e >> 5) << 2); }
+
+// exchange lane bit 4 (ROW = 16) or 5 (ROW = 32) with register bit RB
+template <int RB, int ROW>
+__device__ __forceinline__ void pr_swap(float (&x)[32]) {
+#pragma unroll
+  for (int v = 0; v < 32; ++v) {
+    if (!(v & (1 << RB))) {
+      const uint32_t a = __float_as_uint(x[v]), b = __float_as_uint(x[v | (1 << RB)]);
+      if constexpr (ROW == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      }
+    }
+  }
+}
+
+typedef __attribute__((address_space(4))) const uint64_t* pr_mask_ptr;
+
+// mask[h * 32 + v] bit l: the entry held by register v of lane l at load
+// time in block h has a negative sign (its sign bit is 0: srht.jl's
+// `signs[i] ? scaling : -scaling`)
+__global__ void prune_masks_kernel(const uint32_t* __restrict__ signs32, int nb,
+                                   uint64_t* __restrict__ masks) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nb * 32) return;
+  const int h = idx >> 5, v = idx & 31;
+  uint64_t mk = 0;
+  for (int l = 0; l < 64; ++l) {
+    const int e = (h << 11) | (((l >> 5) & 1) << 10) | (((l >> 4) & 1) << 9) |
+                  (wl_nibble(l & 15) << 2) | ((v >> 2) << 6) | (v & 3);
+    if (!((signs32[e >> 5] >> (e & 31)) & 1u)) mk |= 1ull << l;
+  }
+  masks[idx] = mk;
+}
+
+template <int MS, int MODE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
+                       int nr, const uint64_t* __restrict__ masks_g,
+                       const int32_t* __restrict__ op, int s, double scale,
+                       float* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha,
+                       double beta) {
+  // the masks are read through the scalar cache (they are wave-uniform)
+  const pr_mask_ptr masks = (pr_mask_ptr)masks_g;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* lds = reinterpret_cast<float*>(smem_raw);
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  // this thread's sampled rows r = t + 512 j: LDS index of rho_l, and rho_h
+  // with the lane sign (-1)^popcount(e2..e5) of the DPP stages in bit 30
+  int sidx[MS], shi[MS];
+#pragma unroll
+  for (int j = 0; j < MS; ++j) {
+    const int r = t + 512 * j;
+    sidx[j] = -1;
+    shi[j] = 0;
+    if (r < s) {
+      const int rho = op[r];
+      const int lo = rho & 2047;
+      sidx[j] = pr_pad(lo);
+      shi[j] = (rho >> 11) | ((__builtin_popcount((lo >> 2) & 15) & 1) << 30);
+    }
+  }
+  const int nib = wl_nibble(lane & 15);
+  const float c0 = (nib & 1) ? -1.0f : 1.0f, c1 = (nib & 2) ? -1.0f : 1.0f;
+  const float c2 = (nib & 4) ? -1.0f : 1.0f, c3 = (nib & 8) ? -1.0f : 1.0f;
+  const int off = (((lane >> 5) & 1) << 10) | (((lane >> 4) & 1) << 9) | (nib << 2);
+  // after the transform: registers hold e0, e1, e6, e10, e9 (bits 0..4), lanes
+  // hold e2..e5, e8 (lane bit 4) and e7 (lane bit 5)
+  float* wbase = lds + w * kPrRegion +
+                 pr_pad((nib << 2) | (((lane >> 5) & 1) << 7) | (((lane >> 4) & 1) << 8));
+  float acc[MS];
+#pragma unroll
+  for (int j = 0; j < MS; ++j) acc[j] = 0.0f;
+
+  const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int64_t nitems = my_cols * nr;
+  float xa[32], xb[32];
+  auto load_item = [&](float(&x)[32], int64_t it) {
+    const int64_t col = blockIdx.x + (it / nr) * int64_t(gridDim.x);
+    const int h = 8 * static_cast<int>(it % nr) + w;
+    // opaque per item: keeps the guarded modes' clamps inside the loop
+    int offo = off;
+    asm volatile("" : "+v"(offo));
+    wl_load<MODE>(x, X + col * x_cs, int64_t(h) << 11, m, offo);
+  };
+  if (nitems > 0) load_item(xa, 0);
+
+  auto item = [&](float(&x)[32], float(&nx)[32], int64_t it) {
+    if (it + 1 < nitems) load_item(nx, it + 1);
+    const int i = static_cast<int>(it % nr);
+    const int h = 8 * i + w;
+    pr_mask_ptr mk = masks + int64_t(h) * 32;
+#pragma unroll
+    for (int v = 0; v < 32; ++v) {
+      float o;
+      asm("v_cndmask_b32_e64 %0, %1, -%1, %2" : "=v"(o) : "v"(x[v]), "s"(mk[v]));
+      x[v] = o;
+    }
+    stage<0, 32>(x);
+    stage<1, 32>(x);
+    wl_dpp_stage<0>(x, c0);
+    wl_dpp_stage<1>(x, c1);
+    wl_dpp_stage<2>(x, c2);
+    wl_dpp_stage<3>(x, c3);
+    stage<2, 32>(x);        // e6
+    stage<3, 32>(x);        // e7
+    stage<4, 32>(x);        // e8
+    pr_swap<4, 16>(x);      // lane bit 4 <-> register bit 4: e9 <-> e8
+    stage<4, 32>(x);        // e9
+    pr_swap<3, 32>(x);      // lane bit 5 <-> register bit 3: e10 <-> e7
+    stage<3, 32>(x);        // e10
+    __syncthreads();        // the previous round's readers are done
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const int eo = ((g & 1) << 6) | (((g >> 1) & 1) << 10) | (((g >> 2) & 1) << 9);
+      typedef float v4f __attribute__((ext_vector_type(4)));
+      *reinterpret_cast<v4f*>(wbase + pr_pad(eo)) =
+          v4f{x[4 * g], x[4 * g + 1], x[4 * g + 2], x[4 * g + 3]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MS; ++j) {
+      if (sidx[j] < 0) continue;
+      const float* p = lds + sidx[j];
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = p[q * kPrRegion];
+      const int hi = shi[j];
+      const float s0 = (hi & 1) ? -1.0f : 1.0f, s1 = (hi & 2) ? -1.0f : 1.0f;
+      const float s2 = (hi & 4) ? -1.0f : 1.0f;
+      const float a0 = fmaf(s0, v[1], v[0]), a1 = fmaf(s0, v[3], v[2]);
+      const float a2 = fmaf(s0, v[5], v[4]), a3 = fmaf(s0, v[7], v[6]);
+      const float b0 = fmaf(s1, a1, a0), b1 = fmaf(s1, a3, a2);
+      const float c = fmaf(s2, b1, b0);
+      const int neg = (__builtin_popcount((hi >> 3) & i) ^ (hi >> 30)) & 1;
+      acc[j] += neg ? -c : c;
+    }
+    if (i == nr - 1) {
+      const int64_t col = blockIdx.x + (it / nr) * int64_t(gridDim.x);
+#pragma unroll
+      for (int j = 0; j < MS; ++j) {
+        if (sidx[j] < 0) continue;
+        float* yp = Y + int64_t(t + 512 * j) * y_rs + col * y_cs;
+        const double val = double(acc[j]) * scale;
+        *yp = static_cast<float>(beta == 0.0 ? alpha * val : beta * double(*yp) + alpha * val);
+        acc[j] = 0.0f;
+      }
+    }
+  };
+
+  for (int64_t it = 0; it < nitems; it += 2) {
+    item(xa, xb, it);
+    if (it + 1 < nitems) item(xb, xa, it + 1);
+  }
+}
+
 // Full transform of 2^14-entry chunks (stages 0..13, all outputs) for the
 // generic FWHT: item i is chunk i % nch of column i / nch.  The same
 // wave-local stages and exchange as srht_wave_kernel; after stage 13 each
@@ -811,6 +1001,35 @@ static size_t plan_words(const FusedShape& f, int64_t s) {
   return kPlanHdr + 2 * w + cap * (1 + f.NCH);
 }
 
+// The pruned Float32 kernel: n_pad = 2^L with 14 <= L <= 24, at most 8
+// sampled rows per thread (s <= 4096).  Its plan part is the sign-mask table
+// (n_pad / 2048 blocks x 32 registers x 64-bit lane masks), after the fused
+// kernel's part, 256-byte aligned.
+static bool prune_ok(int elem, int64_t n_pad, int64_t s) {
+  const int L = ilog2(n_pad);
+  return elem == 4 && is_pow2(n_pad) && L >= 14 && L <= 24 && s >= 1 && s <= 4096;
+}
+static size_t prune_mask_words(int64_t n_pad) { return size_t(n_pad / 2048) * 32 * 2; }
+static size_t prune_mask_offset_words(int elem, int64_t n_pad, int64_t s) {
+  FusedShape f;
+  const size_t fused = fused_shape(elem, n_pad, f) ? plan_words(f, s) : 0;
+  return (fused + 63) & ~size_t(63);
+}
+static size_t total_plan_words(int elem, int64_t n_pad, int64_t s) {
+  FusedShape f;
+  const size_t fused = fused_shape(elem, n_pad, f) ? plan_words(f, s) : 0;
+  if (!prune_ok(elem, n_pad, s)) return fused;
+  return prune_mask_offset_words(elem, n_pad, s) + prune_mask_words(n_pad);
+}
+static int build_prune_masks(rla_ctx* ctx, int64_t n_pad, const uint64_t* signs,
+                             uint64_t* masks) {
+  const int nb = static_cast<int>(n_pad / 2048);
+  prune_masks_kernel<<<dim3(unsigned(ceil_div(int64_t(nb) * 32, 256))), 256, 0, ctx->stream>>>(
+      reinterpret_cast<const uint32_t*>(signs), nb, masks);
+  RLA_LAUNCH_CHECK("srht_prune_masks");
+  return RLA_OK;
+}
+
 __global__ void plan_init_kernel(uint32_t* plan, int LC, int NCH, int64_t s, int64_t n_pad,
                                  int elem, int64_t cap) {
   const int64_t w = int64_t(1) << (LC - 5);
@@ -879,7 +1098,13 @@ __global__ void plan_rowmap_kernel(const uint32_t* __restrict__ bitmap,
 }
 
 static int build_plan(rla_ctx* ctx, int elem, int64_t n_pad, int64_t s, const int32_t* op,
-                      uint32_t* plan) {
+                      const uint64_t* signs, uint32_t* plan) {
+  if (signs && prune_ok(elem, n_pad, s)) {
+    int rc = build_prune_masks(
+        ctx, n_pad, signs,
+        reinterpret_cast<uint64_t*>(plan + prune_mask_offset_words(elem, n_pad, s)));
+    if (rc) return rc;
+  }
   FusedShape f;
   if (!fused_shape(elem, n_pad, f)) return RLA_OK;
   const int w = 1 << (f.LC - 5);
@@ -1018,14 +1243,90 @@ static int launch_wave_any(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m
                              beta);
 }
 
+template <int MS, int MODE>
+static int launch_prune_ms(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t n_pad,
+                           int64_t ncols, const uint64_t* masks, const int32_t* op, int64_t s,
+                           double scale, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                           double beta) {
+  const size_t lds = size_t(8) * kPrRegion * sizeof(float);
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&srht_prune_kernel<MS, MODE>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  RLA_HIP(attr_err);
+  const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / lds));
+  const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
+  const int nr = static_cast<int>(n_pad >> 14);
+  srht_prune_kernel<MS, MODE><<<dim3(static_cast<unsigned>(grid)), 512, lds, ctx->stream>>>(
+      X, x_cs, m, ncols, nr, masks, op, static_cast<int>(s),
+      scale, Y, y_rs, y_cs, alpha, beta);
+  RLA_LAUNCH_CHECK("srht_prune");
+  return RLA_OK;
+}
+
+template <int MODE>
+static int launch_prune_mode(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m,
+                             int64_t n_pad, int64_t ncols, const uint64_t* masks,
+                             const int32_t* op, int64_t s, double scale, float* Y, int64_t y_rs,
+                             int64_t y_cs, double alpha, double beta) {
+  if (s <= 512)
+    return launch_prune_ms<1, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                    y_cs, alpha, beta);
+  if (s <= 1024)
+    return launch_prune_ms<2, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                    y_cs, alpha, beta);
+  if (s <= 2048)
+    return launch_prune_ms<4, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                    y_cs, alpha, beta);
+  return launch_prune_ms<8, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                  y_cs, alpha, beta);
+}
+
+static int launch_prune(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t n_pad,
+                        int64_t ncols, const uint64_t* masks, const int32_t* op, int64_t s,
+                        double scale, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                        double beta) {
+  const bool aligned = (x_cs % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+  if (aligned && m == n_pad)
+    return launch_prune_mode<0>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                y_cs, alpha, beta);
+  if (aligned && m % 4 == 0)
+    return launch_prune_mode<1>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+                                y_cs, alpha, beta);
+  return launch_prune_mode<2>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs, y_cs,
+                              alpha, beta);
+}
+
 template <class T>
 int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
                        const int32_t* op, const uint64_t* signs, double scale, const void* plan,
                        const T* X, int64_t x_rs, int64_t x_cs, T* Y, int64_t y_rs, int64_t y_cs,
-                       double alpha, double beta, bool* handled) {
+                       double alpha, double beta, bool exact, bool* handled) {
   *handled = false;
+  if (x_rs != 1) return RLA_OK;
+  if constexpr (sizeof(T) == 4) {
+    if (!exact && prune_ok(4, n_pad, s)) {
+      const uint64_t* masks = nullptr;
+      if (plan) {
+        masks = reinterpret_cast<const uint64_t*>(static_cast<const uint32_t*>(plan) +
+                                                  prune_mask_offset_words(4, n_pad, s));
+      } else {
+        void* ws = nullptr;
+        int rc = workspace(ctx, prune_mask_words(n_pad) * 4, &ws);
+        if (rc) return rc;
+        rc = build_prune_masks(ctx, n_pad, signs, static_cast<uint64_t*>(ws));
+        if (rc) return rc;
+        masks = static_cast<const uint64_t*>(ws);
+      }
+      *handled = true;
+      return launch_prune(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs, y_cs,
+                          alpha, beta);
+    }
+  }
   FusedShape f;
-  if (x_rs != 1 || !fused_shape(sizeof(T), n_pad, f)) return RLA_OK;
+  if (!fused_shape(sizeof(T), n_pad, f)) return RLA_OK;
   if (fused_lds_bytes(f, sizeof(T), s) > 160 * 1024) return RLA_OK;
   if (s > (int64_t(1) << 30)) return RLA_OK;
   const bool lds_scr = scratch_in_lds(f, sizeof(T), s);
@@ -1041,7 +1342,7 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
   }
   const uint32_t* pl = static_cast<const uint32_t*>(plan);
   if (!pl) {
-    int rc = build_plan(ctx, sizeof(T), n_pad, s, op, static_cast<uint32_t*>(ws));
+    int rc = build_plan(ctx, sizeof(T), n_pad, s, op, nullptr, static_cast<uint32_t*>(ws));
     if (rc) return rc;
     pl = static_cast<const uint32_t*>(ws);
   }
@@ -1091,11 +1392,11 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
 template int srht_fused_forward<float>(rla_ctx*, int64_t, int64_t, int64_t, int64_t,
                                        const int32_t*, const uint64_t*, double, const void*,
                                        const float*, int64_t, int64_t, float*, int64_t, int64_t,
-                                       double, double, bool*);
+                                       double, double, bool, bool*);
 template int srht_fused_forward<double>(rla_ctx*, int64_t, int64_t, int64_t, int64_t,
                                         const int32_t*, const uint64_t*, double, const void*,
                                         const double*, int64_t, int64_t, double*, int64_t,
-                                        int64_t, double, double, bool*);
+                                        int64_t, double, double, bool, bool*);
 
 }  // namespace rla
 
@@ -1106,22 +1407,20 @@ extern "C" {
 int rla_srht_plan_bytes(int elem_bytes, int64_t n_pad, int64_t s, size_t* bytes) {
   if (!bytes) return fail(RLA_EARG, "rla_srht_plan_bytes: bytes is NULL");
   if (elem_bytes != 4 && elem_bytes != 8) return fail(RLA_EARG, "elem_bytes must be 4 or 8");
-  FusedShape f;
-  *bytes = fused_shape(elem_bytes, n_pad, f) ? plan_words(f, s) * 4 : 0;
+  *bytes = is_pow2(n_pad) ? total_plan_words(elem_bytes, n_pad, s) * 4 : 0;
   return RLA_OK;
 }
 
 int rla_srht_plan(rla_ctx* ctx, int elem_bytes, int64_t n_pad, int64_t s, const int32_t* op,
-                  void* plan) {
+                  const uint64_t* sign_chunks, void* plan) {
   RLA_CHECK_CTX(ctx);
   if (elem_bytes != 4 && elem_bytes != 8) return fail(RLA_EARG, "elem_bytes must be 4 or 8");
   if (!is_pow2(n_pad) || s < 1 || s > n_pad) return fail(RLA_EARG, "bad SRHT plan shape");
-  FusedShape f;
-  if (!fused_shape(elem_bytes, n_pad, f)) return RLA_OK;  // nothing to plan
-  if (!plan || !op) return fail(RLA_EARG, "rla_srht_plan: NULL pointer");
+  if (total_plan_words(elem_bytes, n_pad, s) == 0) return RLA_OK;  // nothing to plan
+  if (!plan || !op || !sign_chunks) return fail(RLA_EARG, "rla_srht_plan: NULL pointer");
   int rc = use_device(ctx);
   if (rc) return rc;
-  return build_plan(ctx, elem_bytes, n_pad, s, op, static_cast<uint32_t*>(plan));
+  return build_plan(ctx, elem_bytes, n_pad, s, op, sign_chunks, static_cast<uint32_t*>(plan));
 }
 
 }  // extern "C"

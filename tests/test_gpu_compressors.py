@@ -109,7 +109,7 @@ def test_srht_left_bit_exact(rla, dev, dtype, m, n, s):
     rla.seed(m + n)
     A = rnd((m, n), m, NP[dtype])
     Ad = colmajor(A, dev)
-    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=dtype), Ad)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=dtype, exact=True), Ad)
     assert S.n_pad == R.padded_size(m) and S.size() == (s, m)
     C0 = rnd((s, n), s, NP[dtype])
     Cd = colmajor(C0, dev)
@@ -130,11 +130,69 @@ def test_srht_left_strided_columns_bit_exact(rla, dev, m, lda):
     buf[:, :m] = torch.as_tensor(A.T)
     Ad = buf.T[:m]
     assert Ad.stride() == (1, lda)
-    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=F32), Ad)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=F32, exact=True), Ad)
     Cd = torch.zeros((n, s), dtype=F32, device=dev).T
     rla.mul(Cd, S, Ad)
     want = R.srht_left_mul(A, S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float32)
     assert np.array_equal(Cd.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("m,n,s,lda", [(16384, 3, 64, 0), (65536, 512, 2048, 0),
+                                       (40000, 7, 300, 0), (70001, 5, 4096, 0),
+                                       (262144, 2, 1000, 0), (65536, 3, 1024, 65537),
+                                       (30000, 4, 512, 30001), (16385, 2, 3000, 0),
+                                       (131072, 9, 513, 0), (65536, 1, 1, 0)])
+def test_srht_left_pruned_within_fp32_tolerance(rla, dev, m, n, s, lda):
+    # the default Float32 path (the pruned kernel for 2^14 <= n_pad, s <= 4096):
+    # within 1e-5 of the reference's Float32 order (the oracle restatement)
+    # and of the FP64 value, with padded rows, strided columns, alpha and beta;
+    # the exact entry point still gives the reference order bit for bit
+    rla.seed(m + s)
+    A = rnd((m, n), m, np.float32)
+    if lda:
+        buf = torch.zeros((n, lda), dtype=F32, device=dev)
+        buf[:, :m] = torch.as_tensor(A.T)
+        Ad = buf.T[:m]
+    else:
+        Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=F32), Ad)
+    op, sg = S.op.cpu().numpy(), S.sign_vector()
+    C0 = rnd((s, n), s, np.float32)
+    Cd = colmajor(C0, dev)
+    rla.mul(Cd, S, Ad, 1.5, 0.25)
+    got = Cd.cpu().numpy().astype(np.float64)
+    ref32 = R.srht_left_mul(A, op, sg, S.scale, np.float32, C0, 1.5, 0.25)
+    ref64 = R.srht_left_mul(A.astype(np.float64), op, sg, S.scale, np.float64,
+                            C0.astype(np.float64), 1.5, 0.25)
+    for ref in (ref32.astype(np.float64), ref64):
+        assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+        assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    S.exact = True
+    Cd2 = colmajor(C0, dev)
+    rla.mul(Cd2, S, Ad, 1.5, 0.25)
+    assert np.array_equal(Cd2.cpu().numpy(), ref32)
+
+
+def test_srht_pruned_without_plan_and_after_update(rla, dev):
+    # plan = NULL builds the sign masks in the workspace on the call; a recipe
+    # update rebuilds its plan (new signs and rows)
+    rla.seed(21)
+    m, n, s = 32768, 4, 700
+    A = rnd((m, n), 3, np.float32)
+    Ad = colmajor(A, dev)
+    S = rla.complete_compressor(rla.SRHT(compression_dim=s, type=F32), Ad)
+    for _ in range(2):
+        C = torch.zeros(n, s, dtype=F32, device=dev).T
+        rla._lib.call("rla_srht_apply_f32", rla._lib.ctx(dev), 0, m, S.n_pad, s, n,
+                      rla._lib.ptr(S.op), rla._lib.ptr(S.signs), S.scale, None, rla._lib.ptr(Ad),
+                      1, m, rla._lib.ptr(C), 1, s, 1.0, 0.0)
+        C2 = torch.zeros(n, s, dtype=F32, device=dev).T
+        rla.mul(C2, S, Ad)
+        ref = R.srht_left_mul(A.astype(np.float64), S.op.cpu().numpy(), S.sign_vector(), S.scale,
+                              np.float64)
+        for c in (C, C2):
+            assert np.linalg.norm(c.cpu().numpy() - ref) <= 1e-5 * np.linalg.norm(ref)
+        rla.update_compressor(S)
 
 
 def _srht_dense(S, m):
@@ -185,7 +243,7 @@ def test_srht_right_orientation_bit_exact(rla, dev):
     A = rnd((6, 1000), 9, np.float32)
     Ad = colmajor(A, dev)
     S = rla.complete_compressor(rla.SRHT(cardinality=rla.Right(), compression_dim=50,
-                                         type=F32), Ad)
+                                         type=F32, exact=True), Ad)
     C = torch.zeros(6, 50, dtype=F32, device=dev)
     rla.mul(C, Ad, S)
     want = R.srht_left_mul(A.T, S.op.cpu().numpy(), S.sign_vector(), S.scale, np.float32).T

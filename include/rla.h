@@ -437,6 +437,20 @@ int rla_gaussian_apply_f64(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_col
                            int64_t a_cols, const double* A, int64_t a_rs, int64_t a_cs,
                            double alpha, double beta, double* C, int64_t c_rs, int64_t c_cs);
 
+/* Allocation-free rla_gaussian_apply_f32: no counterpart in the reference.  It
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bit for bit) as rla_gaussian_apply_f32, but
+ * the split partial sums go to a file-scope device array of 32 MiB per device
+ * (held until the process ends) instead of the context workspace; the split
+ * rule keeps every shape within it (a larger need would fail with
+ * RLA_ENOTSUP).  Calls that overlap on one device share that array and race. */
+int rla_gaussian_apply_noalloc_f32(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols,
+                                   const float* S, int64_t s_rs, int64_t s_cs, int64_t a_rows,
+                                   int64_t a_cols, const float* A, int64_t a_rs, int64_t a_cs,
+                                   double alpha, double beta, float* C, int64_t c_rs,
+                                   int64_t c_cs);
+
 /* BF16 copies: dst[l + i*ldd] = bf16(src[i, l]) (round to nearest even) for
  * the rows x cols strided view src (rows become contiguous K-runs). */
 int rla_to_bf16_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t s_rs,

@@ -140,6 +140,8 @@ _SIGS = {
                                vp, i64, i64],
     "rla_gaussian_apply_f64": [vp, i32, i64, i64, vp, i64, i64, i64, i64, vp, i64, i64, f64, f64,
                                vp, i64, i64],
+    "rla_gaussian_apply_noalloc_f32": [vp, i32, i64, i64, vp, i64, i64, i64, i64, vp, i64, i64,
+                                       f64, f64, vp, i64, i64],
     "rla_to_bf16_f32": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_to_bf16_f64": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_gemm_bf16": [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64, ctypes.c_float, vp,

@@ -17,6 +17,7 @@
 // long K) split K over workgroups; the partial tiles are summed in split
 // order by a second kernel, so results do not depend on timing.
 #include <algorithm>
+#include <mutex>
 
 #include "rla_internal.hpp"
 
@@ -209,10 +210,13 @@ __global__ void scale_kernel_f64(int64_t M, int64_t N, double* C, int64_t c_rs, 
 
 }  // namespace
 
+// Partial sums of split products go to `scratch` (scratch_elems elements)
+// when it is given, else to the context workspace.
 template <class T>
-int gemm_mfma(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const T* A,
-              int64_t a_rs, int64_t a_cs, const T* B, int64_t b_rs, int64_t b_cs, double beta,
-              T* C, int64_t c_rs, int64_t c_cs) {
+static int gemm_mfma_impl(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha,
+                          const T* A, int64_t a_rs, int64_t a_cs, const T* B, int64_t b_rs,
+                          int64_t b_cs, double beta, T* C, int64_t c_rs, int64_t c_cs,
+                          T* scratch, size_t scratch_elems) {
   if (M < 0 || N < 0 || K < 0) return fail(RLA_EARG, "gemm: negative size");
   if (M == 0 || N == 0) return RLA_OK;
   if (!C || (K > 0 && (!A || !B))) return fail(RLA_EARG, "gemm: NULL operand");
@@ -250,10 +254,19 @@ int gemm_mfma(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const
   g.kps = ceil_div(ceil_div(K, g.ksplit), kGK) * kGK;
   g.ksplit = static_cast<int>(ceil_div(K, g.kps));  // no empty splits
   if (g.ksplit > 1) {
-    void* ws = nullptr;
-    rc = workspace(ctx, size_t(g.ksplit) * M * N * sizeof(T), &ws);
-    if (rc) return rc;
-    g.part = ws;
+    const size_t need = size_t(g.ksplit) * M * N;
+    if (scratch) {
+      if (need > scratch_elems)
+        return fail(RLA_ENOTSUP, "gemm: the split partial sums (" + std::to_string(need * sizeof(T)) +
+                                     " bytes) exceed the allocation-free scratch (" +
+                                     std::to_string(scratch_elems * sizeof(T)) + " bytes)");
+      g.part = scratch;
+    } else {
+      void* ws = nullptr;
+      rc = workspace(ctx, need * sizeof(T), &ws);
+      if (rc) return rc;
+      g.part = ws;
+    }
   }
   const int64_t blocks = tiles * g.ksplit;
   if (blocks > INT32_MAX) return fail(RLA_ENOTSUP, "gemm: too many tiles");
@@ -264,6 +277,14 @@ int gemm_mfma(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const
     RLA_LAUNCH_CHECK("gemm_mfma_reduce");
   }
   return RLA_OK;
+}
+
+template <class T>
+int gemm_mfma(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, double alpha, const T* A,
+              int64_t a_rs, int64_t a_cs, const T* B, int64_t b_rs, int64_t b_cs, double beta,
+              T* C, int64_t c_rs, int64_t c_cs) {
+  return gemm_mfma_impl<T>(ctx, M, N, K, alpha, A, a_rs, a_cs, B, b_rs, b_cs, beta, C, c_rs, c_cs,
+                           nullptr, 0);
 }
 
 template int gemm_mfma<float>(rla_ctx*, int64_t, int64_t, int64_t, double, const float*, int64_t,
@@ -279,7 +300,8 @@ template <class T>
 static int gaussian_apply(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols, const T* S,
                           int64_t s_rs, int64_t s_cs, int64_t a_rows, int64_t a_cols, const T* A,
                           int64_t a_rs, int64_t a_cs, double alpha, double beta, T* C,
-                          int64_t c_rs, int64_t c_cs) {
+                          int64_t c_rs, int64_t c_cs, T* scratch = nullptr,
+                          size_t scratch_elems = 0) {
   RLA_CHECK_CTX(ctx);
   if (side != 0 && side != 1) return fail(RLA_EARG, "rla_gaussian_apply: side is 0 or 1");
   if (side == 0) {
@@ -287,15 +309,43 @@ static int gaussian_apply(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols
       return fail(RLA_EDIM, "Number of columns of S, " + std::to_string(s_cols) +
                                 ", does not match number of rows of A, " +
                                 std::to_string(a_rows) + ".");
-    return gemm_mfma<T>(ctx, s_rows, a_cols, s_cols, alpha, S, s_rs, s_cs, A, a_rs, a_cs, beta,
-                        C, c_rs, c_cs);
+    return gemm_mfma_impl<T>(ctx, s_rows, a_cols, s_cols, alpha, S, s_rs, s_cs, A, a_rs, a_cs,
+                             beta, C, c_rs, c_cs, scratch, scratch_elems);
   }
   if (a_cols != s_rows)
     return fail(RLA_EDIM, "Number of columns of A, " + std::to_string(a_cols) +
                               ", does not match number of rows of S, " +
                               std::to_string(s_rows) + ".");
-  return gemm_mfma<T>(ctx, a_rows, s_cols, a_cols, alpha, A, a_rs, a_cs, S, s_rs, s_cs, beta, C,
-                      c_rs, c_cs);
+  return gemm_mfma_impl<T>(ctx, a_rows, s_cols, a_cols, alpha, A, a_rs, a_cs, S, s_rs, s_cs, beta,
+                           C, c_rs, c_cs, scratch, scratch_elems);
+}
+
+// ---- allocation-free Gaussian apply ------------------------------------------
+// Split partial sums in a file-scope device array instead of the context
+// workspace.  choose_ksplit stops at fewer than 2048 (tile, split) pairs of
+// 64 x 64, so partials stay below 2048 x 4096 = 8 Mi elements and every shape
+// fits; a larger need would be rejected.  Host code uses the array through
+// its device address, looked up once per device.  Calls that overlap on one
+// device share the array and race.
+constexpr size_t kNoallocScratch = size_t(8) << 20;  // elements (32 MiB of Float32)
+__device__ __align__(256) float scratch[kNoallocScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[64];
+}  // namespace
+
+// the calling thread's current device must be `device`
+static int scratch_address(int device, float** out) {
+  if (device < 0 || device >= 64) return fail(RLA_EARG, "scratch: device out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    RLA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return RLA_OK;
 }
 
 }  // namespace rla
@@ -326,6 +376,21 @@ int rla_gaussian_apply_f32(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_col
                            double alpha, double beta, float* C, int64_t c_rs, int64_t c_cs) {
   return gaussian_apply<float>(ctx, side, s_rows, s_cols, S, s_rs, s_cs, a_rows, a_cols, A, a_rs,
                                a_cs, alpha, beta, C, c_rs, c_cs);
+}
+
+int rla_gaussian_apply_noalloc_f32(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols,
+                                   const float* S, int64_t s_rs, int64_t s_cs, int64_t a_rows,
+                                   int64_t a_cols, const float* A, int64_t a_rs, int64_t a_cs,
+                                   double alpha, double beta, float* C, int64_t c_rs,
+                                   int64_t c_cs) {
+  RLA_CHECK_CTX(ctx);
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  float* scr = nullptr;
+  rc = scratch_address(ctx->device, &scr);
+  if (rc) return rc;
+  return gaussian_apply<float>(ctx, side, s_rows, s_cols, S, s_rs, s_cs, a_rows, a_cols, A, a_rs,
+                               a_cs, alpha, beta, C, c_rs, c_cs, scr, kNoallocScratch);
 }
 
 int rla_gaussian_apply_f64(rla_ctx* ctx, int side, int64_t s_rows, int64_t s_cols,

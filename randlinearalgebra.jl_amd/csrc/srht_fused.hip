@@ -695,22 +695,17 @@ constexpr int kPrRegion = 2048 + 256;  // LDS words per block: 4 pad words per 3
 
 __device__ __forceinline__ int pr_pad(int e) { return e + ((e >> 5) << 2); }
 
-// exchange lane bit 4 (ROW = 16) or 5 (ROW = 32) with register bit RB
+// exchange lane bit 4 (ROW = 16) or 5 (ROW = 32) with register bit RB; the
+// swap rewrites both registers in place
 template <int RB, int ROW>
 __device__ __forceinline__ void pr_swap(float (&x)[32]) {
 #pragma unroll
   for (int v = 0; v < 32; ++v) {
     if (!(v & (1 << RB))) {
-      const uint32_t a = __float_as_uint(x[v]), b = __float_as_uint(x[v | (1 << RB)]);
-      if constexpr (ROW == 16) {
-        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
-        x[v] = __uint_as_float(r[0]);
-        x[v | (1 << RB)] = __uint_as_float(r[1]);
-      } else {
-        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
-        x[v] = __uint_as_float(r[0]);
-        x[v | (1 << RB)] = __uint_as_float(r[1]);
-      }
+      if constexpr (ROW == 16)
+        asm volatile("v_permlane16_swap_b32 %0, %1" : "+v"(x[v]), "+v"(x[v | (1 << RB)]));
+      else
+        asm volatile("v_permlane32_swap_b32 %0, %1" : "+v"(x[v]), "+v"(x[v | (1 << RB)]));
     }
   }
 }
@@ -777,19 +772,23 @@ void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int
   const int64_t my_cols = blockIdx.x < ncols ? (ncols - 1 - blockIdx.x) / gridDim.x + 1 : 0;
   const int64_t nitems = my_cols * nr;
   float xa[32], xb[32];
-  auto load_item = [&](float(&x)[32], int64_t it) {
-    const int64_t col = blockIdx.x + (it / nr) * int64_t(gridDim.x);
-    const int h = 8 * static_cast<int>(it % nr) + w;
+  // item it = (column it / nr, round it % nr), tracked by counters
+  auto load_item = [&](float(&x)[32], int64_t col, int i) {
+    const int h = 8 * i + w;
     // opaque per item: keeps the guarded modes' clamps inside the loop
     int offo = off;
     asm volatile("" : "+v"(offo));
     wl_load<MODE>(x, X + col * x_cs, int64_t(h) << 11, m, offo);
   };
-  if (nitems > 0) load_item(xa, 0);
+  if (nitems > 0) load_item(xa, blockIdx.x, 0);
+  int64_t col = blockIdx.x;
+  int i = 0;
 
   auto item = [&](float(&x)[32], float(&nx)[32], int64_t it) {
-    if (it + 1 < nitems) load_item(nx, it + 1);
-    const int i = static_cast<int>(it % nr);
+    if (it + 1 < nitems) {
+      if (i + 1 < nr) load_item(nx, col, i + 1);
+      else load_item(nx, col + gridDim.x, 0);
+    }
     const int h = 8 * i + w;
     pr_mask_ptr mk = masks + int64_t(h) * 32;
 #pragma unroll
@@ -838,7 +837,6 @@ void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int
       acc[j] += neg ? -c : c;
     }
     if (i == nr - 1) {
-      const int64_t col = blockIdx.x + (it / nr) * int64_t(gridDim.x);
 #pragma unroll
       for (int j = 0; j < MS; ++j) {
         if (sidx[j] < 0) continue;
@@ -847,6 +845,10 @@ void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int
         *yp = static_cast<float>(beta == 0.0 ? alpha * val : beta * double(*yp) + alpha * val);
         acc[j] = 0.0f;
       }
+      i = 0;
+      col += gridDim.x;
+    } else {
+      ++i;
     }
   };
 

@@ -76,6 +76,43 @@ def test_results_same_alone_and_in_the_mix(rla, dev, system):
             assert a[1].converged and b[1].converged and a[1].iteration == b[1].iteration
 
 
+def test_same_alone_and_in_the_mix_at_c5_sizes(rla, dev):
+    # the served stream's own sizes: the 100000 x 1000 Kaczmarz system (64-row
+    # L2Norm blocks, rtol 1e-6, FP64 residual every 100) beside a 16384 x 4096
+    # RangeFinder (k = 266, q = 2), both precisions
+    g = torch.Generator(device=dev).manual_seed(5)
+    Arf = lowrank(16384, 4096, 1024, 64.0, g, dev)
+    Akz = torch.randn(1000, 100000, generator=g, device=dev, dtype=F32).T
+    bs = [Akz @ torch.randn(1000, generator=g, device=dev, dtype=F32) for _ in range(2)]
+
+    def reqs():
+        out = []
+        for i in range(2):
+            out.append(rla.RangeFinderRequest("rf", k=266, power_its=2,
+                                              precision="bf16" if i else "plain", seed=10 + i))
+            out.append(rla.KaczmarzRequest("kz", bs[i], block=64, rtol=1e-6, every=100,
+                                           seed=20 + i))
+        return out
+
+    alone = run_alone(rla, Arf, Akz, reqs())
+    with rla.Server(slots=4) as srv:
+        srv.register("rf", Arf, "rangefinder")
+        srv.register("kz", Akz, "kaczmarz")
+        futs = [srv.submit(r) for r in reqs()]
+        mixed = [f.result() for f in futs]
+        assert srv.max_in_flight >= 2
+    for r, a, b in zip(reqs(), alone, mixed):
+        if r.kind == "rangefinder":
+            Pa, Pb = a.double() @ a.double().T, b.double() @ b.double().T
+            assert float(torch.linalg.matrix_norm(Pa - Pb)) <= 1e-5
+        else:
+            xa, xb = a[0].double(), b[0].double()
+            assert float(torch.linalg.vector_norm(xa - xb)) <= 1e-6 * float(
+                torch.linalg.vector_norm(xa))
+            assert a[1].converged and b[1].converged and a[1].iteration == b[1].iteration
+            assert a[1].error <= 1e-6 * float(torch.linalg.vector_norm(r.b.double()))
+
+
 def test_admission_by_hbm_budget(rla, dev, system):
     A1, A2, bs = system
     resident = A2.numel() * 4 * 2 + A2.shape[0] * 8

@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SPARSE_C1_ARGS = []  # the C1 recipe and one A, for its CPU baseline
 M, N, S_DIM = 65536, 512, 2048
 COPIES = 4
 
@@ -135,6 +136,7 @@ def gpu_sparse_sign(rla, torch, steps, warmup):
     except RuntimeError as exc:  # capture unavailable: report the per-call figure only
         print(f"sparse sign graph capture failed: {exc}", file=sys.stderr)
     ms = ms_graph if ms_graph is not None and graph_exact else ms_eager
+    SPARSE_C1_ARGS[:] = [S, As[0]]
     byts = (m * n + s * n) * 8
     return {"workload": "SparseSign left apply 8192x256 f64 -> 1024x256, nnz 8",
             "ms_per_apply": ms, "GB_s": byts / ms / 1e6,
@@ -467,6 +469,159 @@ def serving_c5(rla, torch, qps=80.0, n_req=96):
     return out
 
 
+def _host_threads():
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+
+def _timed_loop(fn, seconds):
+    """Runs fn until `seconds` have passed (at least once); mean seconds per call."""
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return dt / reps, reps
+
+
+def cpu_c1(S, A, seconds=4.0):
+    """C1 on the host cores: the oracle's C restatement of SparseArrays' CSC
+    scatter loop behind sparse_sign.jl:347 (OpenMP over columns), the full
+    8192x256 Float64 apply with the same recipe."""
+    import ctypes
+
+    import numpy as np
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    vp, i64, f64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    lib.oracle_sparse_left_f64.argtypes = [vp, vp, vp, i64, i64, vp, i64, i64, vp, i64, f64, f64,
+                                           i32]
+    op = S.op
+    colptr = op.colptr.cpu().numpy().astype(np.int32)
+    rowidx = op.rowidx.cpu().numpy().astype(np.int32)
+    val = op.val.double().cpu().numpy()
+    Ah = np.asfortranarray(A.cpu().numpy())
+    m, n = Ah.shape
+    C = np.zeros((op.nrows, n), order="F")
+    threads = _host_threads()
+    per, reps = _timed_loop(lambda: lib.oracle_sparse_left_f64(
+        colptr.ctypes.data, rowidx.ctypes.data, val.ctypes.data, op.nrows, m, Ah.ctypes.data, n,
+        m, C.ctypes.data, op.nrows, 1.0, 0.0, threads), seconds)
+    byts = (m * n + op.nrows * n) * 8
+    return {"value": byts / per / 1e9, "unit": "GB/s", "ms_per_apply": per * 1e3,
+            "cores": threads, "kind": "port",
+            "sample": f"full workload (8192x256 f64 SparseSign apply), {reps} reps, C restatement "
+                      "of the SparseArrays CSC scatter loop, OpenMP over columns"}
+
+
+def cpu_c3(seconds_hint=None):
+    """C3 on the host cores: rand_ortho_it (power_its.jl:53-80) restated with
+    numpy / its threaded OpenBLAS, FP32 and FP64, one solve each on the
+    32768x8192 sigma_i = exp(-i/64) matrix (a host draw of the same
+    construction), k = 266, q = 2."""
+    import numpy as np
+    from oracle import restate as R
+    m, n, k, r = 32768, 8192, 266, 2560
+    rng = np.random.default_rng(1234)
+    U = np.linalg.qr(rng.standard_normal((m, r)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    A64 = np.asfortranarray((U * np.exp(-np.arange(r) / 64.0)) @ V.T)
+    del U, V
+    Om64 = rng.standard_normal((n, k)) / np.sqrt(k)
+    out = {"cores": _host_threads(), "kind": "port",
+           "sample": "one solve per precision, numpy restatement of rand_ortho_it with the "
+                     "host BLAS (all threads), Householder QR"}
+    for label, A, Om in (("f64", A64, Om64),
+                         ("f32", np.asfortranarray(A64.astype(np.float32)),
+                          Om64.astype(np.float32))):
+        t0 = time.perf_counter()
+        Q = R.rand_ortho_it(A, Om, 2)
+        dt = time.perf_counter() - t0
+        Qd = Q.astype(np.float64)
+        err = float(np.linalg.norm(A64 - Qd @ (Qd.T @ A64)) / np.linalg.norm(A64))
+        out[label] = {"seconds_per_solve": dt, "solves_per_s": 1.0 / dt, "rel_error": err}
+        del Q, Qd
+    out["value"] = out["f32"]["solves_per_s"]
+    out["unit"] = "solves/s (f32)"
+    return out
+
+
+def cpu_c4(n_sample=64):
+    """C4 on the host cores: per problem the CountSketch mul! (scipy.sparse,
+    like SparseArrays) and the QRSolver ldiv! (qr.jl:36-43: Householder QR of
+    the 4n x n sketch, FP32), over a sample of problems drawn like the 512."""
+    import numpy as np
+    import scipy.linalg
+    import scipy.sparse
+    rng = np.random.default_rng(1234)
+    ms = np.floor(np.exp(rng.uniform(np.log(2000), np.log(50000), 512))).astype(int)[:n_sample]
+    ns = rng.integers(32, 257, 512)[:n_sample]
+    probs = []
+    for m, n in zip(ms, ns):
+        A = (rng.standard_normal((m, n)) * 10.0 ** rng.uniform(-3, 0, n)).astype(np.float32)
+        xs = rng.standard_normal(n).astype(np.float32)
+        b = A @ xs + np.float32(1e-6) * rng.standard_normal(m).astype(np.float32)
+        S = scipy.sparse.csr_matrix((rng.choice([-1.0, 1.0], m).astype(np.float32),
+                                     (rng.integers(0, 4 * n, m), np.arange(m))),
+                                    shape=(4 * n, m))
+        probs.append((np.asfortranarray(A), b, S))
+    t0 = time.perf_counter()
+    for A, b, S in probs:
+        SA, Sb = S @ A, S @ b
+        Q, Rr = scipy.linalg.qr(SA, mode="economic")
+        scipy.linalg.solve_triangular(Rr, Q.T @ Sb)
+    dt = time.perf_counter() - t0
+    per = dt / n_sample
+    return {"value": 1.0 / per, "unit": "solves/s", "seconds_per_512": per * 512,
+            "cores": _host_threads(), "kind": "port",
+            "sample": f"{n_sample} of the 512 problem shapes (host draws), scipy.sparse sketch "
+                      "and LAPACK Householder QR per problem, extrapolated to solves/s"}
+
+
+def cpu_c5b(iters=100, written_iters=5):
+    """C5b on the host cores: kaczmarz.jl:313-368 restated with numpy on the
+    100000x1000 system: 64 rows drawn by squared row norm, the LQ projection
+    (lq.jl:35-46) of each block.  Timed with the FP64 residual every 100
+    iterations, and as written (residual and L2Norm weights every iteration);
+    per-solve figures use the device solve's iteration count (501)."""
+    import numpy as np
+    import scipy.linalg
+    m, n, s = 100000, 1000, 64
+    rng = np.random.default_rng(1234)
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    b = A @ rng.standard_normal(n).astype(np.float32)
+    x = np.zeros(n, np.float32)
+
+    def step(weights):
+        p = weights / weights.sum()
+        rows = np.sort(rng.choice(m, s, replace=False, p=p))
+        At, bt = A[rows], b[rows]
+        Q, Rr = np.linalg.qr(At.T.astype(np.float64))
+        u = Q @ scipy.linalg.solve_triangular(Rr.T, (bt - At @ x).astype(np.float64), lower=True)
+        x[:] += u.astype(np.float32)
+
+    # the reference's FullResidual and L2Norm weights are in the data's type
+    # (full_residual.jl:32-41, strohmer_vershynin.jl:82)
+    w = (A * A).sum(1).astype(np.float64)
+    t0 = time.perf_counter()
+    for i in range(iters):
+        if i % 100 == 0:
+            np.linalg.norm(b - A @ x)
+        step(w)
+    per = (time.perf_counter() - t0) / iters
+    t0 = time.perf_counter()
+    for _ in range(written_iters):
+        np.linalg.norm(b - A @ x)
+        w = (A * A).sum(1).astype(np.float64)
+        step(w)
+    per_written = (time.perf_counter() - t0) / written_iters
+    return {"value": 1.0 / (per * 501), "unit": "solves/s", "us_per_iteration": per * 1e6,
+            "us_per_iteration_as_written": per_written * 1e6, "cores": _host_threads(),
+            "kind": "port",
+            "sample": f"{iters} iterations (residual every 100) and {written_iters} as written "
+                      "(residual and weights every iteration), numpy restatement with the host "
+                      "BLAS; per-solve figures at 501 iterations"}
+
+
 def cpu_baseline(S, A0):
     """Oracle C restatement (OpenMP over columns, all host threads) on the full
     workload: the same A, op, signs and scale."""
@@ -527,6 +682,16 @@ def main():
     ls = None if args.skip_ls else ls_batched_c4(rla, torch)
     kz = None if args.skip_kaczmarz else kaczmarz_c5(rla, torch)
     srv = None if args.skip_serving else serving_c5(rla, torch)
+    if not args.no_cpu:
+        # CPU baselines of the nested workloads (host cores, bounded samples)
+        if extra is not None:
+            extra["cpu_baseline"] = cpu_c1(*SPARSE_C1_ARGS)
+        if rf is not None:
+            rf["cpu_baseline"] = cpu_c3()
+        if ls is not None:
+            ls["cpu_baseline"] = cpu_c4()
+        if kz is not None:
+            kz["cpu_baseline"] = cpu_c5b()
     if args.calib_copy:
         # counter calibration: a plain streaming kernel of known traffic
         # (rla::convert_kernel<float, double>: reads 256 MiB, writes 512 MiB)

@@ -14,14 +14,19 @@
 //
 // Layout: the columns are cut into blocks of 16 (zero columns pad the last
 // block; a zero column is never rotated).  One cooperative launch holds
-// nblk / 2 workgroups of 1024 threads, one per CU.  A sweep is nblk - 1
-// rounds of the round-robin (circle) schedule over blocks: in each round a
-// workgroup loads its two blocks of M and V (k x 32 each) into LDS, runs one
-// sweep of the same schedule over its 32 columns (31 inner rounds of 16
-// disjoint pairs, one wavefront per pair), and stores them back; a grid
-// barrier separates rounds.  The sweep count of rotations decides, after a
-// grid barrier, whether another sweep runs, so every workgroup executes the
-// same number of barriers.  At most max_sweeps sweeps.
+// nblk / 2 workgroups of 1024 threads, one per CU, and M and V in global
+// memory (L2).  A sweep visits every column pair once:
+//   * an intra pass: workgroup w loads blocks 2w, 2w+1 into LDS and runs the
+//     round-robin schedule over each block's 16 columns (15 rounds of 8
+//     disjoint pairs per block, one wavefront per pair);
+//   * nblk - 1 rounds of the round-robin schedule over blocks: workgroup w
+//     loads its blocks I, J and visits the 16 x 16 cross pairs in 16 rounds
+//     (pair (I_a, J_(a+r) mod 16) on wavefront a).
+// A grid barrier follows every pass.  A wavefront keeps its pair's entries
+// in registers between the dot products and the rotation; the three sums
+// are reduced with DPP and permlane swaps.  The sweep's rotation count
+// decides, after a grid barrier, whether another sweep runs, so every
+// workgroup executes the same barriers.  At most 60 sweeps.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -64,6 +69,101 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// sum over the 64 lanes, in every lane: within rows of 16 by DPP (quad
+// swaps, then the half-row and row mirrors), across rows by swapping a copy
+// with itself (v_permlane16_swap / v_permlane32_swap)
+__device__ __forceinline__ double wave_sum_fast(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  int alo = __double2loint(v), ahi = __double2hiint(v);
+  int blo = alo, bhi = ahi;
+  asm("v_permlane16_swap_b32 %0, %1" : "+v"(alo), "+v"(blo));
+  asm("v_permlane16_swap_b32 %0, %1" : "+v"(ahi), "+v"(bhi));
+  v = __hiloint2double(ahi, alo) + __hiloint2double(bhi, blo);
+  alo = __double2loint(v);
+  ahi = __double2hiint(v);
+  blo = alo;
+  bhi = ahi;
+  asm("v_permlane32_swap_b32 %0, %1" : "+v"(alo), "+v"(blo));
+  asm("v_permlane32_swap_b32 %0, %1" : "+v"(ahi), "+v"(bhi));
+  return __hiloint2double(ahi, alo) + __hiloint2double(bhi, blo);
+}
+
+// 1/x and 1/sqrt(x) from the hardware estimates and two Newton steps
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
+constexpr int kJRows = kJMaxK / 64;  // rows per lane (k <= 288)
+
+// rotates columns p and q (LDS, k rows each) of M and V when they are not
+// orthogonal to tolerance; returns 1 if it rotated
+__device__ __forceinline__ int jacobi_pair(double* Ms, double* Vs, int k, int p, int q, int lane,
+                                           double tol) {
+  double* mp = Ms + p * k;
+  double* mq = Ms + q * k;
+  double x[kJRows + 1], y[kJRows + 1];
+  double a = 0.0, b = 0.0, gm = 0.0;
+#pragma unroll
+  for (int r = 0; r <= kJRows; ++r) {
+    const int i = lane + 64 * r;
+    x[r] = i < k ? mp[i] : 0.0;
+    y[r] = i < k ? mq[i] : 0.0;
+    a = fma(x[r], x[r], a);
+    b = fma(y[r], y[r], b);
+    gm = fma(x[r], y[r], gm);
+  }
+  a = wave_sum_fast(a);
+  b = wave_sum_fast(b);
+  gm = wave_sum_fast(gm);
+  if (!(a > 0.0 && b > 0.0 && fabs(gm) > tol * sqrt(a * b))) return 0;
+  // zeta = (b - a) / (2 g), t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+  // c = 1 / sqrt(1 + t^2), s = c t
+  const double zeta = (b - a) * rcp_nr(2.0 * gm);
+  const double az = fabs(zeta);
+  double tt;
+  if (az > 1e150) {
+    tt = 0.5 / az;
+  } else {
+    const double w = fma(zeta, zeta, 1.0);
+    tt = rcp_nr(az + w * rsq_nr(w));
+  }
+  const double tn = zeta < 0.0 ? -tt : tt;
+  const double c = rsq_nr(fma(tn, tn, 1.0));
+  const double sn = c * tn;
+  double* vp = Vs + p * k;
+  double* vq = Vs + q * k;
+#pragma unroll
+  for (int r = 0; r <= kJRows; ++r) {
+    const int i = lane + 64 * r;
+    if (i < k) {
+      mp[i] = c * x[r] - sn * y[r];
+      mq[i] = sn * x[r] + c * y[r];
+      const double u = vp[i], v = vq[i];
+      vp[i] = c * u - sn * v;
+      vq[i] = sn * u + c * v;
+    }
+  }
+  return 1;
+}
+
 __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
   extern __shared__ __align__(16) double jsm[];
   const int k = g.k;
@@ -73,63 +173,72 @@ __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int nblk = g.nblk;
   const int w = blockIdx.x;
+  // a pass moves 32 columns of M and V (at most 9 elements per thread each);
+  // all of a thread's global loads are issued before any is used
+  constexpr int kPer = 32 * kJMaxK / kJThreads;
+  auto gaddr = [&](int I, int J, int e) {
+    const int c = e / k, i = e - c * k;
+    return int64_t(c < kJB ? I * kJB + c : J * kJB + c - kJB) * k + i;
+  };
+  auto load = [&](int I, int J) {
+    double lm[kPer], lv[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * kJThreads;
+      if (e < 32 * k) {
+        const int64_t a = gaddr(I, J, e);
+        lm[q] = g.M[a];
+        lv[q] = g.V[a];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * kJThreads;
+      if (e < 32 * k) {
+        Ms[e] = lm[q];
+        Vs[e] = lv[q];
+      }
+    }
+    __syncthreads();
+  };
+  auto store = [&](int I, int J) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * kJThreads;
+      if (e < 32 * k) {
+        const int64_t a = gaddr(I, J, e);
+        g.M[a] = Ms[e];
+        g.V[a] = Vs[e];
+      }
+    }
+    __threadfence();
+    grid.sync();
+  };
   int sweep = 0;
   for (; sweep < kJMaxSweeps; ++sweep) {
-    int nrot = 0;  // rotations done by this wavefront's lane 0 (one per pair)
-    for (int r = 0; r < nblk - 1; ++r) {
-      const int I = circle_item(w, r, nblk), J = circle_item(nblk - 1 - w, r, nblk);
-      // load blocks I and J: local column c < 16 is global I*16 + c, else J*16 + c - 16
-      for (int e = t; e < 32 * k; e += kJThreads) {
-        const int c = e / k, i = e - c * k;
-        const int gc = (c < kJB ? I * kJB + c : J * kJB + c - kJB);
-        Ms[e] = g.M[int64_t(gc) * k + i];
-        Vs[e] = g.V[int64_t(gc) * k + i];
-      }
-      __syncthreads();
-      for (int ir = 0; ir < 31; ++ir) {
-        const int p = circle_item(wave, ir, 32), q = circle_item(31 - wave, ir, 32);
-        double* mp = Ms + p * k;
-        double* mq = Ms + q * k;
-        double a = 0.0, b = 0.0, gm = 0.0;
-        for (int i = lane; i < k; i += 64) {
-          const double x = mp[i], y = mq[i];
-          a = fma(x, x, a);
-          b = fma(y, y, b);
-          gm = fma(x, y, gm);
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        gm = wave_sum(gm);
-        if (a > 0.0 && b > 0.0 && fabs(gm) > g.tol * sqrt(a * b)) {
-          const double zeta = (b - a) / (2.0 * gm);
-          const double az = fabs(zeta);
-          // for huge |zeta|, sqrt(1 + zeta^2) ~ |zeta| (t ~ 1 / (2 zeta))
-          const double tt = (az > 1e150 ? 0.5 / az : 1.0 / (az + sqrt(fma(zeta, zeta, 1.0))));
-          const double tn = zeta < 0.0 ? -tt : tt;
-          const double c = 1.0 / sqrt(fma(tn, tn, 1.0));
-          const double s = c * tn;
-          double* vp = Vs + p * k;
-          double* vq = Vs + q * k;
-          for (int i = lane; i < k; i += 64) {
-            const double x = mp[i], y = mq[i];
-            mp[i] = c * x - s * y;
-            mq[i] = s * x + c * y;
-            const double u = vp[i], v = vq[i];
-            vp[i] = c * u - s * v;
-            vq[i] = s * u + c * v;
-          }
-          if (lane == 0) ++nrot;
-        }
+    int nrot = 0;  // this wavefront's rotations (lane 0 counts)
+    // intra pass: blocks 2w and 2w+1, 8 wavefronts each
+    load(2 * w, 2 * w + 1);
+    {
+      const int half = wave >> 3, a8 = wave & 7;
+      for (int ir = 0; ir < 15; ++ir) {
+        const int p = 16 * half + circle_item(a8, ir, 16);
+        const int q = 16 * half + circle_item(15 - a8, ir, 16);
+        nrot += jacobi_pair(Ms, Vs, k, p, q, lane, g.tol);
         __syncthreads();
       }
-      for (int e = t; e < 32 * k; e += kJThreads) {
-        const int c = e / k, i = e - c * k;
-        const int gc = (c < kJB ? I * kJB + c : J * kJB + c - kJB);
-        g.M[int64_t(gc) * k + i] = Ms[e];
-        g.V[int64_t(gc) * k + i] = Vs[e];
+    }
+    store(2 * w, 2 * w + 1);
+    // cross passes over the round-robin schedule of blocks
+    for (int r = 0; r < nblk - 1; ++r) {
+      const int I = circle_item(w, r, nblk), J = circle_item(nblk - 1 - w, r, nblk);
+      load(I, J);
+      for (int ir = 0; ir < 16; ++ir) {
+        nrot += jacobi_pair(Ms, Vs, k, wave, 16 + ((wave + ir) & 15), lane, g.tol);
+        __syncthreads();
       }
-      __threadfence();
-      grid.sync();
+      store(I, J);
     }
     if (lane == 0 && nrot) atomicAdd(g.rot + sweep, nrot);
     __threadfence();

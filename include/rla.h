@@ -569,8 +569,28 @@ int rla_kaczmarz_workspace_bytes(int64_t m, int64_t n, int64_t s, int64_t every,
  * explicit s x n block At (strided) and bt (stride bt_inc): the LQ
  * minimum-norm step of kaczmarz_update_block! (kaczmarz.jl:294-311,
  * lq.jl:35-46); s = 1 is kaczmarz_update! (kaczmarz.jl:262-276).  A zero block
- * is skipped (kaczmarz.jl:300); a singular Gram matrix returns RLA_EDOMAIN.
- * s <= 64.  Workspace from rla_kaczmarz_workspace_bytes(s, n, s, 1). */
+ * is skipped (kaczmarz.jl:300); a row dependent on the others gets a zero
+ * coefficient (the projection onto the independent rows); a non-finite Gram
+ * matrix returns RLA_EDOMAIN.  s <= 64.  Workspace from
+ * rla_kaczmarz_workspace_bytes(s, n, s, 1). */
+/* The same projection for any block size s <= n and Float32 / Float64 data,
+ * as lq.jl:35-46 computes it: At' = Q R (FP64 CholeskyQR2, Householder on
+ * breakdown), x += alpha Q (R' \\ (bt - At x)).  A zero block is skipped; rows
+ * dependent on the others (a pivot of R not above sqrt(s eps) max|R_jj|) are
+ * dropped and the block factored again, so the step projects onto the
+ * independent rows.  Synchronises twice (the zero-block test and the pivot
+ * check; once more when it drops rows).  Workspace from
+ * rla_kaczmarz_lq_workspace_bytes(s, n). */
+int rla_kaczmarz_lq_workspace_bytes(int64_t s, int64_t n, size_t* bytes);
+int rla_kaczmarz_lq_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const float* At,
+                               int64_t at_rs, int64_t at_cs, const float* bt, int64_t bt_inc,
+                               float* x, double alpha, void* work, size_t work_bytes);
+int rla_kaczmarz_lq_update_f64(rla_ctx* ctx, int64_t s, int64_t n, const double* At,
+                               int64_t at_rs, int64_t at_cs, const double* bt, int64_t bt_inc,
+                               double* x, double alpha, void* work, size_t work_bytes);
+/* ||b - A x||_2 in FP64 for Float64 data (full_residual.jl:32-41); blocks. */
+int rla_residual_norm_f64(rla_ctx* ctx, int64_t m, int64_t n, const double* A, int64_t a_rs,
+                          int64_t a_cs, const double* x, const double* b, double* out_host);
 int rla_kaczmarz_block_update_f32(rla_ctx* ctx, int64_t s, int64_t n, const float* At,
                                   int64_t at_rs, int64_t at_cs, const float* bt, int64_t bt_inc,
                                   float* x, double alpha, void* work, size_t work_bytes);

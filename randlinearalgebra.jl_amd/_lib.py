@@ -158,6 +158,12 @@ _SIGS = {
     "rla_wsample": [vp, u64, u64, i64, vp, i64, ctypes.c_int, i64, vp],
     "rla_residual_norm_f32": [vp, i64, i64, vp, i64, i64, vp, vp, vp],
     "rla_kaczmarz_workspace_bytes": [i64, i64, i64, i64, vp],
+    "rla_kaczmarz_lq_workspace_bytes": [i64, i64, vp],
+    "rla_kaczmarz_lq_update_f32": [vp, i64, i64, vp, i64, i64, vp, i64, vp, ctypes.c_double, vp,
+                                   ctypes.c_size_t],
+    "rla_kaczmarz_lq_update_f64": [vp, i64, i64, vp, i64, i64, vp, i64, vp, ctypes.c_double, vp,
+                                   ctypes.c_size_t],
+    "rla_residual_norm_f64": [vp, i64, i64, vp, i64, i64, vp, vp, vp],
     "rla_kaczmarz_block_update_f32": [vp, i64, i64, vp, i64, i64, vp, i64, vp, ctypes.c_double,
                                       vp, ctypes.c_size_t],
     "rla_kaczmarz_solve_f32": [vp, i64, i64, vp, i64, vp, i64, vp, vp, i64, ctypes.c_double, vp,

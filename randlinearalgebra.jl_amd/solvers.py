@@ -396,11 +396,11 @@ class LSGradientRecipe:
 
 
 def _norm2(v):
-    """||v||_2 of a contiguous Float32 device vector, accumulated in FP64 by
-    the residual kernel (||v - 0 x|| with a matrix of no columns)."""
+    """||v||_2 of a contiguous Float32 / Float64 device vector, accumulated in
+    FP64 by the residual kernel (||v - 0 x|| with a matrix of no columns)."""
     out = ctypes.c_double(0.0)
-    _lib.call("rla_residual_norm_f32", _lib.ctx(v.device), v.shape[0], 0, _lib.ptr(v), 1, 1,
-              _lib.ptr(v), _lib.ptr(v), ctypes.byref(out))
+    _lib.call(f"rla_residual_norm_{_lib.sfx(v.dtype)}", _lib.ctx(v.device), v.shape[0], 0,
+              _lib.ptr(v), 1, 1, _lib.ptr(v), _lib.ptr(v), ctypes.byref(out))
     return out.value
 
 
@@ -413,8 +413,9 @@ def compute_error(err, solver, A, b):
         return _norm2(err.gradient)
     Av = as_view(A)
     out = ctypes.c_double(0.0)
-    _lib.call("rla_residual_norm_f32", _lib.ctx(Av.t.device), Av.rows, Av.cols, _lib.ptr(Av.t),
-              Av.rs, Av.cs, _lib.ptr(solver.solution_vec), _lib.ptr(b), ctypes.byref(out))
+    _lib.call(f"rla_residual_norm_{_lib.sfx(Av.dtype)}", _lib.ctx(Av.t.device), Av.rows,
+              Av.cols, _lib.ptr(Av.t), Av.rs, Av.cs, _lib.ptr(solver.solution_vec), _lib.ptr(b),
+              ctypes.byref(out))
     return out.value
 
 
@@ -460,11 +461,13 @@ class KaczmarzRecipe:
         self.__dict__.update(kw)
 
 
-def _check_float32(*ts):
+def _check_float(*ts):
+    """Float32 or Float64 data, one type for A, b and x."""
+    dt = ts[0].dtype
     for t in ts:
-        if t.dtype != torch.float32:
-            raise ArgumentError("the GPU Kaczmarz solver runs on Float32 data "
-                                f"(got {t.dtype})")
+        if t.dtype not in (torch.float32, torch.float64) or t.dtype != dt:
+            raise ArgumentError("the GPU Kaczmarz solver takes Float32 or Float64 data of one "
+                                f"type (got {[u.dtype for u in ts]})")
 
 
 def complete_solver(ingredients, x, A, b):
@@ -486,13 +489,14 @@ def complete_solver(ingredients, x, A, b):
     error = FullResidualRecipe(ingredients.error, b)
     if not isinstance(ingredients.sub_solver, LQSolver):
         raise ArgumentError("the GPU Kaczmarz solver projects with LQSolver() only")
-    _check_float32(as_view(A).t, b, x)
+    _check_float(as_view(A).t, b, x)
     sample_size = compressor.n_rows
     n = _size2(A)[1]
     dev = as_view(A).t.device
+    dt = as_view(A).dtype
     # compressed block buffers (kaczmarz.jl:207-224), column-major
-    compressed_mat = torch.zeros(n, sample_size, dtype=torch.float32, device=dev).T
-    compressed_vec = torch.zeros(sample_size, dtype=torch.float32, device=dev)
+    compressed_mat = torch.zeros(n, sample_size, dtype=dt, device=dev).T
+    compressed_vec = torch.zeros(sample_size, dtype=dt, device=dev)
     return KaczmarzRecipe(compressor=compressor, log=logger, error=error,
                           sub_solver=LQSolverRecipe(compressed_mat), alpha=ingredients.alpha,
                           compressed_mat=compressed_mat, compressed_vec=compressed_vec,
@@ -515,7 +519,8 @@ def _row_major(recipe, A):
 def _fast_path(recipe):
     S = recipe.compressor
     return (isinstance(S, SamplingRecipe) and S.cardinality == Left()
-            and S.compression_dim <= 64 and isinstance(recipe.log, BasicLoggerRecipe)
+            and S.compression_dim <= 64 and recipe.compressed_mat.dtype == torch.float32
+            and isinstance(recipe.log, BasicLoggerRecipe)
             and recipe.log.stopping_criterion is threshold_stop
             and np.isscalar(recipe.log.threshold_info))
 
@@ -552,7 +557,7 @@ def _rsolve(recipe, x, A, b):
         raise DimensionMismatch(
             f"Dimension of `b`, {b.shape[0]} is different from number of rows in `A`,"
             f"{a_rows}.")
-    _check_float32(as_view(A).t, b, x)
+    _check_float(as_view(A).t, b, x)
     recipe.solution_vec = x
     if log.max_it == 0:
         return
@@ -601,11 +606,17 @@ def _rsolve_device(recipe, x, A, b):
 
 
 def _rsolve_loop(recipe, x, A, b):
+    """The host-driven loop for the blocks the one-launch device step does not
+    take: other compressors, Float64 data, more than 64 rows.  Blocks of at
+    most 64 Float32 rows use the device step (FP64 Gram, LDL'); the others
+    the QR-based LQ projection (rla_kaczmarz_lq_update_*)."""
     S, log = recipe.compressor, recipe.log
     every = recipe.error.every
     n = _size2(A)[1]
+    f32 = x.dtype == torch.float32
     nb = _dense.query_bytes("rla_kaczmarz_workspace_bytes", 64, n, 64, 1)
     work = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=x.device)
+    lq_work = None
     for i in range(1, log.max_it + 1):
         if (i - 1) % every == 0 or i == log.max_it:
             err = compute_error(recipe.error, recipe, A, b)
@@ -614,16 +625,25 @@ def _rsolve_loop(recipe, x, A, b):
                 return
         update_compressor(S, x, A, b)
         rows_s = S.size(1)
-        if rows_s > 64:
-            raise ArgumentError("the GPU Kaczmarz block update takes at most 64 rows")
         Mt = recipe.compressed_mat[:rows_s]
         bt = recipe.compressed_vec[:rows_s]
         mul(Mt, S, A)
         mul(bt, S, b)
         Mv = as_view(Mt)
-        _lib.call("rla_kaczmarz_block_update_f32", _lib.ctx(x.device), rows_s, n,
+        if f32 and rows_s <= 64:
+            _lib.call("rla_kaczmarz_block_update_f32", _lib.ctx(x.device), rows_s, n,
+                      _lib.ptr(Mv.t), Mv.rs, Mv.cs, _lib.ptr(bt), bt.stride(0), _lib.ptr(x),
+                      recipe.alpha, _lib.ptr(work), work.numel() * 8)
+            continue
+        if rows_s > n:
+            raise ArgumentError("a Kaczmarz block of more rows than columns is not supported")
+        if lq_work is None or lq_work[0] != rows_s:
+            nbl = _dense.query_bytes("rla_kaczmarz_lq_workspace_bytes", rows_s, n)
+            lq_work = (rows_s, _dense.workspace(nbl, x.device))
+        w = lq_work[1]
+        _lib.call(f"rla_kaczmarz_lq_update_{_lib.sfx(x.dtype)}", _lib.ctx(x.device), rows_s, n,
                   _lib.ptr(Mv.t), Mv.rs, Mv.cs, _lib.ptr(bt), bt.stride(0), _lib.ptr(x),
-                  recipe.alpha, _lib.ptr(work), work.numel() * 8)
+                  recipe.alpha, _lib.ptr(w), w.numel() * 8)
 
 
 # ---------------------------------------------------------------------------

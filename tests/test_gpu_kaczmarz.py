@@ -339,3 +339,108 @@ def test_kaczmarz_c5_system_reaches_1e6(rla, dev):
     assert rec.log.error / bn <= 1e-6
     assert float(torch.linalg.vector_norm((x - xs).double())) <= 1e-4 * float(
         torch.linalg.vector_norm(xs.double()))
+
+
+# ---------------------------------------------------------------------------
+# any block size, Float64 data: the QR-based LQ projection
+
+
+def lq_update(rla, dev, At, bt, x, alpha, dtype):
+    s, n = At.shape
+    nb = rla._dense.query_bytes("rla_kaczmarz_lq_workspace_bytes", s, n)
+    work = rla._dense.workspace(nb, dev)
+    Atd, btd, xd = colmajor(At, dev, dtype), colmajor(bt, dev, dtype), colmajor(x, dev, dtype)
+    rla._lib.call(f"rla_kaczmarz_lq_update_{rla._lib.sfx(dtype)}", rla._lib.ctx(dev), s, n,
+                  rla._lib.ptr(Atd), 1, s, rla._lib.ptr(btd), 1, rla._lib.ptr(xd), alpha,
+                  rla._lib.ptr(work), work.numel() * 8)
+    return xd.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("s,n,alpha", [(1, 50, 1.0), (8, 100, 1.0), (64, 1000, 0.7),
+                                       (100, 300, 1.0), (300, 300, 1.25), (257, 2000, 1.0)])
+def test_lq_update_f64_matches_oracle(rla, dev, s, n, alpha):
+    # kaczmarz_update_block! with the LQ sub-solver (qr!(At') \ r) in FP64:
+    # within 1e-12 of the oracle's restatement, for any block size
+    At = rnd((s, n), s + n)
+    x = rnd(n, n)
+    bt = rnd(s, 2 * s)
+    got = lq_update(rla, dev, At, bt, x, alpha, torch.float64)
+    want = R.kaczmarz_update_block(x, At, bt, alpha)
+    assert np.linalg.norm(got - want) <= 1e-12 * max(1.0, np.linalg.norm(want))
+    if s == 1:
+        assert np.allclose(got, R.kaczmarz_update(x, At[0], bt, alpha), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("s,n", [(100, 400), (65, 130)])
+def test_lq_update_f32_matches_oracle(rla, dev, s, n):
+    At = rnd((s, n), 3).astype(np.float32)
+    x = rnd(n, 4).astype(np.float32)
+    bt = rnd(s, 5).astype(np.float32)
+    got = lq_update(rla, dev, At, bt, x, 1.0, F32)
+    want = R.kaczmarz_update_block(x, At, bt, 1.0)
+    assert np.linalg.norm(got - want) <= 1e-5 * max(1.0, np.linalg.norm(want))
+
+
+def test_lq_update_zero_block_and_dependent_rows(rla, dev):
+    x = rnd(40, 1)
+    got = lq_update(rla, dev, np.zeros((30, 40)), np.ones(30), x, 1.0, torch.float64)
+    assert np.array_equal(got, x)
+    x = rnd(100, 1)
+    At = rnd((70, 100), 2)
+    At[50] = At[3]                      # a row drawn twice
+    At[60] = 0.5 * At[1] - 2.0 * At[7]  # a combination of two others
+    bt = rnd(70, 3)
+    bt[50], bt[60] = bt[3], 0.5 * bt[1] - 2.0 * bt[7]
+    got = lq_update(rla, dev, At, bt, x, 1.0, torch.float64)
+    want = x + np.linalg.pinv(At) @ (bt - At @ x)
+    assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)
+
+
+def test_residual_norm_f64(rla, dev):
+    A = rnd((3000, 77), 6)
+    x, b = rnd(77, 7), rnd(3000, 8)
+    out = ctypes.c_double()
+    Ad, xd, bd = colmajor(A, dev, torch.float64), colmajor(x, dev, torch.float64), \
+        colmajor(b, dev, torch.float64)
+    rla._lib.call("rla_residual_norm_f64", rla._lib.ctx(dev), 3000, 77, rla._lib.ptr(Ad), 1, 3000,
+                  rla._lib.ptr(xd), rla._lib.ptr(bd), ctypes.byref(out))
+    assert abs(out.value - R.full_residual(A, x, b)) <= 1e-12 * R.full_residual(A, x, b)
+
+
+@pytest.mark.parametrize("dtype,block,dist", [(torch.float64, 32, "l2norm"),
+                                              (torch.float64, 100, "uniform"),
+                                              (F32, 128, "l2norm")])
+def test_kaczmarz_float64_and_wide_blocks_converge(rla, dev, dtype, block, dist):
+    # Float64 data, and blocks over 64 rows, run the LQ projection per block
+    m, n = 2000, 150
+    A = rnd((m, n), 31)
+    xs = rnd(n, 32)
+    Ad = colmajor(A, dev, dtype)
+    b = Ad @ colmajor(xs, dev, dtype)
+    bn = float(torch.linalg.vector_norm(b.double()))
+    D = rla.L2Norm() if dist == "l2norm" else rla.Uniform()
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    solver = rla.Kaczmarz(compressor=rla.Sampling(compression_dim=block, distribution=D),
+                          log=rla.BasicLogger(max_it=3000, threshold=tol * bn),
+                          error=rla.FullResidual(every=10))
+    x = torch.zeros(n, dtype=dtype, device=dev)
+    rla.seed(2)
+    x, rec = rla.rsolve(solver, x, Ad, b)
+    assert rec.log.converged and rec.log.iteration < 3000
+    err = np.linalg.norm(x.double().cpu().numpy() - xs) / np.linalg.norm(xs)
+    assert err <= (1e-8 if dtype == torch.float64 else 1e-4), err
+
+
+def test_kaczmarz_float64_sparse_sign(rla, dev):
+    m, n = 600, 40
+    A = rnd((m, n), 41)
+    xs = rnd(n, 42)
+    Ad = colmajor(A, dev, torch.float64)
+    b = Ad @ colmajor(xs, dev, torch.float64)
+    bn = float(torch.linalg.vector_norm(b))
+    solver = rla.Kaczmarz(compressor=rla.SparseSign(cardinality=rla.Left(), compression_dim=20),
+                          log=rla.BasicLogger(max_it=4000, threshold=1e-10 * bn))
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    x, rec = rla.rsolve(solver, x, Ad, b)
+    assert rec.log.converged
+    assert np.linalg.norm(x.cpu().numpy() - xs) <= 1e-8 * np.linalg.norm(xs)

@@ -211,6 +211,41 @@ def tall_sparse_sign_row(rla, torch, reps=10):
             "ms_per_apply": ms, "GB_s": b / ms / 1e6, "frac_hbm_peak": b / ms / 1e6 / HBM_PEAK_GBS}
 
 
+def bf16_gemm_c3(rla, torch, reps=20):
+    """The two BF16 MFMA products of a C3 RangeFinder solve on their own
+    (Y = A Z: 32768 x 266 over K = 8192 from the row copy of A; Z = A'Y:
+    8192 x 266 over K = 32768 from the column copy), N(0,1) operands; wall
+    time over `reps` back-to-back products between device synchronisations."""
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(1234)
+    out = {"workload": "BF16 GEMM at the RangeFinder 32768x8192, k=266 product shapes"}
+    lib = rla._lib
+    for name, M, K in (("A_Z", 32768, 8192), ("At_Y", 8192, 32768)):
+        N = 266
+        A = torch.randn(M, K, generator=g, device=dev).to(torch.bfloat16)  # row i = K-run
+        B = torch.randn(N, K, generator=g, device=dev).to(torch.bfloat16)
+        C = torch.empty(N, M, device=dev)  # column-major M x N
+        ctx = lib.ctx(dev)
+
+        def run():
+            lib.call("rla_gemm_bf16", ctx, M, N, K, 1.0, lib.ptr(A), K, lib.ptr(B), K, 0.0,
+                     lib.ptr(C), 1, M)
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        flops = 2.0 * M * N * K
+        out[name] = {"M": M, "N": N, "K": K, "us_per_product": dt * 1e6,
+                     "TFLOP_s": flops / dt / 1e12, "frac_bf16_peak": flops / dt / 2.5e15,
+                     "a_bytes_GB_s": M * K * 2 / dt / 1e9}
+        del A, B, C
+    return out
+
+
 def rangefinder_c3(rla, torch, reps=3, batch=4):
     """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
     32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
@@ -664,6 +699,8 @@ def main():
     ap.add_argument("--skip-ls", action="store_true")
     ap.add_argument("--skip-kaczmarz", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--bf16-gemm-only", action="store_true",
+                    help="time only the two BF16 RangeFinder products (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the SRHT apply "
@@ -673,6 +710,10 @@ def main():
     import torch
     import __graft_entry__
     rla = __graft_entry__.load_package()
+    if args.bf16_gemm_only:
+        print(json.dumps({"metric": "BF16 GEMM TFLOP/s (RangeFinder product shapes)",
+                          "bf16_gemm_c3": bf16_gemm_c3(rla, torch)}))
+        return
     ms, wall, parity, S, A0 = gpu_srht(rla, torch, args.steps, args.warmup)
     gbs = srht_bytes() / ms / 1e6
     extra = None if args.skip_sparse else gpu_sparse_sign(rla, torch, args.steps, args.warmup)

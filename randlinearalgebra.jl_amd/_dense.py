@@ -80,7 +80,9 @@ def gemm_bf16(M, N, K, A_rows, B_cols, C, alpha=1.0, beta=0.0):
 
 
 def convert(src, dtype):
-    """Column-major copy of a column-major device matrix in another precision."""
+    """Column-major copy of a column-major device matrix in another precision.
+    With dtype equal to the source's, a source that is already column-major
+    and dense is returned itself (no copy)."""
     from .compressors import as_view
     v = as_view(src)
     if v.rs != 1:

@@ -7,13 +7,14 @@
 (src/Solvers/SubSolvers/qr.jl:36-43) for a whole batch at once.
 
 `QRSolver` / `complete_sub_solver` / `ldiv` mirror qr.jl for one (already
-sketched) matrix: the same plan with the identity as the sketch.
+sketched) matrix of any size, Float32 or Float64: an FP64 QR of A (CholeskyQR2
+with the Householder fallback), Q'b and a triangular solve.
 """
 import ctypes
 
 import torch
 
-from . import _lib
+from . import _dense, _lib
 from ._lib import ArgumentError
 from .compressors import CountSketch, Left, as_view, complete_compressor
 
@@ -163,7 +164,7 @@ class QRSolverRecipe:
 
     def __init__(self, A):
         self.A = A
-        self._plan = None
+        self._qr = None     # (Q, R) in FP64, for the A of _key
         self._key = None
 
 
@@ -177,25 +178,77 @@ def update_sub_solver(recipe, A):
     recipe.A = A
 
 
-def ldiv(x, recipe, b):
-    """ldiv!(x, ::QRSolverRecipe, b) (qr.jl:36-43): x = argmin ||A x - b||."""
-    A = recipe.A
-    Av = _check_problem(A, b, x)
+def _check_single(A, b, x):
+    Av = as_view(A)
+    if Av.dtype not in (torch.float32, torch.float64):
+        raise ArgumentError("QRSolver runs on Float32 or Float64 data")
+    if b.dtype != Av.dtype or x.dtype != Av.dtype:
+        raise ArgumentError("A, b and x must have the same element type")
+    if Av.rs != 1:
+        raise ArgumentError("A must be column-major (unit row stride)")
+    if b.dim() != 1 or x.dim() != 1:
+        raise ArgumentError("b and x must be vectors")
+    if b.shape[0] != Av.rows or x.shape[0] != Av.cols:
+        raise ArgumentError("b must have size(A, 1) entries and x size(A, 2)")
     if Av.rows < Av.cols:
         raise ArgumentError("QRSolver needs size(A, 1) >= size(A, 2)")
-    key = (Av.t.data_ptr(), Av.rows, Av.cols, Av.cs, b.data_ptr(), x.data_ptr())
-    if recipe._key != key:
-        dev = Av.t.device
-        s = Av.rows
-        rowptr = torch.arange(0, s + 1, dtype=torch.int32, device=dev)
-        colidx = torch.arange(0, s, dtype=torch.int32, device=dev)
-        ones = torch.ones(s, dtype=torch.float32, device=dev)
-        lda = Av.cs if Av.cols > 1 else max(Av.cs, Av.rows)
-        prob = _Problem(Av.rows, Av.cols, lda, _addr(Av.t), _addr(b), _addr(x),
-                        _addr(rowptr), _addr(colidx), _addr(ones), s)
-        recipe._plan = _Plan(dev, [prob], [A, b, x, rowptr, colidx, ones], 0)
-        recipe._key = key
-    recipe._plan.execute()
-    info = recipe._plan.info()[0]
-    if info:
-        raise _lib.DomainError(f"least-squares matrix is rank deficient (column {info})")
+    return Av
+
+
+def _qr_factor(recipe, Av):
+    """Q (m x n, orthonormal columns) and R (n x n upper) of the recipe's A in
+    FP64: CholeskyQR2 on the matrix cores, Householder when the Cholesky factor
+    breaks down (rla_qr_f64).  Cached while A is the same, unchanged tensor;
+    A itself is not overwritten (qr.jl:42's qr! is)."""
+    if recipe._key is not None and _dense.key_matches(recipe._key, recipe.A):
+        return recipe._qr
+    dev = Av.t.device
+    m, n = Av.rows, Av.cols
+    if recipe.A.dtype == torch.float64:
+        Q = torch.empty((n, m), dtype=torch.float64, device=dev).T
+        Q.copy_(recipe.A)  # the QR overwrites its input
+    else:
+        Q = _dense.convert(recipe.A, torch.float64)
+    R = torch.empty((n, n), dtype=torch.float64, device=dev).T
+    nbytes = _dense.query_bytes("rla_qr_workspace_bytes", m, n)
+    work = _dense.workspace(nbytes, dev)
+    hh = ctypes.c_int(0)
+    _lib.call("rla_qr_f64", _lib.ctx(dev), m, n, _lib.ptr(Q), max(1, m), _lib.ptr(R), max(1, n),
+              _lib.ptr(work), nbytes, ctypes.byref(hh))
+    recipe._qr = (Q, R)
+    recipe._key = _dense.SourceKey(recipe.A)
+    return recipe._qr
+
+
+def ldiv(x, recipe, b):
+    """ldiv!(x, ::QRSolverRecipe, b) (qr.jl:36-43): x = argmin ||A x - b|| for
+    Float32 or Float64 A (m >= n, column-major, any size), with A = QR in FP64
+    (_qr_factor), c = Q'b on the FP64 matrix cores and x = R \\ c.  As after
+    Julia's qr!, an exactly zero diagonal entry of R (an exactly rank-deficient
+    A) is an error (the triangular solve's SingularException); a nearly
+    singular R gives the large solution the reference gives."""
+    Av = _check_single(recipe.A, b, x)
+    n = Av.cols
+    if n == 0:
+        return
+    Q, R = _qr_factor(recipe, Av)
+    dev = Av.t.device
+    ctx = _lib.ctx(dev)
+    zero = torch.nonzero(torch.diagonal(R) == 0)
+    if zero.numel():
+        raise _lib.DomainError(f"SingularException({int(zero[0, 0]) + 1}): the least-squares "
+                               "matrix is rank deficient")
+    b64 = _dense.convert(b.contiguous(), torch.float64)
+    direct = x.dtype == torch.float64 and x.stride(0) == 1
+    c = x if direct else torch.empty(n, dtype=torch.float64, device=dev)
+    _dense.gemm(as_view(c), as_view(Q).T, as_view(b64))  # c = Q'b
+    _lib.call("rla_trsv_upper_f64", ctx, 0, n, _lib.ptr(R), n, _lib.ptr(c))
+    if direct:
+        return
+    if x.dtype == torch.float64:
+        x.copy_(c)
+        return
+    out = x if x.stride(0) == 1 else torch.empty(n, dtype=torch.float32, device=dev)
+    _lib.call("rla_convert_f64_f32", ctx, n, 1, _lib.ptr(c), n, _lib.ptr(out), n)
+    if out is not x:
+        x.copy_(out)

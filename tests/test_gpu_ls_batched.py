@@ -125,6 +125,63 @@ def test_qr_sub_solver_ldiv(rla, dev):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= (1e-6 + cond ** 2 * 1e-15) * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("m,n,dt", [(3000, 300, torch.float64), (2000, 280, F32),
+                                    (1500, 1, torch.float64), (64, 64, torch.float64)])
+def test_qr_sub_solver_any_size_and_type(rla, dev, m, n, dt):
+    # qr.jl:36-43 on any Float32/Float64 matrix: sizes beyond the batch limits
+    # (n + 1 > 272, m > 1024) and square systems, against LAPACK lstsq
+    A, b, _ = make_problem(m, n, 11)
+    A = A.astype(np.float64) if dt == torch.float64 else A
+    b = b.astype(np.float64) if dt == torch.float64 else b
+    Ad = torch.as_tensor(A).to(dev).T.contiguous().T
+    bd = torch.as_tensor(b).to(dev)
+    x = torch.zeros(n, dtype=dt, device=dev)
+    rec = rla.complete_sub_solver(rla.QRSolver(), Ad, bd)
+    rla.ldiv(x, rec, bd)
+    xo = np.linalg.lstsq(A.astype(np.float64), b.astype(np.float64), rcond=None)[0]
+    cond = np.linalg.cond(A.astype(np.float64))
+    tol = (1e-12 + cond ** 2 * 1e-16) if dt == torch.float64 else (1e-6 + cond ** 2 * 1e-15)
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= tol * np.linalg.norm(xo)
+    # A unchanged, and a second solve with another b reuses the factor
+    assert np.array_equal(Ad.cpu().numpy(), A)
+    b2 = torch.as_tensor(np.roll(b, 3)).to(dev)
+    rla.ldiv(x, rec, b2)
+    xo2 = np.linalg.lstsq(A.astype(np.float64), np.roll(b, 3).astype(np.float64), rcond=None)[0]
+    assert np.linalg.norm(x.cpu().numpy() - xo2) <= tol * np.linalg.norm(xo2)
+
+
+def test_qr_sub_solver_strided_x_and_changed_a(rla, dev):
+    A, b, _ = make_problem(500, 40, 12)
+    A = A.astype(np.float64)
+    b = b.astype(np.float64)
+    Ad = torch.as_tensor(A).to(dev).T.contiguous().T
+    bd = torch.as_tensor(b).to(dev)
+    buf = torch.zeros(80, dtype=torch.float64, device=dev)
+    x = buf[::2]
+    rec = rla.complete_sub_solver(rla.QRSolver(), Ad, bd)
+    rla.ldiv(x, rec, bd)
+    xo = np.linalg.lstsq(A, b, rcond=None)[0]
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+    assert float(buf[1::2].abs().max()) == 0.0
+    Ad.mul_(2.0)  # changed in place: the cached factor must not be reused
+    rla.ldiv(x, rec, bd)
+    assert np.linalg.norm(x.cpu().numpy() - xo / 2) <= 1e-10 * np.linalg.norm(xo)
+
+
+def test_qr_sub_solver_rank_deficient_raises(rla, dev):
+    A, b, _ = make_problem(300, 20, 13)
+    A = A.astype(np.float64)
+    A[:, 7] = 0.0
+    Ad = torch.as_tensor(A).to(dev).T.contiguous().T
+    bd = torch.as_tensor(b.astype(np.float64)).to(dev)
+    x = torch.zeros(20, dtype=torch.float64, device=dev)
+    rec = rla.complete_sub_solver(rla.QRSolver(), Ad, bd)
+    with pytest.raises(rla._lib.DomainError, match="SingularException"):
+        rla.ldiv(x, rec, bd)
+    with pytest.raises(rla._lib.ArgumentError, match="same element type"):
+        rla.ldiv(torch.zeros(20, dtype=F32, device=dev), rec, bd)
+
+
 def test_c4_batch_512_problems(rla, dev):
     # the batched least-squares workload: 512 problems, m log-uniform in
     # [2000, 50000], n uniform in [32, 256], columns scaled by 10^U(-3,0),

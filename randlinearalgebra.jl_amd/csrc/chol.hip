@@ -8,9 +8,10 @@
 // 18 x 19 / 2 blocks, 25 per wavefront), so the k^3/3 flops of the trailing
 // updates touch no memory.  Block step s:
 //   1. the eighth wavefront factors the diagonal block G_ss (in LDS) in
-//      registers (lane c holds column c; each row of R_ss is passed by
-//      register broadcast) and inverts R_ss, while the block wavefronts
-//      finish the trailing update of step s-1,
+//      registers and inverts R_ss in the same pass (elimination of
+//      [G_ss | I]; four lanes per row, each pivot row broadcast through
+//      LDS), while the block wavefronts finish the trailing update of step
+//      s-1,
 //   2. the owners of G_sj (j > s) form R_sj = inv(R_ss)' G_sj on the MFMA
 //      and publish it in LDS,
 //   3. the owner of G_{s+1,s+1} subtracts R_{s,s+1}' R_{s,s+1} from it first
@@ -46,6 +47,16 @@ constexpr int kMaxCholK = 288;
 // executes a wavefront's instructions in order, and these accesses may alias
 // so the compiler keeps their order.  The wave barrier only pins the code.
 __device__ __forceinline__ void lane_exchange() { __builtin_amdgcn_wave_barrier(); }
+
+// Workgroup barrier for the factorisation steps: the steps exchange data
+// through LDS only, so a wavefront waits for its own LDS operations and not
+// for its global stores of R, Rt and Dinv (no wavefront reads those during
+// the kernel).  __syncthreads() would also wait for every store to reach
+// memory.  The memory clobber keeps the compiler from moving LDS accesses
+// across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 __device__ __forceinline__ void upper_block(int q, int& bi, int& bj) {
   bj = 0;
@@ -95,6 +106,7 @@ struct CholShape {
 // constant (no address or index registers per block).
 template <int NBLK, int W>
 __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> G, int ldg,
+                                                bool gsym,
                                                 global_ptr<double> R, int ldr,
                                                 global_ptr<double> __restrict__ Rt, int ldt,
                                                 global_ptr<double> __restrict__ Dinv,
@@ -107,28 +119,38 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
   constexpr UpperBlocks<NBLK> tab{};
   const int lane = t & 63, lr = lane >> 4, lc = lane & 15;
   f64x4 acc[MAXB > 0 ? MAXB : 1];
+  // clamped (in-bounds) loads, all issued before any result is used (one
+  // wait); the entries outside G are zeroed afterwards.  A symmetric G is read at the
+  // mirror element, so the 16 lanes of a row read 128 contiguous bytes.
 #pragma unroll
   for (int u = 0; u < MAXB; ++u) {
     const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
-    // clamped (in-bounds) loads zeroed afterwards; issued four blocks at a
-    // time so the prologue does not hold an address per element in flight
+    // issued four blocks at a time, so that the addresses of all the loads
+    // in flight are not live at once
     if (u % 4 == 0) asm volatile("" ::: "memory");
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = bi * 16 + lr + 4 * r, j = bj * 16 + lc;
-      const double x = G[min(i, k - 1) + min(j, k - 1) * ldg];
-      acc[u][r] = (i < k && j < k) ? x : 0.0;
+      const int i = min(bi * 16 + lr + 4 * r, k - 1), j = min(bj * 16 + lc, k - 1);
+      acc[u][r] = gsym ? G[j + i * ldg] : G[i + j * ldg];
     }
   }
-  // zero R below the block diagonal (the diagonal blocks are written whole);
-  // G's lower blocks are never read, so R may alias G
+  // every G load has returned before B0, so after it R may be written where
+  // it aliases G
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (!(bi * 16 + lr + 4 * r < k && bj * 16 + lc < k)) acc[u][r] = 0.0;
+  }
+  __syncthreads();  // B0: G read into registers
+  // zero R below the block diagonal (the diagonal blocks are written whole)
   if (R) {
-    for (int e = t; e < k * k; e += BW * 64) {
-      const int i = e % k, j = e / k;
-      if ((i >> 4) > (j >> 4)) R[i + j * ldr] = 0.0;
-    }
+    const int wv = t >> 6, ln = t & 63;
+    for (int j = wv; j < k; j += BW)
+      for (int i = ((j >> 4) + 1) * 16 + ln; i < k; i += 64) R[i + j * ldr] = 0.0;
   }
-  __syncthreads();  // B0: G read into registers, R zeroed
   double* pl = &P[lr][lc];  // this lane's element of a 4 x 16 slab (rows NBLK*16+1 apart)
   constexpr int SR = NBLK * 16 + 1;
   // Barriers per step s: B1 (G_ss in LDS), B2 (R_ss and its inverse in LDS),
@@ -157,7 +179,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
     if (js >= k) break;
-    __syncthreads();  // B1: G_ss in LDS; the factor wavefront starts
+    lds_barrier();  // B1: G_ss in LDS; the factor wavefront starts
     if (s > 0) {  // the rest of step s-1's trailing update
 #pragma unroll
       for (int u = 0; u < MAXB; ++u) {
@@ -165,11 +187,16 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
         if (bi > s - 1 && !(bi == s && bj == s)) update(u, bi, bj);
       }
     }
-    __syncthreads();  // B2: factored by the last wavefront
+    lds_barrier();  // B2: factored by the last wavefront
     if (*fail) break;
     // panel row R_sj = inv(R_ss)' G_sj by its owners (MFMA); the
     // accumulator layout of G_sj is the operand layout the product needs.
-    // The owner of G_ss stores R_ss and inv(R_ss).
+    // The owner of G_ss stores R_ss and inv(R_ss).  The store offsets pass
+    // through empty asm: inside `bi == s` the compiler would otherwise know
+    // the step and hoist every block's addresses out of the loop (hundreds
+    // of registers, spilled).
+    int jso = js, ldro = ldr, ldto = ldt;
+    asm volatile("" : "+s"(jso), "+s"(ldro), "+s"(ldto));
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
       const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
@@ -178,10 +205,10 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = lr + 4 * r;
-          Dinv[s * 256 + i * 16 + c] = Dv[i][c];
+          Dinv[jso * 16 + i * 16 + c] = Dv[i][c];
           if (i < nb && c < nb) {
-            if (R) R[(js + i) + int64_t(js + c) * ldr] = Ds[i][c];
-            Rt[int64_t(js + i) * ldt + js + c] = Ds[i][c];
+            if (R) R[(jso + i) + int64_t(jso + c) * ldro] = Ds[i][c];
+            Rt[int64_t(jso + i) * ldto + jso + c] = Ds[i][c];
           }
         }
       }
@@ -196,13 +223,13 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
           const int i = lr + 4 * r;
           pl[4 * r * SR + bj * 16] = o[r];
           if (j < k) {
-            if (R) R[(js + i) + int64_t(j) * ldr] = o[r];
-            Rt[int64_t(js + i) * ldt + j] = o[r];
+            if (R) R[(jso + i) + int64_t(j) * ldro] = o[r];
+            Rt[int64_t(jso + i) * ldto + j] = o[r];
           }
         }
       }
     }
-    __syncthreads();  // B3: panel row s in LDS
+    lds_barrier();  // B3: panel row s in LDS
     // G_{s+1,s+1} first, to LDS for the next B1 (only diagonal blocks qualify)
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
@@ -218,40 +245,44 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
 
 template <int NBLK>
 __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds)[17],
-                                                 double (*Dv)[17], const double* fl, double* rdl,
-                                                 lds_ptr<volatile int> fail, int lane) {
-  __syncthreads();  // B0: G read into registers, R zeroed
+                                                 double (*Dv)[17], const double* fl,
+                                                 double* rowb, lds_ptr<volatile int> fail,
+                                                 int lane) {
+  __syncthreads();  // B0: G read into registers
+  // Lane (i, g) = (lane / 4, lane % 4) holds columns 4g .. 4g+3 of row i of
+  // the diagonal block G_ss (a) and of an identity beside it (x).  Gaussian
+  // elimination of [G_ss | I] with each pivot row scaled by 1/sqrt(pivot)
+  // leaves [R_ss | R_ss^-T]: the inverse comes out of the same 16 steps.
+  // Step j: the pivot is broadcast from its lane, the four lanes of row j
+  // scale it and publish it in LDS (rowb: 16 entries of R, 16 of R^-T; every
+  // lane reads the same addresses, a conflict-free broadcast), then rows
+  // i > j subtract R_ji times row j.  R_ic receives the same operations in
+  // the same order as a column-by-column right-looking factorisation.
+  const int i = lane >> 2, g = lane & 3;
   for (int s = 0; s < NBLK; ++s) {
     const int js = s * 16;
     if (js >= k) break;
     const int nb = min(16, k - js);
-    __syncthreads();  // B1: G_ss in LDS
-    // 2. factor in registers, lane c holding column c (lanes 16..63 repeat
-    //    lanes 0..15 and write nothing).  Step j: the pivot is lane j's entry
-    //    j; row j of R is entry j of every lane, passed by register broadcast
-    //    (the lanes of one wavefront need no LDS round trip).  Past nb the
-    //    block is padded with the identity, so the fully unrolled loop has no
-    //    data-dependent exits.
-    const int c = lane & 15;
-    double col[16];
-    if (nb == 16) {
+    lds_barrier();  // B1: G_ss in LDS
+    // past nb the block is padded with the identity, so the fully unrolled
+    // loop has no data-dependent exits
+    double a[4], x[4];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) col[i] = Ds[i][c];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const double g = Ds[i][c];
-        col[i] = (i < nb && c < nb) ? g : (i == c ? 1.0 : 0.0);
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * g + q;
+      const double gv = Ds[i][c];
+      a[q] = (i < nb && c < nb) ? gv : (i == c ? 1.0 : 0.0);
+      x[q] = i == c ? 1.0 : 0.0;
     }
-    // Entries below the diagonal (col[i], i > c) are updated like the others
-    // but never read: pivots and broadcasts use entries on or above it, and
-    // R_ss is written masked.
+    // the block's pivot floors, read before the serial loop
+    double flv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) flv[j] = fl[js + j];
     int bad = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      double dj = lane_bcast(col[j], j);
-      const double floor_j = j < nb ? fl[js + j] : 0.0;
+      double dj = lane_bcast(a[j & 3], 4 * j + (j >> 2));  // entry (j, j)
+      const double floor_j = j < nb ? flv[j] : 0.0;
       const bool ok = (dj > floor_j) && (dj > 0.0) && isfinite(dj);
       if (!bad && !ok && js + j < check_k) bad = js + j + 1;
       if (bad || !ok) dj = 1.0;  // unchecked columns past check_k continue
@@ -262,38 +293,47 @@ __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds
       rd = fma(0.5 * rd, fma(-dj * rd, rd, 1.0), rd);
       double d = dj * rd;
       d = fma(0.5 * rd, fma(-d, d, dj), d);
-      if (lane == j) rdl[j] = rd;  // 1 / R_jj
-      col[j] = lane == j ? d : col[j] * rd;
-      // R_ic -= R_ji R_jc for i > j (R_ji is lane i's entry j)
+      if (i == j) {
 #pragma unroll
-      for (int i = j + 1; i < 16; ++i) col[i] -= lane_bcast(col[j], i) * col[j];
+        for (int q = 0; q < 4; ++q) {
+          a[q] = 4 * g + q == j ? d : a[q] * rd;
+          x[q] *= rd;
+          rowb[4 * g + q] = a[q];
+          rowb[16 + 4 * g + q] = x[q];
+        }
+      }
+      lane_exchange();
+      const double rji = rowb[i];  // R_ji
+      double rj[4], xj[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        rj[q] = rowb[4 * g + q];
+        xj[q] = rowb[16 + 4 * g + q];
+      }
+      lane_exchange();  // the next step's writes follow these reads
+      if (i > j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] -= rji * rj[q];
+          x[q] -= rji * xj[q];
+        }
+      }
     }
     if (bad) {
       if (lane == 0) *fail = bad;
     } else {
-      // R_ss to LDS, then column c of inv(R_ss), right-looking:
-      // x_l = v_l / R_ll, then v_i -= R_il x_l for i < l (x_l = 0 for l > c
-      // without masking: v_l stays exactly 0 there)
-      if (lane < 16) {
+      // R_ss (zero below the diagonal) and inv(R_ss) = (R_ss^-T)', zero
+      // outside the nb x nb block
 #pragma unroll
-        for (int i = 0; i < 16; ++i) Ds[i][c] = i <= c ? col[i] : 0.0;
-      }
-      lane_exchange();
-      double v[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) v[i] = i == c ? 1.0 : 0.0;
-      const bool cin = c < nb;
-#pragma unroll
-      for (int l = 15; l >= 0; --l) {
-        const double xl = v[l] * rdl[l];
-        if (lane < 16) Dv[l][c] = (l < nb && cin) ? xl : 0.0;
-#pragma unroll
-        for (int i = 0; i < l; ++i) v[i] -= Ds[i][l] * xl;
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * g + q;
+        Ds[i][c] = i <= c ? a[q] : 0.0;
+        Dv[c][i] = (c <= i && i < nb) ? x[q] : 0.0;
       }
     }
-    __syncthreads();  // B2: done
+    lds_barrier();  // B2: done
     if (*fail) break;
-    __syncthreads();  // B3: panel in LDS
+    lds_barrier();  // B3: panel in LDS
   }
 }
 
@@ -301,6 +341,7 @@ __device__ __forceinline__ void chol_factor_role(int k, int check_k, double (*Ds
 // tested (an augmented column whose pivot is a residual may vanish).
 template <int NBLK>
 __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const double* G, int ldg,
+                                                 bool gsym,
                                                  double* R, int ldr, double* __restrict__ Rt,
                                                  int ldt, double* __restrict__ Dinv,
                                                  int* __restrict__ info) {
@@ -309,7 +350,7 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
   __shared__ double Ds[16][17];     // diagonal block G_ss
   __shared__ double Dv[16][17];     // inv(R_ss)
   __shared__ double fl[CP];         // pivot floors k * eps * G_jj
-  __shared__ double rdl[16];        // 1 / R_jj of the current diagonal block
+  __shared__ __attribute__((aligned(16))) double rowb[32];  // pivot row of the factor step
   __shared__ int fail;
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -322,13 +363,13 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
   // The roles run separate loops with the same barriers (a workgroup barrier
   // counts arrivals, not program points).
   if (wave == BW) {
-    chol_factor_role<NBLK>(k, check_k, Ds, Dv, fl, rdl, failp, t & 63);
+    chol_factor_role<NBLK>(k, check_k, Ds, Dv, fl, rowb, failp, t & 63);
   } else {
     switch (wave) {
 #define RLA_ROLE(W)                                                                        \
   case W:                                                                                  \
     if constexpr (W < BW)                                                                  \
-      chol_block_role<NBLK, W>(k, Gg, ldg, (global_ptr<double>)R, ldr,                  \
+      chol_block_role<NBLK, W>(k, Gg, ldg, gsym, (global_ptr<double>)R, ldr,                  \
                                (global_ptr<double>)Rt, ldt, (global_ptr<double>)Dinv, P, Ds, \
                                Dv, failp, t);                                             \
     break;
@@ -345,9 +386,9 @@ __device__ __forceinline__ void chol_blocks_body(int k, int check_k, const doubl
 
 template <int NBLK>
 __global__ __launch_bounds__(CholShape<NBLK>::threads) void chol_blocks_kernel(
-    int k, const double* G, int ldg, double* R, int ldr, double* __restrict__ Rt, int ldt,
-    double* __restrict__ Dinv, int* __restrict__ info) {
-  chol_blocks_body<NBLK>(k, k, G, ldg, R, ldr, Rt, ldt, Dinv, info);
+    int k, const double* G, int ldg, int gsym, double* R, int ldr, double* __restrict__ Rt,
+    int ldt, double* __restrict__ Dinv, int* __restrict__ info) {
+  chol_blocks_body<NBLK>(k, k, G, ldg, gsym != 0, R, ldr, Rt, ldt, Dinv, info);
 }
 
 // Batched form: workgroup p factors matrix p of a descriptor array.
@@ -355,21 +396,29 @@ template <int NBLK>
 __global__ __launch_bounds__(CholShape<NBLK>::threads) void chol_blocks_batched_kernel(
     const CholBatchItem* __restrict__ items) {
   const CholBatchItem& it = items[blockIdx.x];
-  chol_blocks_body<NBLK>(it.k, it.check_k, it.G, it.k, nullptr, 0, it.Rt, it.k, it.Dinv,
+  chol_blocks_body<NBLK>(it.k, it.check_k, it.G, it.k, false, nullptr, 0, it.Rt, it.k, it.Dinv,
                          it.info);
 }
 
+// Y <- Y R^-1 in place.  Each wavefront owns 16 rows and solves its column
+// blocks in order, Ynew_b = (Y_b - sum_{l<b} Ynew_l R_lb) inv(R_bb), keeping
+// the solved blocks in registers in the MFMA operand layout.  The panel of
+// block column b (-R_lb for l < b, and inv(R_bb)) is staged in LDS once per
+// workgroup; the next panel's loads are in flight while the current one is
+// used.  A panel block is stored row by row (element (r, c) at 16 r + c), so
+// the 64 lanes of an operand read touch 64 consecutive doubles.
 template <int NBLK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trsm_ru_f64_kernel(
-    int64_t m, int k, double* Y, int64_t ldy,
-                                                          const double* __restrict__ Rt, int ldt,
-                                                          const double* __restrict__ Dinv,
-                                                          const int* __restrict__ info) {
+    int64_t m, int k, double* Y, int64_t ldy, const double* __restrict__ Rt, int ldt,
+    const double* __restrict__ Dinv, const int* __restrict__ info) {
   if (info && *info) return;
+  __shared__ double Ps[2][NBLK][256];  // slot l < b: -R_lb; slot NBLK - 1: inv(R_bb)
   __shared__ double Ts[4][16][17];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int lr = lane >> 4, lc = lane & 15;
+  const int pr = t >> 4, pc = t & 15;  // panel element of this thread
   const int64_t rbase = int64_t(blockIdx.x) * 64 + wave * 16;
   const int64_t row = rbase + lc;  // operand layout: row lane%16, column 4q + lane/16
   const bool row_ok = row < m;
@@ -379,6 +428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   double (*ts)[17] = Ts[wave];
   double ya[NBLK][4];
   double yn[4];
+  double pf[NBLK];
   auto load_y = [&](int b) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -387,12 +437,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
       yn[q] = (row_ok && c < k) ? x : 0.0;
     }
   };
+  // panel b into registers: pf[l] = -R(16 l + pr, 16 b + pc), l < b (zero
+  // past column k), pf[NBLK - 1] = inv(R_bb)(pr, pc)
+  auto load_panel = [&](int b) {
+    const int c = b * 16 + pc;
+    const double* rc = Rt + (c < k ? c : k - 1);
+#pragma unroll
+    for (int l = 0; l < NBLK - 1; ++l)
+      if (l < b) {
+        const double x = rc[int64_t(l * 16 + pr) * ldt];
+        pf[l] = c < k ? -x : 0.0;
+      }
+    pf[NBLK - 1] = Dinv[b * 256 + t];
+  };
+  auto store_panel = [&](int buf, int b) {
+#pragma unroll
+    for (int l = 0; l < NBLK - 1; ++l)
+      if (l < b) Ps[buf][l][t] = pf[l];
+    Ps[buf][NBLK - 1][t] = pf[NBLK - 1];
+  };
   load_y(0);
+  load_panel(0);
+  store_panel(0, 0);
+  __syncthreads();
 #pragma unroll
   for (int b = 0; b < NBLK; ++b) {
     if (b * 16 >= k) break;
-    const int cb = b * 16 + lc;
-    const double* rcol = Rt + (cb < k ? cb : k - 1);
+    const int buf = b & 1;
+    const bool more = (b + 1) * 16 < k;
+    if (more) load_panel(b + 1);
     // Y_b into the accumulator layout (through LDS)
 #pragma unroll
     for (int q = 0; q < 4; ++q) ts[lc][4 * q + lr] = yn[q];
@@ -401,27 +474,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = ts[lr + 4 * r][lc];
     lane_exchange();
-    if ((b + 1) * 16 < k) load_y(b + 1);
-    double dv[4];
+    if (more) load_y(b + 1);
+    // element (4q + lr, lc) of panel block l: pb[256 l + 64 q]
+    const double* pb = &Ps[buf][0][0] + 16 * lr + lc;
+    // T = Y_b - sum_{l<b} Ynew_l R_lb
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dv[q] = Dinv[b * 256 + (4 * q + lr) * 16 + lc];
-    // T = Y_b - sum_{l<b} Ynew_l R_lb, R_lb read four row blocks at a time
-#pragma unroll
-    for (int l0 = 0; l0 < b; l0 += 4) {
-      double rv[4][4];
-#pragma unroll
-      for (int l = l0; l < l0 + 4 && l < b; ++l)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double x = rcol[int64_t(l * 16 + 4 * q + lr) * ldt];
-          rv[l - l0][q] = cb < k ? -x : 0.0;
-        }
-#pragma unroll
-      for (int l = l0; l < l0 + 4 && l < b; ++l)
+    for (int l = 0; l < NBLK - 1; ++l)
+      if (l < b) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[l][q], rv[l - l0][q], acc, 0, 0, 0);
-    }
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[l][q], pb[256 * l + 64 * q], acc, 0, 0,
+                                                     0);
+      }
     // Ynew_b = T inv(R_bb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) ts[lr + 4 * r][lc] = acc[r];
@@ -432,7 +496,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     lane_exchange();
     acc = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[q], dv[q], acc, 0, 0, 0);
+    for (int q = 0; q < 4; ++q)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[q], pb[256 * (NBLK - 1) + 64 * q], acc, 0, 0,
+                                                 0);
 #pragma unroll
     for (int r = 0; r < 4; ++r) ts[lr + 4 * r][lc] = acc[r];
     lane_exchange();
@@ -443,6 +509,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
       if (row_ok && c < k) Y[row + int64_t(c) * ldy] = ya[b][q];
     }
     lane_exchange();
+    if (more) {
+      // Ps[buf ^ 1] held panel b - 1: every wavefront finished with it before
+      // the previous iteration's barrier
+      store_panel(buf ^ 1, b + 1);
+      __syncthreads();  // panel b + 1 in LDS
+    }
   }
 }
 
@@ -457,16 +529,18 @@ bool cholqr_small_k(int64_t k) { return chol_nblk(k) != 0; }
 
 // R (column-major, ldr; may alias G), Rt (row-major, ldt) and Dinv (16 x 16
 // row-major inverse of each diagonal block, 256 * ceil(k/16) doubles).
+// g_symmetric: G holds both triangles (faster, coalesced reads); otherwise
+// only its upper triangle is read.
 int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R, int64_t ldr,
-                double* Rt, int64_t ldt, double* Dinv, int* info_dev) {
+                double* Rt, int64_t ldt, double* Dinv, int* info_dev, bool g_symmetric) {
   const int nb = chol_nblk(k);
   if (!nb) return fail(RLA_ENOTSUP, "chol_blocks: k must be in 1..288");
   const int ki = static_cast<int>(k);
 #define RLA_CHOL(NB)                                                                       \
   case NB:                                                                                 \
     chol_blocks_kernel<NB><<<1, CholShape<NB>::threads, 0, ctx->stream>>>(                           \
-        ki, G, static_cast<int>(ldg), R, static_cast<int>(ldr), Rt, static_cast<int>(ldt), \
-        Dinv, info_dev);                                                                   \
+        ki, G, static_cast<int>(ldg), g_symmetric ? 1 : 0, R, static_cast<int>(ldr), Rt,    \
+        static_cast<int>(ldt), Dinv, info_dev);                                           \
     break;
   switch (nb) {
     RLA_CHOL(2) RLA_CHOL(4) RLA_CHOL(8) RLA_CHOL(12) RLA_CHOL(16) RLA_CHOL(17) RLA_CHOL(18)
@@ -532,7 +606,7 @@ extern "C" int rla_chol_inv_f64(rla_ctx* ctx, int64_t k, const double* G, int64_
   int* info = static_cast<int*>(ws);
   double* Rt = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
   double* Dinv = reinterpret_cast<double*>(static_cast<char*>(ws) + 256 + rt_bytes);
-  rc = chol_blocks(ctx, k, G, ldg, R, ldr, Rt, k, Dinv, info);
+  rc = chol_blocks(ctx, k, G, ldg, R, ldr, Rt, k, Dinv, info, false);
   if (rc) return rc;
   if (X) {
     // X = I R^-1 (skipped on the device after a breakdown)

@@ -38,7 +38,7 @@ size_t gram_f64_part_bytes(int64_t m, int64_t k);
 int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, double* G,
              int64_t ldg, double* part, size_t part_bytes);
 int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R, int64_t ldr,
-                double* Rt, int64_t ldt, double* Dinv, int* info_dev);
+                double* Rt, int64_t ldt, double* Dinv, int* info_dev, bool g_symmetric);
 int trsm_ru_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, const double* Rt,
                 int64_t ldt, const double* Dinv, const int* info);
 
@@ -195,7 +195,8 @@ static int cholqr_pass(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ld
   if (cholqr_small_k(k)) {
     int rc = gram_f64(ctx, m, k, Y, ldy, q.G, k, q.part, q.part_bytes);
     if (rc) return rc;
-    rc = chol_blocks(ctx, k, q.G, k, q.G, k, q.X, k, q.Dinv, info);
+    // gram_f64 writes both triangles of G
+    rc = chol_blocks(ctx, k, q.G, k, q.G, k, q.X, k, q.Dinv, info, true);
     if (rc) return rc;
     return trsm_ru_f64(ctx, m, k, Y, ldy, q.X, k, q.Dinv, info);
   }

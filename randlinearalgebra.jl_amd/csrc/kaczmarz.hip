@@ -308,6 +308,91 @@ __device__ __forceinline__ void kz_gram_tiles(const float (&op)[4][16], kz_f64x4
   }
 }
 
+// LDL' of the (padded) Gram matrix in Gs by one wavefront, lane i holding
+// row i: unit lower L left-looking over 16-column block columns (the update
+// of a block column from the factored ones on the MFMA, then a right-looking
+// factor of its 16 columns).  A pivot d_j not above the floor of its row
+// (fl, lane j) is a row dependent on the rows before it: its column of L and
+// 1/d_j are zero, so solves project onto the independent rows.  With RHS,
+// z (lane i: entry i) is solved forward with unit L on the way.  On return Gs
+// holds L strictly below the diagonal, Dv the pivots; lane i has 1/d_i (0
+// for a dependent pivot) in invme and isfinite(d_i) in okme.
+template <bool RHS>
+__device__ __forceinline__ void kz_ldl(double* Gs, double* Dv, int nb, int lane, double fl,
+                                       double& z, double& invme, bool& okme) {
+  const int i = lane;
+  double dme = 1.0;
+  // Only entries strictly below the diagonal of L are read back (by the
+  // MFMA update and the back-solve); lanes above a pivot compute values
+  // that nothing reads.
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    if (kb < nb) {
+      const int r0 = 16 * kb;
+      // G(r0:, r0:r0+15) -= L(r0:, 0:r0-1) D L(r0:r0+15, 0:r0-1)'
+#pragma unroll
+      for (int I = kb; I < 4; ++I) {
+        if (kb > 0 && I < nb) {
+          kz_f64x4 acc4;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            acc4[r] = Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)];
+#pragma unroll
+          for (int k0 = 0; k0 < r0; k0 += 4) {
+            const int kk = k0 + (lane >> 4);
+            const double av_ = -Gs[(16 * I + (lane & 15)) * kKzGs + kk] * Dv[kk];
+            const double bv_ = Gs[(r0 + (lane & 15)) * kKzGs + kk];
+            acc4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av_, bv_, acc4, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)] = acc4[r];
+        }
+      }
+      kz_lane_order();
+      double a[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) a[c] = Gs[i * kKzGs + r0 + c];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int jj = r0 + j;
+        const double d = kz_bcast(a[j], jj);     // pivot d_jj
+        const double flj = kz_bcast(fl, jj);     // its floor
+        // A pivot not above its floor is a row dependent on the rows before
+        // it (a row drawn twice, or a combination of others): its column of
+        // L and its coefficient are zero, so the projection is onto the span
+        // of the independent rows (exact for a consistent block).
+        const bool dep = !(d > flj);
+        // 1/d: reciprocal estimate refined by two Newton steps e = 1 - d r, r += r e
+        double inv = __builtin_amdgcn_rcp(d);
+        inv = fma(inv, fma(-d, inv, 1.0), inv);
+        inv = fma(inv, fma(-d, inv, 1.0), inv);
+        if (dep) inv = 0.0;
+        if (i == jj) {
+          okme = isfinite(d);
+          dme = d;
+          invme = inv;
+        }
+        const double u = dep ? 0.0 : a[j];  // L_ij d_j
+        const double l = u * inv;
+        a[j] = l;
+        // forward solve with unit L, right-looking
+        if constexpr (RHS) {
+          const double zj = kz_bcast(z, jj);
+          if (i > jj) z = fma(-l, zj, z);
+        }
+        // trailing columns of this block column: a_ic -= L_ij d_j L_cj
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c) a[c] = fma(-u, kz_bcast(l, r0 + c), a[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) Gs[i * kKzGs + r0 + c] = a[c];
+      if (i >= r0 && i < r0 + 16) Dv[i] = dme;
+      kz_lane_order();
+    }
+  }
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
   __shared__ __align__(16) float T[kKzMaxBlock * kKzTs];
@@ -540,75 +625,9 @@ __global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
   }
   const double fl = 4.0 * s * 2.220446049250313e-16 * gii;  // pivot floor of row i
   double z = rv[i];
-  double dme = 1.0, invme = 1.0;  // lane j: d_j and 1 / d_j
-  bool okme = true;               // lane j: pivot j above its floor
-  // Only entries strictly below the diagonal of L are read back (by the
-  // MFMA update and the back-solve); lanes above a pivot compute values
-  // that nothing reads.
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    if (kb < nb) {
-      const int r0 = 16 * kb;
-      // G(r0:, r0:r0+15) -= L(r0:, 0:r0-1) D L(r0:r0+15, 0:r0-1)'
-#pragma unroll
-      for (int I = kb; I < 4; ++I) {
-        if (kb > 0 && I < nb) {
-          kz_f64x4 acc4;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            acc4[r] = Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)];
-#pragma unroll
-          for (int k0 = 0; k0 < r0; k0 += 4) {
-            const int kk = k0 + (lane >> 4);
-            const double av_ = -Gs[(16 * I + (lane & 15)) * kKzGs + kk] * Dv[kk];
-            const double bv_ = Gs[(r0 + (lane & 15)) * kKzGs + kk];
-            acc4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av_, bv_, acc4, 0, 0, 0);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            Gs[(16 * I + (lane >> 4) + 4 * r) * kKzGs + r0 + (lane & 15)] = acc4[r];
-        }
-      }
-      kz_lane_order();
-      double a[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) a[c] = Gs[i * kKzGs + r0 + c];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int jj = r0 + j;
-        const double d = kz_bcast(a[j], jj);     // pivot d_jj
-        const double flj = kz_bcast(fl, jj);     // its floor
-        // A pivot not above its floor is a row dependent on the rows before
-        // it (a row drawn twice, or a combination of others): its column of
-        // L and its coefficient are zero, so the projection is onto the span
-        // of the independent rows (exact for a consistent block).
-        const bool dep = !(d > flj);
-        // 1/d: reciprocal estimate refined by two Newton steps e = 1 - d r, r += r e
-        double inv = __builtin_amdgcn_rcp(d);
-        inv = fma(inv, fma(-d, inv, 1.0), inv);
-        inv = fma(inv, fma(-d, inv, 1.0), inv);
-        if (dep) inv = 0.0;
-        if (i == jj) {
-          okme = isfinite(d);
-          dme = d;
-          invme = inv;
-        }
-        const double u = dep ? 0.0 : a[j];  // L_ij d_j
-        const double l = u * inv;
-        a[j] = l;
-        // forward solve with unit L, right-looking
-        const double zj = kz_bcast(z, jj);
-        if (i > jj) z = fma(-l, zj, z);
-        // trailing columns of this block column: a_ic -= L_ij d_j L_cj
-#pragma unroll
-        for (int c = j + 1; c < 16; ++c) a[c] = fma(-u, kz_bcast(l, r0 + c), a[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) Gs[i * kKzGs + r0 + c] = a[c];
-      if (i >= r0 && i < r0 + 16) Dv[i] = dme;
-      kz_lane_order();
-    }
-  }
+  double invme = 1.0;  // lane j: 1 / d_j
+  bool okme = true;    // lane j: pivot j finite
+  kz_ldl<true>(Gs, Dv, nb, lane, fl, z, invme, okme);
   const unsigned long long badm = __ballot(!okme && i < s);
   if (badm) {  // non-finite block: skipped, and recorded (info is sticky)
     if (i < s) p.y_out[i] = 0.0;
@@ -633,6 +652,375 @@ __global__ __launch_bounds__(256) void kz_step_kernel(KzStep p) {
   }
   if (i < s) p.y_out[i] = z;
   if (i == 0) *p.ticket = 0u;
+}
+
+// ---- Kaczmarz over a chunk of pre-drawn row sets ------------------------------
+//
+// The row sets of a residual interval are drawn before its steps, and a
+// block's Gram matrix does not depend on x.  So kz_prep_kernel forms, for
+// every set of the chunk at once (one workgroup each), G = A~ A~' (FP64 MFMA
+// on FP32 entries: exact products), its LDL' with the dependent-pivot rule of
+// kz_ldl and the explicit M = L^-T D^+ L^-1, the operator with y = M r exactly
+// the coefficients kz_step_kernel's solve gives.  kz_run_kernel then runs
+// the chunk's steps in one workgroup: per step the 64 rows are gathered,
+// r = b~ - A~ x (FP64 sums of exact products), y = M r and
+// x += alpha A~' y, with nothing exchanged between workgroups.
+
+struct KzPrep {
+  const float* A;   // row-major rows (row i at A + i * ld)
+  int64_t ld, n;
+  const int* sets;  // cnt x s row indices
+  int s;
+  double* M;        // cnt x 64 x 64 (rows and columns past s zero)
+  int* info;        // sticky: 1-based failing pivot of a non-finite block
+};
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void kz_prep_kernel(KzPrep p) {
+  __shared__ __align__(16) float T[kKzMaxBlock * kKzTs];
+  __shared__ double Gs[kKzMaxBlock * kKzGs];
+  __shared__ double Dv[kKzMaxBlock];
+  __shared__ double Iv[kKzMaxBlock];
+  __shared__ int64_t crow[kKzMaxBlock];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int s = p.s, nb = (s + 15) >> 4;
+  const int* idx = p.sets + int64_t(blockIdx.x) * s;
+  if (t < kKzMaxBlock) crow[t] = int64_t(idx[min(t, s - 1)]) * p.ld;
+  __syncthreads();
+  kz_f64x4 acc[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) acc[q] = kz_f64x4{0.0, 0.0, 0.0, 0.0};
+  const int64_t nchunks = (p.n + kKzCols - 1) / kKzCols;
+  // rows past s repeat row s - 1 and columns past n are clamped (in bounds);
+  // both are zeroed when staged
+  auto load = [&](int64_t ch, float(&lc)[16]) {
+    const int64_t c0 = ch * kKzCols;
+    if constexpr (VEC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = t + 256 * k, row = e >> 4;
+        const int64_t c = min<int64_t>(c0 + 4 * (e & 15), p.n - 4);
+        const float4 w = *reinterpret_cast<const float4*>(p.A + crow[row] + c);
+        lc[4 * k] = w.x, lc[4 * k + 1] = w.y, lc[4 * k + 2] = w.z, lc[4 * k + 3] = w.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, row = e >> 6;
+        lc[k] = p.A[crow[row] + c0 + min<int64_t>(e & 63, p.n - c0 - 1)];
+      }
+    }
+  };
+  float buf[2][16];
+  load(0, buf[0]);
+  for (int64_t ch = 0; ch < nchunks; ++ch) {
+    float(&lc)[16] = buf[ch & 1];
+    if (ch + 1 < nchunks) load(ch + 1, buf[(ch + 1) & 1]);
+    const int64_t c0 = ch * kKzCols;
+    const int ncol = static_cast<int>(min<int64_t>(kKzCols, p.n - c0));
+    if constexpr (VEC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = t + 256 * k, row = e >> 4, cb = 4 * (e & 15);
+        const bool rok = row < s;
+        *reinterpret_cast<float4*>(&T[row * kKzTs + cb]) =
+            float4{rok && cb < ncol ? lc[4 * k] : 0.f, rok && cb + 1 < ncol ? lc[4 * k + 1] : 0.f,
+                   rok && cb + 2 < ncol ? lc[4 * k + 2] : 0.f,
+                   rok && cb + 3 < ncol ? lc[4 * k + 3] : 0.f};
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int e = t + 256 * k, row = e >> 6, cc = e & 63;
+        T[row * kKzTs + cc] = (row < s && cc < ncol) ? lc[k] : 0.f;
+      }
+    }
+    __syncthreads();
+    float op[4][16];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int kq = 0; kq < 16; ++kq)
+        op[rb][kq] = T[(16 * rb + (lane & 15)) * kKzTs + 4 * kq + (lane >> 4)];
+    switch (wv) {
+      case 0: kz_gram_tiles<0>(op, acc); break;
+      case 1: kz_gram_tiles<1>(op, acc); break;
+      case 2: kz_gram_tiles<2>(op, acc); break;
+      default: kz_gram_tiles<3>(op, acc); break;
+    }
+    __syncthreads();  // T is rewritten by the next chunk
+  }
+  // tiles b = wv + 4q of the Gram matrix; accumulator element r of lane l is
+  // row l/16 + 4r, column l%16 of the tile
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int b = wv + 4 * q;
+    if (b >= kKzTiles) break;
+    const int I = kz_ti(b), J = kz_tj(b);
+    if (J >= nb) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * I + (lane >> 4) + 4 * r, col = 16 * J + (lane & 15);
+      Gs[row * kKzGs + col] = acc[q][r];
+      if (I != J) Gs[col * kKzGs + row] = acc[q][r];
+    }
+  }
+  __syncthreads();
+  if (wv != 0) return;
+
+  // ---- one wavefront: the factor, then column `lane` of M ----
+  const int i = lane;
+  const int np = 16 * nb;
+  double* Mo = p.M + int64_t(blockIdx.x) * (kKzMaxBlock * kKzMaxBlock);
+  if (i >= s && i < np) Gs[i * kKzGs + i] = 1.0;
+  const double gii = i < s ? Gs[i * kKzGs + i] : 0.0;
+  bool zero = __ballot(gii != 0.0) == 0;  // A~ == 0: no update (kaczmarz.jl:300)
+  if (!zero) {
+    const double fl = 4.0 * s * 2.220446049250313e-16 * gii;
+    double z = 0.0, invme = 1.0;
+    bool okme = true;
+    kz_ldl<false>(Gs, Dv, nb, lane, fl, z, invme, okme);
+    const unsigned long long badm = __ballot(!okme && i < s);
+    if (badm) {  // non-finite block: skipped, and recorded (info is sticky)
+      if (i == 0) *p.info = __ffsll(badm);
+      zero = true;
+    } else {
+      Iv[i] = invme;
+      kz_lane_order();
+    }
+  }
+  if (zero) {
+#pragma unroll 8
+    for (int r = 0; r < kKzMaxBlock; ++r) Mo[r * kKzMaxBlock + lane] = 0.0;
+    return;
+  }
+  // v = L^-1 e_c (unit lower, forward), v <- D^+ v, v = L^-T v (backward);
+  // L_ij is read by every lane from the same LDS word (a broadcast)
+  double v[kKzMaxBlock];
+#pragma unroll
+  for (int j = 0; j < kKzMaxBlock; ++j) v[j] = j == lane ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = 0; j < kKzMaxBlock; ++j) {
+    if (j < np) {
+#pragma unroll
+      for (int r = j + 1; r < kKzMaxBlock; ++r)
+        if (r < np) v[r] = fma(-Gs[r * kKzGs + j], v[j], v[r]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kKzMaxBlock; ++j) v[j] *= j < np ? Iv[j] : 0.0;
+#pragma unroll
+  for (int j = kKzMaxBlock - 1; j >= 0; --j) {
+    if (j < np) {
+#pragma unroll
+      for (int r = 0; r < j; ++r) v[r] = fma(-Gs[j * kKzGs + r], v[j], v[r]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kKzMaxBlock; ++r)
+    Mo[r * kKzMaxBlock + lane] = (r < s && lane < s) ? v[r] : 0.0;
+}
+
+// Swap lanes 32..63 of a with lanes 0..31 of b (v_permlane32_swap), or lanes
+// 16..31 / 48..63 of a with lanes 0..15 / 32..47 of b (v_permlane16_swap),
+// for both halves of a double.  The builtins let the compiler insert the
+// wait states these instructions need after a VALU write of their operands.
+template <int ROW>
+__device__ __forceinline__ void kz_swap(double& a, double& b) {
+  const unsigned al = __double2loint(a), ah = __double2hiint(a);
+  const unsigned bl = __double2loint(b), bh = __double2hiint(b);
+  if constexpr (ROW == 32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+    a = __hiloint2double(int(hi[0]), int(lo[0]));
+    b = __hiloint2double(int(hi[1]), int(lo[1]));
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+    a = __hiloint2double(int(hi[0]), int(lo[0]));
+    b = __hiloint2double(int(hi[1]), int(lo[1]));
+  }
+}
+
+template <int CTRL>
+__device__ __forceinline__ double kz_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One DPP stage of the transpose-reduce: values q and q + H are paired; lane l
+// keeps the one selected by its bit SEL and adds the partner lane's copy
+// (partner = l ^ mask of the DPP pattern, which flips bit SEL).
+template <int H, int SEL, int CTRL>
+__device__ __forceinline__ void kz_tr_stage(double (&v)[16], int lane) {
+  const bool hi = (lane >> SEL) & 1;
+#pragma unroll
+  for (int q = 0; q < H; ++q) {
+    const double keep = hi ? v[q + H] : v[q];
+    const double send = hi ? v[q] : v[q + H];
+    v[q] = keep + kz_dpp<CTRL>(send);
+  }
+}
+
+struct KzRun {
+  const float* A;   // row-major rows
+  int64_t ld, n;
+  const float* b;
+  float* x;
+  float alpha;
+  const int* sets;  // cnt x s
+  int s, cnt;
+  const double* M;  // cnt x 64 x 64
+};
+
+constexpr int kKzRunThreads = 1024;
+constexpr int kKzRunCols = 1024;  // columns one run workgroup holds
+
+// The steps of one chunk in one workgroup (n <= 1024).  Thread t takes rows
+// 16 g .. 16 g + 15 (g = t / 256) of the block and columns 4 c .. 4 c + 3
+// (c = t % 256): one 16-byte load per row.  A~x: per row the lane's four
+// products (exact in FP64), summed over the wavefront by a transpose-reduce
+// that leaves row l / 4 of the group in lane l, then over the group's four
+// wavefronts in a fixed order.  x += alpha A~' y: per column the lane's
+// sixteen rows, then the four groups' partial sums in a fixed order; every
+// group applies the same update to its own copy of x (identical bits).
+template <bool VEC>
+__global__ __launch_bounds__(kKzRunThreads) void kz_run_kernel(KzRun p) {
+  constexpr int NW = kKzRunThreads / 64;
+  __shared__ int64_t roff[2][kKzMaxBlock];
+  __shared__ double bts[2][kKzMaxBlock];
+  __shared__ double red[NW][16];
+  __shared__ double ypart[NW][kKzMaxBlock];
+  __shared__ double rr[kKzMaxBlock];
+  __shared__ double yy[kKzMaxBlock];
+  __shared__ double upd[4][kKzRunCols];
+  __shared__ double Ms[2][kKzMaxBlock * kKzMaxBlock];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int g = wv >> 2;  // row group
+  const int s = p.s;
+  const int64_t n = p.n;
+  const int c0 = 4 * (t & 255);  // first column of this thread
+  bool okc[4];
+  float xq[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    okc[e] = c0 + e < n;
+    xq[e] = okc[e] ? p.x[c0 + e] : 0.f;
+  }
+  // clamped columns (in bounds; x = 0 there and never written)
+  const int64_t cq = VEC ? min<int64_t>(c0, n - 4) : 0;
+  auto fetch = [&](int it, int bf) {
+    if (t < kKzMaxBlock) {
+      const int r = p.sets[int64_t(it) * s + min(t, s - 1)];
+      roff[bf][t] = int64_t(r) * p.ld;
+      bts[bf][t] = t < s ? static_cast<double>(p.b[r]) : 0.0;
+    }
+    const double* Mi = p.M + int64_t(it) * (kKzMaxBlock * kKzMaxBlock);
+#pragma unroll
+    for (int k = 0; k < kKzMaxBlock * kKzMaxBlock / kKzRunThreads; ++k)
+      Ms[bf][t + kKzRunThreads * k] = Mi[t + kKzRunThreads * k];
+  };
+  fetch(0, 0);
+  __syncthreads();
+  for (int it = 0; it < p.cnt; ++it) {
+    const int bf = it & 1;
+    // rows 16 g + j of the block; rows past s repeat row s - 1 (their
+    // residual entries are set to zero and M's rows past s are zero)
+    float a[16][4];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int64_t ro = roff[bf][16 * g + j];  // the same in every lane
+      const int64_t rou = (int64_t(__builtin_amdgcn_readfirstlane(int(ro >> 32))) << 32) |
+                          uint32_t(__builtin_amdgcn_readfirstlane(int(ro)));
+      const float* row = p.A + rou;
+      if constexpr (VEC) {
+        const float4 v = *reinterpret_cast<const float4*>(row + cq);
+        a[j][0] = v.x, a[j][1] = v.y, a[j][2] = v.z, a[j][3] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[j][e] = row[min<int64_t>(c0 + e, n - 1)];
+      }
+    }
+    // the other buffer was last read before the previous step's barriers
+    if (it + 1 < p.cnt) fetch(it + 1, bf ^ 1);
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      double d = 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) d = fma(static_cast<double>(a[j][e]), static_cast<double>(xq[e]), d);
+      v[j] = d;
+    }
+    // the FP64 copies of the block are not kept for the update (that would
+    // hold 128 registers); it converts the floats again
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) asm volatile("" : "+v"(a[j][e]));
+    // transpose-reduce: row bits 3, 2 by lane bits 5, 4 (permlane swaps),
+    // row bits 1, 0 by lane bits 3, 2 (DPP), then lane bits 1, 0 summed
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      kz_swap<32>(v[j], v[j + 8]);
+      v[j] = v[j] + v[j + 8];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      kz_swap<16>(v[j], v[j + 4]);
+      v[j] = v[j] + v[j + 4];
+    }
+    kz_tr_stage<2, 3, 0x128>(v, lane);  // row_ror:8, partner lane ^ 8
+    kz_tr_stage<1, 2, 0x141>(v, lane);  // row_half_mirror, partner lane ^ 7
+    v[0] = v[0] + kz_dpp<0x4E>(v[0]);   // quad_perm [2,3,0,1], lane ^ 2
+    v[0] = v[0] + kz_dpp<0xB1>(v[0]);   // quad_perm [1,0,3,2], lane ^ 1
+    if ((lane & 3) == 0) red[wv][lane >> 2] = v[0];
+    __syncthreads();
+    if (t < kKzMaxBlock) {
+      const int w0 = 4 * (t >> 4), j = t & 15;
+      const double ax = ((red[w0][j] + red[w0 + 1][j]) + red[w0 + 2][j]) + red[w0 + 3][j];
+      rr[t] = t < s ? bts[bf][t] - ax : 0.0;
+    }
+    __syncthreads();
+    // y = M r (M symmetric): thread t adds 4 terms of entry t % 64
+    {
+      const int q = t & 63, j0 = (t >> 6) * (kKzMaxBlock / NW);
+      double y = 0.0;
+#pragma unroll
+      for (int j = 0; j < kKzMaxBlock / NW; ++j)
+        y = fma(Ms[bf][(j0 + j) * kKzMaxBlock + q], rr[j0 + j], y);
+      ypart[t >> 6][q] = y;
+    }
+    __syncthreads();
+    if (t < kKzMaxBlock) {
+      double y = ypart[0][t];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) y += ypart[w][t];
+      yy[t] = y;
+    }
+    __syncthreads();
+    // x += alpha A~' y (kaczmarz.jl:308-309)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double u = 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u = fma(static_cast<double>(a[j][e]), yy[16 * g + j], u);
+      upd[g][c0 + e] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double u = ((upd[0][c0 + e] + upd[1][c0 + e]) + upd[2][c0 + e]) + upd[3][c0 + e];
+      if (okc[e]) xq[e] = p.alpha * static_cast<float>(u) + xq[e];
+    }
+  }
+  if (g == 0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (okc[e]) p.x[c0 + e] = xq[e];
+  }
 }
 
 __global__ void set_int_kernel(int* p, int v) {
@@ -839,7 +1227,12 @@ struct KzBufs {
   double* resv;       // residual norm
   double *k1, *k2, *cdf;
   int *i1, *i2;
+  double* M;          // kz_prep_kernel output: up to kKzPrepMax x 64 x 64
+  int* sets2;         // second row-set buffer (prepared interval ahead)
+  double* M2;         // second operator buffer
 };
+
+constexpr int64_t kKzPrepMax = 256;  // row sets prepared per kz_prep / kz_run pair
 
 // workgroups of a step over n columns, and the 64-column chunks each sweeps
 static void kz_grid(int64_t n, int* wgs, int* chunks_per_wg) {
@@ -864,6 +1257,13 @@ static size_t kz_layout(int64_t m, int64_t n, int64_t s, int64_t every, void* wo
   k.res = b.take<double>(std::max<int64_t>(residual_parts(m), ceil_div(m, 256)));
   k.resv = b.take<double>(1);
   sample_scratch(m, static_cast<int>(s), every, b, &k.k1, &k.i1, &k.k2, &k.i2, &k.cdf);
+  if (n <= kKzRunCols) {
+    k.M = b.take<double>(std::min(every, kKzPrepMax) * kKzMaxBlock * kKzMaxBlock);
+    if (every <= kKzPrepMax) {
+      k.sets2 = b.take<int>(every * s);
+      k.M2 = b.take<double>(every * kKzMaxBlock * kKzMaxBlock);
+    }
+  }
   if (out) *out = k;
   return b.need;
 }
@@ -896,6 +1296,183 @@ static int kz_step(rla_ctx* ctx, const KzBufs& k, int64_t n, const RowSrc& prev,
   else
     kz_step_kernel<false><<<dim3(unsigned(W)), 256, 0, ctx->stream>>>(st);
   RLA_LAUNCH_CHECK("kz_step");
+  return RLA_OK;
+}
+
+// kz_solve_chunked with the next interval's draws and operators prepared on
+// a second stream while the current interval runs (intervals of at most
+// kKzPrepMax steps): the draws and kz_prep_kernel depend only on the counter
+// stream and A, and kz_run_kernel occupies one CU.  Two buffer pairs
+// alternate; a buffer is refilled only after the run that read it.
+static int kz_solve_overlapped(rla_ctx* ctx, KzBufs& k, int64_t m, int64_t n, const float* A_rm,
+                               int64_t ld_rm, const float* b, float* x, int s, float alpha,
+                               const double* w, int replace, uint64_t seed, uint64_t epoch0,
+                               int64_t max_it, int64_t every, double threshold, double* hist,
+                               int64_t hist_cap, int64_t* n_hist, int64_t* iterations,
+                               int* converged) {
+  struct Side {
+    hipStream_t st = nullptr;
+    hipEvent_t prep[2] = {nullptr, nullptr}, run[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    ~Side() {
+      if (st) (void)hipStreamSynchronize(st);
+      for (int q = 0; q < 2; ++q) {
+        if (prep[q]) (void)hipEventDestroy(prep[q]);
+        if (run[q]) (void)hipEventDestroy(run[q]);
+      }
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } side;
+  RLA_HIP(hipStreamCreateWithFlags(&side.st, hipStreamNonBlocking));
+  for (int q = 0; q < 2; ++q) {
+    RLA_HIP(hipEventCreateWithFlags(&side.prep[q], hipEventDisableTiming));
+    RLA_HIP(hipEventCreateWithFlags(&side.run[q], hipEventDisableTiming));
+  }
+  int* sets[2] = {k.sets, k.sets2};
+  double* Ms[2] = {k.M, k.M2};
+  const bool vec = n % 4 == 0 && ld_rm % 4 == 0 && (reinterpret_cast<uintptr_t>(A_rm) & 15) == 0;
+  bool cdf_ready = false;
+  set_int_kernel<<<1, 1, 0, ctx->stream>>>(k.info, 0);
+  RLA_LAUNCH_CHECK("set_int");
+  // the side stream starts after the work already queued (x, info)
+  RLA_HIP(hipEventRecord(side.run[1], ctx->stream));
+  RLA_HIP(hipStreamWaitEvent(side.st, side.run[1], 0));
+  // row sets of iterations ic .. ic + every - 1 and the operators of the
+  // first cnt of them, into buffer q, on the side stream
+  auto prepare = [&](int64_t ic, int64_t cnt, int q) -> int {
+    if (side.used[q]) RLA_HIP(hipStreamWaitEvent(side.st, side.run[q], 0));
+    hipStream_t main_st = ctx->stream;
+    ctx->stream = side.st;
+    int r = draw_sets(ctx, seed, epoch0 + uint64_t(ic - 1), m, w, s, replace, every, sets[q],
+                      k.k1, k.i1, k.k2, k.i2, k.cdf, cdf_ready);
+    ctx->stream = main_st;
+    if (r) return r;
+    cdf_ready = true;
+    KzPrep pp{A_rm, ld_rm, n, sets[q], s, Ms[q], k.info};
+    if (vec)
+      kz_prep_kernel<true><<<dim3(unsigned(cnt)), 256, 0, side.st>>>(pp);
+    else
+      kz_prep_kernel<false><<<dim3(unsigned(cnt)), 256, 0, side.st>>>(pp);
+    RLA_LAUNCH_CHECK("kz_prep");
+    RLA_HIP(hipEventRecord(side.prep[q], side.st));
+    return RLA_OK;
+  };
+  int64_t nh = 0, it = 0;
+  int conv = 0;
+  int rc;
+  int q = 0;
+  if (max_it > 1) {
+    rc = prepare(1, std::min(every, max_it - 1), 0);
+    if (rc) return rc;
+  }
+  for (int64_t i = 1; max_it >= 1;) {
+    it = i;
+    rc = residual(ctx, m, n, A_rm, ld_rm, 1, x, b, k.res, k.resv);
+    if (rc) return rc;
+    double err = 0.0;
+    int info = 0;
+    RLA_HIP(hipMemcpyAsync(&err, k.resv, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RLA_HIP(hipStreamSynchronize(ctx->stream));
+    if (info > 0)
+      return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
+    if (hist && nh < hist_cap) hist[nh] = err;
+    ++nh;
+    conv = (i < max_it) ? (err < threshold) : 1;  // update_logger!, basic_logger.jl:93-98
+    if (conv) break;
+    const int64_t cnt = std::min(every, max_it - i);
+    // the run is queued first, so that its one workgroup has a CU before the
+    // next interval's draws fill the device
+    RLA_HIP(hipStreamWaitEvent(ctx->stream, side.prep[q], 0));
+    KzRun pr{A_rm, ld_rm, n, b, x, alpha, sets[q], s, static_cast<int>(cnt), Ms[q]};
+    if (vec)
+      kz_run_kernel<true><<<1, kKzRunThreads, 0, ctx->stream>>>(pr);
+    else
+      kz_run_kernel<false><<<1, kKzRunThreads, 0, ctx->stream>>>(pr);
+    RLA_LAUNCH_CHECK("kz_run");
+    RLA_HIP(hipEventRecord(side.run[q], ctx->stream));
+    side.used[q] = true;
+    if (i + cnt < max_it) {  // the next interval, prepared while this one runs
+      rc = prepare(i + cnt, std::min(every, max_it - i - cnt), q ^ 1);
+      if (rc) return rc;
+    }
+    q ^= 1;
+    i += cnt;
+  }
+  RLA_HIP(hipStreamSynchronize(side.st));
+  int info = 0;
+  RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
+  if (n_hist) *n_hist = nh;
+  if (iterations) *iterations = it;
+  if (converged) *converged = conv;
+  return RLA_OK;
+}
+
+// rla_kaczmarz_solve_f32 for n <= 1024: per residual interval, the row sets
+// are drawn, kz_prep_kernel forms the operators M of up to kKzPrepMax sets at
+// once and kz_run_kernel applies their steps in one workgroup.  The residual
+// checks, draws and iteration count are those of the per-step loop.
+static int kz_solve_chunked(rla_ctx* ctx, KzBufs& k, int64_t m, int64_t n, const float* A_rm,
+                            int64_t ld_rm, const float* b, float* x, int s, float alpha,
+                            const double* w, int replace, uint64_t seed, uint64_t epoch0,
+                            int64_t max_it, int64_t every, double threshold, double* hist,
+                            int64_t hist_cap, int64_t* n_hist, int64_t* iterations,
+                            int* converged) {
+  set_int_kernel<<<1, 1, 0, ctx->stream>>>(k.info, 0);
+  RLA_LAUNCH_CHECK("set_int");
+  const bool vec = n % 4 == 0 && ld_rm % 4 == 0 && (reinterpret_cast<uintptr_t>(A_rm) & 15) == 0;
+  int64_t nh = 0, it = 0;
+  int conv = 0;
+  bool cdf_ready = false;
+  int rc;
+  for (int64_t i = 1; max_it >= 1;) {
+    // error and logger at i (i = 1, 1 + every, ... and max_it)
+    it = i;
+    rc = residual(ctx, m, n, A_rm, ld_rm, 1, x, b, k.res, k.resv);
+    if (rc) return rc;
+    double err = 0.0;
+    int info = 0;
+    RLA_HIP(hipMemcpyAsync(&err, k.resv, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    RLA_HIP(hipStreamSynchronize(ctx->stream));
+    if (info > 0)
+      return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
+    if (hist && nh < hist_cap) hist[nh] = err;
+    ++nh;
+    conv = (i < max_it) ? (err < threshold) : 1;  // update_logger!, basic_logger.jl:93-98
+    if (conv) break;
+    // row sets of iterations i .. i + every - 1 (epochs epoch0 + i - 1 ...)
+    rc = draw_sets(ctx, seed, epoch0 + uint64_t(i - 1), m, w, s, replace, every, k.sets, k.k1,
+                   k.i1, k.k2, k.i2, k.cdf, cdf_ready);
+    if (rc) return rc;
+    cdf_ready = true;
+    const int64_t cnt = std::min(every, max_it - i);  // steps before the next check
+    for (int64_t o = 0; o < cnt; o += kKzPrepMax) {
+      const int c = static_cast<int>(std::min(kKzPrepMax, cnt - o));
+      KzPrep pp{A_rm, ld_rm, n, k.sets + o * s, s, k.M, k.info};
+      if (vec)
+        kz_prep_kernel<true><<<dim3(unsigned(c)), 256, 0, ctx->stream>>>(pp);
+      else
+        kz_prep_kernel<false><<<dim3(unsigned(c)), 256, 0, ctx->stream>>>(pp);
+      RLA_LAUNCH_CHECK("kz_prep");
+      KzRun pr{A_rm, ld_rm, n, b, x, alpha, k.sets + o * s, s, c, k.M};
+      if (vec)
+        kz_run_kernel<true><<<1, kKzRunThreads, 0, ctx->stream>>>(pr);
+      else
+        kz_run_kernel<false><<<1, kKzRunThreads, 0, ctx->stream>>>(pr);
+      RLA_LAUNCH_CHECK("kz_run");
+    }
+    i += cnt;
+  }
+  int info = 0;
+  RLA_HIP(hipMemcpyAsync(&info, k.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (info > 0) return fail(RLA_EDOMAIN, "block Gram matrix is not finite (NaN or Inf in the block or x)");
+  if (n_hist) *n_hist = nh;
+  if (iterations) *iterations = it;
+  if (converged) *converged = conv;
   return RLA_OK;
 }
 
@@ -1037,6 +1614,14 @@ int rla_kaczmarz_solve_f32(rla_ctx* ctx, int64_t m, int64_t n, const float* A, i
   if (kz_layout(m, n, s, every, work, work_bytes, &k) > work_bytes)
     return fail(RLA_EARG, "Kaczmarz workspace too small");
   const int si = static_cast<int>(s);
+  if (n <= kKzRunCols && every <= kKzPrepMax)
+    return kz_solve_overlapped(ctx, k, m, n, A_rm, ld_rm, b, x, si, static_cast<float>(alpha), w,
+                               replace, seed, epoch0, max_it, every, threshold, hist, hist_cap,
+                               n_hist, iterations, converged);
+  if (n <= kKzRunCols)
+    return kz_solve_chunked(ctx, k, m, n, A_rm, ld_rm, b, x, si, static_cast<float>(alpha), w,
+                            replace, seed, epoch0, max_it, every, threshold, hist, hist_cap,
+                            n_hist, iterations, converged);
   const RowSrc none{nullptr, 0, 0, nullptr, 0};
   const float al = static_cast<float>(alpha);
   RowSrc pending = none;

@@ -702,10 +702,18 @@ __device__ __forceinline__ void pr_swap(float (&x)[32]) {
 #pragma unroll
   for (int v = 0; v < 32; ++v) {
     if (!(v & (1 << RB))) {
-      if constexpr (ROW == 16)
-        asm volatile("v_permlane16_swap_b32 %0, %1" : "+v"(x[v]), "+v"(x[v | (1 << RB)]));
-      else
-        asm volatile("v_permlane32_swap_b32 %0, %1" : "+v"(x[v]), "+v"(x[v | (1 << RB)]));
+      // the builtins let the compiler insert the wait states these
+      // instructions need after a VALU write of their operands
+      const unsigned a = __float_as_uint(x[v]), b = __float_as_uint(x[v | (1 << RB)]);
+      if constexpr (ROW == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        x[v] = __uint_as_float(r[0]);
+        x[v | (1 << RB)] = __uint_as_float(r[1]);
+      }
     }
   }
 }

@@ -108,61 +108,149 @@ __host__ __device__ inline int pow2_at_least(int n) {
   return p;
 }
 
+// Order-preserving code of a key: a larger double gives a larger code.
+__device__ __forceinline__ uint64_t topk_code(double k) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(k));
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
 // Pass over candidates (n_rows items in all).  first: the candidates are items
-// [0, n_items) with keys
-// drawn here (set r uses epoch0 + r); otherwise they are the (key, index) pairs
-// in_key/in_idx[r][0..n_items).  Each workgroup sorts a chunk of kSortN and
-// keeps its k best.  final (one chunk): the k best are sorted by index and
-// written to out_set[r][0..k).
+// [0, n_items) with keys drawn here (set r uses epoch0 + r); otherwise they
+// are the (key, index) pairs in_key/in_idx[r][0..n_items).  Each workgroup
+// takes a chunk of kSortN and keeps its k largest keys, found by a radix
+// select on the keys' order-preserving codes: 8-bit digits from the top, a
+// histogram of the candidates still tied with the selected prefix per digit,
+// until the boundary digit holds exactly the number still to take.  The kept
+// set does not depend on timing (keys are continuous draws: equal codes occur
+// only for zero weights, key -inf, which the final pass never returns when
+// at least k weights are positive); its order in the output does.  final
+// (one chunk): the k best are sorted by index and written to out_set[r][0..k).
 __global__ __launch_bounds__(kSortThreads) void topk_pass_kernel(
     uint64_t seed, uint64_t epoch0, int64_t n_items, int64_t n_rows, const double* __restrict__ w,
     int k,
     const double* __restrict__ in_key, const int* __restrict__ in_idx, double* __restrict__ out_key,
     int* __restrict__ out_idx, int* __restrict__ out_set, int first, int final_pass) {
-  __shared__ double key[kSortN];
-  __shared__ int idx[kSortN];
+  constexpr int E = kSortN / kSortThreads;
+  __shared__ int hist[256];
+  __shared__ int sel_bin, sel_rem, sel_done, outc, tiec;
+  __shared__ int fin[1024];
+  const int t = threadIdx.x, lane = t & 63;
   const int r = blockIdx.y, chunk = blockIdx.x;
   const int64_t base = int64_t(chunk) * kSortN;
-  const int64_t cnt = min<int64_t>(kSortN, n_items - base);
-  const int N = pow2_at_least(static_cast<int>(cnt));
-  for (int e = threadIdx.x; e < N; e += blockDim.x) {
-    double kk = -INFINITY;
-    int ii = 0x7fffffff;
+  const int cnt = static_cast<int>(min<int64_t>(kSortN, n_items - base));
+  double kk[E];
+  int ii[E];
+  uint64_t code[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = t + q * kSortThreads;
+    kk[q] = -INFINITY;
+    ii[q] = 0x7fffffff;
     if (e < cnt) {
       const int64_t item = base + e;
       if (first) {
         Stream st(seed, epoch0 + r, DOM_WSAMPLE, static_cast<uint64_t>(item));
         const double u = st.open01();
         const double wi = w ? w[item] : 1.0;
-        kk = wi > 0.0 ? log(u) / wi : -INFINITY;
-        ii = static_cast<int>(item);
+        kk[q] = wi > 0.0 ? log(u) / wi : -INFINITY;
+        ii[q] = static_cast<int>(item);
       } else {
-        kk = in_key[int64_t(r) * n_items + item];
-        ii = in_idx[int64_t(r) * n_items + item];
+        kk[q] = in_key[int64_t(r) * n_items + item];
+        ii[q] = in_idx[int64_t(r) * n_items + item];
       }
     }
-    key[e] = kk;
-    idx[e] = ii;
+    code[q] = topk_code(kk[q]);
+  }
+  const int take = min(k, cnt);
+  uint64_t prefix = 0, pmask = 0;
+  bool all = take == cnt;  // every candidate is kept
+  int rem = take;
+  if (t == 0) {
+    outc = 0;
+    tiec = 0;
   }
   __syncthreads();
-  bitonic_sort(key, idx, N);
+  for (int shift = 56; shift >= 0 && !all; shift -= 8) {
+    if (t < 256) hist[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+      if (t + q * kSortThreads < cnt && (code[q] & pmask) == prefix)
+        atomicAdd(&hist[int((code[q] >> shift) & 255u)], 1);
+    __syncthreads();
+    if (t < 64) {
+      // suffix sums over bins (lane l: bins 4l .. 4l + 3), from the top
+      int h[4], s4 = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s4 += (h[b] = hist[4 * lane + b]);
+      int suf = s4;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_down(suf, d);
+        if (lane + d < 64) suf += o;
+      }
+      int above = __shfl_down(suf, 1);
+      if (lane == 63) above = 0;
+      if (suf >= rem && above < rem) {  // exactly one lane
+        int cum = above;
+#pragma unroll
+        for (int b = 3; b >= 0; --b) {
+          if (cum + h[b] >= rem) {
+            const int nr = rem - cum;
+            sel_bin = 4 * lane + b;
+            sel_rem = nr;
+            sel_done = h[b] == nr;
+            break;
+          }
+          cum += h[b];
+        }
+      }
+    }
+    __syncthreads();
+    prefix |= uint64_t(sel_bin) << shift;
+    pmask |= uint64_t(255u) << shift;
+    rem = sel_rem;
+    all = sel_done != 0;  // the boundary bin is taken whole
+  }
+  // kept: candidates above the selected prefix, those equal to it (all of
+  // them when the boundary bin was taken whole, else rem of the exact ties)
+  const bool whole = take == cnt || all;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = t + q * kSortThreads;
+    if (e >= cnt) continue;
+    const uint64_t c = code[q] & pmask;
+    bool keep = take == cnt || c > prefix;
+    if (!keep && c == prefix) keep = whole || atomicAdd(&tiec, 1) < rem;
+    if (!keep) continue;
+    const int slot = atomicAdd(&outc, 1);
+    if (!final_pass) {
+      const int64_t nout = int64_t(gridDim.x) * k;
+      out_key[int64_t(r) * nout + int64_t(chunk) * k + slot] = kk[q];
+      out_idx[int64_t(r) * nout + int64_t(chunk) * k + slot] = ii[q];
+    } else {
+      fin[slot] = ii[q];
+    }
+  }
   if (!final_pass) {
+    // slots past the kept ones (a chunk of fewer than k candidates)
     const int64_t nout = int64_t(gridDim.x) * k;
-    for (int e = threadIdx.x; e < k; e += blockDim.x) {
-      out_key[int64_t(r) * nout + int64_t(chunk) * k + e] = e < cnt ? key[e] : -INFINITY;
-      out_idx[int64_t(r) * nout + int64_t(chunk) * k + e] = e < cnt ? idx[e] : 0x7fffffff;
+    for (int e = take + t; e < k; e += kSortThreads) {
+      out_key[int64_t(r) * nout + int64_t(chunk) * k + e] = -INFINITY;
+      out_idx[int64_t(r) * nout + int64_t(chunk) * k + e] = 0x7fffffff;
     }
     return;
   }
-  // k best by index, ascending (ordered = true); entries past k are padding
+  // k best by index, ascending (ordered = true); entries past take are padding
   const int K = pow2_at_least(k);
-  for (int e = k + threadIdx.x; e < K; e += blockDim.x) idx[e] = 0x7fffffff;
   __syncthreads();
-  bitonic_sort_int(idx, K);
+  for (int e = take + t; e < K; e += kSortThreads) fin[e] = 0x7fffffff;
+  __syncthreads();
+  bitonic_sort_int(fin, K);
   // the host guarantees at least k items of positive weight; the clamp only
   // keeps a violated precondition in bounds
-  for (int e = threadIdx.x; e < k; e += blockDim.x)
-    out_set[int64_t(r) * k + e] = static_cast<int>(min<int64_t>(idx[e], n_rows - 1));
+  for (int e = t; e < k; e += kSortThreads)
+    out_set[int64_t(r) * k + e] = static_cast<int>(min<int64_t>(fin[e], n_rows - 1));
 }
 
 // ---- sampling with replacement ----------------------------------------------

@@ -47,6 +47,35 @@ def test_wsample_uniform_distinct_sorted(rla, dev, n, k):
     assert np.array_equal(S[3], wsample(rla, dev, n, k, epoch=3)[0])
 
 
+def _host_topk(n, k, w, seed, epoch):
+    """The k items with the largest A-Res keys log(u_i) / w_i (u_i from the
+    sampler's counter stream, domain 5), sorted by index."""
+    keys = np.full(n, -np.inf)
+    for i in range(n):
+        if w is None or w[i] > 0:
+            u = R.Stream(seed, epoch, 5, i).open01()
+            keys[i] = np.log(u) / (1.0 if w is None else w[i])
+    return np.sort(np.argsort(-keys, kind="stable")[:k])
+
+
+@pytest.mark.parametrize("n,k,wkind", [(5000, 64, "none"), (9000, 700, "rand"),
+                                       (12000, 64, "sparse"), (3000, 1024, "rand")])
+def test_wsample_is_exact_topk_of_the_keys(rla, dev, n, k, wkind):
+    # the device selection (per-chunk radix select, then a merge pass) keeps
+    # exactly the k largest keys, whatever their spread over the chunks
+    rng = np.random.default_rng(n + k)
+    if wkind == "none":
+        w = None
+    elif wkind == "rand":
+        w = rng.uniform(0.1, 5.0, n)
+    else:  # mostly zero weights: many equal keys (-inf) below the k kept
+        w = np.zeros(n)
+        w[rng.choice(n, 150, replace=False)] = rng.uniform(0.5, 2.0, 150)
+    S = wsample(rla, dev, n, k, w=w, nsets=2, seed=21, epoch=4)
+    for r in range(2):
+        assert np.array_equal(S[r], _host_topk(n, k, w, 21, 4 + r))
+
+
 def test_wsample_uniform_inclusion_frequencies(rla, dev):
     n, k, sets = 40, 6, 6000
     S = wsample(rla, dev, n, k, nsets=sets)

@@ -253,6 +253,26 @@ int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64
                               int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
                               double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
 
+/* The same product for tall X by blocks: one workgroup per (64 KiB block of
+ * X, column of X) writes each row's FP64 sum over the block, and the blocks
+ * are added in ascending order, so the result is within rounding of
+ * rla_sparse_apply (not identical: sums are formed per block).  Workspace:
+ * rla_sparse_apply_bell_split_bytes (nblocks * n_rhs * rows doubles).
+ * Replaces the same mul! as rla_sparse_apply_bell (sparse_sign.jl:339-348,
+ * count_sketch.jl:166-213) for operands taller than one block. */
+int rla_sparse_apply_bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,
+                                      size_t* bytes);
+int rla_sparse_apply_bell_split_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                                    const int32_t* widths, const int32_t* bidx,
+                                    const float* bval, int64_t n_rhs, const float* X,
+                                    int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                                    int64_t c_cs, double alpha, double beta);
+int rla_sparse_apply_bell_split_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                                    const int32_t* widths, const int32_t* bidx,
+                                    const double* bval, int64_t n_rhs, const double* X,
+                                    int64_t x_rs, int64_t x_cs, double* C, int64_t c_rs,
+                                    int64_t c_cs, double alpha, double beta);
+
 /* Agmon maximal-residual selection, the top-k step of sample_distribution!
  * for the Agmon distribution (src/Compressors/Distributions/agmon.jl:354-416):
  * out[t] (t < k) = the candidate with the t-th largest |v[c]|, ties to the

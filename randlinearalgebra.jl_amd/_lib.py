@@ -89,6 +89,11 @@ _SIGS = {
                                   f64, f64],
     "rla_sparse_apply_bell_f64": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64, i64,
                                   f64, f64],
+    "rla_sparse_apply_bell_split_f32": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64,
+                                        i64, f64, f64],
+    "rla_sparse_apply_bell_split_f64": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64,
+                                        i64, f64, f64],
+    "rla_sparse_apply_bell_split_bytes": [i64, i64, i64, i64, vp],
     "rla_fjlt_apply_bell_f32": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,
                                 i64, i64, f64, f64],
     "rla_fjlt_apply_bell_f64": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,

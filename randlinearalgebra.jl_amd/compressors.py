@@ -335,12 +335,20 @@ def _device_of(*xs):
 # sparse operators: SparseSign, CountSketch
 
 
+# partial sums the block-split sparse gather may hold (larger: block by block)
+_BELL_SPLIT_MAX_BYTES = 64 << 20
+
+
 class _SparseOp:
     """An nrows x ncols CSC matrix M on the device and its CSR twin.
     `_uniform`: all nonzeros share one magnitude (the sign-packed gather is
-    tried first; its builder verifies it)."""
+    tried first; its builder verifies it).  `exact`: tall X (columns above
+    64 KiB) take the block-by-block gather whose sums are added in
+    SparseArrays' entry order (identical bits); otherwise the block-split
+    gather, whose per-block FP64 partial sums are within rounding of it."""
 
     _uniform = False
+    exact = False
 
     def __init__(self, nrows, ncols, colptr, rowidx, val):
         self.nrows, self.ncols = nrows, ncols
@@ -455,7 +463,11 @@ class _SparseOp:
                 bell = self._bell_for(trans_m, dt)
                 if bell is not None:
                     rb, widths, bidx, bval = bell
-                    _lib.call(f"rla_sparse_apply_bell_{_lib.sfx(dt)}", _lib.ctx(X.t.device),
+                    nblk = (X.rows + rb - 1) // rb
+                    split = (not self.exact and nblk >= 2
+                             and nblk * X.cols * C.rows * 8 <= _BELL_SPLIT_MAX_BYTES)
+                    name = "rla_sparse_apply_bell_split_" if split else "rla_sparse_apply_bell_"
+                    _lib.call(f"{name}{_lib.sfx(dt)}", _lib.ctx(X.t.device),
                               C.rows, X.rows, rb, _lib.ptr(widths), _lib.ptr(bidx),
                               _lib.ptr(bval), X.cols, _lib.ptr(X.t), X.rs, X.cs, _lib.ptr(C.t),
                               C.rs, C.cs, alpha, beta)
@@ -507,9 +519,13 @@ class _SparseRecipe(CompressorRecipe):
 
 class SparseSign(Compressor):
     """SparseSign(cardinality=Left(), compression_dim=2, nnz=min(8, compression_dim),
-    type=Float64) — src/Compressors/sparse_sign.jl:65-127."""
+    type=Float64) — src/Compressors/sparse_sign.jl:65-127.  exact=True keeps
+    SparseArrays' summation order (identical bits) for tall operands as well;
+    by default X with columns above 64 KiB is applied block-split, within
+    rounding of it."""
 
-    def __init__(self, cardinality=None, compression_dim=2, nnz=None, type=torch.float64):
+    def __init__(self, cardinality=None, compression_dim=2, nnz=None, type=torch.float64,
+                 exact=False):
         cardinality = Left() if cardinality is None else cardinality
         nnz = min(8, compression_dim) if nnz is None else nnz
         if compression_dim <= 0:
@@ -524,6 +540,7 @@ class SparseSign(Compressor):
         object.__setattr__(self, "compression_dim", int(compression_dim))
         object.__setattr__(self, "nnz", int(nnz))
         object.__setattr__(self, "type", _dtype(type))
+        object.__setattr__(self, "exact", bool(exact))
 
     def __setattr__(self, name, val):
         if name == "compression_dim":
@@ -544,7 +561,9 @@ class SparseSign(Compressor):
 
     def _complete(self, *args):
         A = args[0] if len(args) <= 2 else args[1]
-        return SparseSignRecipe(self.cardinality, self.compression_dim, A, self.nnz, self.type)
+        rec = SparseSignRecipe(self.cardinality, self.compression_dim, A, self.nnz, self.type)
+        rec.op.exact = self.exact
+        return rec
 
 
 class SparseSignRecipe(_SparseRecipe):
@@ -589,9 +608,9 @@ class SparseSignRecipe(_SparseRecipe):
 
 class CountSketch(Compressor):
     """CountSketch(cardinality=Left(), compression_dim=2, type=Float64) —
-    src/Compressors/count_sketch.jl:60-81."""
+    src/Compressors/count_sketch.jl:60-81.  exact: as for SparseSign."""
 
-    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64):
+    def __init__(self, cardinality=None, compression_dim=2, type=torch.float64, exact=False):
         cardinality = Left() if cardinality is None else cardinality
         if compression_dim <= 0:
             raise ArgumentError("Field 'compression_dim' must be positive.")
@@ -601,10 +620,13 @@ class CountSketch(Compressor):
         self.cardinality = cardinality
         self.compression_dim = int(compression_dim)
         self.type = _dtype(type)
+        self.exact = bool(exact)
 
     def _complete(self, *args):
         A = args[0] if len(args) <= 2 else args[1]
-        return CountSketchRecipe(self.cardinality, self.compression_dim, A, self.type)
+        rec = CountSketchRecipe(self.cardinality, self.compression_dim, A, self.type)
+        rec.op.exact = self.exact
+        return rec
 
 
 class CountSketchRecipe(_SparseRecipe):

@@ -824,6 +824,146 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_bell_kernel(
   }
 }
 
+// Block-split form of rowgather_bell_kernel for tall X: one workgroup per
+// (64 KiB block of X, column of X) stages that block and writes each output
+// row's FP64 sum over the block's slots (part[(k * nblocks + blk) * rows + i]);
+// bell_combine_kernel adds the blocks in ascending order.  The block of
+// workgroup id: XCD id % 8 takes blocks x, x + 8, ..., all columns of one
+// block before the next, so a block's slots are read from that XCD's L2.
+// Rows are summed block by block rather than entry by entry in one
+// accumulator, so the result is within rounding of rla_sparse_apply, not
+// identical to it.
+template <class T>
+__global__ __launch_bounds__(kGatherThreads) void rowgather_bell_split_kernel(
+    int64_t rows, int64_t in_rows, int64_t rb, int64_t nblocks, int64_t n_rhs,
+    const int32_t* __restrict__ widths, const int32_t* __restrict__ bidx,
+    const T* __restrict__ bval, const T* __restrict__ X, int64_t x_cs, int vec_ok,
+    double* __restrict__ part) {
+  constexpr int B = kBellRoundSlots;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  __shared__ int64_t slot0;
+  const int64_t id = blockIdx.x;
+  const int64_t xcd = id % 8, seq = id / 8;
+  const int64_t blk = (seq / n_rhs) * 8 + xcd;
+  const int64_t k = seq % n_rhs;
+  if (blk >= nblocks) return;
+  if (threadIdx.x == 0) {
+    int64_t w = 0;
+    for (int64_t b = 0; b < blk; ++b) w += widths[b];
+    slot0 = w;
+  }
+  const int64_t j0 = blk * rb;
+  const int64_t jn = in_rows - j0 < rb ? in_rows - j0 : rb;
+  const T* xc = X + k * x_cs + j0;
+  if (vec_ok) {
+    constexpr int VW = 16 / sizeof(T);
+    const int64_t nv = jn / VW;
+    for (int64_t vb = 0; vb < nv; vb += 8 * int64_t(blockDim.x)) {
+      uint4 tmp[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
+        tmp[u] = v < nv ? reinterpret_cast<const uint4*>(xc)[v] : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
+        if (v < nv) reinterpret_cast<uint4*>(lds)[v] = tmp[u];
+      }
+    }
+    for (int64_t j = nv * VW + threadIdx.x; j < jn; j += blockDim.x) lds[j] = xc[j];
+  } else {
+    for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j] = xc[j];
+  }
+  __syncthreads();
+  const int32_t w = widths[blk];
+  const int64_t base = slot0 * rows;
+  double* out = part + (k * nblocks + blk) * rows;
+  for (int64_t i = threadIdx.x; i < rows; i += blockDim.x) {
+    double acc = 0.0;
+    for (int32_t s0 = 0; s0 < w; s0 += B) {
+      int32_t jj[B];
+      T vv[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const bool in = s0 + u < w;
+        const int64_t e = base + int64_t(s0 + u) * rows + i;
+        jj[u] = in ? bidx[e] : -1;
+        vv[u] = in ? bval[e] : T(0);
+      }
+      bool done = false;
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int64_t d = int64_t(jj[u]) - j0;
+        const T xv = lds[(d >= 0 && d < jn) ? d : 0];
+        if (jj[u] < 0) done = true;  // first empty slot: the row is done
+        if (!done) acc += double(vv[u]) * double(xv);
+      }
+      if (done) break;
+    }
+    out[i] = acc;
+  }
+}
+
+// C(i, k) = alpha * sum over blocks (ascending) of part + beta * C(i, k)
+template <class T>
+__global__ void bell_combine_kernel(int64_t rows, int64_t nblocks, int64_t n_rhs,
+                                    const double* __restrict__ part, T* __restrict__ C,
+                                    int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= rows * n_rhs) return;
+  const int64_t i = e % rows, k = e / rows;
+  const double* p = part + k * nblocks * rows + i;
+  double s = 0.0;
+  for (int64_t b = 0; b < nblocks; ++b) s += p[b * rows];
+  T* c = C + i * c_rs + k * c_cs;
+  *c = static_cast<T>(beta == 0.0 ? alpha * s : alpha * s + beta * double(*c));
+}
+
+// partial sums of the block-split apply
+static size_t bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs) {
+  return sizeof(double) * size_t(ceil_div(in_rows, rb)) * size_t(n_rhs) * size_t(rows);
+}
+
+template <class T>
+static int sparse_apply_bell_split(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                                   const int32_t* widths, const int32_t* bidx, const T* bval,
+                                   int64_t n_rhs, const T* X, int64_t x_rs, int64_t x_cs, T* C,
+                                   int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  if (in_rows < 1 || rb < 1)
+    return fail(RLA_EARG, "block-interleaved apply: need in_rows >= 1 and rb >= 1");
+  if (rows == 0 || n_rhs == 0) return RLA_OK;
+  if (!widths || !bidx || !bval || !X || !C)
+    return fail(RLA_EARG, "block-interleaved apply: NULL pointer");
+  if (x_rs != 1)
+    return fail(RLA_EARG, "block-interleaved apply needs X column-major (x_rs == 1)");
+  if (size_t(rb) * sizeof(T) > kGatherLds)
+    return fail(RLA_ENOTSUP, "block-interleaved apply: rb * element size must be <= 64 KiB");
+  const int64_t nblocks = ceil_div(in_rows, rb);
+  const int64_t ids = ceil_div(nblocks, 8) * 8 * n_rhs;
+  if (ids > INT32_MAX || rows * n_rhs > INT32_MAX * int64_t(256))
+    return fail(RLA_ENOTSUP, "block-interleaved apply: grid too large");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = workspace(ctx, bell_split_bytes(rows, in_rows, rb, n_rhs), &ws);
+  if (rc) return rc;
+  double* part = static_cast<double*>(ws);
+  const size_t lds = size_t(std::min(rb, in_rows)) * sizeof(T);
+  const int vec_ok = reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
+                     (x_cs * int64_t(sizeof(T))) % 16 == 0 && (rb * int64_t(sizeof(T))) % 16 == 0;
+  rowgather_bell_split_kernel<T><<<dim3(unsigned(ids)), kGatherThreads, lds, ctx->stream>>>(
+      rows, in_rows, rb, nblocks, n_rhs, widths, bidx, bval, X, x_cs, vec_ok, part);
+  RLA_LAUNCH_CHECK("rowgather_bell_split");
+  bell_combine_kernel<T><<<dim3(unsigned(ceil_div(rows * n_rhs, 256))), 256, 0, ctx->stream>>>(
+      rows, nblocks, n_rhs, part, C, c_rs, c_cs, alpha, beta);
+  RLA_LAUNCH_CHECK("bell_combine");
+  return RLA_OK;
+}
+
 static int bell_widths(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
                        const int32_t* ptr, const int32_t* idx, int32_t* widths,
                        int64_t* total) {
@@ -1063,6 +1203,32 @@ int rla_sparse_apply_bell_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64
                               int64_t c_rs, int64_t c_cs, double alpha, double beta) {
   return sparse_apply_bell<float>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X, x_rs,
                                   x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_bell_split_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                                    const int32_t* widths, const int32_t* bidx,
+                                    const float* bval, int64_t n_rhs, const float* X,
+                                    int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                                    int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_bell_split<float>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X,
+                                        x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_bell_split_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,
+                                    const int32_t* widths, const int32_t* bidx,
+                                    const double* bval, int64_t n_rhs, const double* X,
+                                    int64_t x_rs, int64_t x_cs, double* C, int64_t c_rs,
+                                    int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_bell_split<double>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X,
+                                         x_rs, x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_apply_bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,
+                                      size_t* bytes) {
+  if (!bytes || rows < 0 || in_rows < 1 || rb < 1 || n_rhs < 0)
+    return fail(RLA_EARG, "rla_sparse_apply_bell_split_bytes: bad arguments");
+  *bytes = bell_split_bytes(rows, in_rows, rb, n_rhs);
+  return RLA_OK;
 }
 
 int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rb,

@@ -741,17 +741,19 @@ template <int MS, int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int64_t ncols,
                        int nr, const uint64_t* __restrict__ masks_g,
-                       const int32_t* __restrict__ op, int s, double scale,
-                       float* __restrict__ Y, int64_t y_rs, int64_t y_cs, double alpha,
-                       double beta) {
+                       const int32_t* __restrict__ op, const int32_t* __restrict__ ord, int s,
+                       double scale, float* __restrict__ Y, int64_t y_rs, int64_t y_cs,
+                       double alpha, double beta) {
   // the masks are read through the scalar cache (they are wave-uniform)
   const pr_mask_ptr masks = (pr_mask_ptr)masks_g;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float* lds = reinterpret_cast<float*>(smem_raw);
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-  // this thread's sampled rows r = t + 512 j: LDS index of rho_l, and rho_h
-  // with the lane sign (-1)^popcount(e2..e5) of the DPP stages in bit 30
+  // this thread's sampled rows ord[t + 512 j] (ord lists the rows by rho_l,
+  // so the lanes of a wavefront read nearly consecutive LDS words; NULL: the
+  // rows in order): LDS index of rho_l, and rho_h with the lane sign
+  // (-1)^popcount(e2..e5) of the DPP stages in bit 30
   int sidx[MS], shi[MS];
 #pragma unroll
   for (int j = 0; j < MS; ++j) {
@@ -759,7 +761,7 @@ void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int
     sidx[j] = -1;
     shi[j] = 0;
     if (r < s) {
-      const int rho = op[r];
+      const int rho = op[ord ? ord[r] : r];
       const int lo = rho & 2047;
       sidx[j] = pr_pad(lo);
       shi[j] = (rho >> 11) | ((__builtin_popcount((lo >> 2) & 15) & 1) << 30);
@@ -848,7 +850,8 @@ void srht_prune_kernel(const float* __restrict__ X, int64_t x_cs, int64_t m, int
 #pragma unroll
       for (int j = 0; j < MS; ++j) {
         if (sidx[j] < 0) continue;
-        float* yp = Y + int64_t(t + 512 * j) * y_rs + col * y_cs;
+        const int r = ord ? ord[t + 512 * j] : t + 512 * j;
+        float* yp = Y + int64_t(r) * y_rs + col * y_cs;
         const double val = double(acc[j]) * scale;
         *yp = static_cast<float>(beta == 0.0 ? alpha * val : beta * double(*yp) + alpha * val);
         acc[j] = 0.0f;
@@ -1025,12 +1028,44 @@ static size_t prune_mask_offset_words(int elem, int64_t n_pad, int64_t s) {
   const size_t fused = fused_shape(elem, n_pad, f) ? plan_words(f, s) : 0;
   return (fused + 63) & ~size_t(63);
 }
+// after the masks: the row order (s words), the rows sorted by op & 2047
+static size_t prune_order_offset_words(int elem, int64_t n_pad, int64_t s) {
+  return prune_mask_offset_words(elem, n_pad, s) + prune_mask_words(n_pad);
+}
 static size_t total_plan_words(int elem, int64_t n_pad, int64_t s) {
   FusedShape f;
   const size_t fused = fused_shape(elem, n_pad, f) ? plan_words(f, s) : 0;
   if (!prune_ok(elem, n_pad, s)) return fused;
-  return prune_mask_offset_words(elem, n_pad, s) + prune_mask_words(n_pad);
+  return prune_order_offset_words(elem, n_pad, s) + size_t(s);
 }
+// ord = the rows 0..s-1 sorted by (op[r] & 2047, r): a bitonic sort of the
+// keys (op[r] & 2047) << 12 | r in LDS by one workgroup (s <= 4096)
+__global__ __launch_bounds__(1024) void prune_order_kernel(const int32_t* __restrict__ op,
+                                                           int s, int32_t* __restrict__ ord) {
+  __shared__ uint32_t key[4096];
+  int n = 1;
+  while (n < s) n <<= 1;
+  for (int i = threadIdx.x; i < n; i += 1024)
+    key[i] = i < s ? (uint32_t(op[i] & 2047) << 12) | uint32_t(i) : 0xFFFFFFFFu;
+  __syncthreads();
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += 1024) {
+        const int p = i ^ j;
+        if (p > i) {
+          const uint32_t a = key[i], b = key[p];
+          if (((i & k) == 0) == (a > b)) {
+            key[i] = b;
+            key[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < s; i += 1024) ord[i] = static_cast<int32_t>(key[i] & 4095u);
+}
+
 static int build_prune_masks(rla_ctx* ctx, int64_t n_pad, const uint64_t* signs,
                              uint64_t* masks) {
   const int nb = static_cast<int>(n_pad / 2048);
@@ -1114,6 +1149,10 @@ static int build_plan(rla_ctx* ctx, int elem, int64_t n_pad, int64_t s, const in
         ctx, n_pad, signs,
         reinterpret_cast<uint64_t*>(plan + prune_mask_offset_words(elem, n_pad, s)));
     if (rc) return rc;
+    prune_order_kernel<<<1, 1024, 0, ctx->stream>>>(
+        op, static_cast<int>(s),
+        reinterpret_cast<int32_t*>(plan + prune_order_offset_words(elem, n_pad, s)));
+    RLA_LAUNCH_CHECK("srht_prune_order");
   }
   FusedShape f;
   if (!fused_shape(elem, n_pad, f)) return RLA_OK;
@@ -1255,8 +1294,8 @@ static int launch_wave_any(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m
 
 template <int MS, int MODE>
 static int launch_prune_ms(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t n_pad,
-                           int64_t ncols, const uint64_t* masks, const int32_t* op, int64_t s,
-                           double scale, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
+                           int64_t ncols, const uint64_t* masks, const int32_t* op,
+                           const int32_t* ord, int64_t s, double scale, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
                            double beta) {
   const size_t lds = size_t(8) * kPrRegion * sizeof(float);
   static std::once_flag once;
@@ -1270,7 +1309,7 @@ static int launch_prune_ms(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m
   const int64_t grid = std::min<int64_t>(ncols, int64_t(num_cus(ctx->device)) * per_cu);
   const int nr = static_cast<int>(n_pad >> 14);
   srht_prune_kernel<MS, MODE><<<dim3(static_cast<unsigned>(grid)), 512, lds, ctx->stream>>>(
-      X, x_cs, m, ncols, nr, masks, op, static_cast<int>(s),
+      X, x_cs, m, ncols, nr, masks, op, ord, static_cast<int>(s),
       scale, Y, y_rs, y_cs, alpha, beta);
   RLA_LAUNCH_CHECK("srht_prune");
   return RLA_OK;
@@ -1279,33 +1318,34 @@ static int launch_prune_ms(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m
 template <int MODE>
 static int launch_prune_mode(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m,
                              int64_t n_pad, int64_t ncols, const uint64_t* masks,
-                             const int32_t* op, int64_t s, double scale, float* Y, int64_t y_rs,
+                             const int32_t* op, const int32_t* ord, int64_t s, double scale, float* Y, int64_t y_rs,
                              int64_t y_cs, double alpha, double beta) {
   if (s <= 512)
-    return launch_prune_ms<1, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+    return launch_prune_ms<1, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                     y_cs, alpha, beta);
   if (s <= 1024)
-    return launch_prune_ms<2, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+    return launch_prune_ms<2, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                     y_cs, alpha, beta);
   if (s <= 2048)
-    return launch_prune_ms<4, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+    return launch_prune_ms<4, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                     y_cs, alpha, beta);
-  return launch_prune_ms<8, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+  return launch_prune_ms<8, MODE>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                   y_cs, alpha, beta);
 }
 
 static int launch_prune(rla_ctx* ctx, const float* X, int64_t x_cs, int64_t m, int64_t n_pad,
-                        int64_t ncols, const uint64_t* masks, const int32_t* op, int64_t s,
+                        int64_t ncols, const uint64_t* masks, const int32_t* op,
+                        const int32_t* ord, int64_t s,
                         double scale, float* Y, int64_t y_rs, int64_t y_cs, double alpha,
                         double beta) {
   const bool aligned = (x_cs % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
   if (aligned && m == n_pad)
-    return launch_prune_mode<0>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+    return launch_prune_mode<0>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                 y_cs, alpha, beta);
   if (aligned && m % 4 == 0)
-    return launch_prune_mode<1>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs,
+    return launch_prune_mode<1>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs,
                                 y_cs, alpha, beta);
-  return launch_prune_mode<2>(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs, y_cs,
+  return launch_prune_mode<2>(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs, y_cs,
                               alpha, beta);
 }
 
@@ -1319,9 +1359,12 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
   if constexpr (sizeof(T) == 4) {
     if (!exact && prune_ok(4, n_pad, s)) {
       const uint64_t* masks = nullptr;
+      const int32_t* ord = nullptr;  // no plan: the rows in order
       if (plan) {
         masks = reinterpret_cast<const uint64_t*>(static_cast<const uint32_t*>(plan) +
                                                   prune_mask_offset_words(4, n_pad, s));
+        ord = reinterpret_cast<const int32_t*>(static_cast<const uint32_t*>(plan) +
+                                               prune_order_offset_words(4, n_pad, s));
       } else {
         void* ws = nullptr;
         int rc = workspace(ctx, prune_mask_words(n_pad) * 4, &ws);
@@ -1331,7 +1374,7 @@ int srht_fused_forward(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_
         masks = static_cast<const uint64_t*>(ws);
       }
       *handled = true;
-      return launch_prune(ctx, X, x_cs, m, n_pad, ncols, masks, op, s, scale, Y, y_rs, y_cs,
+      return launch_prune(ctx, X, x_cs, m, n_pad, ncols, masks, op, ord, s, scale, Y, y_rs, y_cs,
                           alpha, beta);
     }
   }

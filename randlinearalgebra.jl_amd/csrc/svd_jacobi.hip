@@ -14,19 +14,22 @@
 //
 // Layout: the columns are cut into blocks of 16 (zero columns pad the last
 // block; a zero column is never rotated).  One cooperative launch holds
-// nblk / 2 workgroups of 1024 threads, one per CU, and M and V in global
-// memory (L2).  A sweep visits every column pair once:
+// nblk / 2 workgroups of 4 wavefronts, and M and V in global memory (L2).
+// A column pair is worked by one 16-lane DPP row: lane i holds rows
+// 2i, 2i + 1, 2i + 32, 2i + 33, ..., so a wavefront works 4 pairs at once,
+// moves columns through LDS with 16-byte accesses, and its sums are four DPP
+// steps within the row.  A sweep visits every column pair once:
 //   * an intra pass: workgroup w loads blocks 2w, 2w+1 into LDS and runs the
 //     round-robin schedule over each block's 16 columns (15 rounds of 8
-//     disjoint pairs per block, one wavefront per pair);
-//   * nblk - 1 rounds of the round-robin schedule over blocks: workgroup w
-//     loads its blocks I, J and visits the 16 x 16 cross pairs in 16 rounds
-//     (pair (I_a, J_(a+r) mod 16) on wavefront a).
-// A grid barrier follows every pass.  A wavefront keeps its pair's entries
-// in registers between the dot products and the rotation; the three sums
-// are reduced with DPP and permlane swaps.  The sweep's rotation count
-// decides, after a grid barrier, whether another sweep runs, so every
-// workgroup executes the same barriers.  At most 60 sweeps.
+//     disjoint pairs per block, 16 pairs per round on the 16 rows);
+//   * nblk - 1 passes of the round-robin schedule over blocks: workgroup w
+//     takes its blocks I, J; row u (wavefront u / 4, row u % 4) keeps
+//     column u of I (and of V) in registers for the whole pass and meets
+//     column (u + r) mod 16 of J, read from and written back to LDS, in
+//     round r (16 rounds).
+// A grid barrier follows every pass.  The sweep's rotation count decides,
+// after a grid barrier, whether another sweep runs, so every workgroup
+// executes the same barriers.  At most 60 sweeps.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -42,7 +45,10 @@ namespace cg = cooperative_groups;
 
 constexpr int kJB = 16;            // columns per block
 constexpr int kJMaxK = 288;
-constexpr int kJThreads = 1024;    // 16 wavefronts: one column pair each
+constexpr int kJT = kJMaxK / 16;   // rows per lane of a 16-lane row
+constexpr int kJLd = 304;          // LDS column stride (doubles): the 4 rows of a
+                                   // wavefront reading 4 columns fill two bank passes
+constexpr int kJThreads = 256;     // 4 wavefronts x 4 rows: 16 column pairs per round
 constexpr int kJMaxSweeps = 60;
 
 struct JacobiArgs {
@@ -76,26 +82,18 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// sum over the 64 lanes, in every lane: within rows of 16 by DPP (quad
-// swaps, then the half-row and row mirrors), across rows by swapping a copy
-// with itself (v_permlane16_swap / v_permlane32_swap)
-__device__ __forceinline__ double wave_sum_fast(double v) {
+// row of M held by entry q of lane li of a 16-lane row: rows 2 li, 2 li + 1,
+// then the same 32 rows further down, so LDS moves are 16-byte accesses
+__device__ __forceinline__ int jrow(int li, int q) { return 2 * li + (q & 1) + 32 * (q >> 1); }
+
+// sum over the 16 lanes of a DPP row, in every lane of the row: quad swaps,
+// then the half-row and row mirrors
+__device__ __forceinline__ double row_sum16(double v) {
   v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
   v += dpp_f64<0x141>(v);  // row_half_mirror
   v += dpp_f64<0x140>(v);  // row_mirror
-  int alo = __double2loint(v), ahi = __double2hiint(v);
-  int blo = alo, bhi = ahi;
-  asm("v_permlane16_swap_b32 %0, %1" : "+v"(alo), "+v"(blo));
-  asm("v_permlane16_swap_b32 %0, %1" : "+v"(ahi), "+v"(bhi));
-  v = __hiloint2double(ahi, alo) + __hiloint2double(bhi, blo);
-  alo = __double2loint(v);
-  ahi = __double2hiint(v);
-  blo = alo;
-  bhi = ahi;
-  asm("v_permlane32_swap_b32 %0, %1" : "+v"(alo), "+v"(blo));
-  asm("v_permlane32_swap_b32 %0, %1" : "+v"(ahi), "+v"(bhi));
-  return __hiloint2double(ahi, alo) + __hiloint2double(bhi, blo);
+  return v;
 }
 
 // 1/x and 1/sqrt(x) from the hardware estimates and two Newton steps
@@ -111,138 +109,233 @@ __device__ __forceinline__ double rsq_nr(double x) {
   return y * fma(-h * y, y, 1.5);
 }
 
-constexpr int kJRows = kJMaxK / 64;  // rows per lane (k <= 288)
-
-// rotates columns p and q (LDS, k rows each) of M and V when they are not
-// orthogonal to tolerance; returns 1 if it rotated
-__device__ __forceinline__ int jacobi_pair(double* Ms, double* Vs, int k, int p, int q, int lane,
-                                           double tol) {
-  double* mp = Ms + p * k;
-  double* mq = Ms + q * k;
-  double x[kJRows + 1], y[kJRows + 1];
-  double a = 0.0, b = 0.0, gm = 0.0;
-#pragma unroll
-  for (int r = 0; r <= kJRows; ++r) {
-    const int i = lane + 64 * r;
-    x[r] = i < k ? mp[i] : 0.0;
-    y[r] = i < k ? mq[i] : 0.0;
-    a = fma(x[r], x[r], a);
-    b = fma(y[r], y[r], b);
-    gm = fma(x[r], y[r], gm);
-  }
-  a = wave_sum_fast(a);
-  b = wave_sum_fast(b);
-  gm = wave_sum_fast(gm);
-  if (!(a > 0.0 && b > 0.0 && fabs(gm) > tol * sqrt(a * b))) return 0;
-  // zeta = (b - a) / (2 g), t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
-  // c = 1 / sqrt(1 + t^2), s = c t
-  const double zeta = (b - a) * rcp_nr(2.0 * gm);
+// the rotation (c, s) of a pair from its sums; false (c = 1, s = 0) when the
+// pair is orthogonal to tolerance or has a zero column
+__device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol,
+                                                double& c, double& tn) {
+  c = 1.0;
+  tn = 0.0;
+  // |g| > tol sqrt(a b), squared and divided by a^2 so nothing leaves the
+  // exponent range: (g / a)^2 > tol^2 (b / a)
+  if (!(a > 0.0 && b > 0.0)) return false;
+  const double ra = rcp_nr(a), gr = g * ra;
+  if (!(gr * gr > tol * tol * (b * ra))) return false;
+  const double zeta = (b - a) * rcp_nr(2.0 * g);
   const double az = fabs(zeta);
   double tt;
   if (az > 1e150) {
-    tt = 0.5 / az;
+    tt = 0.5 * rcp_nr(az);
   } else {
     const double w = fma(zeta, zeta, 1.0);
     tt = rcp_nr(az + w * rsq_nr(w));
   }
-  const double tn = zeta < 0.0 ? -tt : tt;
-  const double c = rsq_nr(fma(tn, tn, 1.0));
-  const double sn = c * tn;
-  double* vp = Vs + p * k;
-  double* vq = Vs + q * k;
+  tn = zeta < 0.0 ? -tt : tt;
+  c = rsq_nr(fma(tn, tn, 1.0));
+  return true;
+}
+
+// Columns x and y of one pair (a lane's 18 rows) are held scaled: the true
+// columns are dx * x and dy * y (likewise for V).  A rotation by the angle
+// with tangent t and cosine c becomes, on the stored columns,
+//   x <- x - t (dy / dx) y,  y <- y + t (dx / dy) x,  dx <- c dx,  dy <- c dy
+// (one FMA per entry; the scales are folded back into the columns at the end
+// of every pass, after at most 16 rotations, so they stay above 2^-8).
+// Every lane of the wavefront calls these (the rows of a wavefront work 4
+// different pairs).
+
+// the pair's rotation from its sums (c = 1, t = 0 and false when none)
+__device__ __forceinline__ bool pair_rotation(const double (&x)[kJT], const double (&y)[kJT],
+                                              double dx, double dy, double tol, double& c,
+                                              double& tn) {
+  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0, g0 = 0.0, g1 = 0.0;
 #pragma unroll
-  for (int r = 0; r <= kJRows; ++r) {
-    const int i = lane + 64 * r;
-    if (i < k) {
-      mp[i] = c * x[r] - sn * y[r];
-      mq[i] = sn * x[r] + c * y[r];
-      const double u = vp[i], v = vq[i];
-      vp[i] = c * u - sn * v;
-      vq[i] = sn * u + c * v;
-    }
+  for (int t = 0; t < kJT; t += 2) {
+    a0 = fma(x[t], x[t], a0);
+    b0 = fma(y[t], y[t], b0);
+    g0 = fma(x[t], y[t], g0);
+    a1 = fma(x[t + 1], x[t + 1], a1);
+    b1 = fma(y[t + 1], y[t + 1], b1);
+    g1 = fma(x[t + 1], y[t + 1], g1);
   }
-  return 1;
+  const double a = row_sum16(a0 + a1) * (dx * dx);
+  const double b = row_sum16(b0 + b1) * (dy * dy);
+  const double g = row_sum16(g0 + g1) * (dx * dy);
+  return jacobi_rotation(a, b, g, tol, c, tn);
+}
+
+// applies the rotation to the stored M and V columns of the pair
+__device__ __forceinline__ void apply_rotation(double (&x)[kJT], double (&vx)[kJT],
+                                               double (&y)[kJT], double (&vy)[kJT], double& dx,
+                                               double& dy, double c, double tn) {
+  const double tp = tn * dy * rcp_nr(dx), tq = tn * dx * rcp_nr(dy);
+#pragma unroll
+  for (int t = 0; t < kJT; ++t) {
+    const double xp = x[t], u = vx[t];
+    x[t] = fma(-tp, y[t], xp);
+    y[t] = fma(tq, xp, y[t]);
+    vx[t] = fma(-tp, vy[t], u);
+    vy[t] = fma(tq, u, vy[t]);
+  }
+  dx *= c;
+  dy *= c;
 }
 
 __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
   extern __shared__ __align__(16) double jsm[];
   const int k = g.k;
-  double* Ms = jsm;               // 32 columns of M, column c at c * k
-  double* Vs = jsm + 32 * k;      // 32 columns of V
+  double* Ms = jsm;                  // 32 columns of M, column c at c * kJLd
+  double* Vs = jsm + 32 * kJLd;      // 32 columns of V
+  __shared__ double dsc[32];         // scales of the LDS columns
   cg::grid_group grid = cg::this_grid();
-  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int t = threadIdx.x, lane = t & 63;
+  const int u = (t >> 6) * 4 + (lane >> 4);  // this row's pair slot 0..15
+  const int li = lane & 15;
   const int nblk = g.nblk;
   const int w = blockIdx.x;
-  // a pass moves 32 columns of M and V (at most 9 elements per thread each);
-  // all of a thread's global loads are issued before any is used
-  constexpr int kPer = 32 * kJMaxK / kJThreads;
-  auto gaddr = [&](int I, int J, int e) {
-    const int c = e / k, i = e - c * k;
-    return int64_t(c < kJB ? I * kJB + c : J * kJB + c - kJB) * k + i;
+  // the 16 columns of block B of M and V <-> LDS columns c0.. (zero rows from
+  // k up to 288).  Row u of the workgroup moves column u; every load is
+  // issued before the first LDS store.
+  auto load_block = [&](int B, int c0) {
+    if (li == 0) dsc[c0 + u] = 1.0;
+    double lm[kJT], lv[kJT];
+    const int64_t base = int64_t(B * kJB + u) * k;
+#pragma unroll
+    for (int q = 0; q < kJT; ++q) {
+      const int i = jrow(li, q);
+      lm[q] = i < k ? g.M[base + i] : 0.0;
+      lv[q] = i < k ? g.V[base + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kJT; ++q) {
+      Ms[(c0 + u) * kJLd + jrow(li, q)] = lm[q];
+      Vs[(c0 + u) * kJLd + jrow(li, q)] = lv[q];
+    }
   };
-  auto load = [&](int I, int J) {
-    double lm[kPer], lv[kPer];
+  auto store_block = [&](int B, int c0) {
+    const int64_t base = int64_t(B * kJB + u) * k;
+    const double d = dsc[c0 + u];
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = t + q * kJThreads;
-      if (e < 32 * k) {
-        const int64_t a = gaddr(I, J, e);
-        lm[q] = g.M[a];
-        lv[q] = g.V[a];
+    for (int q = 0; q < kJT; ++q) {
+      const int i = jrow(li, q);
+      if (i < k) {
+        g.M[base + i] = d * Ms[(c0 + u) * kJLd + i];
+        g.V[base + i] = d * Vs[(c0 + u) * kJLd + i];
       }
     }
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = t + q * kJThreads;
-      if (e < 32 * k) {
-        Ms[e] = lm[q];
-        Vs[e] = lv[q];
-      }
-    }
-    __syncthreads();
   };
-  auto store = [&](int I, int J) {
-    __syncthreads();
+  // a lane's 18 entries of LDS column c as 9 16-byte reads and writes
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  auto lds_col = [&](const double* base, int c, double (&x)[kJT]) {
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = t + q * kJThreads;
-      if (e < 32 * k) {
-        const int64_t a = gaddr(I, J, e);
-        g.M[a] = Ms[e];
-        g.V[a] = Vs[e];
-      }
+    for (int q = 0; q < kJT; q += 2) {
+      const f64x2 v = *reinterpret_cast<const f64x2*>(base + c * kJLd + jrow(li, q));
+      x[q] = v.x;
+      x[q + 1] = v.y;
     }
+  };
+  auto lds_put = [&](double* base, int c, const double (&x)[kJT]) {
+#pragma unroll
+    for (int q = 0; q < kJT; q += 2)
+      *reinterpret_cast<f64x2*>(base + c * kJLd + jrow(li, q)) = f64x2{x[q], x[q + 1]};
+  };
+  auto grid_barrier = [&]() {
     __threadfence();
     grid.sync();
   };
   int sweep = 0;
   for (; sweep < kJMaxSweeps; ++sweep) {
-    int nrot = 0;  // this wavefront's rotations (lane 0 counts)
-    // intra pass: blocks 2w and 2w+1, 8 wavefronts each
-    load(2 * w, 2 * w + 1);
+    int nrot = 0;  // this lane's rotations (lane 0 of each row counts)
+    // intra pass: blocks 2w and 2w+1 in LDS columns 0..15 and 16..31
+    load_block(2 * w, 0);
+    load_block(2 * w + 1, kJB);
+    __syncthreads();
     {
-      const int half = wave >> 3, a8 = wave & 7;
+      const int half = u >> 3, a8 = u & 7;
       for (int ir = 0; ir < 15; ++ir) {
         const int p = 16 * half + circle_item(a8, ir, 16);
         const int q = 16 * half + circle_item(15 - a8, ir, 16);
-        nrot += jacobi_pair(Ms, Vs, k, p, q, lane, g.tol);
+        double x[kJT], y[kJT];
+        lds_col(Ms, p, x);
+        lds_col(Ms, q, y);
+        double dx = dsc[p], dy = dsc[q], c, tn;
+        const bool r = pair_rotation(x, y, dx, dy, g.tol, c, tn);
+        if (__builtin_amdgcn_ballot_w64(r)) {
+          double vx[kJT], vy[kJT];
+          lds_col(Vs, p, vx);
+          lds_col(Vs, q, vy);
+          apply_rotation(x, vx, y, vy, dx, dy, c, tn);
+          if (r) {
+            lds_put(Ms, p, x);
+            lds_put(Vs, p, vx);
+            lds_put(Ms, q, y);
+            lds_put(Vs, q, vy);
+            if (li == 0) {
+              dsc[p] = dx;
+              dsc[q] = dy;
+            }
+          }
+        }
+        nrot += r && li == 0;
         __syncthreads();
       }
     }
-    store(2 * w, 2 * w + 1);
+    store_block(2 * w, 0);
+    store_block(2 * w + 1, kJB);
+    grid_barrier();
     // cross passes over the round-robin schedule of blocks
-    for (int r = 0; r < nblk - 1; ++r) {
-      const int I = circle_item(w, r, nblk), J = circle_item(nblk - 1 - w, r, nblk);
-      load(I, J);
+    for (int rr = 0; rr < nblk - 1; ++rr) {
+      const int I = circle_item(w, rr, nblk), J = circle_item(nblk - 1 - w, rr, nblk);
+      // column u of I (and of V) in registers, J in LDS columns 0..15
+      double x[kJT], vx[kJT];
+#pragma unroll
+      for (int q = 0; q < kJT; ++q) {
+        const int i = jrow(li, q);
+        const int64_t a = int64_t(I * kJB + u) * k + i;
+        x[q] = i < k ? g.M[a] : 0.0;
+        vx[q] = i < k ? g.V[a] : 0.0;
+      }
+      double dx = 1.0;
+      load_block(J, 0);
+      __syncthreads();
       for (int ir = 0; ir < 16; ++ir) {
-        nrot += jacobi_pair(Ms, Vs, k, wave, 16 + ((wave + ir) & 15), lane, g.tol);
+        const int q = (u + ir) & 15;
+        double y[kJT];
+        lds_col(Ms, q, y);
+        double dy = dsc[q], c, tn;
+        const bool r = pair_rotation(x, y, dx, dy, g.tol, c, tn);
+        if (__builtin_amdgcn_ballot_w64(r)) {
+          double vy[kJT];
+          lds_col(Vs, q, vy);
+          apply_rotation(x, vx, y, vy, dx, dy, c, tn);
+          if (r) {
+            lds_put(Ms, q, y);
+            lds_put(Vs, q, vy);
+            if (li == 0) dsc[q] = dy;
+          }
+        }
+        nrot += r && li == 0;
         __syncthreads();
       }
-      store(I, J);
+#pragma unroll
+      for (int q = 0; q < kJT; ++q) {
+        const int i = jrow(li, q);
+        if (i < k) {
+          const int64_t a = int64_t(I * kJB + u) * k + i;
+          g.M[a] = dx * x[q];
+          g.V[a] = dx * vx[q];
+        }
+      }
+      store_block(J, 0);
+      grid_barrier();
     }
-    if (lane == 0 && nrot) atomicAdd(g.rot + sweep, nrot);
-    __threadfence();
-    grid.sync();
+    // one atomic per wavefront
+    const int wrot = __builtin_amdgcn_readfirstlane(
+        static_cast<int>(__builtin_popcountll(__builtin_amdgcn_ballot_w64(nrot != 0))));
+    int tot = nrot;
+    tot += __shfl_xor(tot, 16, 64);
+    tot += __shfl_xor(tot, 32, 64);
+    if (lane == 0 && wrot) atomicAdd(g.rot + sweep, tot);
+    grid_barrier();
     const int total = __hip_atomic_load(g.rot + sweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (total == 0) {
       ++sweep;
@@ -327,12 +420,13 @@ int svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int tra
   svd_init_kernel<<<dim3(unsigned(ceil_div(n, 256))), 256, 0, ctx->stream>>>(
       int(k), kp, X, ldx, transpose, M, Vw, rot);
   RLA_LAUNCH_CHECK("svd_init");
-  const size_t lds = size_t(64) * k * sizeof(double);
+  const size_t lds = size_t(64) * kJLd * sizeof(double);
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&svd_jacobi_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(lds));
   });
   RLA_HIP(attr_err);
   JacobiArgs a{};

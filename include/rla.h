@@ -527,7 +527,10 @@ int rla_chol_inv_f64(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, doub
 /* RangeFinder iterations on Y = A*S (m x k, FP64, column-major, in/out):
  * orthogonalize != 0 runs rand_ortho_it (power_its.jl:53-80), otherwise
  * rand_power_it (power_its.jl:15-36); Y leaves as Q.
- * precision 0: products in A's type (a_elem 4 or 8; A column-major, lda);
+ * precision 0: products in A's type (a_elem 4 or 8; A column-major, lda) on
+ * the vendor BLAS; A_rm, when not NULL, is a second copy of A in A's type
+ * with rows contiguous (row stride ld_rm >= n), which turns every A'*X into
+ * a plain (non-transposed) product; A_cm is unused.
  * precision 1: products on the BF16 MFMA with FP32 accumulation, reading the
  * two resident BF16 copies of A (A_rm: rows contiguous, ld_rm; A_cm:
  * columns contiguous, ld_cm; from rla_to_bf16_*).  QR is FP64 throughout. */
@@ -535,7 +538,7 @@ int rla_rangefinder_workspace_bytes(int precision, int a_elem, int64_t m, int64_
                                     size_t* bytes);
 int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
                     int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
-                    const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
+                    const void* A_rm, int64_t ld_rm, const void* A_cm, int64_t ld_cm,
                     double* Y, int64_t ldy, void* work, size_t work_bytes);
 
 /* FP64 SVD of a k x k matrix, k <= 288 (the `svd(QA)` of randsvd.jl:186 once
@@ -551,12 +554,14 @@ int rla_svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int
 
 /* RandSVD finish (randsvd.jl:174-189) from the range Q (m x k): B' = A'Q,
  * B' = Qb Rb (CholeskyQR2), Rb' = Ur S Vr' (rla_svd_small_f64, k <= 288),
- * U = Q Ur, V = Qb Vr.
+ * U = Q Ur, V = Qb Vr.  A_cm: for precision 1 the columns-contiguous BF16
+ * copy of A; for precision 0 NULL or the rows-contiguous copy of A in A's
+ * type (as A_rm of rla_rangefinder).
  * Outputs U (m x k), S (k), V (n x k), FP64. */
 int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,
                                 size_t* bytes);
 int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_t k, int a_elem,
-                       const void* A, int64_t lda, const uint16_t* A_cm, int64_t ld_cm,
+                       const void* A, int64_t lda, const void* A_cm, int64_t ld_cm,
                        const double* Q, int64_t ldq, double* U, int64_t ldu, double* S, double* V,
                        int64_t ldv, void* work, size_t work_bytes);
 

@@ -95,6 +95,28 @@ class _Bf16Copies:
         return self.rm, self.cm
 
 
+class _RowCopy:
+    """The plain path's resident copy of A in A's type with rows contiguous:
+    A'*X then reads it as a column-major A' (a plain product) instead of a
+    transposed A.  A matrix that is already row-major is its own copy.
+    Rebuilt only when A's storage or version changes."""
+
+    def __init__(self):
+        self.key = None
+        self.rows = None
+
+    def get(self, A):
+        if A.dtype not in (torch.float32, torch.float64):
+            return None
+        v = as_view(A)
+        if v.cs == 1 and v.rs >= v.cols:   # rows already contiguous
+            return A
+        if not _dense.key_matches(self.key, A):
+            self.rows = A.contiguous()
+            self.key = _dense.SourceKey(A)
+        return self.rows
+
+
 def _range(recipe, A):
     """Q (m x k, FP64, column-major) for recipe's compressor, power_its and
     orthogonalize settings (power_its.jl:15-80)."""
@@ -121,7 +143,8 @@ def _range(recipe, A):
             mul(Yt, A_, S)
             Y = _dense.convert(Yt, torch.float64)
     else:
-        rm = cm = None
+        # the row copy makes A'*X a plain product, as A*X is on A itself
+        rm, cm = recipe._rows.get(A), None
         Yt = torch.empty(k, m, dtype=A.dtype, device=dev).T
         mul(Yt, A_, S)
         Y = _dense.convert(Yt, torch.float64)
@@ -129,9 +152,9 @@ def _range(recipe, A):
     p = PRECISIONS[prec]
     nbytes = _dense.query_bytes("rla_rangefinder_workspace_bytes", p, a_elem, m, n, k)
     work = _dense.workspace(nbytes, dev)
+    ld_rm = 0 if rm is None else (rm.shape[1] if prec == "bf16" else as_view(rm).rs)
     _lib.call("rla_rangefinder", _lib.ctx(dev), p, int(bool(recipe.orthogonalize)), m, n, k,
-              recipe.power_its, a_elem, _lib.ptr(A_), v.cs, _lib.ptr(rm),
-              rm.shape[1] if rm is not None else 0, _lib.ptr(cm),
+              recipe.power_its, a_elem, _lib.ptr(A_), v.cs, _lib.ptr(rm), ld_rm, _lib.ptr(cm),
               cm.shape[1] if cm is not None else 0, _lib.ptr(Y), m, _lib.ptr(work),
               work.numel() * 8)
     return Y, A_, rm, cm
@@ -175,6 +198,7 @@ class RangeFinderRecipe(ApproximatorRecipe):
         self.precision = precision
         self.range = None
         self._bf16 = _Bf16Copies()
+        self._rows = _RowCopy()
 
     def _run(self, A):
         Q, _, _, _ = _range(self, A)
@@ -238,6 +262,7 @@ class RandSVDRecipe(ApproximatorRecipe):
         self.precision = precision
         self.U = self.S = self.V = None
         self._bf16 = _Bf16Copies()
+        self._rows = _RowCopy()
 
     def _run(self, A):
         Q, A_, rm, cm = _range(self, A)
@@ -253,8 +278,11 @@ class RandSVDRecipe(ApproximatorRecipe):
         S = torch.empty(k, dtype=torch.float64, device=dev)
         nbytes = _dense.query_bytes("rla_randsvd_workspace_bytes", p, a_elem, m, n, k)
         work = _dense.workspace(nbytes, dev)
+        # A'Q reads the BF16 columns copy, or the plain path's row copy
+        at = cm if self.precision == "bf16" else rm
+        ld_at = 0 if at is None else (at.shape[1] if self.precision == "bf16" else as_view(at).rs)
         _lib.call("rla_randsvd_finish", _lib.ctx(dev), p, m, n, k, a_elem, _lib.ptr(A_),
-                  as_view(A_).cs, _lib.ptr(cm), cm.shape[1] if cm is not None else 0,
+                  as_view(A_).cs, _lib.ptr(at), ld_at,
                   _lib.ptr(Q), m, _lib.ptr(U), m, _lib.ptr(S), _lib.ptr(V), n, _lib.ptr(work),
                   work.numel() * 8)
         self.U, self.S, self.V = U, S, V

@@ -352,47 +352,55 @@ static size_t rf_layout(int precision, int a_elem, int64_t m, int64_t n, int64_t
   return b.need;
 }
 
-// Z (n x k, f64) = A' * Y (Y m x k, f64)
+// Z (n x k, f64) = A' * Y (Y m x k, f64).  At: for bf16 the columns-
+// contiguous BF16 copy of A; for plain, NULL or a copy of A in A's type with
+// rows contiguous (row stride ld_t), i.e. A' column-major, which makes A'Y a
+// plain NN product instead of a transposed read of A.
 static int rf_at_y(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n, int64_t k,
-                   const void* A, int64_t lda, const uint16_t* A_cm, int64_t ld_cm,
+                   const void* A, int64_t lda, const void* At, int64_t ld_t,
                    const double* Y, int64_t ldy, RfBufs& w) {
   rocblas_handle h;
   int rc;
   if (precision == PREC_BF16) {
     rc = to_bf16_rows<double>(ctx, k, m, Y, ldy, 1, w.Ybf, ld8(m));
     if (rc) return rc;
-    rc = gemm_bf16_tn(ctx, n, k, m, 1.f, A_cm, ld_cm, w.Ybf, ld8(m), 0.f, w.Z32, 1, n);
+    rc = gemm_bf16_tn(ctx, n, k, m, 1.f, static_cast<const uint16_t*>(At), ld_t, w.Ybf, ld8(m),
+                      0.f, w.Z32, 1, n);
     if (rc) return rc;
     return convert<float, double>(ctx, n, k, w.Z32, n, w.Z64, n);
   }
   rc = blas(ctx, &h);
   if (rc) return rc;
+  const rocblas_operation op_a = At ? rocblas_operation_none : rocblas_operation_transpose;
+  const void* Aop = At ? At : A;
+  const int64_t ld_op = At ? ld_t : lda;
   if (a_elem == 8) {
     const double one = 1.0, zero = 0.0;
-    RLA_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, int(n), int(k),
-                           int(m), &one, static_cast<const double*>(A), int(lda), Y, int(ldy),
-                           &zero, w.Z64, int(n)));
+    RLA_BLAS(rocblas_dgemm(h, op_a, rocblas_operation_none, int(n), int(k), int(m), &one,
+                           static_cast<const double*>(Aop), int(ld_op), Y, int(ldy), &zero,
+                           w.Z64, int(n)));
     return RLA_OK;
   }
   const float one = 1.f, zero = 0.f;
   rc = convert<double, float>(ctx, m, k, Y, ldy, w.Y32, m);
   if (rc) return rc;
-  RLA_BLAS(rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, int(n), int(k),
-                         int(m), &one, static_cast<const float*>(A), int(lda), w.Y32, int(m),
-                         &zero, w.Z32, int(n)));
+  RLA_BLAS(rocblas_sgemm(h, op_a, rocblas_operation_none, int(n), int(k), int(m), &one,
+                         static_cast<const float*>(Aop), int(ld_op), w.Y32, int(m), &zero,
+                         w.Z32, int(n)));
   return convert<float, double>(ctx, n, k, w.Z32, n, w.Z64, n);
 }
 
 // Y (m x k, f64) = A * Z (Z n x k, f64 in w.Z64)
 static int rf_a_z(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n, int64_t k,
-                  const void* A, int64_t lda, const uint16_t* A_rm, int64_t ld_rm, double* Y,
+                  const void* A, int64_t lda, const void* A_rm, int64_t ld_rm, double* Y,
                   int64_t ldy, RfBufs& w) {
   rocblas_handle h;
   int rc;
   if (precision == PREC_BF16) {
     rc = to_bf16_rows<double>(ctx, k, n, w.Z64, n, 1, w.Zbf, ld8(n));
     if (rc) return rc;
-    rc = gemm_bf16_tn(ctx, m, k, n, 1.f, A_rm, ld_rm, w.Zbf, ld8(n), 0.f, w.Y32, 1, m);
+    rc = gemm_bf16_tn(ctx, m, k, n, 1.f, static_cast<const uint16_t*>(A_rm), ld_rm, w.Zbf,
+                      ld8(n), 0.f, w.Y32, 1, m);
     if (rc) return rc;
     return convert<float, double>(ctx, m, k, w.Y32, m, Y, ldy);
   }
@@ -415,7 +423,7 @@ static int rf_a_z(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n,
 }
 
 static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, const void* A,
-                    const uint16_t* A_rm, const uint16_t* A_cm) {
+                    const void* A_rm, const void* A_cm) {
   if (precision != PREC_PLAIN && precision != PREC_BF16)
     return fail(RLA_EARG, "precision must be 0 (plain) or 1 (bf16 products)");
   if (a_elem != 4 && a_elem != 8) return fail(RLA_EARG, "A element size must be 4 or 8");
@@ -425,6 +433,13 @@ static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, 
   if (precision == PREC_PLAIN && !A) return fail(RLA_EARG, "plain precision needs A");
   if (precision == PREC_BF16 && (!A_rm || !A_cm))
     return fail(RLA_EARG, "bf16 precision needs both BF16 copies of A");
+  return RLA_OK;
+}
+
+// the plain path's optional rows-contiguous copy of A (row stride ld >= n)
+static int check_rows_copy(int precision, const void* At, int64_t n, int64_t ld) {
+  if (precision == PREC_PLAIN && At && ld < n)
+    return fail(RLA_EARG, "the row copy of A needs a row stride >= n");
   return RLA_OK;
 }
 
@@ -445,9 +460,12 @@ static int64_t rf_info_slots(int orthogonalize, int64_t power_its) {
 // only the returned Q takes the second pass of CholeskyQR2.
 static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
                   int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
-                  const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
+                  const void* A_rm, int64_t ld_rm, const void* A_cm, int64_t ld_cm,
                   double* Y, int64_t ldy, RfBufs& w, int* info) {
   int rc;
+  // the copy that A'X reads: bf16 columns-contiguous, or plain rows-contiguous
+  const void* At = precision == PREC_BF16 ? A_cm : A_rm;
+  const int64_t ld_t = precision == PREC_BF16 ? ld_cm : ld_rm;
   int64_t ns = 0;
   auto qr = [&](int64_t rows, double* M, int64_t ld, bool last) {
     const int passes = last ? 2 : 1;
@@ -460,7 +478,7 @@ static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int
     rc = qr(m, Y, ldy, power_its == 0);
     if (rc) return rc;
     for (int64_t it = 0; it < power_its; ++it) {
-      rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
+      rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, At, ld_t, Y, ldy, w);
       if (rc) return rc;
       rc = qr(n, w.Z64, n, false);
       if (rc) return rc;
@@ -473,7 +491,7 @@ static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int
   }
   // rand_power_it: products only, one QR at the end (power_its.jl:15-36)
   for (int64_t it = 0; it < power_its; ++it) {
-    rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, A_cm, ld_cm, Y, ldy, w);
+    rc = rf_at_y(ctx, precision, a_elem, m, n, k, A, lda, At, ld_t, Y, ldy, w);
     if (rc) return rc;
     rc = rf_a_z(ctx, precision, a_elem, m, n, k, A, lda, A_rm, ld_rm, Y, ldy, w);
     if (rc) return rc;
@@ -566,10 +584,12 @@ int rla_rangefinder_workspace_bytes(int precision, int a_elem, int64_t m, int64_
 
 int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
                     int64_t k, int64_t power_its, int a_elem, const void* A, int64_t lda,
-                    const uint16_t* A_rm, int64_t ld_rm, const uint16_t* A_cm, int64_t ld_cm,
+                    const void* A_rm, int64_t ld_rm, const void* A_cm, int64_t ld_cm,
                     double* Y, int64_t ldy, void* work, size_t work_bytes) {
   RLA_CHECK_CTX(ctx);
   int rc = check_rf(precision, a_elem, m, n, k, A, A_rm, A_cm);
+  if (rc) return rc;
+  rc = check_rows_copy(precision, A_rm, n, ld_rm);
   if (rc) return rc;
   if (power_its < 0) return fail(RLA_EARG, "Field `power_its` must be non-negative.");
   rc = use_device(ctx);
@@ -617,11 +637,13 @@ int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n,
 }
 
 int rla_randsvd_finish(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_t k, int a_elem,
-                       const void* A, int64_t lda, const uint16_t* A_cm, int64_t ld_cm,
+                       const void* A, int64_t lda, const void* A_cm, int64_t ld_cm,
                        const double* Q, int64_t ldq, double* U, int64_t ldu, double* S, double* V,
                        int64_t ldv, void* work, size_t work_bytes) {
   RLA_CHECK_CTX(ctx);
   int rc = check_rf(precision, a_elem, m, n, k, A, A_cm, A_cm);
+  if (rc) return rc;
+  rc = check_rows_copy(precision, A_cm, n, ld_cm);
   if (rc) return rc;
   if (k > n) return fail(RLA_EARG, "RandSVD needs k <= n");
   if (k > 288) return fail(RLA_ENOTSUP, "RandSVD: the small SVD takes k <= 288");

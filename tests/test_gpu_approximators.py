@@ -317,6 +317,50 @@ def test_rangefinder_plain_matches_oracle(rla, dev, ortho, q, dtype, tol):
     assert np.linalg.norm(Q @ Q.T - want @ want.T) < tol * 25
 
 
+@pytest.mark.parametrize("dtype,tol", [(F64, 1e-12), (F32, 1e-5)])
+def test_rangefinder_plain_row_copy_matches_transposed_read(rla, dev, dtype, tol):
+    # the plain path's A'X through the rows-contiguous copy (an NN product) and
+    # through a transposed read of A give the same range; the recipe keeps the
+    # copy of a column-major A and uses a row-major A as its own copy
+    import ctypes
+    A, _ = _spectrum_matrix(500, 300, 20.0, 5)
+    m, n, k = 500, 300, 24
+    Ad = colmajor(A, dev, dtype)
+    Om = torch.tensor(rnd((n, k), 8), dtype=F64, device=dev)
+    Y0 = (Ad.double() @ Om).T.contiguous().T
+    a_elem = 4 if dtype == F32 else 8
+    nb = ctypes.c_size_t(0)
+    rla._lib.call("rla_rangefinder_workspace_bytes", 0, a_elem, m, n, k, ctypes.byref(nb))
+    work = torch.empty(nb.value // 8 + 1, dtype=F64, device=dev)
+    rows = Ad.contiguous()
+    out = {}
+    for name, copy in (("transposed", None), ("copy", rows)):
+        Y = Y0.clone().T.contiguous().T
+        rla._lib.call("rla_rangefinder", rla._lib.ctx(dev), 0, 1, m, n, k, 2, a_elem,
+                      rla._lib.ptr(Ad), m, rla._lib.ptr(copy), n if copy is not None else 0,
+                      None, 0, rla._lib.ptr(Y), m, rla._lib.ptr(work), work.numel() * 8)
+        out[name] = Y.cpu().numpy()
+    P0 = out["transposed"] @ out["transposed"].T
+    P1 = out["copy"] @ out["copy"].T
+    assert np.linalg.norm(P0 - P1) < tol * k
+    with pytest.raises(rla.ArgumentError):
+        rla._lib.call("rla_rangefinder", rla._lib.ctx(dev), 0, 1, m, n, k, 2, a_elem,
+                      rla._lib.ptr(Ad), m, rla._lib.ptr(rows), n - 1, None, 0,
+                      rla._lib.ptr(Y0), m, rla._lib.ptr(work), work.numel() * 8)
+    rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=k,
+                                                 type=dtype), power_its=2, orthogonalize=True)
+    rec = rla.complete_approximator(rf, Ad)
+    rla.rapproximate_(rec, Ad)
+    assert rec._rows.rows is not None and rec._rows.rows.is_contiguous()
+    Ar = Ad.contiguous()                   # row-major: its own row copy
+    rec2 = rla.complete_approximator(rf, Ar)
+    rec2.compressor = rec.compressor
+    rla.rapproximate_(rec2, Ar)
+    assert rec2._rows.rows is None
+    Q, Q2 = rec.range.double().cpu().numpy(), rec2.range.double().cpu().numpy()
+    assert np.linalg.norm(Q @ Q.T - Q2 @ Q2.T) < tol * k
+
+
 def test_rangefinder_exact_recovery_sparse_sign(rla, dev):
     # default compressor SparseSign(Right()), rank 6 < k: A = QQ'A to 1e-10
     A = R._q(rnd((400, 6), 1)) @ rnd((6, 150), 2)

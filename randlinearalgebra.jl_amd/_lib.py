@@ -5,6 +5,7 @@ CPU fallback: if the shared library is missing, or no GPU is visible, calls
 raise instead of computing anything on the host.
 """
 import ctypes
+import itertools
 import os
 import threading
 
@@ -250,6 +251,7 @@ _ctxs_lock = threading.Lock()
 def _destroy(keys):
     lib = load()
     for key in keys:
+        _ctx_use.pop(key, None)
         c = _ctxs.pop(key, None)
         if c is not None and c.handle:
             lib.rla_ctx_destroy(c.handle)  # waits for its stream, frees its workspace
@@ -262,23 +264,40 @@ def drop_thread_contexts():
         _destroy([k for k in _ctxs if k[0] == me])
 
 
+# contexts one thread keeps per device (one per stream it uses), least
+# recently used destroyed first
+MAX_STREAM_CONTEXTS = 8
+_ctx_use = {}                 # key -> tick of its last use
+_ticks = itertools.count(1)
+
+
 def ctx(device=None):
-    """The calling thread's context for `device`, bound to torch's current stream."""
+    """The calling thread's context for `device` and torch's current stream.
+
+    A context owns a workspace that its calls use asynchronously, so one
+    context serves one stream: calls on two streams from one thread then run
+    concurrently instead of being ordered through a shared workspace."""
     if not torch.cuda.is_available():
         raise RuntimeError("no GPU visible: the rla kernels run on the device only")
     if device is None:
         device = torch.cuda.current_device()
     elif isinstance(device, torch.device):
         device = device.index if device.index is not None else torch.cuda.current_device()
-    key = (threading.get_ident(), device)
+    me = threading.get_ident()
+    stream = torch.cuda.current_stream(device).cuda_stream
+    key = (me, device, stream)
     c = _ctxs.get(key)
     if c is None:
         with _ctxs_lock:
             # contexts of threads that have ended hold workspaces nobody uses
             alive = {t.ident for t in threading.enumerate()}
             _destroy([k for k in _ctxs if k[0] not in alive])
+            mine = sorted((k for k in _ctxs if k[0] == me and k[1] == device),
+                          key=lambda k: _ctx_use.get(k, 0))
+            _destroy(mine[:max(0, len(mine) + 1 - MAX_STREAM_CONTEXTS)])
             c = _Ctx(device)
             _ctxs[key] = c
+    _ctx_use[key] = next(_ticks)
     return c.bind()
 
 

@@ -480,7 +480,8 @@ def ls_batched_c4(rla, torch, reps=5):
             "sketch_kernel_ms": sketch_ms, "sketch_GB_s": sbytes / sketch_ms / 1e6,
             "sketch_frac_hbm_peak": sbytes / sketch_ms / 1e6 / HBM_PEAK_GBS,
             "tile_config": {"cols_per_tile": cfg["cols_per_tile"],
-                            "rows_per_chunk": cfg["rows_per_chunk"]},
+                            "rows_per_chunk": cfg["rows_per_chunk"],
+                            "bucket": batch.tile_key, "from_tile_cache": batch.tile_cached},
             "breakdowns": int(sum(1 for i in info if i)),
             "rel_error_median": float(np.median(err)), "rel_error_max": float(err.max())}
 
@@ -584,10 +585,18 @@ def serving_c5(rla, torch, qps=80.0, n_req=96):
             st["kaczmarz_all_converged"] = all(r[1].converged for r, k in zip(res, kinds) if k)
             out[label] = st
             del res, futs
-    out["p99_ratio_plain_over_served"] = (out["queue_plain_1slot"]["p99_ms"] /
-                                          out["served_bf16_4slots"]["p99_ms"])
-    out["solves_per_s_ratio_served_over_plain"] = (out["served_bf16_4slots"]["solves_per_s"] /
-                                                   out["queue_plain_1slot"]["solves_per_s"])
+    # one variable at a time: continuous batching (4 slots vs a 1-slot queue,
+    # both plain), then precision (BF16 vs plain products, both 4 slots);
+    # the last pair changes both
+    q, sp, sb = out["queue_plain_1slot"], out["served_plain_4slots"], out["served_bf16_4slots"]
+    out["batching_p99_ratio_queue_over_served_plain"] = q["p99_ms"] / sp["p99_ms"]
+    out["batching_solves_per_s_ratio"] = sp["solves_per_s"] / q["solves_per_s"]
+    out["precision_p99_ratio_plain_over_bf16_4slots"] = sp["p99_ms"] / sb["p99_ms"]
+    out["precision_solves_per_s_ratio_4slots"] = sb["solves_per_s"] / sp["solves_per_s"]
+    out["precision_rangefinder_p99_ratio_4slots"] = (sp["rangefinder"]["p99_ms"] /
+                                                      sb["rangefinder"]["p99_ms"])
+    out["p99_ratio_plain_over_served"] = q["p99_ms"] / sb["p99_ms"]
+    out["solves_per_s_ratio_served_over_plain"] = sb["solves_per_s"] / q["solves_per_s"]
     return out
 
 

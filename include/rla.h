@@ -659,7 +659,7 @@ typedef struct rla_ls_plan rla_ls_plan;
 /* Plans a batch: validates, allocates the per-problem state (SA, Gram,
  * factor) in one device allocation, picks the sketch tile shape (autotune: 0
  * default, 1 time the three candidates on this batch, 2..4 force candidate
- * 0..2), and captures the batch (persistent
+ * 0..2, e.g. from a tile cache, and time it alone), and captures the batch (persistent
  * sketch kernel, batched FP64 Gram, batched Cholesky, back-substitution) in a
  * hipGraph.  The problem data must stay valid while the plan is used. */
 int rla_ls_plan_create(rla_ctx* ctx, int64_t nprob, const rla_ls_problem* probs, int autotune,

@@ -11,6 +11,10 @@ sketched) matrix of any size, Float32 or Float64: an FP64 QR of A (CholeskyQR2
 with the Householder fallback), Q'b and a triangular solve.
 """
 import ctypes
+import json
+import math
+import os
+import threading
 
 import torch
 
@@ -94,6 +98,37 @@ class _Plan:
             pass
 
 
+# Tile shapes chosen by the autotuner, by batch bucket: a committed JSON file
+# beside this module (read), or the file named by RLA_TILE_CACHE (read, and
+# written when a new bucket is tuned).  A hit skips the plan-time timing.
+_TILE_CACHE_DEFAULT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ls_tiles.json")
+_tile_lock = threading.Lock()
+
+
+def _tile_cache_path():
+    return os.environ.get("RLA_TILE_CACHE", _TILE_CACHE_DEFAULT)
+
+
+def _load_tiles(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def tile_bucket(device, shapes):
+    """The autotuner's key for a batch: the GPU's architecture and the batch's
+    (m, n) bucket, i.e. the problem count, the median log2(m) and the median
+    n in steps of 32 (rounded), so batches of the same make share a tile."""
+    arch = torch.cuda.get_device_properties(device).gcnArchName.split(":")[0]
+    ms = sorted(int(round(math.log2(max(m, 1)))) for m, _, _ in shapes)
+    ns = sorted((n + 16) // 32 for _, n, _ in shapes)
+    cnt = int(round(math.log2(len(shapes))))
+    return f"{arch}|count2^{cnt}|m2^{ms[len(ms) // 2]}|n{32 * ns[len(ns) // 2]}"
+
+
 def _check_problem(A, b, x):
     Av = as_view(A)
     if Av.dtype != torch.float32 or b.dtype != torch.float32 or x.dtype != torch.float32:
@@ -132,7 +167,39 @@ class SketchSolveBatch:
                                   _addr(op.rowptr), _addr(op.colidx), _addr(op.val_t), s))
             self.shapes.append((Av.rows, Av.cols, s))
         keep.append(self.recipes)
-        self.plan = _Plan(As[0].device, probs, keep, autotune)
+        dev = As[0].device
+        self.tile_key, self.tile_cached = None, False
+        if autotune == 1:
+            self.tile_key = tile_bucket(dev, self.shapes)
+            path = _tile_cache_path()
+            hit = _load_tiles(path).get(self.tile_key)
+            if hit is None and path != _TILE_CACHE_DEFAULT:
+                hit = _load_tiles(_TILE_CACHE_DEFAULT).get(self.tile_key)
+            if hit is not None and hit.get("shape") in (0, 1, 2):
+                autotune = 2 + int(hit["shape"])   # the cached shape, no timing
+                self.tile_cached = True
+        self.plan = _Plan(dev, probs, keep, autotune)
+        if self.tile_key is not None and not self.tile_cached:
+            cfg = self.plan.config()
+            ms = cfg["tuned_ms"]
+            best = min(range(3), key=lambda c: ms[c] if ms[c] >= 0 else float("inf"))
+            self._store_tile(self.tile_key, {"shape": best, "cols_per_tile": cfg["cols_per_tile"],
+                                             "rows_per_chunk": cfg["rows_per_chunk"],
+                                             "tuned_ms": ms})
+
+    @staticmethod
+    def _store_tile(key, entry):
+        path = _tile_cache_path()
+        with _tile_lock:
+            d = _load_tiles(path)
+            d[key] = entry
+            try:
+                tmp = path + ".tmp"
+                with open(tmp, "w") as f:
+                    json.dump(d, f, indent=1, sort_keys=True)
+                os.replace(tmp, path)
+            except OSError:
+                pass  # a read-only install keeps tuning at plan time
 
     def run(self):
         self.plan.execute()

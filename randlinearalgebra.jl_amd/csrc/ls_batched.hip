@@ -489,14 +489,16 @@ int rla_ls_plan_create(rla_ctx* ctx, int64_t nprob, const rla_ls_problem* probs,
     return cleanup(fail(RLA_EHIP, "rla_ls_plan_create: descriptor upload failed"));
   // autotune the sketch tile shape on this batch (min of 3 timed runs each)
   // autotune: 0 = default shape, 1 = time all three, 2..4 = force shape 0..2
+  // (a shape from the caller's tile cache: only that shape is timed, for
+  // rla_ls_plan_config)
   const int forced = autotune >= 2 && autotune <= 4 ? autotune - 2 : -1;
   if (autotune > 4 || autotune < 0) return cleanup(fail(RLA_EARG, "autotune must be 0..4"));
   const bool tune = autotune == 1;
   plan->cfg = forced >= 0 ? forced : 1;
   for (int c = 0; c < 3; ++c) plan->tuned_ms[c] = -1.f;
-  const int ncfg = tune ? 3 : 0;
+  const int ncfg = tune ? 3 : (forced >= 0 ? 1 : 0);
   for (int ci = 0; ci < ncfg; ++ci) {
-    const int c = ci;
+    const int c = tune ? ci : forced;
     build_sketch_tiles(v, kCfgCB[c], st);
     plan->nstiles = static_cast<int>(st.size());
     if (hipMemcpy(plan->stiles, st.data(), sizeof(SketchTile) * st.size(), hipMemcpyHostToDevice))

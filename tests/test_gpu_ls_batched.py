@@ -67,8 +67,11 @@ def test_batch_sketch_bit_exact_and_solution(rla, dev, cfg):
     batch.close()
 
 
-def test_autotuned_plan_and_custom_sketch_sizes(rla, dev):
-    # s = 123 is not a multiple of 4 (the Gram kernel's unaligned-column path)
+def test_autotuned_plan_and_custom_sketch_sizes(rla, dev, monkeypatch, tmp_path):
+    # s = 123 is not a multiple of 4 (the Gram kernel's unaligned-column path);
+    # an empty tile cache, so all three shapes are timed
+    monkeypatch.setenv("RLA_TILE_CACHE", str(tmp_path / "tiles.json"))
+    monkeypatch.setattr(rla.batched, "_TILE_CACHE_DEFAULT", str(tmp_path / "none.json"))
     probs = [make_problem(m, n, 50 + i)
              for i, (m, n) in enumerate([(3000, 40), (800, 64), (2000, 33)])]
     As = [colmajor(A, dev) for A, _, _ in probs]
@@ -88,6 +91,37 @@ def test_autotuned_plan_and_custom_sketch_sizes(rla, dev):
         xls = np.linalg.lstsq(SA, Sb, rcond=None)[0]
         got = xs[p].double().cpu().numpy()
         assert np.linalg.norm(got - xls) <= 1e-5 * np.linalg.norm(xls), p
+
+
+def test_tile_cache_written_then_used(rla, dev, monkeypatch, tmp_path):
+    # the first plan of a bucket times the three shapes and records the best;
+    # a second batch of the same bucket takes the recorded shape (timing only
+    # that one) and still sketches bit-exactly
+    import json
+    path = tmp_path / "tiles.json"
+    monkeypatch.setenv("RLA_TILE_CACHE", str(path))
+    monkeypatch.setattr(rla.batched, "_TILE_CACHE_DEFAULT", str(tmp_path / "none.json"))
+    probs = [make_problem(m, n, 70 + i) for i, (m, n) in enumerate([(3000, 40), (2500, 48)])]
+    As = [colmajor(A, dev) for A, _, _ in probs]
+    bs = [torch.as_tensor(b).to(dev) for _, b, _ in probs]
+    x1 = [torch.zeros(A.shape[1], dtype=F32, device=dev) for A, _, _ in probs]
+    x2 = [torch.zeros(A.shape[1], dtype=F32, device=dev) for A, _, _ in probs]
+    b1 = rla.SketchSolveBatch(As, bs, x1, autotune=1)
+    assert not b1.tile_cached and min(b1.config()["tuned_ms"]) > 0
+    saved = json.loads(path.read_text())
+    assert b1.tile_key in saved and saved[b1.tile_key]["shape"] in (0, 1, 2)
+    shape = saved[b1.tile_key]["shape"]
+    rla.seed(5)
+    b2 = rla.SketchSolveBatch(As, bs, x2, autotune=1)
+    assert b2.tile_cached and b2.tile_key == b1.tile_key
+    ms = b2.config()["tuned_ms"]
+    assert ms[shape] > 0 and all(ms[c] == -1 for c in range(3) if c != shape)
+    assert b2.config()["cols_per_tile"] == b1.config()["cols_per_tile"]
+    b1.run()
+    b2.run()
+    assert b1.info() == [0, 0] and b2.info() == [0, 0]
+    for p, (A, b, _) in enumerate(probs):
+        assert np.array_equal(b2.sketch(p).cpu().numpy(), sketch_oracle(b2.recipes[p], A, b))
 
 
 def test_rank_deficient_problem_reports_breakdown(rla, dev):

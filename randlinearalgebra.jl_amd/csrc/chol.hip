@@ -400,18 +400,25 @@ __global__ __launch_bounds__(CholShape<NBLK>::threads) void chol_blocks_batched_
                          it.info);
 }
 
-// Y <- Y R^-1 in place.  Each wavefront owns 16 rows and solves its column
+// Yout <- Yin R^-1.  Each wavefront owns 16 rows and solves its column
 // blocks in order, Ynew_b = (Y_b - sum_{l<b} Ynew_l R_lb) inv(R_bb), keeping
 // the solved blocks in registers in the MFMA operand layout.  The panel of
 // block column b (-R_lb for l < b, and inv(R_bb)) is staged in LDS once per
 // workgroup; the next panel's loads are in flight while the current one is
 // used.  A panel block is stored row by row (element (r, c) at 16 r + c), so
 // the 64 lanes of an operand read touch 64 consecutive doubles.
-template <int NBLK>
+// Yin is FP64 or FP32 (widened exactly).  OUT64: the result to Yout (FP64;
+// Yout may be Yin itself: a wavefront reads a block of its rows before it
+// writes it).  OUTBF: its BF16 rounding f32_to_bf16_bits(float(x)) to Ybf
+// (column c at Ybf + c * ld_bf), the operand of a mixed-precision product.
+// Nothing is written when any of the ninfo breakdown flags is set.
+template <int NBLK, class Tin, bool OUT64, bool OUTBF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trsm_ru_f64_kernel(
-    int64_t m, int k, double* Y, int64_t ldy, const double* __restrict__ Rt, int ldt,
-    const double* __restrict__ Dinv, const int* __restrict__ info) {
-  if (info && *info) return;
+    int64_t m, int k, const Tin* Yin, int64_t ldy, double* Yout, int64_t ldo, uint16_t* Ybf,
+    int64_t ld_bf, const double* __restrict__ Rt, int ldt, const double* __restrict__ Dinv,
+    const int* __restrict__ info, int ninfo) {
+  for (int i = 0; i < ninfo; ++i)
+    if (info[i]) return;
   __shared__ double Ps[2][NBLK][256];  // slot l < b: -R_lb; slot NBLK - 1: inv(R_bb)
   __shared__ double Ts[4][16][17];
   const int t = threadIdx.x;
@@ -424,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   const bool row_ok = row < m;
   // loads are unconditional from clamped (in-bounds) addresses and zeroed
   // afterwards, so they carry no branches and can all be in flight at once
-  const double* yrow = Y + (row_ok ? row : m - 1);
+  const Tin* yrow = Yin + (row_ok ? row : m - 1);
   double (*ts)[17] = Ts[wave];
   double ya[NBLK][4];
   double yn[4];
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = b * 16 + 4 * q + lr;
-      const double x = yrow[int64_t(c < k ? c : k - 1) * ldy];
+      const double x = static_cast<double>(yrow[int64_t(c < k ? c : k - 1) * ldy]);
       yn[q] = (row_ok && c < k) ? x : 0.0;
     }
   };
@@ -506,7 +513,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     for (int q = 0; q < 4; ++q) {
       ya[b][q] = ts[lc][4 * q + lr];
       const int c = b * 16 + 4 * q + lr;
-      if (row_ok && c < k) Y[row + int64_t(c) * ldy] = ya[b][q];
+      if (row_ok && c < k) {
+        if (OUT64) Yout[row + int64_t(c) * ldo] = ya[b][q];
+        if (OUTBF) Ybf[row + int64_t(c) * ld_bf] = f32_to_bf16_bits(static_cast<float>(ya[b][q]));
+      }
     }
     lane_exchange();
     if (more) {
@@ -558,25 +568,48 @@ int chol_blocks_batched(hipStream_t stream, int64_t nitems, const CholBatchItem*
   return RLA_OK;
 }
 
-int trsm_ru_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, const double* Rt,
-                int64_t ldt, const double* Dinv, const int* info) {
+template <class Tin>
+int trsm_ru(rla_ctx* ctx, int64_t m, int64_t k, const Tin* Yin, int64_t ldy, double* Yout,
+            int64_t ldo, uint16_t* Ybf, int64_t ld_bf, const double* Rt, int64_t ldt,
+            const double* Dinv, const int* info, int ninfo) {
   const int nb = chol_nblk(k);
   if (!nb) return fail(RLA_ENOTSUP, "trsm_ru_f64: k must be in 1..288");
+  if ((Yout != nullptr) == (Ybf != nullptr))
+    return fail(RLA_EARG, "trsm_ru_f64: exactly one of the FP64 and BF16 outputs");
   if (m <= 0) return RLA_OK;
+  if (!info) ninfo = 0;
   const int ki = static_cast<int>(k);
   const unsigned grid = static_cast<unsigned>(ceil_div(m, 64));
-#define RLA_TRSM(NB)                                                                         \
-  case NB:                                                                                   \
-    trsm_ru_f64_kernel<NB><<<dim3(grid), 256, 0, ctx->stream>>>(m, ki, Y, ldy, Rt,           \
-                                                                static_cast<int>(ldt), Dinv, \
-                                                                info);                       \
+#define RLA_TRSM_GO(NB, O64, OBF)                                                            \
+  trsm_ru_f64_kernel<NB, Tin, O64, OBF><<<dim3(grid), 256, 0, ctx->stream>>>(                \
+      m, ki, Yin, ldy, Yout, ldo, Ybf, ld_bf, Rt, static_cast<int>(ldt), Dinv, info, ninfo)
+#define RLA_TRSM(NB)                    \
+  case NB:                              \
+    if (Yout)                           \
+      RLA_TRSM_GO(NB, true, false);     \
+    else                                \
+      RLA_TRSM_GO(NB, false, true);     \
     break;
   switch (nb) {
     RLA_TRSM(2) RLA_TRSM(4) RLA_TRSM(8) RLA_TRSM(12) RLA_TRSM(16) RLA_TRSM(17) RLA_TRSM(18)
   }
 #undef RLA_TRSM
+#undef RLA_TRSM_GO
   RLA_LAUNCH_CHECK("trsm_ru_f64");
   return RLA_OK;
+}
+
+template int trsm_ru<double>(rla_ctx*, int64_t, int64_t, const double*, int64_t, double*,
+                             int64_t, uint16_t*, int64_t, const double*, int64_t, const double*,
+                             const int*, int);
+template int trsm_ru<float>(rla_ctx*, int64_t, int64_t, const float*, int64_t, double*, int64_t,
+                            uint16_t*, int64_t, const double*, int64_t, const double*,
+                            const int*, int);
+
+// Y <- Y R^-1 in place; skipped when *info != 0
+int trsm_ru_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, const double* Rt,
+                int64_t ldt, const double* Dinv, const int* info) {
+  return trsm_ru<double>(ctx, m, k, Y, ldy, Y, ldy, nullptr, 0, Rt, ldt, Dinv, info, 1);
 }
 
 __global__ void identity_kernel(int k, double* X, int64_t ldx) {

@@ -33,14 +33,6 @@ constexpr int kBM = 128;        // rows per workgroup
 constexpr int kBK = 64;         // K per step
 constexpr int kBstride = 72;    // bf16 per staged B column (64 + 8 pad)
 
-// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
-__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return uint16_t((u >> 16) | 0x40);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
-
 // dst[l + i*ldd] = bf16(src[i, l]) for i < rows, l < cols of a strided
 // source (the source's (rows x cols) view); i.e. rows of the view become
 // contiguous runs.  Tiled through LDS so both sides stay coalesced.

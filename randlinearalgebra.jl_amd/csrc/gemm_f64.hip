@@ -140,7 +140,8 @@ size_t gemm_f64_split_bytes(int64_t M, int64_t N) { return size_t(16) * M * N * 
 //
 // Gram G = Y'Y: each workgroup takes a contiguous range of rows of Y and
 // computes every upper-triangular 16 x 16 block of G for it, so Y is read
-// from HBM exactly once.  Rows are staged 16 at a time in LDS (double
+// from HBM exactly once.  Y is FP64, or FP32 (widened exactly on load, so
+// the result equals the Gram matrix of the FP64 copy).  Rows are staged 16 at a time in LDS (double
 // buffered, next chunk prefetched into registers); the 8 wavefronts share the
 // NBLK(NBLK+1)/2 blocks round-robin.  Per-workgroup partial blocks go to
 // scratch and a second kernel sums them in workgroup order (deterministic).
@@ -160,9 +161,9 @@ __device__ __forceinline__ void tri_block(int q, int& bi, int& bj) {
   bi = q;
 }
 
-template <int NBLK>
+template <int NBLK, class TY>
 __global__ __launch_bounds__(gram_threads(NBLK)) void gram_f64_kernel(int64_t m, int k,
-                                                                const double* __restrict__ Y,
+                                                                const TY* __restrict__ Y,
                                                                 int64_t ldy, int64_t rows_per_wg,
                                                                 double* __restrict__ part) {
   constexpr int CP = NBLK * 16;
@@ -198,11 +199,11 @@ __global__ __launch_bounds__(gram_threads(NBLK)) void gram_f64_kernel(int64_t m,
   auto load = [&](int64_t r0) {
     const int64_t gr = r0 + lr;
     const bool row_ok = gr < r_end;
-    const double* src = Y + (row_ok ? gr : r_begin);
+    const TY* src = Y + (row_ok ? gr : r_begin);
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int c = lc + CSTEP * p;
-      const double x = src[int64_t(c < k ? c : k - 1) * ldy];
+      const double x = static_cast<double>(src[int64_t(c < k ? c : k - 1) * ldy]);
       v[p] = (row_ok && c < k) ? x : 0.0;
     }
   };
@@ -281,7 +282,8 @@ size_t gram_f64_part_bytes(int64_t m, int64_t k) {
   return size_t(gram_parts(m)) * (nb * (nb + 1) / 2) * 256 * sizeof(double);
 }
 
-int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, double* G,
+template <class TY>
+int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const TY* Y, int64_t ldy, double* G,
              int64_t ldg, double* part, size_t part_bytes) {
   const int nb = qr_nblk(k);
   if (!nb) return fail(RLA_ENOTSUP, "gram_f64: more than 288 columns");
@@ -293,7 +295,7 @@ int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, d
   const int ki = static_cast<int>(k);
 #define RLA_GRAM(NB)                                                                          \
   case NB:                                                                                    \
-    gram_f64_kernel<NB><<<dim3(unsigned(nparts)), gram_threads(NB), 0, ctx->stream>>>( \
+    gram_f64_kernel<NB, TY><<<dim3(unsigned(nparts)), gram_threads(NB), 0, ctx->stream>>>( \
         m, ki, Y, ldy, rows, part);                                                          \
     break;
   switch (nb) {
@@ -306,6 +308,11 @@ int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, d
   RLA_LAUNCH_CHECK("gram_f64/reduce");
   return RLA_OK;
 }
+
+template int gram_f64<double>(rla_ctx*, int64_t, int64_t, const double*, int64_t, double*,
+                              int64_t, double*, size_t);
+template int gram_f64<float>(rla_ctx*, int64_t, int64_t, const float*, int64_t, double*,
+                             int64_t, double*, size_t);
 
 }  // namespace rla
 

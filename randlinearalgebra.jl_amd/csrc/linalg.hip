@@ -35,12 +35,17 @@ int gemm_f64(rla_ctx* ctx, bool ta, bool tb, int64_t M, int64_t N, int64_t K, do
 size_t gemm_f64_split_bytes(int64_t M, int64_t N);
 bool cholqr_small_k(int64_t k);
 size_t gram_f64_part_bytes(int64_t m, int64_t k);
-int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const double* Y, int64_t ldy, double* G,
+template <class T>
+int gram_f64(rla_ctx* ctx, int64_t m, int64_t k, const T* Y, int64_t ldy, double* G,
              int64_t ldg, double* part, size_t part_bytes);
 int chol_blocks(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, double* R, int64_t ldr,
                 double* Rt, int64_t ldt, double* Dinv, int* info_dev, bool g_symmetric);
 int trsm_ru_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, const double* Rt,
                 int64_t ldt, const double* Dinv, const int* info);
+template <class Tin>
+int trsm_ru(rla_ctx* ctx, int64_t m, int64_t k, const Tin* Yin, int64_t ldy, double* Yout,
+            int64_t ldo, uint16_t* Ybf, int64_t ld_bf, const double* Rt, int64_t ldt,
+            const double* Dinv, const int* info, int ninfo);
 
 #define RLA_BLAS(call)                                                                       \
   do {                                                                                       \
@@ -193,7 +198,7 @@ __global__ void zero_lower_kernel(int64_t k, double* R, int64_t ldr) {
 static int cholqr_pass(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, QrBufs& q,
                        int* info, bool* broke) {
   if (cholqr_small_k(k)) {
-    int rc = gram_f64(ctx, m, k, Y, ldy, q.G, k, q.part, q.part_bytes);
+    int rc = gram_f64<double>(ctx, m, k, Y, ldy, q.G, k, q.part, q.part_bytes);
     if (rc) return rc;
     // gram_f64 writes both triangles of G
     rc = chol_blocks(ctx, k, q.G, k, q.G, k, q.X, k, q.Dinv, info, true);
@@ -222,6 +227,22 @@ static int cholqr_pass(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ld
   rc = gemm_f64(ctx, false, false, m, k, k, 1.0, Y, ldy, q.X, k, 0.0, q.T, m, nullptr, 0);
   if (rc) return rc;
   return convert<double, double>(ctx, m, k, q.T, m, Y, ldy);
+}
+
+// One CholeskyQR pass with separate input and output, for the mixed-precision
+// RangeFinder: Yin (FP64, or an FP32 product widened exactly on load) gives G,
+// R and Yin R^-1, which goes to Yout (FP64) or to Ybf (its BF16 rounding, the
+// next product's operand).  This pass's breakdown flag is info[nflags - 1];
+// the solve writes nothing unless all of info[0 .. nflags) are 0.
+template <class Tin>
+static int cholqr_to(rla_ctx* ctx, int64_t m, int64_t k, const Tin* Yin, int64_t ldy,
+                     double* Yout, int64_t ldo, uint16_t* Ybf, int64_t ld_bf, QrBufs& q,
+                     int* info, int nflags) {
+  int rc = gram_f64<Tin>(ctx, m, k, Yin, ldy, q.G, k, q.part, q.part_bytes);
+  if (rc) return rc;
+  rc = chol_blocks(ctx, k, q.G, k, q.G, k, q.X, k, q.Dinv, info + nflags - 1, true);
+  if (rc) return rc;
+  return trsm_ru<Tin>(ctx, m, k, Yin, ldy, Yout, ldo, Ybf, ld_bf, q.X, k, q.Dinv, info, nflags);
 }
 
 // Y (m x k, ldy) <- Q; optionally R (k x k upper, ldr) with Y_in = Q R.
@@ -422,6 +443,47 @@ static int rf_a_z(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n,
   return convert<float, double>(ctx, m, k, w.Y32, m, Y, ldy);
 }
 
+// rand_ortho_it (power_its.jl:53-80) on the BF16 path with device-side
+// breakdown flags (power_its >= 1, k <= 288).  Each CholeskyQR pass reads the
+// product's FP32 output directly and writes only what comes next: the BF16
+// operand of the next product, or, for the returned Q, the first pass of
+// CholeskyQR2 into Ybak and the second into Y.  The arithmetic is that of
+// rf_run's passes (the same Gram, factor and solve on exactly widened
+// values, the same BF16 rounding), so Q is bit-identical; the conversion
+// kernels between the passes and the products are gone.  Y holds the sketch
+// until the last solve, which writes nothing if any pass broke down, so a
+// rerun needs no saved copy.
+static int rf_run_bf16_fused(rla_ctx* ctx, int64_t m, int64_t n, int64_t k, int64_t power_its,
+                             const void* A_rm, int64_t ld_rm, const void* A_cm, int64_t ld_cm,
+                             double* Y, int64_t ldy, RfBufs& w, int* info) {
+  QrBufs q;
+  if (qr_layout(std::max(m, n), k, w.qr, w.qr_bytes, &q) > w.qr_bytes)
+    return fail(RLA_EARG, "RangeFinder workspace too small");
+  const uint16_t* Arm = static_cast<const uint16_t*>(A_rm);
+  const uint16_t* Acm = static_cast<const uint16_t*>(A_cm);
+  int ns = 0;
+  int rc = cholqr_to<double>(ctx, m, k, Y, ldy, nullptr, 0, w.Ybf, ld8(m), q, info, ++ns);
+  if (rc) return rc;
+  for (int64_t it = 0; it < power_its; ++it) {
+    rc = gemm_bf16_tn(ctx, n, k, m, 1.f, Acm, ld_cm, w.Ybf, ld8(m), 0.f, w.Z32, 1, n);
+    if (rc) return rc;
+    rc = cholqr_to<float>(ctx, n, k, w.Z32, n, nullptr, 0, w.Zbf, ld8(n), q, info, ++ns);
+    if (rc) return rc;
+    rc = gemm_bf16_tn(ctx, m, k, n, 1.f, Arm, ld_rm, w.Zbf, ld8(n), 0.f, w.Y32, 1, m);
+    if (rc) return rc;
+    if (it + 1 < power_its) {
+      rc = cholqr_to<float>(ctx, m, k, w.Y32, m, nullptr, 0, w.Ybf, ld8(m), q, info, ++ns);
+      if (rc) return rc;
+      continue;
+    }
+    rc = cholqr_to<float>(ctx, m, k, w.Y32, m, w.Ybak, m, nullptr, 0, q, info, ++ns);
+    if (rc) return rc;
+    rc = cholqr_to<double>(ctx, m, k, w.Ybak, m, Y, ldy, nullptr, 0, q, info, ++ns);
+    if (rc) return rc;
+  }
+  return RLA_OK;
+}
+
 static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, const void* A,
                     const void* A_rm, const void* A_cm) {
   if (precision != PREC_PLAIN && precision != PREC_BF16)
@@ -599,7 +661,24 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
     return fail(RLA_EARG, "RangeFinder workspace too small");
   const int64_t nslots = rf_info_slots(orthogonalize, power_its);
   const bool async = cholqr_small_k(k) && nslots <= w.ninfo;
-  if (async) {
+  auto flags_ok = [&](bool* ok) -> int {
+    std::vector<int> h(nslots);
+    RLA_HIP(hipMemcpyAsync(h.data(), w.info, sizeof(int) * nslots, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    RLA_HIP(hipStreamSynchronize(ctx->stream));
+    *ok = true;
+    for (int v : h) *ok = *ok && v == 0;
+    return RLA_OK;
+  };
+  if (async && precision == PREC_BF16 && orthogonalize && power_its >= 1) {
+    // Y is left as it was unless every pass succeeded: a rerun starts from it
+    rc = rf_run_bf16_fused(ctx, m, n, k, power_its, A_rm, ld_rm, A_cm, ld_cm, Y, ldy, w,
+                           w.info);
+    if (rc) return rc;
+    bool ok = false;
+    rc = flags_ok(&ok);
+    if (rc || ok) return rc;
+  } else if (async) {
     // no host round trip inside the loop; a breakdown anywhere reruns the
     // whole solve with host-checked QRs (Householder fallback) from the copy
     RLA_HIP(hipMemcpy2DAsync(w.Ybak, sizeof(double) * m, Y, sizeof(double) * ldy,
@@ -607,13 +686,9 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
     rc = rf_run(ctx, precision, orthogonalize, m, n, k, power_its, a_elem, A, lda, A_rm, ld_rm,
                 A_cm, ld_cm, Y, ldy, w, w.info);
     if (rc) return rc;
-    std::vector<int> h(nslots);
-    RLA_HIP(hipMemcpyAsync(h.data(), w.info, sizeof(int) * nslots, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    RLA_HIP(hipStreamSynchronize(ctx->stream));
-    bool ok = true;
-    for (int v : h) ok = ok && v == 0;
-    if (ok) return RLA_OK;
+    bool ok = false;
+    rc = flags_ok(&ok);
+    if (rc || ok) return rc;
     RLA_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldy, w.Ybak, sizeof(double) * m,
                              sizeof(double) * m, k, hipMemcpyDeviceToDevice, ctx->stream));
   }

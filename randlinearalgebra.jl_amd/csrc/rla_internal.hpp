@@ -37,6 +37,16 @@ using global_ptr = __attribute__((address_space(1))) T*;
 template <class T>
 using lds_ptr = __attribute__((address_space(3))) T*;
 
+// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).  The BF16
+// operands of the mixed-precision products are made from FP64 values as
+// f32_to_bf16_bits(float(x)) wherever they are written.
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return uint16_t((u >> 16) | 0x40);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
 struct CholBatchItem {
   int k, check_k;
   const double* G;

@@ -445,6 +445,25 @@ def test_rangefinder_rank_deficient_reruns_with_householder(rla, dev, prec, orth
     assert np.linalg.norm(A - Q @ (Q.T @ A)) < tol * np.linalg.norm(A)
 
 
+@pytest.mark.parametrize("q", [1, 2])
+def test_rangefinder_bf16_breakdown_reruns_from_untouched_sketch(rla, dev, q):
+    # only 8 rows of A are nonzero, so even its BF16 copy has rank 8 and the
+    # sketch's Gram matrix is exactly singular: the orthogonalised BF16 run
+    # (which keeps no copy of the sketch) breaks down and must leave the
+    # sketch intact for the host-checked rerun
+    A = np.zeros((700, 260))
+    A[[3, 50, 51, 200, 333, 480, 610, 699]] = rnd((8, 260), 6)
+    Ad = colmajor(A, dev)
+    rla.seed(11)
+    rf = rla.RangeFinder(compressor=rla.Gaussian(cardinality=rla.Right(), compression_dim=30),
+                         power_its=q, orthogonalize=True, precision="bf16")
+    Q = rla.rapproximate(rf, Ad).range.cpu().numpy()
+    assert np.isfinite(Q).all()
+    assert np.linalg.norm(Q.T @ Q - np.eye(30)) < 1e-10
+    # range(A) lies in span(Q) up to FP64 rounding: the zero rows stay zero
+    assert np.linalg.norm(A - Q @ (Q.T @ A)) < 1e-10 * np.linalg.norm(A)
+
+
 def small_svd(rla, dev, X, transpose=False):
     import ctypes
     k = X.shape[0]

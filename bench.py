@@ -384,26 +384,31 @@ def rangefinder_c3(rla, torch, reps=3, batch=4):
             recs.append(rla.complete_approximator(rf, A32))
             recs[-1]._bf16 = rec._bf16
         streams = [torch.cuda.Stream(device=dev) for _ in recs]
+        n_batch = 3 * reps   # solves per stream in the timed batch
+        go = threading.Barrier(batch + 1)
 
-        def work(r, s, n_solves):
+        def work(r, s, n_solves, wait):
             with torch.cuda.stream(s):
+                rla.rapproximate_(r, A32)   # this thread's context and workspaces
+                s.synchronize()
+                if wait:
+                    go.wait()
                 for _ in range(n_solves):
                     rla.rapproximate_(r, A32)
                 s.synchronize()
 
-        for r, s in zip(recs, streams):
-            work(r, s, 1)
-        torch.cuda.synchronize()
-        threads = [threading.Thread(target=work, args=(r, s, reps)) for r, s in zip(recs, streams)]
-        t0 = time.perf_counter()
+        threads = [threading.Thread(target=work, args=(r, s, n_batch, True))
+                   for r, s in zip(recs, streams)]
         for t in threads:
             t.start()
+        go.wait()   # every thread warmed up; the timed solves start together
+        t0 = time.perf_counter()
         for t in threads:
             t.join()
         torch.cuda.synchronize()
         bdt = time.perf_counter() - t0
-        out[prec]["batch"] = {"in_flight": batch, "solves": batch * reps,
-                              "solves_per_s": batch * reps / bdt}
+        out[prec]["batch"] = {"in_flight": batch, "solves": batch * n_batch,
+                              "solves_per_s": batch * n_batch / bdt}
         del recs, rec
     # RandSVD (randsvd.jl:174-189) on the same A, BF16 products: the range, then
     # B = Q'A and its FP64 SVD; singular values against the planted ones

@@ -74,19 +74,23 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
-// Upper-triangular 16 x 16 block number q -> (block row, block column),
-// enumerated column by column.
-template <int NBLK>
-struct UpperBlocks {
-  int bi[NBLK * (NBLK + 1) / 2];
-  int bj[NBLK * (NBLK + 1) / 2];
-  constexpr UpperBlocks() : bi(), bj() {
-    int q = 0;
+// The upper-triangular 16 x 16 blocks (bi, bj) owned by each of BW block
+// wavefronts, column by column.  Owner (bi + 2 bj) mod BW spreads every block
+// row (the panel of a step) and every trailing region evenly (for BW = 7 no
+// wavefront holds more than ceil(T / 7) blocks or more than 3 blocks of a
+// panel row).
+template <int NBLK, int BW>
+struct BlockOwners {
+  int bi[BW][NBLK * (NBLK + 1) / 2];
+  int bj[BW][NBLK * (NBLK + 1) / 2];
+  int n[BW];
+  constexpr BlockOwners() : bi(), bj(), n() {
     for (int j = 0; j < NBLK; ++j)
       for (int i = 0; i <= j; ++i) {
-        bi[q] = i;
-        bj[q] = j;
-        ++q;
+        const int w = (i + 2 * j) % BW;
+        bi[w][n[w]] = i;
+        bj[w][n[w]] = j;
+        ++n[w];
       }
   }
 };
@@ -99,9 +103,8 @@ struct CholShape {
   static constexpr int block_waves = threads / 64 - 1;
 };
 
-// The trailing-matrix role of block wavefront W: its blocks are
-// W, W + BW, W + 2 BW, ... of the column-by-column enumeration (BW block
-// wavefronts).  W is a template
+// The trailing-matrix role of block wavefront W: the blocks BlockOwners
+// gives it (BW block wavefronts).  W is a template
 // parameter so every block coordinate and LDS offset is a compile-time
 // constant (no address or index registers per block).
 template <int NBLK, int W>
@@ -113,10 +116,9 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
                                                 double (*P)[NBLK * 16 + 1], double (*Ds)[17],
                                                 double (*Dv)[17], lds_ptr<volatile int> fail,
                                                 int t) {
-  constexpr int T = NBLK * (NBLK + 1) / 2;
   constexpr int BW = CholShape<NBLK>::block_waves;
-  constexpr int MAXB = (T - W + BW - 1) / BW;  // blocks of this wavefront
-  constexpr UpperBlocks<NBLK> tab{};
+  constexpr BlockOwners<NBLK, BW> own{};
+  constexpr int MAXB = own.n[W];  // blocks of this wavefront
   const int lane = t & 63, lr = lane >> 4, lc = lane & 15;
   f64x4 acc[MAXB > 0 ? MAXB : 1];
   // clamped (in-bounds) loads, all issued before any result is used (one
@@ -124,7 +126,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
   // mirror element, so the 16 lanes of a row read 128 contiguous bytes.
 #pragma unroll
   for (int u = 0; u < MAXB; ++u) {
-    const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+    const int bi = own.bi[W][u], bj = own.bj[W][u];
     // issued four blocks at a time, so that the addresses of all the loads
     // in flight are not live at once
     if (u % 4 == 0) asm volatile("" ::: "memory");
@@ -139,7 +141,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 #pragma unroll
   for (int u = 0; u < MAXB; ++u) {
-    const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+    const int bi = own.bi[W][u], bj = own.bj[W][u];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (!(bi * 16 + lr + 4 * r < k && bj * 16 + lc < k)) acc[u][r] = 0.0;
@@ -172,7 +174,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
   // G_00 to LDS
 #pragma unroll
   for (int u = 0; u < MAXB; ++u)
-    if (tab.bi[W + BW * u] == 0 && tab.bj[W + BW * u] == 0) {
+    if (own.bi[W][u] == 0 && own.bj[W][u] == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) Ds[lr + 4 * r][lc] = acc[u][r];
     }
@@ -183,7 +185,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
     if (s > 0) {  // the rest of step s-1's trailing update
 #pragma unroll
       for (int u = 0; u < MAXB; ++u) {
-        const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+        const int bi = own.bi[W][u], bj = own.bj[W][u];
         if (bi > s - 1 && !(bi == s && bj == s)) update(u, bi, bj);
       }
     }
@@ -199,7 +201,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
     asm volatile("" : "+s"(jso), "+s"(ldro), "+s"(ldto));
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
-      const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+      const int bi = own.bi[W][u], bj = own.bj[W][u];
       if (bi == s && bj == s) {
         const int c = lc, nb = min(16, k - js);
 #pragma unroll
@@ -233,7 +235,7 @@ __device__ __forceinline__ void chol_block_role(int k, global_ptr<const double> 
     // G_{s+1,s+1} first, to LDS for the next B1 (only diagonal blocks qualify)
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
-      const int bi = tab.bi[W + BW * u], bj = tab.bj[W + BW * u];
+      const int bi = own.bi[W][u], bj = own.bj[W][u];
       if (bi == bj && bi == s + 1) {
         update(u, bi, bj);
 #pragma unroll

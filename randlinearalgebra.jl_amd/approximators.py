@@ -134,7 +134,7 @@ def _range(recipe, A):
     if prec == "bf16":
         rm, cm = recipe._bf16.get(A)
         if isinstance(S, GaussianRecipe):
-            om = _dense.bf16_rows(as_view(S.op).T)    # columns of the sketch, K-contiguous
+            om = S.bf16_operand(1)    # columns of the sketch, K-contiguous
             Y32 = torch.empty(k, m, dtype=torch.float32, device=dev).T
             _dense.gemm_bf16(m, k, n, rm, om, as_view(Y32))
             Y = _dense.convert(Y32, torch.float64)

@@ -1023,6 +1023,18 @@ class GaussianRecipe(CompressorRecipe):
     def _op_as(self, dtype):
         return self.op if dtype == self.op.dtype else self.op.to(dtype)
 
+    def bf16_operand(self, side):
+        """BF16 copy of op read K-contiguous by the BF16 GEMM: rows contiguous
+        for S*A (side 0), columns for A*S (side 1).  Kept until op changes
+        (in place by _gen, which drops it, or by the caller)."""
+        from . import _dense
+        c = self._bf16_op
+        if c is None or c[2] != side or not _dense.key_matches(c[0], self.op):
+            S = as_view(self.op)
+            ob = _dense.bf16_rows(S if side == 0 else S.T)
+            self._bf16_op = c = (_dense.SourceKey(self.op), ob, side)
+        return c[1]
+
     def _apply(self, side, C, A, alpha, beta):
         """side 0: C = alpha*S*A + beta*C; side 1: C = alpha*A*S + beta*C
         (gaussian.jl:145-170), on the library's MFMA kernels."""
@@ -1037,14 +1049,7 @@ class GaussianRecipe(CompressorRecipe):
         from . import _dense
         if C.dtype != torch.float32 or A.dtype != torch.float32:
             raise ArgumentError("the bf16 Gaussian apply takes Float32 data")
-        S = as_view(self.op)
-        # the BF16 GEMM reads both operands K-contiguous: S with its rows
-        # contiguous for S*A, its columns for A*S; kept while op is unchanged
-        c = self._bf16_op
-        if c is None or c[2] != side or not _dense.key_matches(c[0], self.op):
-            ob = _dense.bf16_rows(S if side == 0 else S.T)
-            self._bf16_op = c = (_dense.SourceKey(self.op), ob, side)
-        ob = c[1]
+        ob = self.bf16_operand(side)
         if side == 0:   # C (s x n) = S (s x m) * A (m x n)
             Ab = _dense.bf16_rows(A.T)
             _dense.gemm_bf16(C.rows, C.cols, A.rows, ob, Ab, C, alpha, beta)

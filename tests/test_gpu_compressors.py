@@ -708,7 +708,8 @@ def _bell_layout(rowptr, colidx, vals, rows, in_rows, rb):
 @pytest.mark.parametrize("rows,in_rows,ncols,rb", [(96, 20000, 5, None), (3000, 40001, 2, None),
                                                   (700, 33000, 40, 1000), (5, 9, 3, 4),
                                                   (2049, 17000, 1, None), (1000, 17000, 512, None),
-                                                  (2048, 17000, 512, None)])
+                                                  (2048, 17000, 512, None),
+                                                  (1500, 40001, 515, None)])
 def test_sparse_block_interleaved_apply_equals_csr_apply(rla, dev, dtype, rows, in_rows, ncols,
                                                          rb):
     # the block-interleaved layout against its definition (built on the host),

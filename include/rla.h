@@ -487,6 +487,26 @@ int rla_gemm_bf16(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, co
                   int64_t lda, const uint16_t* B, int64_t ldb, float beta, float* C, int64_t c_rs,
                   int64_t c_cs);
 
+/* Tiled BF16 copy of the rows x cols strided view src, the layout in which
+ * the BF16 GEMM streams its A operand with one contiguous kilobyte per
+ * wavefront load: tiles of 128 rows x 64 columns, tile (t, s) at element
+ * (t * ceil(cols/64) + s) * 8192; inside a tile the 8 entries of row
+ * 32 g + r (r < 32), columns 16 q + 8 h .. + 8 (h < 2), at element
+ * ((4 g + q) * 64 + r + 32 h) * 8.  Entries past the view are written as
+ * zero.  Same rounding as rla_to_bf16_*.  rla_to_bf16_tiled_bytes gives the
+ * size of dst (16-byte aligned).  The resident copies of A for precision 2
+ * of rla_rangefinder / rla_randsvd_finish (power_its.jl:21,28-29,59,68,72). */
+int rla_to_bf16_tiled_bytes(int64_t rows, int64_t cols, size_t* bytes);
+int rla_to_bf16_tiled_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src,
+                          int64_t s_rs, int64_t s_cs, uint16_t* dst);
+int rla_to_bf16_tiled_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src,
+                          int64_t s_rs, int64_t s_cs, uint16_t* dst);
+/* rla_gemm_bf16 with A (M x K) in the tiled layout above, N <= 288 (else
+ * RLA_ENOTSUP): the same products in the same order, so the same bits. */
+int rla_gemm_bf16_tiled(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha,
+                        const uint16_t* At, const uint16_t* B, int64_t ldb, float beta, float* C,
+                        int64_t c_rs, int64_t c_cs);
+
 int rla_convert_f32_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src, int64_t lds,
                         double* dst, int64_t ldd);
 int rla_convert_f64_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src, int64_t lds,
@@ -533,7 +553,11 @@ int rla_chol_inv_f64(rla_ctx* ctx, int64_t k, const double* G, int64_t ldg, doub
  * a plain (non-transposed) product; A_cm is unused.
  * precision 1: products on the BF16 MFMA with FP32 accumulation, reading the
  * two resident BF16 copies of A (A_rm: rows contiguous, ld_rm; A_cm:
- * columns contiguous, ld_cm; from rla_to_bf16_*).  QR is FP64 throughout. */
+ * columns contiguous, ld_cm; from rla_to_bf16_*).
+ * precision 2: as 1 with the copies in the GEMM's tiled layout (A_rm =
+ * rla_to_bf16_tiled of A, m x n; A_cm = rla_to_bf16_tiled of A', n x m;
+ * ld_rm and ld_cm unused): the same products, bit-identical Q, with each
+ * operand load one contiguous kilobyte.  QR is FP64 throughout. */
 int rla_rangefinder_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,
                                     size_t* bytes);
 int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int64_t n,
@@ -555,7 +579,8 @@ int rla_svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int
 /* RandSVD finish (randsvd.jl:174-189) from the range Q (m x k): B' = A'Q,
  * B' = Qb Rb (CholeskyQR2), Rb' = Ur S Vr' (rla_svd_small_f64, k <= 288),
  * U = Q Ur, V = Qb Vr.  A_cm: for precision 1 the columns-contiguous BF16
- * copy of A; for precision 0 NULL or the rows-contiguous copy of A in A's
+ * copy of A, for precision 2 the tiled BF16 copy of A' (n x m); for
+ * precision 0 NULL or the rows-contiguous copy of A in A's
  * type (as A_rm of rla_rangefinder).
  * Outputs U (m x k), S (k), V (n x k), FP64. */
 int rla_randsvd_workspace_bytes(int precision, int a_elem, int64_t m, int64_t n, int64_t k,

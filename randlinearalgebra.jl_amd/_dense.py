@@ -71,6 +71,24 @@ def bf16_rows(v):
     return out
 
 
+def bf16_tiled(v):
+    """Tiled BF16 copy of the (rows x cols) View: the layout in which the BF16
+    GEMM streams its A operand (rla_to_bf16_tiled: 128 x 64 tiles, one
+    contiguous kilobyte per wavefront load, padding zero)."""
+    nbytes = query_bytes("rla_to_bf16_tiled_bytes", v.rows, v.cols)
+    out = torch.empty(max(1, nbytes // 2), dtype=torch.bfloat16, device=v.t.device)
+    _lib.call(f"rla_to_bf16_tiled_{_lib.sfx(v.dtype)}", _lib.ctx(v.t.device), v.rows, v.cols,
+              _lib.ptr(v.t), v.rs, v.cs, _lib.ptr(out))
+    return out
+
+
+def gemm_bf16_tiled(M, N, K, A_tiled, B_cols, C, alpha=1.0, beta=0.0):
+    """gemm_bf16 with A (M x K) as a tiled copy from bf16_tiled (N <= 288)."""
+    _lib.call("rla_gemm_bf16_tiled", _lib.ctx(C.t.device), M, N, K, float(alpha),
+              _lib.ptr(A_tiled), _lib.ptr(B_cols), B_cols.shape[1], float(beta), _lib.ptr(C.t),
+              C.rs, C.cs)
+
+
 def gemm_bf16(M, N, K, A_rows, B_cols, C, alpha=1.0, beta=0.0):
     """C (Float32 View, M x N) = alpha * A_rows[:M, :K] @ B_cols[:N, :K]' + beta*C,
     with both operands BF16 and K-contiguous (from bf16_rows)."""

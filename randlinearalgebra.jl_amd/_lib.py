@@ -152,6 +152,11 @@ _SIGS = {
     "rla_to_bf16_f64": [vp, i64, i64, vp, i64, i64, vp, i64],
     "rla_gemm_bf16": [vp, i64, i64, i64, ctypes.c_float, vp, i64, vp, i64, ctypes.c_float, vp,
                       i64, i64],
+    "rla_to_bf16_tiled_bytes": [i64, i64, vp],
+    "rla_to_bf16_tiled_f32": [vp, i64, i64, vp, i64, i64, vp],
+    "rla_to_bf16_tiled_f64": [vp, i64, i64, vp, i64, i64, vp],
+    "rla_gemm_bf16_tiled": [vp, i64, i64, i64, ctypes.c_float, vp, vp, i64, ctypes.c_float, vp,
+                            i64, i64],
     "rla_convert_f32_f64": [vp, i64, i64, vp, i64, vp, i64],
     "rla_convert_f64_f32": [vp, i64, i64, vp, i64, vp, i64],
     "rla_scale_cols_f64": [vp, i64, i64, vp, i64, vp, vp, i64],

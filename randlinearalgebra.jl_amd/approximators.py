@@ -18,7 +18,9 @@ from ._lib import ArgumentError, DomainError
 from .compressors import (GaussianRecipe, Left, Right, SparseSign, _register_mul_hook, as_view,
                           complete_compressor, left_mul_dimcheck, mul, right_mul_dimcheck)
 
-PRECISIONS = {"plain": 0, "bf16": 1}
+# host precision -> the rla_rangefinder code ("bf16": products on the BF16
+# MFMA reading tiled BF16 copies of A)
+PRECISIONS = {"plain": 0, "bf16": 2}
 
 
 class Approximator:
@@ -77,8 +79,8 @@ def rapproximate(approx, A):
 
 
 class _Bf16Copies:
-    """The two resident BF16 copies of A used by the bf16 products: rows
-    contiguous (for A*X) and columns contiguous (for A'*X).  Rebuilt only
+    """The two resident BF16 copies of A used by the bf16 products, in the
+    GEMM's tiled layout: of A (for A*X) and of A' (for A'*X).  Rebuilt only
     when A's storage or version changes."""
 
     def __init__(self):
@@ -89,8 +91,8 @@ class _Bf16Copies:
         """A is the caller's matrix (any strides), not a copy of it."""
         if not _dense.key_matches(self.key, A):
             v = as_view(A)
-            self.rm = _dense.bf16_rows(v)      # m x ld8(n)
-            self.cm = _dense.bf16_rows(v.T)    # n x ld8(m)
+            self.rm = _dense.bf16_tiled(v)      # A, m x n
+            self.cm = _dense.bf16_tiled(v.T)    # A', n x m
             self.key = _dense.SourceKey(A)
         return self.rm, self.cm
 
@@ -136,7 +138,7 @@ def _range(recipe, A):
         if isinstance(S, GaussianRecipe):
             om = S.bf16_operand(1)    # columns of the sketch, K-contiguous
             Y32 = torch.empty(k, m, dtype=torch.float32, device=dev).T
-            _dense.gemm_bf16(m, k, n, rm, om, as_view(Y32))
+            _dense.gemm_bf16_tiled(m, k, n, rm, om, as_view(Y32))
             Y = _dense.convert(Y32, torch.float64)
         else:
             Yt = torch.empty(k, m, dtype=A.dtype, device=dev).T
@@ -152,10 +154,11 @@ def _range(recipe, A):
     p = PRECISIONS[prec]
     nbytes = _dense.query_bytes("rla_rangefinder_workspace_bytes", p, a_elem, m, n, k)
     work = _dense.workspace(nbytes, dev)
-    ld_rm = 0 if rm is None else (rm.shape[1] if prec == "bf16" else as_view(rm).rs)
+    # tiled BF16 copies carry no leading dimension
+    ld_rm = 0 if (rm is None or prec == "bf16") else as_view(rm).rs
     _lib.call("rla_rangefinder", _lib.ctx(dev), p, int(bool(recipe.orthogonalize)), m, n, k,
               recipe.power_its, a_elem, _lib.ptr(A_), v.cs, _lib.ptr(rm), ld_rm, _lib.ptr(cm),
-              cm.shape[1] if cm is not None else 0, _lib.ptr(Y), m, _lib.ptr(work),
+              0, _lib.ptr(Y), m, _lib.ptr(work),
               work.numel() * 8)
     return Y, A_, rm, cm
 
@@ -280,7 +283,7 @@ class RandSVDRecipe(ApproximatorRecipe):
         work = _dense.workspace(nbytes, dev)
         # A'Q reads the BF16 columns copy, or the plain path's row copy
         at = cm if self.precision == "bf16" else rm
-        ld_at = 0 if at is None else (at.shape[1] if self.precision == "bf16" else as_view(at).rs)
+        ld_at = 0 if (at is None or self.precision == "bf16") else as_view(at).rs
         _lib.call("rla_randsvd_finish", _lib.ctx(dev), p, m, n, k, a_elem, _lib.ptr(A_),
                   as_view(A_).cs, _lib.ptr(at), ld_at,
                   _lib.ptr(Q), m, _lib.ptr(U), m, _lib.ptr(S), _lib.ptr(V), n, _lib.ptr(work),

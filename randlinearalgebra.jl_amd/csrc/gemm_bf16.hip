@@ -61,6 +61,50 @@ __global__ __launch_bounds__(256) void to_bf16_rows_kernel(int64_t rows, int64_t
   }
 }
 
+// The tiled BF16 copy of a (rows x cols) operand that the pipelined kernel
+// streams as its A: tiles of 128 rows x 64 columns (K), tile (t, s) at
+// element (t * ceil(cols / 64) + s) * 8192; inside a tile, the 16-byte piece
+// of row 32 g + r (r < 32) and columns 16 q + 8 h .. + 8 (h < 2) sits at
+// element ((4 g + q) * 64 + r + 32 h) * 8, i.e. in the order one wavefront's
+// operand load reads it, so every load is one contiguous kilobyte.  Entries
+// past the operand are zero.  The source is read through LDS as in
+// to_bf16_rows_kernel; the same rounding (float, then nearest-even bf16).
+constexpr int kTileRows = 128;
+constexpr int kTileK = 64;
+constexpr int kTileElems = kTileRows * kTileK;
+
+__device__ __forceinline__ int64_t tiled_index(int64_t i, int64_t l, int64_t nkt) {
+  const int64_t t = i / kTileRows, s = l / kTileK;
+  const int r = static_cast<int>(i % kTileRows), c = static_cast<int>(l % kTileK);
+  const int g = r >> 5, q = c >> 4, h = (c >> 3) & 1;
+  return (t * nkt + s) * kTileElems + ((4 * g + q) * 64 + (r & 31) + 32 * h) * 8 + (c & 7);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void to_bf16_tiled_kernel(int64_t rows, int64_t cols,
+                                                            const T* __restrict__ src,
+                                                            int64_t s_rs, int64_t s_cs,
+                                                            uint16_t* __restrict__ dst,
+                                                            int64_t nkt) {
+  __shared__ float tile[64][65];
+  const int64_t i0 = int64_t(blockIdx.y) * 64, l0 = int64_t(blockIdx.x) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const bool rows_fast = s_rs == 1;
+  for (int r = ty; r < 64; r += 4) {
+    const int64_t i = rows_fast ? i0 + tx : i0 + r;
+    const int64_t l = rows_fast ? l0 + r : l0 + tx;
+    float v = 0.f;
+    if (i < rows && l < cols) v = static_cast<float>(src[i * s_rs + l * s_cs]);
+    if (rows_fast) tile[tx][r] = v; else tile[r][tx] = v;
+  }
+  __syncthreads();
+  // the grid covers the padded extent, so padding is written as zeros
+  for (int r = ty; r < 64; r += 4) {
+    const int64_t i = i0 + r, l = l0 + tx;
+    dst[tiled_index(i, l, nkt)] = f32_to_bf16_bits(tile[r][tx]);
+  }
+}
+
 // 8 bf16 at p with entries at or beyond n_valid replaced by zero
 __device__ __forceinline__ u32x4 load8_tail(const uint16_t* p, int64_t n_valid) {
   uint32_t w[4];
@@ -275,7 +319,7 @@ struct PipeCfg {
   static constexpr int BUF = BN * kBstride;  // bf16 per LDS buffer
 };
 
-template <int NB, int NB0, int NB1, int RT>
+template <int NB, int NB0, int NB1, int RT, bool TILED>
 __device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* __restrict__ A,
                                          int64_t lda, const uint16_t* __restrict__ B,
                                          int64_t ldb, float* __restrict__ C, int64_t c_rs,
@@ -295,10 +339,16 @@ __device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* _
       for (int r = 0; r < 16; ++r) acc[rt][nb][r] = 0.f;
 
   const int64_t row_w = tm * kPM + wr * 32 * RT;  // first row of this wavefront
+  // row layout: row pointers (lda the row stride); tiled layout (lda = the
+  // number of 64-wide K tiles): this lane's piece of tile row tm, K tile 0
   const uint16_t* ap[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
-    ap[rt] = A + std::min<int64_t>(row_w + rt * 32 + (lane & 31), M - 1) * lda;
+    ap[rt] = TILED ? A + tm * lda * kTileElems + ((wr * RT + rt) * 4 * 64 + lane) * 8
+                   : A + std::min<int64_t>(row_w + rt * 32 + (lane & 31), M - 1) * lda;
+  auto a_piece = [&](int rt, int64_t k0, int kk) {
+    return TILED ? ap[rt] + (k0 / kTileK) * kTileElems + kk * 64 * 8 : ap[rt] + k0 + kk * 16 + akq;
+  };
   // piece u of this thread: column (tid >> 3) + u * NT/8 (clamped to N-1 in
   // global memory), K offset (tid & 7) * 8; LDS offset boff0 + u * NT/8 * kBstride
   const int bcol0 = tid >> 3;
@@ -316,8 +366,8 @@ __device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* _
     for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
-        af[rt][kk] = __builtin_bit_cast(
-            bf16x8, *reinterpret_cast<const u32x4*>(ap[rt] + k0 + kk * 16 + akq));
+        af[rt][kk] = __builtin_bit_cast(bf16x8,
+                                        *reinterpret_cast<const u32x4*>(a_piece(rt, k0, kk)));
   };
   auto load_b = [&](int64_t k0, u32x4(&br)[P::PER_T]) {
 #pragma unroll
@@ -388,7 +438,8 @@ __device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* _
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
         u32x4 v = {0u, 0u, 0u, 0u};
-        if (gk + 8 <= ke) v = *reinterpret_cast<const u32x4*>(ap[rt] + gk);
+        if (TILED) v = *reinterpret_cast<const u32x4*>(a_piece(rt, k0, kk));  // zero-padded
+        else if (gk + 8 <= ke) v = *reinterpret_cast<const u32x4*>(ap[rt] + gk);
         else if (gk < ke) v = load8_tail(ap[rt] + gk, ke - gk);
         af[rt][kk] = __builtin_bit_cast(bf16x8, v);
       }
@@ -431,7 +482,7 @@ __device__ __forceinline__ void pipe_run(int64_t M, int64_t N, const uint16_t* _
   }
 }
 
-template <int NB0, int NB1, int RT>
+template <int NB0, int NB1, int RT, bool TILED>
 __global__ __launch_bounds__(256 * (2 / RT))
 __attribute__((amdgpu_waves_per_eu(2 / RT, 2 / RT)))
 void gemm_bf16_pipe_kernel(int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A,
@@ -449,14 +500,14 @@ void gemm_bf16_pipe_kernel(int64_t M, int64_t N, int64_t K, const uint16_t* __re
   if (kb >= ke) return;  // uniform over the workgroup
   const int wr = wave % P::WROWS;
   if (wave / P::WROWS)
-    pipe_run<NB1, NB0, NB1, RT>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
+    pipe_run<NB1, NB0, NB1, RT, TILED>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
                                 kb, ke, tm, wr, NB0, part, pipe_lds);
   else
-    pipe_run<NB0, NB0, NB1, RT>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
+    pipe_run<NB0, NB0, NB1, RT, TILED>(M, N, A, lda, B, ldb, C, c_rs, c_cs, alpha, beta, ksplit, split,
                                 kb, ke, tm, wr, 0, part, pipe_lds);
 }
 
-template <int NB0, int NB1, int RT>
+template <int NB0, int NB1, int RT, bool TILED = false>
 static int launch_pipe(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint16_t* A,
                        int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t c_rs,
                        int64_t c_cs, float alpha, float beta) {
@@ -466,7 +517,7 @@ static int launch_pipe(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
     attr_err = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&gemm_bf16_pipe_kernel<NB0, NB1, RT>),
+        reinterpret_cast<const void*>(&gemm_bf16_pipe_kernel<NB0, NB1, RT, TILED>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   });
   RLA_HIP(attr_err);
@@ -485,7 +536,7 @@ static int launch_pipe(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint
     if (rc) return rc;
     part = static_cast<float*>(ws);
   }
-  gemm_bf16_pipe_kernel<NB0, NB1, RT><<<dim3(static_cast<unsigned>(tiles * ksplit)), P::NT, lds,
+  gemm_bf16_pipe_kernel<NB0, NB1, RT, TILED><<<dim3(static_cast<unsigned>(tiles * ksplit)), P::NT, lds,
                                         ctx->stream>>>(M, N, K, A, lda, B, ldb, C, c_rs, c_cs,
                                                        alpha, beta, ksplit, kps, part);
   RLA_LAUNCH_CHECK("gemm_bf16_pipe");
@@ -541,6 +592,54 @@ template int to_bf16_rows<float>(rla_ctx*, int64_t, int64_t, const float*, int64
 template int to_bf16_rows<double>(rla_ctx*, int64_t, int64_t, const double*, int64_t, int64_t,
                                   uint16_t*, int64_t);
 
+int64_t bf16_tiled_elems(int64_t rows, int64_t cols) {
+  return ceil_div(rows, kTileRows) * ceil_div(cols, kTileK) * kTileElems;
+}
+
+template <class T>
+int to_bf16_tiled(rla_ctx* ctx, int64_t rows, int64_t cols, const T* src, int64_t s_rs,
+                  int64_t s_cs, uint16_t* dst) {
+  if (rows <= 0 || cols <= 0) return RLA_OK;
+  const int64_t nkt = ceil_div(cols, kTileK);
+  dim3 grid(static_cast<unsigned>(nkt), static_cast<unsigned>(ceil_div(rows, kTileRows) * 2));
+  to_bf16_tiled_kernel<T><<<grid, 256, 0, ctx->stream>>>(rows, cols, src, s_rs, s_cs, dst, nkt);
+  RLA_LAUNCH_CHECK("to_bf16_tiled");
+  return RLA_OK;
+}
+
+template int to_bf16_tiled<float>(rla_ctx*, int64_t, int64_t, const float*, int64_t, int64_t,
+                                  uint16_t*);
+template int to_bf16_tiled<double>(rla_ctx*, int64_t, int64_t, const double*, int64_t, int64_t,
+                                   uint16_t*);
+
+// gemm_bf16_tn with A in the tiled layout of to_bf16_tiled (M x K, N <= 288)
+int gemm_bf16_tn_tiled(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha,
+                       const uint16_t* At, const uint16_t* B, int64_t ldb, float beta, float* C,
+                       int64_t c_rs, int64_t c_cs) {
+  if (M <= 0 || N <= 0) return RLA_OK;
+  if (K <= 0) return fail(RLA_EARG, "gemm_bf16: K must be positive");
+  if ((ldb % 8) || (reinterpret_cast<uintptr_t>(At) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
+    return fail(RLA_EARG, "gemm_bf16: operands must be 16-byte aligned with ld a multiple of 8");
+  const int64_t nkt = ceil_div(K, kTileK);
+#define RLA_TPIPE(A0, A1)                                                                  \
+  return launch_pipe<A0, A1, kPipeRT, true>(ctx, M, N, K, At, nkt, B, ldb, C, c_rs, c_cs, \
+                                            alpha, beta)
+  switch (ceil_div(N, 32)) {
+    case 1:
+    case 2: RLA_TPIPE(1, 1);
+    case 3: RLA_TPIPE(2, 1);
+    case 4: RLA_TPIPE(2, 2);
+    case 5: RLA_TPIPE(3, 2);
+    case 6: RLA_TPIPE(3, 3);
+    case 7: RLA_TPIPE(4, 3);
+    case 8: RLA_TPIPE(4, 4);
+    case 9: RLA_TPIPE(5, 4);
+    default: break;
+  }
+#undef RLA_TPIPE
+  return fail(RLA_ENOTSUP, "gemm_bf16 with a tiled A takes N <= 288");
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -572,6 +671,40 @@ int rla_gemm_bf16(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, co
   int rc = use_device(ctx);
   if (rc) return rc;
   return gemm_bf16_tn(ctx, M, N, K, alpha, A, lda, B, ldb, beta, C, c_rs, c_cs);
+}
+
+int rla_to_bf16_tiled_bytes(int64_t rows, int64_t cols, size_t* bytes) {
+  if (!bytes) return fail(RLA_EARG, "bytes is NULL");
+  if (rows < 0 || cols < 0) return fail(RLA_EARG, "rla_to_bf16_tiled: negative dimensions");
+  *bytes = size_t(bf16_tiled_elems(rows, cols)) * sizeof(uint16_t);
+  return RLA_OK;
+}
+
+int rla_to_bf16_tiled_f32(rla_ctx* ctx, int64_t rows, int64_t cols, const float* src,
+                          int64_t s_rs, int64_t s_cs, uint16_t* dst) {
+  RLA_CHECK_CTX(ctx);
+  if (rows > 0 && cols > 0 && (!src || !dst)) return fail(RLA_EARG, "rla_to_bf16_tiled: NULL");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  return to_bf16_tiled<float>(ctx, rows, cols, src, s_rs, s_cs, dst);
+}
+
+int rla_to_bf16_tiled_f64(rla_ctx* ctx, int64_t rows, int64_t cols, const double* src,
+                          int64_t s_rs, int64_t s_cs, uint16_t* dst) {
+  RLA_CHECK_CTX(ctx);
+  if (rows > 0 && cols > 0 && (!src || !dst)) return fail(RLA_EARG, "rla_to_bf16_tiled: NULL");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  return to_bf16_tiled<double>(ctx, rows, cols, src, s_rs, s_cs, dst);
+}
+
+int rla_gemm_bf16_tiled(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha,
+                        const uint16_t* At, const uint16_t* B, int64_t ldb, float beta, float* C,
+                        int64_t c_rs, int64_t c_cs) {
+  RLA_CHECK_CTX(ctx);
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  return gemm_bf16_tn_tiled(ctx, M, N, K, alpha, At, B, ldb, beta, C, c_rs, c_cs);
 }
 
 }  // extern "C"

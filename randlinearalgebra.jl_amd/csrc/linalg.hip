@@ -26,6 +26,9 @@ namespace rla {
 int gemm_bf16_tn(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha, const uint16_t* A,
                  int64_t lda, const uint16_t* B, int64_t ldb, float beta, float* C, int64_t c_rs,
                  int64_t c_cs);
+int gemm_bf16_tn_tiled(rla_ctx* ctx, int64_t M, int64_t N, int64_t K, float alpha,
+                       const uint16_t* At, const uint16_t* B, int64_t ldb, float beta, float* C,
+                       int64_t c_rs, int64_t c_cs);
 template <class T>
 int to_bf16_rows(rla_ctx* ctx, int64_t rows, int64_t cols, const T* src, int64_t s_rs,
                  int64_t s_cs, uint16_t* dst, int64_t ldd);
@@ -334,7 +337,25 @@ int qr_f64(rla_ctx* ctx, int64_t m, int64_t k, double* Y, int64_t ldy, double* R
 
 // ---- RangeFinder -------------------------------------------------------------
 
-enum { PREC_PLAIN = 0, PREC_BF16 = 1 };
+// BF16 products read the resident BF16 copies of A: rows-contiguous
+// (PREC_BF16) or in the GEMM's tiled layout (PREC_BF16_TILED, the same
+// products in the same order)
+enum { PREC_PLAIN = 0, PREC_BF16 = 1, PREC_BF16_TILED = 2 };
+
+static bool is_bf16(int precision) {
+  return precision == PREC_BF16 || precision == PREC_BF16_TILED;
+}
+
+// C (M x N, FP32) = Acopy (M x K, a BF16 copy of A or A') * X, X BF16 with
+// K-contiguous columns (ld ldx)
+static int bf16_product(rla_ctx* ctx, int precision, int64_t M, int64_t N, int64_t K,
+                        const void* Acopy, int64_t ld_a, const uint16_t* X, int64_t ldx, float* C,
+                        int64_t c_rs, int64_t c_cs) {
+  const uint16_t* a = static_cast<const uint16_t*>(Acopy);
+  if (precision == PREC_BF16_TILED)
+    return gemm_bf16_tn_tiled(ctx, M, N, K, 1.f, a, X, ldx, 0.f, C, c_rs, c_cs);
+  return gemm_bf16_tn(ctx, M, N, K, 1.f, a, ld_a, X, ldx, 0.f, C, c_rs, c_cs);
+}
 
 static int64_t ld8(int64_t n) { return (n + 7) & ~int64_t(7); }
 
@@ -356,11 +377,11 @@ static size_t rf_layout(int precision, int a_elem, int64_t m, int64_t n, int64_t
   Bump b{static_cast<char*>(work), bytes};
   RfBufs r{};
   r.Z64 = b.take<double>(n * k);
-  if (precision == PREC_BF16 || a_elem == 4) {
+  if (is_bf16(precision) || a_elem == 4) {
     r.Y32 = b.take<float>(m * k);
     r.Z32 = b.take<float>(n * k);
   }
-  if (precision == PREC_BF16) {
+  if (is_bf16(precision)) {
     r.Ybf = b.take<uint16_t>(ld8(m) * k);
     r.Zbf = b.take<uint16_t>(ld8(n) * k);
   }
@@ -382,11 +403,10 @@ static int rf_at_y(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n
                    const double* Y, int64_t ldy, RfBufs& w) {
   rocblas_handle h;
   int rc;
-  if (precision == PREC_BF16) {
+  if (is_bf16(precision)) {
     rc = to_bf16_rows<double>(ctx, k, m, Y, ldy, 1, w.Ybf, ld8(m));
     if (rc) return rc;
-    rc = gemm_bf16_tn(ctx, n, k, m, 1.f, static_cast<const uint16_t*>(At), ld_t, w.Ybf, ld8(m),
-                      0.f, w.Z32, 1, n);
+    rc = bf16_product(ctx, precision, n, k, m, At, ld_t, w.Ybf, ld8(m), w.Z32, 1, n);
     if (rc) return rc;
     return convert<float, double>(ctx, n, k, w.Z32, n, w.Z64, n);
   }
@@ -417,11 +437,10 @@ static int rf_a_z(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n,
                   int64_t ldy, RfBufs& w) {
   rocblas_handle h;
   int rc;
-  if (precision == PREC_BF16) {
+  if (is_bf16(precision)) {
     rc = to_bf16_rows<double>(ctx, k, n, w.Z64, n, 1, w.Zbf, ld8(n));
     if (rc) return rc;
-    rc = gemm_bf16_tn(ctx, m, k, n, 1.f, static_cast<const uint16_t*>(A_rm), ld_rm, w.Zbf,
-                      ld8(n), 0.f, w.Y32, 1, m);
+    rc = bf16_product(ctx, precision, m, k, n, A_rm, ld_rm, w.Zbf, ld8(n), w.Y32, 1, m);
     if (rc) return rc;
     return convert<float, double>(ctx, m, k, w.Y32, m, Y, ldy);
   }
@@ -453,23 +472,22 @@ static int rf_a_z(rla_ctx* ctx, int precision, int a_elem, int64_t m, int64_t n,
 // kernels between the passes and the products are gone.  Y holds the sketch
 // until the last solve, which writes nothing if any pass broke down, so a
 // rerun needs no saved copy.
-static int rf_run_bf16_fused(rla_ctx* ctx, int64_t m, int64_t n, int64_t k, int64_t power_its,
+static int rf_run_bf16_fused(rla_ctx* ctx, int precision, int64_t m, int64_t n, int64_t k,
+                             int64_t power_its,
                              const void* A_rm, int64_t ld_rm, const void* A_cm, int64_t ld_cm,
                              double* Y, int64_t ldy, RfBufs& w, int* info) {
   QrBufs q;
   if (qr_layout(std::max(m, n), k, w.qr, w.qr_bytes, &q) > w.qr_bytes)
     return fail(RLA_EARG, "RangeFinder workspace too small");
-  const uint16_t* Arm = static_cast<const uint16_t*>(A_rm);
-  const uint16_t* Acm = static_cast<const uint16_t*>(A_cm);
   int ns = 0;
   int rc = cholqr_to<double>(ctx, m, k, Y, ldy, nullptr, 0, w.Ybf, ld8(m), q, info, ++ns);
   if (rc) return rc;
   for (int64_t it = 0; it < power_its; ++it) {
-    rc = gemm_bf16_tn(ctx, n, k, m, 1.f, Acm, ld_cm, w.Ybf, ld8(m), 0.f, w.Z32, 1, n);
+    rc = bf16_product(ctx, precision, n, k, m, A_cm, ld_cm, w.Ybf, ld8(m), w.Z32, 1, n);
     if (rc) return rc;
     rc = cholqr_to<float>(ctx, n, k, w.Z32, n, nullptr, 0, w.Zbf, ld8(n), q, info, ++ns);
     if (rc) return rc;
-    rc = gemm_bf16_tn(ctx, m, k, n, 1.f, Arm, ld_rm, w.Zbf, ld8(n), 0.f, w.Y32, 1, m);
+    rc = bf16_product(ctx, precision, m, k, n, A_rm, ld_rm, w.Zbf, ld8(n), w.Y32, 1, m);
     if (rc) return rc;
     if (it + 1 < power_its) {
       rc = cholqr_to<float>(ctx, m, k, w.Y32, m, nullptr, 0, w.Ybf, ld8(m), q, info, ++ns);
@@ -486,14 +504,15 @@ static int rf_run_bf16_fused(rla_ctx* ctx, int64_t m, int64_t n, int64_t k, int6
 
 static int check_rf(int precision, int a_elem, int64_t m, int64_t n, int64_t k, const void* A,
                     const void* A_rm, const void* A_cm) {
-  if (precision != PREC_PLAIN && precision != PREC_BF16)
-    return fail(RLA_EARG, "precision must be 0 (plain) or 1 (bf16 products)");
+  if (precision != PREC_PLAIN && !is_bf16(precision))
+    return fail(RLA_EARG,
+                "precision must be 0 (plain), 1 (bf16 products) or 2 (bf16, tiled copies)");
   if (a_elem != 4 && a_elem != 8) return fail(RLA_EARG, "A element size must be 4 or 8");
   if (m < 1 || n < 1 || k < 1) return fail(RLA_EARG, "RangeFinder dimensions must be positive");
   if (k > m) return fail(RLA_EARG, "compression dimension larger than the row count");
   if (m > INT32_MAX || n > INT32_MAX) return fail(RLA_ENOTSUP, "dimensions exceed int32");
   if (precision == PREC_PLAIN && !A) return fail(RLA_EARG, "plain precision needs A");
-  if (precision == PREC_BF16 && (!A_rm || !A_cm))
+  if (is_bf16(precision) && (!A_rm || !A_cm))
     return fail(RLA_EARG, "bf16 precision needs both BF16 copies of A");
   return RLA_OK;
 }
@@ -526,8 +545,8 @@ static int rf_run(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, int
                   double* Y, int64_t ldy, RfBufs& w, int* info) {
   int rc;
   // the copy that A'X reads: bf16 columns-contiguous, or plain rows-contiguous
-  const void* At = precision == PREC_BF16 ? A_cm : A_rm;
-  const int64_t ld_t = precision == PREC_BF16 ? ld_cm : ld_rm;
+  const void* At = is_bf16(precision) ? A_cm : A_rm;
+  const int64_t ld_t = is_bf16(precision) ? ld_cm : ld_rm;
   int64_t ns = 0;
   auto qr = [&](int64_t rows, double* M, int64_t ld, bool last) {
     const int passes = last ? 2 : 1;
@@ -670,9 +689,9 @@ int rla_rangefinder(rla_ctx* ctx, int precision, int orthogonalize, int64_t m, i
     for (int v : h) *ok = *ok && v == 0;
     return RLA_OK;
   };
-  if (async && precision == PREC_BF16 && orthogonalize && power_its >= 1) {
+  if (async && is_bf16(precision) && orthogonalize && power_its >= 1) {
     // Y is left as it was unless every pass succeeded: a rerun starts from it
-    rc = rf_run_bf16_fused(ctx, m, n, k, power_its, A_rm, ld_rm, A_cm, ld_cm, Y, ldy, w,
+    rc = rf_run_bf16_fused(ctx, precision, m, n, k, power_its, A_rm, ld_rm, A_cm, ld_cm, Y, ldy, w,
                            w.info);
     if (rc) return rc;
     bool ok = false;

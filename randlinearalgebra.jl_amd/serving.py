@@ -24,7 +24,8 @@ import torch
 
 from . import _dense, _lib
 from ._lib import AdmissionError, ArgumentError
-from .approximators import RangeFinder, _Bf16Copies, complete_approximator, rapproximate_
+from .approximators import (PRECISIONS, RangeFinder, _Bf16Copies, complete_approximator,
+                            rapproximate_)
 from .compressors import Gaussian, Right, as_view, seed
 from .solvers import (BasicLogger, FullResidual, Kaczmarz, L2Norm, Sampling, complete_solver,
                       rsolve)
@@ -42,6 +43,8 @@ class RangeFinderRequest:
 
     def __init__(self, matrix, k, power_its=2, orthogonalize=True, precision="bf16", seed=0,
                  arrival=None):
+        if precision not in PRECISIONS:
+            raise ArgumentError(f"precision must be one of {sorted(PRECISIONS)}")
         self.matrix, self.k, self.power_its = matrix, int(k), int(power_its)
         self.orthogonalize, self.precision, self.seed = orthogonalize, precision, int(seed)
         self.arrival = arrival
@@ -147,7 +150,7 @@ class Server:
         r = self.resident[req.matrix]
         m, n = r.A.shape
         if req.kind == "rangefinder":
-            p = 1 if req.precision == "bf16" else 0
+            p = PRECISIONS[req.precision]
             ws = _dense.query_bytes("rla_rangefinder_workspace_bytes", p, 4, m, n, req.k)
             return ws + 8 * m * req.k * 3 + 4 * n * req.k * 2 + _SLOT_SCRATCH
         ws = _dense.query_bytes("rla_kaczmarz_workspace_bytes", m, n, req.block, req.every)

@@ -171,6 +171,76 @@ def test_gemm_bf16_matches_fp32_reference(rla, dev, M, N, K):
     assert err < 2e-6, err
 
 
+def _tiled_host(Ab):
+    """The tiled BF16 layout from its definition: 128 x 64 tiles, tile (t, s)
+    at (t * nkt + s) * 8192, the 8 entries of row 32 g + r and columns
+    16 q + 8 h .. + 8 at ((4 g + q) * 64 + r + 32 h) * 8; zero padding."""
+    rows, cols = Ab.shape
+    nmt, nkt = -(-rows // 128), -(-cols // 64)
+    out = np.zeros(nmt * nkt * 8192, Ab.dtype)
+    i, l = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    r, c = i % 128, l % 64
+    idx = ((i // 128) * nkt + l // 64) * 8192 + ((4 * (r // 32) + c // 16) * 64 + r % 32
+                                                  + 32 * ((c // 8) % 2)) * 8 + c % 8
+    out[idx] = Ab
+    return out
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_bf16_tiled_copy_layout(rla, dev, dtype):
+    # a strided 130 x 70 view (two row tiles, two K tiles, both partial)
+    rows, cols = 130, 70
+    X = rnd((rows + 5, cols), 3)
+    Xd = colmajor(X, dev, dtype)[:rows]
+    got = rla._dense.bf16_tiled(rla.as_view(Xd)).view(torch.int16).cpu().numpy()
+    # the same rounding as the row copies (through float, nearest even)
+    Xb = torch.as_tensor(X[:rows]).to(dtype).float().to(torch.bfloat16).view(torch.int16).numpy()
+    assert np.array_equal(got, _tiled_host(Xb))
+
+
+@pytest.mark.parametrize("M,N,K", [(1000, 266, 3000), (130, 40, 70), (1000, 266, 40003),
+                                   (8192, 266, 32768), (300, 288, 1000)])
+def test_gemm_bf16_tiled_equals_row_layout(rla, dev, M, N, K):
+    # the same products in the same order whatever A's layout: identical bits,
+    # with and without split K, partial K steps and partial row tiles
+    A = rnd((M, K), 1).astype(np.float32)
+    B = rnd((K, N), 2).astype(np.float32)
+    Ad, Bd = colmajor(A, dev), colmajor(B, dev)
+    cols = rla._dense.bf16_rows(rla.as_view(Bd).T)
+    C1 = torch.zeros(N, M, dtype=F32, device=dev).T
+    rla._dense.gemm_bf16(M, N, K, rla._dense.bf16_rows(rla.as_view(Ad)), cols, rla.as_view(C1))
+    C2 = torch.zeros(N, M, dtype=F32, device=dev).T
+    rla._dense.gemm_bf16_tiled(M, N, K, rla._dense.bf16_tiled(rla.as_view(Ad)), cols,
+                               rla.as_view(C2))
+    assert torch.equal(C1, C2)
+
+
+def test_rangefinder_tiled_copies_give_identical_q(rla, dev):
+    # rla_rangefinder with the row BF16 copies (precision 1) and the tiled
+    # ones (precision 2) on the same sketch: the same Q, bit for bit
+    m, n, k = 3000, 700, 40
+    A = R._q(rnd((m, 60), 4)) @ np.diag(np.exp(-np.arange(60) / 8.0)) @ rnd((60, n), 5)
+    Ad = colmajor(A, dev, F32)
+    v = rla.as_view(Ad)
+    Y0 = colmajor(rnd((m, k), 6), dev, F64)
+    lib = rla._lib
+    nb = rla._dense.query_bytes("rla_rangefinder_workspace_bytes", 1, 4, m, n, k)
+    work = rla._dense.workspace(nb, dev)
+    Qs = []
+    for prec, rm, cm, ld_rm, ld_cm in (
+            (1, rla._dense.bf16_rows(v), rla._dense.bf16_rows(v.T), None, None),
+            (2, rla._dense.bf16_tiled(v), rla._dense.bf16_tiled(v.T), 0, 0)):
+        Y = Y0.clone()
+        lib.call("rla_rangefinder", lib.ctx(dev), prec, 1, m, n, k, 2, 4, lib.ptr(Ad), m,
+                 lib.ptr(rm), rm.shape[1] if ld_rm is None else ld_rm, lib.ptr(cm),
+                 cm.shape[1] if ld_cm is None else ld_cm, lib.ptr(Y), m, lib.ptr(work),
+                 work.numel() * 8)
+        Qs.append(Y)
+    assert torch.equal(Qs[0], Qs[1])
+    Q = Qs[1].cpu().numpy()
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-10
+
+
 @pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 64, 64, 16), (1, 0, 64, 64, 16),
                                          (1, 0, 266, 266, 5000), (0, 0, 5000, 266, 266),
                                          (0, 1, 70, 33, 100), (1, 1, 17, 65, 1030)])

@@ -300,8 +300,9 @@ def tall_sparse_sign_row(rla, torch, reps=10):
 
 def bf16_gemm_c3(rla, torch, reps=20):
     """The two BF16 MFMA products of a C3 RangeFinder solve on their own
-    (Y = A Z: 32768 x 266 over K = 8192 from the row copy of A; Z = A'Y:
-    8192 x 266 over K = 32768 from the column copy), N(0,1) operands; wall
+    (Y = A Z: 32768 x 266 over K = 8192 from a copy of A; Z = A'Y: 8192 x 266
+    over K = 32768 from a copy of A'), N(0,1) operands, with A's copy in the
+    row layout and in the tiled layout the RangeFinder keeps resident; wall
     time over `reps` back-to-back products between device synchronisations."""
     dev = torch.device("cuda", 0)
     g = torch.Generator(device=dev).manual_seed(1234)
@@ -309,27 +310,37 @@ def bf16_gemm_c3(rla, torch, reps=20):
     lib = rla._lib
     for name, M, K in (("A_Z", 32768, 8192), ("At_Y", 8192, 32768)):
         N = 266
-        A = torch.randn(M, K, generator=g, device=dev).to(torch.bfloat16)  # row i = K-run
+        Af = torch.randn(M, K, generator=g, device=dev)
+        A = Af.to(torch.bfloat16)  # row i = K-run
         B = torch.randn(N, K, generator=g, device=dev).to(torch.bfloat16)
         C = torch.empty(N, M, device=dev)  # column-major M x N
         ctx = lib.ctx(dev)
 
-        def run():
+        At = rla._dense.bf16_tiled(rla.as_view(Af))   # the same A, tiled
+        del Af
+
+        def run_rows():
             lib.call("rla_gemm_bf16", ctx, M, N, K, 1.0, lib.ptr(A), K, lib.ptr(B), K, 0.0,
                      lib.ptr(C), 1, M)
-        for _ in range(3):
-            run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            run()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-        flops = 2.0 * M * N * K
-        out[name] = {"M": M, "N": N, "K": K, "us_per_product": dt * 1e6,
-                     "TFLOP_s": flops / dt / 1e12, "frac_bf16_peak": flops / dt / 2.5e15,
-                     "a_bytes_GB_s": M * K * 2 / dt / 1e9}
-        del A, B, C
+
+        def run_tiled():
+            lib.call("rla_gemm_bf16_tiled", ctx, M, N, K, 1.0, lib.ptr(At), lib.ptr(B), K, 0.0,
+                     lib.ptr(C), 1, M)
+        for layout, run in (("tiled", run_tiled), ("rows", run_rows)):
+            for _ in range(3):
+                run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            flops = 2.0 * M * N * K
+            out[f"{name}_{layout}"] = {"M": M, "N": N, "K": K, "us_per_product": dt * 1e6,
+                                       "TFLOP_s": flops / dt / 1e12,
+                                       "frac_bf16_peak": flops / dt / 2.5e15,
+                                       "a_bytes_GB_s": M * K * 2 / dt / 1e9}
+        del A, At, B, C
     return out
 
 

@@ -12,21 +12,23 @@
 //   m_p <- c m_p - s m_q,  m_q <- s m_p + c m_q   (and likewise V),
 // skipped when |g| <= tol sqrt(a b), tol = sqrt(k) eps.
 //
-// Layout: the columns are cut into blocks of 16 (zero columns pad the last
-// block; a zero column is never rotated).  One cooperative launch holds
-// nblk / 2 workgroups of 4 wavefronts, and M and V in global memory (L2).
+// Layout: the columns are cut into blocks of kJB = 16 (zero columns pad the
+// last block; a zero column is never rotated).  One cooperative launch holds
+// nblk / 2 workgroups of kJB / 4 wavefronts, and M and V in global memory
+// (L2).  (Blocks of 8 give twice the workgroups and half the LDS traffic per
+// round, but twice the grid barriers per sweep: slower at k = 266.)
 // A column pair is worked by one 16-lane DPP row: lane i holds rows
 // 2i, 2i + 1, 2i + 32, 2i + 33, ..., so a wavefront works 4 pairs at once,
 // moves columns through LDS with 16-byte accesses, and its sums are four DPP
 // steps within the row.  A sweep visits every column pair once:
 //   * an intra pass: workgroup w loads blocks 2w, 2w+1 into LDS and runs the
-//     round-robin schedule over each block's 16 columns (15 rounds of 8
-//     disjoint pairs per block, 16 pairs per round on the 16 rows);
+//     round-robin schedule over each block's kJB columns (kJB - 1 rounds of
+//     kJB / 2 disjoint pairs per block, kJB pairs per round on the kJB rows);
 //   * nblk - 1 passes of the round-robin schedule over blocks: workgroup w
 //     takes its blocks I, J; row u (wavefront u / 4, row u % 4) keeps
 //     column u of I (and of V) in registers for the whole pass and meets
-//     column (u + r) mod 16 of J, read from and written back to LDS, in
-//     round r (16 rounds).
+//     column (u + r) mod kJB of J, read from and written back to LDS, in
+//     round r (kJB rounds).
 // A grid barrier follows every pass.  The sweep's rotation count decides,
 // after a grid barrier, whether another sweep runs, so every workgroup
 // executes the same barriers.  At most 60 sweeps.
@@ -43,19 +45,19 @@ namespace {
 
 namespace cg = cooperative_groups;
 
-constexpr int kJB = 16;            // columns per block
+constexpr int kJB = 16;            // columns per block (a multiple of 4)
 constexpr int kJMaxK = 288;
 constexpr int kJT = kJMaxK / 16;   // rows per lane of a 16-lane row
 constexpr int kJLd = 304;          // LDS column stride (doubles): the 4 rows of a
                                    // wavefront reading 4 columns fill two bank passes
-constexpr int kJThreads = 256;     // 4 wavefronts x 4 rows: 16 column pairs per round
+constexpr int kJThreads = kJB * 16;  // one 16-lane row per pair: kJB pairs per round
 constexpr int kJMaxSweeps = 60;
 
 struct JacobiArgs {
   int k;           // rows of M and V, and real columns
   int nblk;        // column blocks (even)
-  double* M;       // k x (nblk * 16), ld k
-  double* V;       // k x (nblk * 16), ld k
+  double* M;       // k x (nblk * kJB), ld k
+  double* V;       // k x (nblk * kJB), ld k
   int* rot;        // kJMaxSweeps counters, zero on entry
   int* sweeps;     // out: sweeps run
   double tol;
@@ -183,16 +185,16 @@ __device__ __forceinline__ void apply_rotation(double (&x)[kJT], double (&vx)[kJ
 __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
   extern __shared__ __align__(16) double jsm[];
   const int k = g.k;
-  double* Ms = jsm;                  // 32 columns of M, column c at c * kJLd
-  double* Vs = jsm + 32 * kJLd;      // 32 columns of V
-  __shared__ double dsc[32];         // scales of the LDS columns
+  double* Ms = jsm;                  // 2 kJB columns of M, column c at c * kJLd
+  double* Vs = jsm + 2 * kJB * kJLd;  // 2 kJB columns of V
+  __shared__ double dsc[2 * kJB];    // scales of the LDS columns
   cg::grid_group grid = cg::this_grid();
   const int t = threadIdx.x, lane = t & 63;
-  const int u = (t >> 6) * 4 + (lane >> 4);  // this row's pair slot 0..15
+  const int u = (t >> 6) * 4 + (lane >> 4);  // this row's pair slot 0 .. kJB-1
   const int li = lane & 15;
   const int nblk = g.nblk;
   const int w = blockIdx.x;
-  // the 16 columns of block B of M and V <-> LDS columns c0.. (zero rows from
+  // the kJB columns of block B of M and V <-> LDS columns c0.. (zero rows from
   // k up to 288).  Row u of the workgroup moves column u; every load is
   // issued before the first LDS store.
   auto load_block = [&](int B, int c0) {
@@ -245,15 +247,15 @@ __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
   int sweep = 0;
   for (; sweep < kJMaxSweeps; ++sweep) {
     int nrot = 0;  // this lane's rotations (lane 0 of each row counts)
-    // intra pass: blocks 2w and 2w+1 in LDS columns 0..15 and 16..31
+    // intra pass: blocks 2w and 2w+1 in LDS columns 0.. kJB-1 and kJB.. 2kJB-1
     load_block(2 * w, 0);
     load_block(2 * w + 1, kJB);
     __syncthreads();
     {
-      const int half = u >> 3, a8 = u & 7;
-      for (int ir = 0; ir < 15; ++ir) {
-        const int p = 16 * half + circle_item(a8, ir, 16);
-        const int q = 16 * half + circle_item(15 - a8, ir, 16);
+      const int half = u / (kJB / 2), ap = u % (kJB / 2);
+      for (int ir = 0; ir < kJB - 1; ++ir) {
+        const int p = kJB * half + circle_item(ap, ir, kJB);
+        const int q = kJB * half + circle_item(kJB - 1 - ap, ir, kJB);
         double x[kJT], y[kJT];
         lds_col(Ms, p, x);
         lds_col(Ms, q, y);
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
     // cross passes over the round-robin schedule of blocks
     for (int rr = 0; rr < nblk - 1; ++rr) {
       const int I = circle_item(w, rr, nblk), J = circle_item(nblk - 1 - w, rr, nblk);
-      // column u of I (and of V) in registers, J in LDS columns 0..15
+      // column u of I (and of V) in registers, J in LDS columns 0 .. kJB-1
       double x[kJT], vx[kJT];
 #pragma unroll
       for (int q = 0; q < kJT; ++q) {
@@ -297,8 +299,8 @@ __global__ __launch_bounds__(kJThreads) void svd_jacobi_kernel(JacobiArgs g) {
       double dx = 1.0;
       load_block(J, 0);
       __syncthreads();
-      for (int ir = 0; ir < 16; ++ir) {
-        const int q = (u + ir) & 15;
+      for (int ir = 0; ir < kJB; ++ir) {
+        const int q = (u + ir) % kJB;
         double y[kJT];
         lds_col(Ms, q, y);
         double dy = dsc[q], c, tn;
@@ -420,7 +422,7 @@ int svd_small_f64(rla_ctx* ctx, int64_t k, const double* X, int64_t ldx, int tra
   svd_init_kernel<<<dim3(unsigned(ceil_div(n, 256))), 256, 0, ctx->stream>>>(
       int(k), kp, X, ldx, transpose, M, Vw, rot);
   RLA_LAUNCH_CHECK("svd_init");
-  const size_t lds = size_t(64) * kJLd * sizeof(double);
+  const size_t lds = size_t(4 * kJB) * kJLd * sizeof(double);
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {

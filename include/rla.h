@@ -253,11 +253,14 @@ int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64
                               int64_t n_rhs, const double* X, int64_t x_rs, int64_t x_cs,
                               double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta);
 
-/* The same product for tall X by blocks: one workgroup per (64 KiB block of
- * X, column of X) writes each row's FP64 sum over the block, and the blocks
- * are added in ascending order, so the result is within rounding of
- * rla_sparse_apply (not identical: sums are formed per block).  Workspace:
- * rla_sparse_apply_bell_split_bytes (nblocks * n_rhs * rows doubles).
+/* The same product for tall X by runs of blocks: a workgroup takes up to 8
+ * columns of X (as many rb-row blocks as fit 128 KiB of LDS; each entry of M
+ * it reads serves all of them) and, for at most 1024 output rows, a run of
+ * consecutive blocks, writing each row's FP64 sum over its run; the runs are
+ * added in ascending order, so the result is within rounding of
+ * rla_sparse_apply (not identical: sums are formed per run).  Smaller blocks
+ * (rb) let more columns share the entries read.  The context workspace holds
+ * the runs' sums: at most rla_sparse_apply_bell_split_bytes bytes.
  * Replaces the same mul! as rla_sparse_apply_bell (sparse_sign.jl:339-348,
  * count_sketch.jl:166-213) for operands taller than one block. */
 int rla_sparse_apply_bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,

@@ -337,6 +337,9 @@ def _device_of(*xs):
 
 # partial sums the block-split sparse gather may hold (larger: block by block)
 _BELL_SPLIT_MAX_BYTES = 64 << 20
+# output rows up to which the block-split gather carries each row's sums over
+# a run of blocks (2 per thread of a 512-thread workgroup)
+_BELL_CARRY_ROWS = 1024
 
 
 class _SparseOp:
@@ -419,17 +422,17 @@ class _SparseOp:
         self._ell[key] = out
         return out
 
-    def _bell_for(self, trans_m, dtype):
-        """Block-interleaved gather arrays of op(M) (X blocks of 64 KiB) in
-        `dtype`, built once per operator and cached: (rb, widths, bidx, bval),
-        or None when X fits one block, rows average under 4 entries, or the
-        copy would hold more than four times the entries plus one slot per row
-        and block."""
-        key = ("bell", int(trans_m), dtype)
+    def _bell_for(self, trans_m, dtype, block_bytes=65536):
+        """Block-interleaved gather arrays of op(M) (X blocks of `block_bytes`)
+        in `dtype`, built once per operator and block size and cached: (rb,
+        widths, bidx, bval), or None when X fits one block, rows average under
+        4 entries, or the copy would hold more than four times the entries
+        plus one slot per row and block."""
+        key = ("bell", int(trans_m), dtype, block_bytes)
         if key in self._ell:
             return self._ell[key]
         rows, in_rows = (self.ncols, self.nrows) if trans_m else (self.nrows, self.ncols)
-        rb = 65536 // (torch.finfo(dtype).bits // 8)
+        rb = block_bytes // (torch.finfo(dtype).bits // 8)
         out = None
         if rows > 0 and in_rows > rb and self.nnz >= 4 * rows:
             dev = self.rowidx.device
@@ -460,12 +463,17 @@ class _SparseOp:
             if col_bytes > 64 * 1024 and not (self._uniform and col_bytes <= 160 * 1024):
                 # X taller than one LDS block (and no whole-column sign path):
                 # the block-interleaved copy keeps every metadata load coalesced
-                bell = self._bell_for(trans_m, dt)
+                # the block-split apply shares each entry read among up to 8
+                # columns staged side by side: 16 KiB blocks let 8 fit the LDS
+                from . import _dense
+                small = not self.exact and X.cols >= 2 and C.rows <= _BELL_CARRY_ROWS
+                bell = self._bell_for(trans_m, dt, 16384 if small else 65536)
                 if bell is not None:
                     rb, widths, bidx, bval = bell
                     nblk = (X.rows + rb - 1) // rb
                     split = (not self.exact and nblk >= 2
-                             and nblk * X.cols * C.rows * 8 <= _BELL_SPLIT_MAX_BYTES)
+                             and _dense.query_bytes("rla_sparse_apply_bell_split_bytes", C.rows,
+                                                    X.rows, rb, X.cols) <= _BELL_SPLIT_MAX_BYTES)
                     name = "rla_sparse_apply_bell_split_" if split else "rla_sparse_apply_bell_"
                     _lib.call(f"{name}{_lib.sfx(dt)}", _lib.ctx(X.t.device),
                               C.rows, X.rows, rb, _lib.ptr(widths), _lib.ptr(bidx),

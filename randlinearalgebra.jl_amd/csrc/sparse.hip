@@ -824,89 +824,192 @@ __global__ __launch_bounds__(kGatherThreads) void rowgather_bell_kernel(
   }
 }
 
-// Block-split form of rowgather_bell_kernel for tall X: one workgroup per
-// (64 KiB block of X, column of X) stages that block and writes each output
-// row's FP64 sum over the block's slots (part[(k * nblocks + blk) * rows + i]);
-// bell_combine_kernel adds the blocks in ascending order.  The block of
-// workgroup id: XCD id % 8 takes blocks x, x + 8, ..., all columns of one
-// block before the next, so a block's slots are read from that XCD's L2.
-// Rows are summed block by block rather than entry by entry in one
-// accumulator, so the result is within rounding of rla_sparse_apply, not
-// identical to it.
-template <class T>
+// Block-split form of rowgather_bell_kernel for tall X.  A workgroup takes NC
+// columns of X (the entries of M it reads serve all of them) and a run of G
+// consecutive rb-row blocks: per block it stages the NC column blocks in LDS
+// and walks each output row's slots, adding entry by entry in ascending
+// order into FP64 sums; with CARRY (at most 2 rows per thread) the sums run
+// on across the G blocks and are written once, otherwise G = 1 and each
+// block's sums are written.  bell_combine_kernel adds the runs in ascending
+// order (part[(k * nruns + run) * rows + i]).  Workgroup id: XCD id % 8 takes
+// runs x, x + 8, ..., all column groups of one run before the next, so a
+// run's slots are read from that XCD's L2.  Rows are summed run by run rather
+// than entry by entry in one accumulator, so the result is within rounding
+// of rla_sparse_apply, not identical to it.
+template <class T, int NC, bool CARRY>
 __global__ __launch_bounds__(kGatherThreads) void rowgather_bell_split_kernel(
-    int64_t rows, int64_t in_rows, int64_t rb, int64_t nblocks, int64_t n_rhs,
-    const int32_t* __restrict__ widths, const int32_t* __restrict__ bidx,
-    const T* __restrict__ bval, const T* __restrict__ X, int64_t x_cs, int vec_ok,
-    double* __restrict__ part) {
+    int64_t rows, int64_t in_rows, int64_t rb, int64_t nblocks, int64_t n_rhs, int64_t G,
+    int64_t nruns, int64_t ncg, const int32_t* __restrict__ widths,
+    const int32_t* __restrict__ bidx, const T* __restrict__ bval, const T* __restrict__ X,
+    int64_t x_cs, int vec_ok, double* __restrict__ part) {
   constexpr int B = kBellRoundSlots;
+  constexpr int RP = CARRY ? 2 : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   __shared__ int64_t slot0;
   const int64_t id = blockIdx.x;
   const int64_t xcd = id % 8, seq = id / 8;
-  const int64_t blk = (seq / n_rhs) * 8 + xcd;
-  const int64_t k = seq % n_rhs;
-  if (blk >= nblocks) return;
+  const int64_t run = (seq / ncg) * 8 + xcd;
+  const int64_t k0 = (seq % ncg) * NC;
+  if (run >= nruns) return;
+  const int64_t b_first = run * G;
+  const int64_t b_end = b_first + G < nblocks ? b_first + G : nblocks;
+  const int64_t ls = rb < in_rows ? rb : in_rows;  // LDS stride between the columns
   if (threadIdx.x == 0) {
     int64_t w = 0;
-    for (int64_t b = 0; b < blk; ++b) w += widths[b];
+    for (int64_t b = 0; b < b_first; ++b) w += widths[b];
     slot0 = w;
   }
-  const int64_t j0 = blk * rb;
-  const int64_t jn = in_rows - j0 < rb ? in_rows - j0 : rb;
-  const T* xc = X + k * x_cs + j0;
-  if (vec_ok) {
-    constexpr int VW = 16 / sizeof(T);
-    const int64_t nv = jn / VW;
-    for (int64_t vb = 0; vb < nv; vb += 8 * int64_t(blockDim.x)) {
-      uint4 tmp[8];
+  double acc[RP][NC];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
-        tmp[u] = v < nv ? reinterpret_cast<const uint4*>(xc)[v] : make_uint4(0, 0, 0, 0);
+  for (int q = 0; q < RP; ++q)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[q][c] = 0.0;
+  int64_t sl = -1;  // first slot of the current block (read from slot0 after a barrier)
+  for (int64_t blk = b_first; blk < b_end; ++blk) {
+    const int64_t j0 = blk * rb;
+    const int64_t jn = in_rows - j0 < rb ? in_rows - j0 : rb;
+    __syncthreads();  // the previous block's reads of LDS are done; slot0 is visible
+    if (sl < 0) sl = slot0;
+    // the NC column blocks as one flat range, 8 loads per thread in flight
+    const int64_t ncv = n_rhs - k0 < NC ? n_rhs - k0 : NC;  // columns present
+    if (vec_ok) {
+      constexpr int VW = 16 / sizeof(T);
+      const int64_t nv = jn / VW, tot = ncv * nv;
+      for (int64_t vb = 0; vb < tot; vb += 8 * int64_t(blockDim.x)) {
+        uint4 tmp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t f = vb + u * int64_t(blockDim.x) + threadIdx.x;
+          const int64_t c = f / nv, v = f - c * nv;
+          tmp[u] = f < tot ? reinterpret_cast<const uint4*>(X + (k0 + c) * x_cs + j0)[v]
+                           : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t f = vb + u * int64_t(blockDim.x) + threadIdx.x;
+          const int64_t c = f / nv, v = f - c * nv;
+          if (f < tot) reinterpret_cast<uint4*>(lds + c * ls)[v] = tmp[u];
+        }
       }
+      for (int64_t c = 0; c < ncv; ++c)
+        for (int64_t j = nv * VW + threadIdx.x; j < jn; j += blockDim.x)
+          lds[c * ls + j] = X[(k0 + c) * x_cs + j0 + j];
+    } else {
+      const int64_t tot = ncv * jn;
+      for (int64_t fb = 0; fb < tot; fb += 8 * int64_t(blockDim.x)) {
+        T tmp[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t v = vb + u * int64_t(blockDim.x) + threadIdx.x;
-        if (v < nv) reinterpret_cast<uint4*>(lds)[v] = tmp[u];
+        for (int u = 0; u < 8; ++u) {
+          const int64_t f = fb + u * int64_t(blockDim.x) + threadIdx.x;
+          const int64_t c = f / jn, j = f - c * jn;
+          tmp[u] = f < tot ? X[(k0 + c) * x_cs + j0 + j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t f = fb + u * int64_t(blockDim.x) + threadIdx.x;
+          const int64_t c = f / jn, j = f - c * jn;
+          if (f < tot) lds[c * ls + j] = tmp[u];
+        }
       }
     }
-    for (int64_t j = nv * VW + threadIdx.x; j < jn; j += blockDim.x) lds[j] = xc[j];
-  } else {
-    for (int64_t j = threadIdx.x; j < jn; j += blockDim.x) lds[j] = xc[j];
+    __syncthreads();
+    const int32_t w = widths[blk];
+    const int64_t base = sl * rows;
+    // one output row's slots in this block, added to a[0..NC)
+    auto walk = [&](int64_t i, double (&a)[NC]) {
+      for (int32_t s0 = 0; s0 < w; s0 += B) {
+        int32_t jj[B];
+        T vv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const bool in = s0 + u < w;
+          const int64_t e = base + int64_t(s0 + u) * rows + i;
+          jj[u] = in ? bidx[e] : -1;
+          vv[u] = in ? bval[e] : T(0);
+        }
+        bool done = false;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const int64_t d = int64_t(jj[u]) - j0;
+          const int64_t dd = (d >= 0 && d < jn) ? d : 0;
+          if (jj[u] < 0) done = true;  // first empty slot: the row is done
+          if (!done) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a[c] += double(vv[u]) * double(lds[c * ls + dd]);
+          }
+        }
+        if (done) break;
+      }
+    };
+    if (CARRY) {
+      // the thread's rows walk together: per round, BS slots of each of them
+      // are loaded at once, then added in slot order per row
+      constexpr int BS = 2 * B;
+      bool act[RP];
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < RP; ++q) {
+        act[q] = threadIdx.x + int64_t(q) * blockDim.x < rows && w > 0;
+        any = any || act[q];
+      }
+      for (int32_t s0 = 0; any; s0 += BS) {
+        int32_t jj[RP][BS];
+        T vv[RP][BS];
+#pragma unroll
+        for (int q = 0; q < RP; ++q)
+#pragma unroll
+          for (int u = 0; u < BS; ++u) {
+            const bool in = act[q] && s0 + u < w;
+            const int64_t e =
+                base + int64_t(s0 + u) * rows + threadIdx.x + int64_t(q) * blockDim.x;
+            jj[q][u] = in ? bidx[e] : -1;
+            vv[q][u] = in ? bval[e] : T(0);
+          }
+        any = false;
+#pragma unroll
+        for (int q = 0; q < RP; ++q) {
+          bool more = act[q];
+#pragma unroll
+          for (int u = 0; u < BS; ++u) {
+            if (more && jj[q][u] >= 0) {
+              const int64_t d = int64_t(jj[q][u]) - j0;
+              const int64_t dd = (d >= 0 && d < jn) ? d : 0;
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+                acc[q][c] += double(vv[q][u]) * double(lds[c * ls + dd]);
+            } else {
+              more = false;  // first empty slot (or the block's width): the row is done
+            }
+          }
+          act[q] = more && s0 + BS < w;
+          any = any || act[q];
+        }
+      }
+    } else {
+      for (int64_t i = threadIdx.x; i < rows; i += blockDim.x) {
+        double a[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = 0.0;
+        walk(i, a);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (k0 + c < n_rhs) part[((k0 + c) * nruns + run) * rows + i] = a[c];
+      }
+    }
+    sl += w;
   }
-  __syncthreads();
-  const int32_t w = widths[blk];
-  const int64_t base = slot0 * rows;
-  double* out = part + (k * nblocks + blk) * rows;
-  for (int64_t i = threadIdx.x; i < rows; i += blockDim.x) {
-    double acc = 0.0;
-    for (int32_t s0 = 0; s0 < w; s0 += B) {
-      int32_t jj[B];
-      T vv[B];
+  if (CARRY) {
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const bool in = s0 + u < w;
-        const int64_t e = base + int64_t(s0 + u) * rows + i;
-        jj[u] = in ? bidx[e] : -1;
-        vv[u] = in ? bval[e] : T(0);
-      }
-      bool done = false;
+    for (int q = 0; q < RP; ++q) {
+      const int64_t i = threadIdx.x + int64_t(q) * blockDim.x;
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int64_t d = int64_t(jj[u]) - j0;
-        const T xv = lds[(d >= 0 && d < jn) ? d : 0];
-        if (jj[u] < 0) done = true;  // first empty slot: the row is done
-        if (!done) acc += double(vv[u]) * double(xv);
-      }
-      if (done) break;
+      for (int c = 0; c < NC; ++c)
+        if (i < rows && k0 + c < n_rhs) part[((k0 + c) * nruns + run) * rows + i] = acc[q][c];
     }
-    out[i] = acc;
   }
 }
 
-// C(i, k) = alpha * sum over blocks (ascending) of part + beta * C(i, k)
 template <class T>
 __global__ void bell_combine_kernel(int64_t rows, int64_t nblocks, int64_t n_rhs,
                                     const double* __restrict__ part, T* __restrict__ C,
@@ -921,9 +1024,55 @@ __global__ void bell_combine_kernel(int64_t rows, int64_t nblocks, int64_t n_rhs
   *c = static_cast<T>(beta == 0.0 ? alpha * s : alpha * s + beta * double(*c));
 }
 
-// partial sums of the block-split apply
-static size_t bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs) {
-  return sizeof(double) * size_t(ceil_div(in_rows, rb)) * size_t(n_rhs) * size_t(rows);
+// Shape of the block-split apply: NC columns per workgroup (as many rb-row
+// blocks as fit kSplitLds of LDS, at most 8), runs of G blocks when every
+// thread carries at most 2 output rows (about 256 workgroups in all).
+constexpr size_t kSplitLds = 128 * 1024;
+
+struct SplitShape {
+  int nc;
+  bool carry;
+  int64_t g, nruns, ncg;
+};
+
+static SplitShape split_shape(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,
+                              size_t elem) {
+  SplitShape sh;
+  const int64_t nblocks = ceil_div(in_rows, rb);
+  const size_t col_bytes = size_t(std::min(rb, in_rows)) * elem;
+  sh.nc = 1;
+  while (sh.nc < 8 && sh.nc < n_rhs && size_t(2 * sh.nc) * col_bytes <= kSplitLds) sh.nc *= 2;
+  sh.ncg = ceil_div(n_rhs, sh.nc);
+  sh.carry = rows <= 2 * int64_t(kGatherThreads);
+  sh.g = sh.carry ? std::max<int64_t>(1, ceil_div(nblocks * sh.ncg, 256)) : 1;
+  sh.nruns = ceil_div(nblocks, sh.g);
+  return sh;
+}
+
+// partial sums of the block-split apply (one per run, column and row)
+static size_t bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb, int64_t n_rhs,
+                               size_t elem) {
+  const SplitShape sh = split_shape(rows, in_rows, rb, n_rhs, elem);
+  return sizeof(double) * size_t(sh.nruns) * size_t(n_rhs) * size_t(rows);
+}
+
+template <class T, int NC, bool CARRY>
+static int launch_split(rla_ctx* ctx, const SplitShape& sh, unsigned ids, size_t lds,
+                        int64_t rows, int64_t in_rows, int64_t rb, int64_t nblocks,
+                        int64_t n_rhs, const int32_t* widths, const int32_t* bidx, const T* bval,
+                        const T* X, int64_t x_cs, int vec_ok, double* part) {
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&rowgather_bell_split_kernel<T, NC, CARRY>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(kSplitLds));
+  });
+  RLA_HIP(attr_err);
+  rowgather_bell_split_kernel<T, NC, CARRY><<<dim3(ids), kGatherThreads, lds, ctx->stream>>>(
+      rows, in_rows, rb, nblocks, n_rhs, sh.g, sh.nruns, sh.ncg, widths, bidx, bval, X, x_cs,
+      vec_ok, part);
+  return RLA_OK;
 }
 
 template <class T>
@@ -943,23 +1092,38 @@ static int sparse_apply_bell_split(rla_ctx* ctx, int64_t rows, int64_t in_rows, 
   if (size_t(rb) * sizeof(T) > kGatherLds)
     return fail(RLA_ENOTSUP, "block-interleaved apply: rb * element size must be <= 64 KiB");
   const int64_t nblocks = ceil_div(in_rows, rb);
-  const int64_t ids = ceil_div(nblocks, 8) * 8 * n_rhs;
+  const SplitShape sh = split_shape(rows, in_rows, rb, n_rhs, sizeof(T));
+  const int64_t ids = ceil_div(sh.nruns, 8) * 8 * sh.ncg;
   if (ids > INT32_MAX || rows * n_rhs > INT32_MAX * int64_t(256))
     return fail(RLA_ENOTSUP, "block-interleaved apply: grid too large");
   int rc = use_device(ctx);
   if (rc) return rc;
   void* ws = nullptr;
-  rc = workspace(ctx, bell_split_bytes(rows, in_rows, rb, n_rhs), &ws);
+  rc = workspace(ctx, bell_split_bytes(rows, in_rows, rb, n_rhs, sizeof(T)), &ws);
   if (rc) return rc;
   double* part = static_cast<double*>(ws);
-  const size_t lds = size_t(std::min(rb, in_rows)) * sizeof(T);
+  const int64_t ls = std::min(rb, in_rows);
+  const size_t lds = size_t(sh.nc) * size_t(ls) * sizeof(T);
   const int vec_ok = reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
-                     (x_cs * int64_t(sizeof(T))) % 16 == 0 && (rb * int64_t(sizeof(T))) % 16 == 0;
-  rowgather_bell_split_kernel<T><<<dim3(unsigned(ids)), kGatherThreads, lds, ctx->stream>>>(
-      rows, in_rows, rb, nblocks, n_rhs, widths, bidx, bval, X, x_cs, vec_ok, part);
+                     (x_cs * int64_t(sizeof(T))) % 16 == 0 &&
+                     (rb * int64_t(sizeof(T))) % 16 == 0 && (ls * int64_t(sizeof(T))) % 16 == 0;
+  const unsigned g = static_cast<unsigned>(ids);
+#define RLA_SPLIT(NC)                                                                          \
+  rc = sh.carry ? launch_split<T, NC, true>(ctx, sh, g, lds, rows, in_rows, rb, nblocks, n_rhs, \
+                                            widths, bidx, bval, X, x_cs, vec_ok, part)         \
+                : launch_split<T, NC, false>(ctx, sh, g, lds, rows, in_rows, rb, nblocks,      \
+                                             n_rhs, widths, bidx, bval, X, x_cs, vec_ok, part)
+  switch (sh.nc) {
+    case 8: RLA_SPLIT(8); break;
+    case 4: RLA_SPLIT(4); break;
+    case 2: RLA_SPLIT(2); break;
+    default: RLA_SPLIT(1); break;
+  }
+#undef RLA_SPLIT
+  if (rc) return rc;
   RLA_LAUNCH_CHECK("rowgather_bell_split");
   bell_combine_kernel<T><<<dim3(unsigned(ceil_div(rows * n_rhs, 256))), 256, 0, ctx->stream>>>(
-      rows, nblocks, n_rhs, part, C, c_rs, c_cs, alpha, beta);
+      rows, sh.nruns, n_rhs, part, C, c_rs, c_cs, alpha, beta);
   RLA_LAUNCH_CHECK("bell_combine");
   return RLA_OK;
 }
@@ -1227,7 +1391,9 @@ int rla_sparse_apply_bell_split_bytes(int64_t rows, int64_t in_rows, int64_t rb,
                                       size_t* bytes) {
   if (!bytes || rows < 0 || in_rows < 1 || rb < 1 || n_rhs < 0)
     return fail(RLA_EARG, "rla_sparse_apply_bell_split_bytes: bad arguments");
-  *bytes = bell_split_bytes(rows, in_rows, rb, n_rhs);
+  // the shape depends on the element size: the larger of the two
+  *bytes = std::max(bell_split_bytes(rows, in_rows, rb, n_rhs, sizeof(float)),
+                    bell_split_bytes(rows, in_rows, rb, n_rhs, sizeof(double)));
   return RLA_OK;
 }
 

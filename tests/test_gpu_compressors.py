@@ -358,7 +358,8 @@ def test_tall_left_apply_block_interleaved(rla, dev, dtype, kind, exact):
     C0 = rnd((s, n), 8, NP[dtype])
     C = colmajor(C0, dev)
     rla.mul(C, S, Ad, 0.75, -2.0)
-    assert S.op._ell[("bell", 0, dtype)] is not None
+    # the default apply shares each entry among the 3 columns over 16 KiB blocks
+    assert S.op._ell[("bell", 0, dtype, 65536 if exact else 16384)] is not None
     want = R.sparse_mul(S.op.dense(), A, 0.75, -2.0, C0)
     if exact:
         assert np.array_equal(C.cpu().numpy(), want)
@@ -664,7 +665,7 @@ def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
     C1 = torch.zeros(n, s, dtype=dtype, device=dev).T
     rla.mul(C1, S, Ad, 1.25, 0.0)
     op = S.op
-    assert op._ell[("bell", 0, dtype)] is not None
+    assert op._ell[("bell", 0, dtype, 65536)] is not None
     val, val_t = op._vals_for(dtype)
     C2 = torch.zeros(n, s, dtype=dtype, device=dev).T
     Av = rla.as_view(Ad)
@@ -709,7 +710,8 @@ def _bell_layout(rowptr, colidx, vals, rows, in_rows, rb):
                                                   (700, 33000, 40, 1000), (5, 9, 3, 4),
                                                   (2049, 17000, 1, None), (1000, 17000, 512, None),
                                                   (2048, 17000, 512, None),
-                                                  (1500, 40001, 515, None)])
+                                                  (1500, 40001, 515, None),
+                                                  (1024, 300001, 24, 2048)])
 def test_sparse_block_interleaved_apply_equals_csr_apply(rla, dev, dtype, rows, in_rows, ncols,
                                                          rb):
     # the block-interleaved layout against its definition (built on the host),

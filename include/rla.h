@@ -276,6 +276,36 @@ int rla_sparse_apply_bell_split_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows,
                                     int64_t x_rs, int64_t x_cs, double* C, int64_t c_rs,
                                     int64_t c_cs, double alpha, double beta);
 
+/* Row-group tiles: the same left product for tall Float32 X when M (rows x
+ * in_rows, CSR ptr/idx with strictly ascending columns per row) has at most
+ * 1024 rows and nonzeros of one magnitude (SparseSign, CountSketch).  The
+ * lanes of a wavefront are 64 columns of X; a wavefront keeps the FP32 sums
+ * of its output rows in registers while blocks of 384 rows of X pass through
+ * LDS, each entry of M one 32-bit word read once for all 64 columns.  Each
+ * workgroup sums a range of blocks; the ranges' sums are added in ascending
+ * order in FP64, so the result is within rounding of rla_sparse_apply (not
+ * identical).  Replaces the same mul! as rla_sparse_apply_bell
+ * (sparse_sign.jl:339-348, count_sketch.jl:166-213).
+ *   rla_sparse_tile_shape: words per record and record count of the copy
+ *     (RLA_ENOTSUP above 1024 rows or when a record would not fit the LDS,
+ *     RLA_EDOMAIN for columns out of range or not strictly ascending);
+ *   rla_sparse_tile_build_f32: fills the copy (16-byte aligned, nrec *
+ *     rec_words words) and returns the magnitude (RLA_EDOMAIN if the
+ *     nonzeros do not share one);
+ *   rla_sparse_apply_tile_f32: C = alpha*M*X + beta*C for X column-major
+ *     (x_rs == 1); the context workspace holds the ranges' sums, at most
+ *     rla_sparse_apply_tile_bytes bytes. */
+int rla_sparse_tile_shape(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                          const int32_t* idx, int64_t* rec_words, int64_t* nrec);
+int rla_sparse_tile_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                              const int32_t* idx, const float* val, int64_t rec_words,
+                              uint32_t* tiles, double* magnitude);
+int rla_sparse_apply_tile_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs, size_t* bytes);
+int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
+                              const uint32_t* tiles, double magnitude, int64_t n_rhs,
+                              const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                              int64_t c_cs, double alpha, double beta);
+
 /* Agmon maximal-residual selection, the top-k step of sample_distribution!
  * for the Agmon distribution (src/Compressors/Distributions/agmon.jl:354-416):
  * out[t] (t < k) = the candidate with the t-th largest |v[c]|, ties to the

@@ -340,6 +340,12 @@ _BELL_SPLIT_MAX_BYTES = 64 << 20
 # output rows up to which the block-split gather carries each row's sums over
 # a run of blocks (2 per thread of a 512-thread workgroup)
 _BELL_CARRY_ROWS = 1024
+# Float32 X at least this tall and wide (and M of one magnitude with at most
+# 1024 rows) takes the row-group tile gather: each workgroup's sums (256 KiB
+# for 1024 rows and 64 columns) then stay well below the X it reads
+_TILE_MIN_ROWS = 1 << 19
+_TILE_MIN_COLS = 16
+_TILE_MAX_ROWS = 1024
 
 
 class _SparseOp:
@@ -455,9 +461,48 @@ class _SparseOp:
         self._ell[key] = out
         return out
 
+    def _tile_for(self, trans_m):
+        """Row-group tile copy of op(M) for Float32 X (built once per operator
+        and cached): (words per record, records, magnitude), or None when M has
+        more than 1024 rows, nonzeros of several magnitudes, or a record would
+        not fit the LDS."""
+        key = ("tile", int(trans_m))
+        if key in self._ell:
+            return self._ell[key]
+        rows, in_rows = (self.ncols, self.nrows) if trans_m else (self.nrows, self.ncols)
+        out = None
+        if self._uniform and 0 < rows <= _TILE_MAX_ROWS:
+            dev = self.rowidx.device
+            ctx = _lib.ctx(dev)
+            ptr, idx = (self.colptr, self.rowidx) if trans_m else (self.rowptr, self.colidx)
+            words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
+            rc = _lib.call_rc("rla_sparse_tile_shape", ctx, rows, in_rows, _lib.ptr(ptr),
+                              _lib.ptr(idx), ctypes.byref(words), ctypes.byref(nrec))
+            if rc == 0:
+                val, val_t = self._vals_for(torch.float32)
+                v = val if trans_m else val_t
+                tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
+                mag = ctypes.c_double(0.0)
+                rc = _lib.call_rc("rla_sparse_tile_build_f32", ctx, rows, in_rows, _lib.ptr(ptr),
+                                  _lib.ptr(idx), _lib.ptr(v), words.value, _lib.ptr(tiles),
+                                  ctypes.byref(mag))
+                if rc == 0:
+                    out = (words.value, tiles, mag.value)
+        self._ell[key] = out
+        return out
+
     def apply(self, trans_m, X, C, alpha, beta):
         """C = alpha*op(M)*X + beta*C on strided views (op = M or M')."""
         dt = X.dtype
+        if (dt == torch.float32 and X.rs == 1 and not self.exact and X.rows >= _TILE_MIN_ROWS
+                and X.cols >= _TILE_MIN_COLS and C.rows <= _TILE_MAX_ROWS):
+            tile = self._tile_for(trans_m)
+            if tile is not None:
+                words, tiles, mag = tile
+                _lib.call("rla_sparse_apply_tile_f32", _lib.ctx(X.t.device), C.rows, X.rows,
+                          words, _lib.ptr(tiles), mag, X.cols, _lib.ptr(X.t), X.rs, X.cs,
+                          _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
+                return
         if X.rs == 1 and X.rows > 0 and X.cols > 0:
             col_bytes = X.rows * (torch.finfo(dt).bits // 8)
             if col_bytes > 64 * 1024 and not (self._uniform and col_bytes <= 160 * 1024):

@@ -1246,6 +1246,468 @@ static int sparse_apply_bell(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_
   return RLA_OK;
 }
 
+// ---- row-group tiles: tall X, one magnitude, at most 1024 output rows ------
+//
+// The lanes of a wavefront are 64 columns of X and the wavefront owns RW
+// output rows, whose FP32 sums sit in registers for the whole launch.  X is
+// read in blocks of kTgJ rows: a block of 64 columns is staged in LDS column
+// by column (column stride kTgJ + 1 words, so a wavefront reading row j of all
+// 64 columns hits 64 distinct banks, and word kTgJ of every column holds 0).
+// Each entry of M is then one wave-uniform 32-bit word: the block-local row
+// of X times 4 (the LDS byte offset) in the low half and its sign as the
+// half-precision number +-1 in the high half, read once for all 64 columns:
+// per entry one LDS read, one address add and one mixed-precision FMA
+// (+-1 * x + sum, exact in the multiply) per lane.
+//
+// Record (block b, row group g) of a tiled copy, in 32-bit words:
+//   header: (RWG + 1) 16-bit step offsets, row i of the group owning steps
+//           [ptr[i], ptr[i+1]) (rounded up to whole words, then to 4 words);
+//   steps:  two entry words each, a row's entries in ascending column order,
+//           padded to an even count with the word of the zero row kTgJ;
+// records are l32 words apart (l32 a multiple of 4).  The workgroup of
+// (range q, row group g, column group) stages records and X blocks of its
+// range in turn (the next block's loads are in flight while the current one
+// is gathered), writes its FP32 sums, and a combine kernel adds the ranges'
+// sums in ascending order in FP64 and applies alpha*magnitude and beta.
+constexpr int kTgJ = 384;
+constexpr int kTgThreads = 1024;
+constexpr int kTgWaves = kTgThreads / 64;
+constexpr int kTgStride = kTgJ + 1;
+constexpr uint32_t kTgPlus = 0x3c000000u, kTgMinus = 0xbc000000u;  // +-1 in the high half
+constexpr uint32_t kTgDummy = uint32_t(kTgJ) * 4u | kTgPlus;
+constexpr size_t kTgLdsX = size_t(64) * kTgStride * sizeof(float);
+constexpr size_t kTgLdsMax = 160 * 1024;
+constexpr size_t kTgSlack = 16;  // LDS past a record: the last step reads two words ahead
+constexpr int64_t kTgMaxRows = int64_t(kTgWaves) * 64;
+constexpr int kTgWorkgroups = 256;  // one per CU
+
+struct TileShape {
+  int rw, rwg, g, h32, ncg, q;
+  int64_t nblocks;
+};
+
+static TileShape tile_shape(int64_t rows, int64_t in_rows, int64_t n_rhs) {
+  TileShape s;
+  s.rw = 4;
+  while (s.rw < 64 && int64_t(kTgWaves) * s.rw < rows) s.rw *= 2;
+  s.rwg = kTgWaves * s.rw;
+  s.g = int(ceil_div(std::max<int64_t>(rows, 1), s.rwg));
+  s.h32 = int(ceil_div((s.rwg + 2) / 2, 4) * 4);
+  s.nblocks = ceil_div(in_rows, kTgJ);
+  s.ncg = int(ceil_div(std::max<int64_t>(n_rhs, 1), 64));
+  const int64_t q = kTgWorkgroups / (int64_t(s.g) * s.ncg);
+  s.q = int(std::max<int64_t>(1, std::min<int64_t>(q, s.nblocks)));
+  return s;
+}
+
+static size_t tile_part_bytes(const TileShape& s) {
+  return sizeof(float) * size_t(s.ncg) * size_t(s.g) * size_t(s.q) * size_t(s.rwg) * 64;
+}
+
+// pc[b * rows + i] = pairs of row i in block b (pc zeroed before); *bad set
+// when a row's columns are out of range or not strictly ascending.
+__global__ void tile_count_kernel(int64_t rows, int64_t in_rows, const int32_t* __restrict__ ptr,
+                                  const int32_t* __restrict__ idx, int32_t* __restrict__ pc,
+                                  int32_t* __restrict__ bad) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  int64_t prev = -1, cur = -1;
+  int32_t cnt = 0;
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+    const int64_t c = idx[p];
+    if (c <= prev || c >= in_rows) {
+      atomicOr(bad, 1);
+      return;
+    }
+    prev = c;
+    const int64_t b = c / kTgJ;
+    if (b != cur) {
+      if (cur >= 0) pc[cur * rows + i] = (cnt + 1) / 2;
+      cur = b;
+      cnt = 0;
+    }
+    ++cnt;
+  }
+  if (cur >= 0) pc[cur * rows + i] = (cnt + 1) / 2;
+}
+
+// One workgroup per record: the maximum of the records' pair counts in *mx.
+__global__ __launch_bounds__(kTgThreads) void tile_stats_kernel(int64_t rows, int rwg,
+                                                                int g_count,
+                                                                const int32_t* __restrict__ pc,
+                                                                int32_t* __restrict__ mx) {
+  __shared__ int32_t s[kTgThreads];
+  const int64_t b = blockIdx.x / g_count;
+  const int g = blockIdx.x % g_count;
+  const int t = threadIdx.x;
+  const int64_t i = int64_t(g) * rwg + t;
+  s[t] = (t < rwg && i < rows) ? pc[b * rows + i] : 0;
+  __syncthreads();
+  if (t == 0) {
+    int32_t tot = 0;
+    for (int k = 0; k < rwg; ++k) tot += s[k];
+    atomicMax(mx, tot);
+  }
+}
+
+// Record headers: exclusive prefix sums of each row group's pair counts.
+__global__ __launch_bounds__(kTgThreads) void tile_header_kernel(int64_t rows, int rwg,
+                                                                 int g_count, int64_t l32,
+                                                                 const int32_t* __restrict__ pc,
+                                                                 uint32_t* __restrict__ tiles) {
+  __shared__ int32_t s[kTgThreads];
+  const int64_t b = blockIdx.x / g_count;
+  const int g = blockIdx.x % g_count;
+  const int t = threadIdx.x;
+  const int64_t i = int64_t(g) * rwg + t;
+  s[t] = (t < rwg && i < rows) ? pc[b * rows + i] : 0;
+  __syncthreads();
+  if (t == 0) {
+    uint16_t* h = reinterpret_cast<uint16_t*>(tiles + int64_t(blockIdx.x) * l32);
+    int32_t run = 0;
+    for (int k = 0; k < rwg; ++k) {
+      h[k] = static_cast<uint16_t>(run);
+      run += s[k];
+    }
+    h[rwg] = static_cast<uint16_t>(run);
+  }
+}
+
+// Entries: thread per output row, walking its CSR row once.
+template <class T>
+__global__ void tile_fill_kernel(int64_t rows, int rwg, int g_count, int h32, int64_t l32,
+                                 const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                 const T* __restrict__ val, uint32_t* __restrict__ tiles,
+                                 int32_t* __restrict__ mixed) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+  if (p0 == p1) return;
+  const T mag = val[0] < T(0) ? -val[0] : val[0];
+  const int g = int(i / rwg), r = int(i % rwg);
+  bool bad = false;
+  int64_t cur = -1;
+  uint32_t* out = nullptr;
+  int32_t k = 0;
+  for (int32_t p = p0; p < p1; ++p) {
+    const int64_t c = idx[p];
+    const int64_t b = c / kTgJ;
+    if (b != cur) {
+      if (cur >= 0 && (k & 1)) out[k] = kTgDummy;
+      cur = b;
+      uint32_t* rec = tiles + (b * g_count + g) * l32;
+      const uint16_t* h = reinterpret_cast<const uint16_t*>(rec);
+      out = rec + h32 + 2 * int64_t(h[r]);
+      k = 0;
+    }
+    const T v = val[p];
+    bad |= !(v == mag || v == -mag);
+    out[k++] = uint32_t(c - b * kTgJ) * 4u | (v < T(0) ? kTgMinus : kTgPlus);
+  }
+  if (k & 1) out[k] = kTgDummy;
+  if (bad) atomicOr(mixed, 1);
+}
+
+__global__ void tile_zero_kernel(uint32_t* p, int64_t n) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * blockDim.x)
+    p[e] = 0u;
+}
+
+template <int RW>
+__global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
+    int64_t in_rows, int64_t nblocks, int G, int Q, int NCG, int64_t n_rhs, int l32, int h32,
+    const uint32_t* __restrict__ tiles, const float* __restrict__ X, int64_t x_cs,
+    float* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* xl = reinterpret_cast<float*>(smem_raw);
+  uint32_t* rec = reinterpret_cast<uint32_t*>(smem_raw + kTgLdsX);
+  // the G row groups of one range and column group are ids 8 apart (one XCD)
+  const int id = blockIdx.x;
+  const int xcd = id % 8, seq = id / 8;
+  const int g = seq % G;
+  const int64_t qc = int64_t(seq / G) * 8 + xcd;
+  if (qc >= int64_t(Q) * NCG) return;
+  const int q = int(qc % Q), cg = int(qc / Q);
+  const int64_t b0 = int64_t(q) * nblocks / Q, b1 = int64_t(q + 1) * nblocks / Q;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t k0 = int64_t(cg) * 64;
+  const int n4 = l32 / 4;
+  if (threadIdx.x < 64) xl[threadIdx.x * kTgStride + kTgJ] = 0.f;
+
+  // staging registers: wavefront w holds columns w + 16 t, rows lane + 64 i
+  constexpr int NI = kTgJ / 64;
+  float sx[4 * NI];
+  uint4 sr[2];
+  auto load = [&](int64_t b) {
+    const int64_t j0 = b * kTgJ;
+    const bool full = j0 + kTgJ <= in_rows;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t col = k0 + w + 16 * t;
+      const bool colok = col < n_rhs;
+      const float* xc = X + (colok ? col : 0) * x_cs + j0 + lane;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const bool ok = colok && (full || j0 + lane + 64 * i < in_rows);
+        sx[t * NI + i] = ok ? xc[64 * i] : 0.f;
+      }
+    }
+    const uint4* src = reinterpret_cast<const uint4*>(tiles + (b * G + g) * int64_t(l32));
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int k = threadIdx.x + u * kTgThreads;
+      sr[u] = k < n4 ? src[k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < NI; ++i) xl[(w + 16 * t) * kTgStride + lane + 64 * i] = sx[t * NI + i];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int k = threadIdx.x + u * kTgThreads;
+      if (k < n4) reinterpret_cast<uint4*>(rec)[k] = sr[u];
+    }
+  };
+
+  float acc[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) acc[r] = 0.f;
+  const unsigned char* colb = smem_raw + size_t(lane) * kTgStride * sizeof(float);
+  if (b0 < b1) load(b0);
+  for (int64_t b = b0; b < b1; ++b) {
+    __syncthreads();  // the previous block's gathers are done
+    store();
+    __syncthreads();
+    if (b + 1 < b1) load(b + 1);
+    const uint16_t* rp = reinterpret_cast<const uint16_t*>(rec) + w * RW;
+    const int wbase = __builtin_amdgcn_readfirstlane(int(rp[0]));
+    const int nsteps = __builtin_amdgcn_readfirstlane(int(rp[RW])) - wbase;
+    const int vp = int(rp[lane < RW ? lane : RW]) - wbase;  // lane r: row r's first step
+    const uint2* ps = reinterpret_cast<const uint2*>(rec + h32) + wbase;
+    auto xat = [&](uint32_t word) -> float {
+      return *reinterpret_cast<const float*>(colb + (word & 0xffffu));
+    };
+    auto sgn = [](uint32_t word) -> float {
+      return float(__builtin_bit_cast(_Float16, static_cast<uint16_t>(word >> 16)));
+    };
+    // the next step's two words are read while this one gathers
+    uint2 nx = ps[0];
+    int p = 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int p1 = r + 1 < RW ? __builtin_amdgcn_readlane(vp, r + 1) : nsteps;
+      float a = acc[r];
+      for (; p + 2 <= p1; p += 2) {  // two steps, the reads alternating registers
+        const uint2 nb = ps[p + 1];
+        const float x0 = xat(nx.x), x1 = xat(nx.y);
+        a = __builtin_fmaf(sgn(nx.x), x0, a);
+        a = __builtin_fmaf(sgn(nx.y), x1, a);
+        nx = ps[p + 2];
+        const float x2 = xat(nb.x), x3 = xat(nb.y);
+        a = __builtin_fmaf(sgn(nb.x), x2, a);
+        a = __builtin_fmaf(sgn(nb.y), x3, a);
+      }
+      if (p < p1) {
+        const uint2 cu = nx;
+        nx = ps[p + 1];
+        const float x0 = xat(cu.x), x1 = xat(cu.y);
+        a = __builtin_fmaf(sgn(cu.x), x0, a);
+        a = __builtin_fmaf(sgn(cu.y), x1, a);
+        ++p;
+      }
+      acc[r] = a;
+    }
+  }
+  float* pp = part + ((int64_t(cg) * G + g) * Q + q) * int64_t(kTgWaves * RW) * 64 +
+              int64_t(w) * RW * 64 + lane;
+#pragma unroll
+  for (int r = 0; r < RW; ++r) pp[r * 64] = acc[r];
+}
+
+// C[i, k] = alpha*magnitude*(sum over ranges, ascending, in FP64) + beta*C.
+__global__ void tile_combine_kernel(int64_t rows, int64_t n_rhs, int G, int Q, int rwg,
+                                    const float* __restrict__ part, double scale,
+                                    float* __restrict__ C, int64_t c_rs, int64_t c_cs,
+                                    double beta) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int lane = int(e & 63);
+  const int64_t rc = e >> 6;
+  const int64_t i = rc % rows, cg = rc / rows;
+  const int64_t col = cg * 64 + lane;
+  if (col >= n_rhs) return;
+  const int g = int(i / rwg), r = int(i % rwg);
+  const int64_t stride = int64_t(rwg) * 64;
+  const float* p = part + (cg * G + g) * int64_t(Q) * stride + int64_t(r) * 64 + lane;
+  double s = 0.0;
+  for (int k = 0; k < Q; ++k) s += double(p[k * stride]);
+  float* c = C + i * c_rs + col * c_cs;
+  *c = static_cast<float>(beta == 0.0 ? scale * s : scale * s + beta * double(*c));
+}
+
+static int tile_shape_query(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                            const int32_t* idx, int64_t* rec_words, int64_t* nrec) {
+  RLA_CHECK_CTX(ctx);
+  if (!rec_words || !nrec) return fail(RLA_EARG, "row-group tiles: NULL output");
+  *rec_words = 0;
+  *nrec = 0;
+  if (rows < 1 || in_rows < 1)
+    return fail(RLA_EARG, "row-group tiles: need rows >= 1 and in_rows >= 1");
+  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tiles: more than 1024 rows");
+  if (!ptr || !idx) return fail(RLA_EARG, "row-group tiles: NULL pointer");
+  const TileShape sh = tile_shape(rows, in_rows, 1);
+  if (sh.nblocks * int64_t(sh.g) > INT32_MAX)
+    return fail(RLA_ENOTSUP, "row-group tiles: too many records");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const size_t pc_bytes = sizeof(int32_t) * size_t(sh.nblocks) * size_t(rows);
+  void* ws = nullptr;
+  rc = workspace(ctx, 16 + pc_bytes, &ws);
+  if (rc) return rc;
+  int32_t* flags = static_cast<int32_t*>(ws);  // bad columns, largest record
+  int32_t* pc = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + 16);
+  zero_i32_kernel<<<1, 64, 0, ctx->stream>>>(flags, 4);
+  tile_zero_kernel<<<dim3(unsigned(std::min<int64_t>(4096, ceil_div(sh.nblocks * rows, 256)))),
+                    256, 0, ctx->stream>>>(reinterpret_cast<uint32_t*>(pc), sh.nblocks * rows);
+  tile_count_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, in_rows, ptr, idx, pc, flags);
+  tile_stats_kernel<<<dim3(unsigned(sh.nblocks * sh.g)), kTgThreads, 0, ctx->stream>>>(
+      rows, sh.rwg, sh.g, pc, flags + 1);
+  RLA_LAUNCH_CHECK("tile_stats");
+  int32_t h[2] = {0, 0};
+  RLA_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (h[0])
+    return fail(RLA_EDOMAIN,
+                "row-group tiles: columns out of range or not strictly ascending in a row");
+  const int64_t words = sh.h32 + ceil_div(2 * int64_t(std::max<int32_t>(h[1], 1)), 4) * 4;
+  if (kTgLdsX + size_t(words) * 4 + kTgSlack > kTgLdsMax)
+    return fail(RLA_ENOTSUP, "row-group tiles: a record does not fit the LDS");
+  *rec_words = words;
+  *nrec = sh.nblocks * sh.g;
+  return RLA_OK;
+}
+
+template <class T>
+static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                      const int32_t* idx, const T* val, int64_t rec_words, uint32_t* tiles,
+                      double* magnitude) {
+  RLA_CHECK_CTX(ctx);
+  if (!magnitude) return fail(RLA_EARG, "row-group tiles: magnitude is NULL");
+  *magnitude = 0.0;
+  if (rows < 1 || in_rows < 1 || rows > kTgMaxRows)
+    return fail(RLA_EARG, "row-group tiles: need 1 <= rows <= 1024 and in_rows >= 1");
+  if (!ptr || !idx || !val || !tiles) return fail(RLA_EARG, "row-group tiles: NULL pointer");
+  const TileShape sh = tile_shape(rows, in_rows, 1);
+  if (rec_words < sh.h32 + 4 || rec_words % 4 ||
+      kTgLdsX + size_t(rec_words) * 4 + kTgSlack > kTgLdsMax)
+    return fail(RLA_EARG, "row-group tiles: bad record length");
+  if (reinterpret_cast<uintptr_t>(tiles) % 16)
+    return fail(RLA_EARG, "row-group tiles: the copy must be 16-byte aligned");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  int32_t nnz = 0;
+  RLA_HIP(hipMemcpyAsync(&nnz, ptr + rows, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  const size_t pc_bytes = sizeof(int32_t) * size_t(sh.nblocks) * size_t(rows);
+  void* ws = nullptr;
+  rc = workspace(ctx, 16 + pc_bytes, &ws);
+  if (rc) return rc;
+  int32_t* flags = static_cast<int32_t*>(ws);  // bad columns, mixed magnitudes
+  int32_t* pc = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + 16);
+  const int64_t nrec = sh.nblocks * sh.g;
+  zero_i32_kernel<<<1, 64, 0, ctx->stream>>>(flags, 4);
+  tile_zero_kernel<<<dim3(unsigned(std::min<int64_t>(4096, ceil_div(sh.nblocks * rows, 256)))),
+                    256, 0, ctx->stream>>>(reinterpret_cast<uint32_t*>(pc), sh.nblocks * rows);
+  tile_zero_kernel<<<dim3(unsigned(std::min<int64_t>(4096, ceil_div(nrec * rec_words, 256)))),
+                    256, 0, ctx->stream>>>(tiles, nrec * rec_words);
+  tile_count_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, in_rows, ptr, idx, pc, flags);
+  tile_header_kernel<<<dim3(unsigned(nrec)), kTgThreads, 0, ctx->stream>>>(rows, sh.rwg, sh.g,
+                                                                           rec_words, pc, tiles);
+  RLA_LAUNCH_CHECK("tile_header");
+  int32_t h[2] = {0, 0};
+  RLA_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (h[0])
+    return fail(RLA_EDOMAIN,
+                "row-group tiles: columns out of range or not strictly ascending in a row");
+  if (nnz == 0) return RLA_OK;
+  tile_fill_kernel<T><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+      rows, sh.rwg, sh.g, sh.h32, rec_words, ptr, idx, val, tiles, flags + 1);
+  RLA_LAUNCH_CHECK("tile_fill");
+  T v0 = T(0);
+  RLA_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipMemcpyAsync(&v0, val, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  RLA_HIP(hipStreamSynchronize(ctx->stream));
+  if (h[1]) return fail(RLA_EDOMAIN, "row-group tiles: the nonzeros do not share one magnitude");
+  *magnitude = double(v0 < T(0) ? -v0 : v0);
+  return RLA_OK;
+}
+
+template <int RW>
+static int launch_tile(rla_ctx* ctx, const TileShape& sh, unsigned ids, int64_t in_rows,
+                       int64_t n_rhs, int l32, const uint32_t* tiles, const float* X,
+                       int64_t x_cs, float* part) {
+  static std::once_flag once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(once, [&] {
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&tilegather_kernel<RW>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(kTgLdsMax));
+  });
+  RLA_HIP(attr_err);
+  const size_t lds = kTgLdsX + size_t(l32) * 4 + kTgSlack;
+  tilegather_kernel<RW><<<dim3(ids), kTgThreads, lds, ctx->stream>>>(
+      in_rows, sh.nblocks, sh.g, sh.q, sh.ncg, n_rhs, l32, sh.h32, tiles, X, x_cs, part);
+  return RLA_OK;
+}
+
+static int sparse_apply_tile(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
+                             const uint32_t* tiles, double magnitude, int64_t n_rhs,
+                             const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                             int64_t c_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
+  if (in_rows < 1) return fail(RLA_EARG, "row-group tile apply: need in_rows >= 1");
+  if (rows == 0 || n_rhs == 0) return RLA_OK;
+  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tile apply: more than 1024 rows");
+  if (!tiles || !X || !C) return fail(RLA_EARG, "row-group tile apply: NULL pointer");
+  if (x_rs != 1) return fail(RLA_EARG, "row-group tile apply needs X column-major (x_rs == 1)");
+  const TileShape sh = tile_shape(rows, in_rows, n_rhs);
+  if (rec_words < sh.h32 + 4 || rec_words % 4 ||
+      kTgLdsX + size_t(rec_words) * 4 + kTgSlack > kTgLdsMax)
+    return fail(RLA_EARG, "row-group tile apply: bad record length");
+  if (reinterpret_cast<uintptr_t>(tiles) % 16)
+    return fail(RLA_EARG, "row-group tile apply: the copy must be 16-byte aligned");
+  const int64_t ids = ceil_div(int64_t(sh.q) * sh.ncg, 8) * 8 * sh.g;
+  if (ids > INT32_MAX || rows * sh.ncg * 64 > INT32_MAX * int64_t(256))
+    return fail(RLA_ENOTSUP, "row-group tile apply: grid too large");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = workspace(ctx, tile_part_bytes(sh), &ws);
+  if (rc) return rc;
+  float* part = static_cast<float*>(ws);
+  const unsigned g = static_cast<unsigned>(ids);
+  const int l32 = static_cast<int>(rec_words);
+  switch (sh.rw) {
+    case 64: rc = launch_tile<64>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
+    case 32: rc = launch_tile<32>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
+    case 16: rc = launch_tile<16>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
+    case 8: rc = launch_tile<8>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
+    default: rc = launch_tile<4>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
+  }
+  if (rc) return rc;
+  RLA_LAUNCH_CHECK("tilegather");
+  tile_combine_kernel<<<dim3(unsigned(ceil_div(rows * sh.ncg * 64, 256))), 256, 0,
+                        ctx->stream>>>(rows, n_rhs, sh.g, sh.q, sh.rwg, part, alpha * magnitude,
+                                       C, c_rs, c_cs, beta);
+  RLA_LAUNCH_CHECK("tile_combine");
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
@@ -1403,6 +1865,32 @@ int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64
                               double* C, int64_t c_rs, int64_t c_cs, double alpha, double beta) {
   return sparse_apply_bell<double>(ctx, rows, in_rows, rb, widths, bidx, bval, n_rhs, X, x_rs,
                                    x_cs, C, c_rs, c_cs, alpha, beta);
+}
+
+int rla_sparse_tile_shape(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                          const int32_t* idx, int64_t* rec_words, int64_t* nrec) {
+  return tile_shape_query(ctx, rows, in_rows, ptr, idx, rec_words, nrec);
+}
+
+int rla_sparse_tile_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
+                              const int32_t* idx, const float* val, int64_t rec_words,
+                              uint32_t* tiles, double* magnitude) {
+  return tile_build<float>(ctx, rows, in_rows, ptr, idx, val, rec_words, tiles, magnitude);
+}
+
+int rla_sparse_apply_tile_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs, size_t* bytes) {
+  if (!bytes || rows < 0 || in_rows < 1 || n_rhs < 0)
+    return fail(RLA_EARG, "rla_sparse_apply_tile_bytes: bad arguments");
+  *bytes = (rows == 0 || n_rhs == 0) ? 0 : tile_part_bytes(tile_shape(rows, in_rows, n_rhs));
+  return RLA_OK;
+}
+
+int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
+                              const uint32_t* tiles, double magnitude, int64_t n_rhs,
+                              const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
+                              int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_tile(ctx, rows, in_rows, rec_words, tiles, magnitude, n_rhs, X, x_rs, x_cs,
+                           C, c_rs, c_cs, alpha, beta);
 }
 
 }  // extern "C"

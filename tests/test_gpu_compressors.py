@@ -790,3 +790,166 @@ def test_sparse_block_interleaved_rejects_unsorted_rows(rla, dev):
         rc = lib.call_rc("rla_sparse_bell_widths", ctx, 2, 9, 4, lib.ptr(rowptr),
                          lib.ptr(colidx), lib.ptr(widths), ctypes.byref(total))
         assert rc == lib.RLA_EDOMAIN and total.value == 0
+
+
+# ---- row-group tiles (tall Float32 X, one magnitude, <= 1024 rows) -----------
+
+TILE_J = 384   # rows of X per block
+
+
+def _tile_layout(rowptr, colidx, val, rows, in_rows, words):
+    """The row-group tile copy from its definition: per (block, row group) a
+    16-bit step header (exclusive sums of each row's entries rounded up to
+    even, halved), then the rows' entry words (block-local row * 4 | +-1 as
+    a half-precision number << 16), odd rows padded with the zero row's word."""
+    rw = 4
+    while rw < 64 and 16 * rw < rows:
+        rw *= 2
+    rwg = 16 * rw
+    ng = -(-rows // rwg)
+    h32 = -(-((rwg + 2) // 2) // 4) * 4
+    nb = -(-in_rows // TILE_J)
+    plus, minus = np.uint32(0x3C00 << 16), np.uint32(0xBC00 << 16)
+    dummy = np.uint32(TILE_J * 4) | plus
+    out = np.zeros((nb * ng, words), np.uint32)
+    per = {}
+    for i in range(rows):
+        c = colidx[rowptr[i]:rowptr[i + 1]].astype(np.int64)
+        v = val[rowptr[i]:rowptr[i + 1]]
+        blk = c // TILE_J
+        for b in np.unique(blk):
+            sel = blk == b
+            w = (c[sel] - b * TILE_J).astype(np.uint32) * 4 | np.where(v[sel] < 0, minus, plus)
+            if w.size % 2:
+                w = np.append(w, dummy)
+            per[(int(b), i)] = w.astype(np.uint32)
+    for b in range(nb):
+        for g in range(ng):
+            rec = out[b * ng + g]
+            hdr = rec.view(np.uint16)
+            run = 0
+            for r in range(rwg):
+                hdr[r] = run
+                w = per.get((b, g * rwg + r))
+                if w is not None:
+                    rec[h32 + 2 * run:h32 + 2 * run + w.size] = w
+                    run += w.size // 2
+            hdr[rwg] = run
+    return out.reshape(-1)
+
+
+def _tile_operator(dev, rows, in_rows, k, seed, mag=None):
+    """CSR of an operator with k distinct rows per column and values
+    +-mag (mag = 1/sqrt(k) by default)."""
+    g = np.random.default_rng(seed)
+    mag = 1.0 / math.sqrt(k) if mag is None else mag
+    cols = np.repeat(np.arange(in_rows), k)
+    rws = np.concatenate([g.choice(rows, k, replace=False) for _ in range(in_rows)])
+    vals = np.where(g.integers(0, 2, cols.size) == 1, mag, -mag).astype(np.float32)
+    order = np.lexsort((cols, rws))
+    rws, cols, vals = rws[order], cols[order], vals[order]
+    rowptr = np.zeros(rows + 1, np.int32)
+    np.add.at(rowptr, rws + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    t = lambda a: torch.as_tensor(a, device=dev)
+    return rowptr, cols.astype(np.int32), vals, t(rowptr), t(cols.astype(np.int32)), t(vals)
+
+
+@pytest.mark.parametrize("rows,in_rows,ncols,k", [(1024, TILE_J * 20 + 77, 64, 8),
+                                                  (1000, 50000, 70, 8), (300, 40001, 16, 3),
+                                                  (17, 9000, 5, 2), (1, 5000, 3, 1),
+                                                  (1024, 100000, 64, 1), (64, 1000, 129, 4),
+                                                  (1024, TILE_J, 64, 8)])
+def test_sparse_tile_apply_within_rounding(rla, dev, rows, in_rows, ncols, k):
+    # the row-group tile copy against its definition (built on the host), and
+    # its apply against the FP64 product: ragged last block, rows not a
+    # multiple of a wavefront's, empty rows, 1-3 groups of 64 columns with a
+    # partial last one, one block, beta != 0, strided C
+    import ctypes
+    import scipy.sparse as sp
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    print(f"tile case rows={rows} in_rows={in_rows} ncols={ncols} k={k}", flush=True)
+    rp_h, ci_h, v_h, rowptr, colidx, val = _tile_operator(dev, rows, in_rows, k, rows + ncols)
+    words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
+    lib.call("rla_sparse_tile_shape", ctx, rows, in_rows, lib.ptr(rowptr), lib.ptr(colidx),
+             ctypes.byref(words), ctypes.byref(nrec))
+    tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
+    mag = ctypes.c_double(0.0)
+    lib.call("rla_sparse_tile_build_f32", ctx, rows, in_rows, lib.ptr(rowptr), lib.ptr(colidx),
+             lib.ptr(val), words.value, lib.ptr(tiles), ctypes.byref(mag))
+    assert mag.value == float(np.float32(1.0 / math.sqrt(k)))
+    want = _tile_layout(rp_h, ci_h, v_h, rows, in_rows, words.value)
+    assert want.size == tiles.numel()
+    assert np.array_equal(tiles.cpu().numpy().view(np.uint32), want)
+    X = colmajor(rnd((in_rows, ncols), 3, np.float32), dev)
+    C0 = colmajor(rnd((rows + 3, ncols), 4, np.float32), dev)[:rows]
+    C = C0.clone()
+    lib.call("rla_sparse_apply_tile_f32", ctx, rows, in_rows, words.value, lib.ptr(tiles),
+             mag.value, ncols, lib.ptr(X), 1, in_rows, lib.ptr(C), C.stride(0), C.stride(1),
+             1.5, -0.5)
+    S = sp.csr_matrix((v_h.astype(np.float64), ci_h, rp_h), shape=(rows, in_rows))
+    ref = 1.5 * (S @ X.cpu().numpy().astype(np.float64)) - 0.5 * C0.cpu().numpy()
+    d = np.abs(C.cpu().numpy() - ref).max()
+    assert d <= 1e-5 * (np.abs(ref).max() + 1.0), d
+
+
+def test_sparse_tile_rejects_what_it_cannot_take(rla, dev):
+    import ctypes
+    lib = rla._lib
+    ctx = lib.ctx(dev)
+    words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
+    # more than 1024 rows
+    _, _, _, rowptr, colidx, val = _tile_operator(dev, 1025, 2000, 2, 1)
+    rc = lib.call_rc("rla_sparse_tile_shape", ctx, 1025, 2000, lib.ptr(rowptr), lib.ptr(colidx),
+                     ctypes.byref(words), ctypes.byref(nrec))
+    assert rc == lib.RLA_ENOTSUP and words.value == 0
+    # columns repeated / descending / out of range
+    rp = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)
+    for cols in ([3, 1, 0], [1, 1, 0], [0, 900, 2]):
+        ci = torch.tensor(cols, dtype=torch.int32, device=dev)
+        rc = lib.call_rc("rla_sparse_tile_shape", ctx, 2, 800, lib.ptr(rp), lib.ptr(ci),
+                         ctypes.byref(words), ctypes.byref(nrec))
+        assert rc == lib.RLA_EDOMAIN
+    # two magnitudes
+    _, _, v_h, rowptr, colidx, val = _tile_operator(dev, 64, 3000, 2, 2)
+    val[5] = 2 * val[5]
+    lib.call("rla_sparse_tile_shape", ctx, 64, 3000, lib.ptr(rowptr), lib.ptr(colidx),
+             ctypes.byref(words), ctypes.byref(nrec))
+    tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
+    mag = ctypes.c_double(-1.0)
+    rc = lib.call_rc("rla_sparse_tile_build_f32", ctx, 64, 3000, lib.ptr(rowptr),
+                     lib.ptr(colidx), lib.ptr(val), words.value, lib.ptr(tiles),
+                     ctypes.byref(mag))
+    assert rc == lib.RLA_EDOMAIN and mag.value == 0.0
+    # X not column-major, a record length that is not the copy's
+    X = torch.zeros(3000, 4, dtype=F32, device=dev)
+    C = torch.zeros(64, 4, dtype=F32, device=dev)
+    for x_rs, w in ((4, words.value), (1, words.value + 2)):
+        rc = lib.call_rc("rla_sparse_apply_tile_f32", ctx, 64, 3000, w, lib.ptr(tiles), 1.0, 4,
+                         lib.ptr(X), x_rs, 1, lib.ptr(C), 1, 64, 1.0, 0.0)
+        assert rc == lib.RLA_EARG
+
+
+@pytest.mark.parametrize("kind,s", [("sparse_sign", 1024), ("sparse_sign", 1000),
+                                    ("count_sketch", 512)])
+def test_tall_float32_left_apply_takes_row_group_tiles(rla, dev, kind, s):
+    # a Float32 operand of 2^19 + 1001 rows and 64 columns: the recipe's left
+    # apply takes the row-group tiles, within 1e-5 of the FP64 product
+    import scipy.sparse as sp
+    m, n = (1 << 19) + 1001, 64
+    print(f"tile recipe {kind} s={s}", flush=True)
+    rla.seed(11)
+    Ad = torch.randn(n, m, device=dev, dtype=F32).T
+    tech = (rla.SparseSign(compression_dim=s, nnz=8, type=F32) if kind == "sparse_sign"
+            else rla.CountSketch(compression_dim=s, type=F32))
+    S = rla.complete_compressor(tech, Ad)
+    C0 = torch.randn(n, s, device=dev, dtype=F32).T
+    C = C0.clone()
+    rla.mul(C, S, Ad, 0.75, -2.0)
+    assert S.op._ell[("tile", 0)] is not None
+    rp = S.op.rowptr.cpu().numpy()
+    M = sp.csr_matrix((S.op._vals_for(F32)[1].double().cpu().numpy(),
+                       S.op.colidx.cpu().numpy(), rp), shape=(s, m))
+    ref = 0.75 * (M @ Ad.cpu().numpy().astype(np.float64)) - 2.0 * C0.cpu().numpy()
+    assert np.abs(C.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max()

@@ -1542,7 +1542,15 @@ __global__ void tile_combine_kernel(int64_t rows, int64_t n_rhs, int G, int Q, i
   const int64_t stride = int64_t(rwg) * 64;
   const float* p = part + (cg * G + g) * int64_t(Q) * stride + int64_t(r) * 64 + lane;
   double s = 0.0;
-  for (int k = 0; k < Q; ++k) s += double(p[k * stride]);
+  int k = 0;
+  for (; k + 16 <= Q; k += 16) {  // 16 loads in flight, added in range order
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[(k + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += double(v[u]);
+  }
+  for (; k < Q; ++k) s += double(p[k * stride]);
   float* c = C + i * c_rs + col * c_cs;
   *c = static_cast<float>(beta == 0.0 ? scale * s : scale * s + beta * double(*c));
 }

@@ -344,6 +344,32 @@ def bf16_gemm_c3(rla, torch, reps=20):
     return out
 
 
+def _in_flight(rla, torch, dev, recs, A, n_solves):
+    """Seconds for n_solves solves of every recipe in recs, one host thread
+    and HIP stream per recipe, all warmed up before the clock starts."""
+    streams = [torch.cuda.Stream(device=dev) for _ in recs]
+    go = threading.Barrier(len(recs) + 1)
+
+    def work(r, s):
+        with torch.cuda.stream(s):
+            rla.rapproximate_(r, A)   # this thread's context and workspaces
+            s.synchronize()
+            go.wait()
+            for _ in range(n_solves):
+                rla.rapproximate_(r, A)
+            s.synchronize()
+
+    threads = [threading.Thread(target=work, args=(r, s)) for r, s in zip(recs, streams)]
+    for t in threads:
+        t.start()
+    go.wait()   # every thread warmed up; the timed solves start together
+    t0 = time.perf_counter()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def rangefinder_c3(rla, torch, reps=3, batch=4):
     """RangeFinder mixed precision vs the plain FP32 path (C3): A = U diag(s) V'
     32768 x 8192, s_i = exp(-i/64) (singular values below 1e-17 s_1 dropped),
@@ -394,30 +420,8 @@ def rangefinder_c3(rla, torch, reps=3, batch=4):
             rla.seed(7 + i)
             recs.append(rla.complete_approximator(rf, A32))
             recs[-1]._bf16 = rec._bf16
-        streams = [torch.cuda.Stream(device=dev) for _ in recs]
         n_batch = 3 * reps   # solves per stream in the timed batch
-        go = threading.Barrier(batch + 1)
-
-        def work(r, s, n_solves, wait):
-            with torch.cuda.stream(s):
-                rla.rapproximate_(r, A32)   # this thread's context and workspaces
-                s.synchronize()
-                if wait:
-                    go.wait()
-                for _ in range(n_solves):
-                    rla.rapproximate_(r, A32)
-                s.synchronize()
-
-        threads = [threading.Thread(target=work, args=(r, s, n_batch, True))
-                   for r, s in zip(recs, streams)]
-        for t in threads:
-            t.start()
-        go.wait()   # every thread warmed up; the timed solves start together
-        t0 = time.perf_counter()
-        for t in threads:
-            t.join()
-        torch.cuda.synchronize()
-        bdt = time.perf_counter() - t0
+        bdt = _in_flight(rla, torch, dev, recs, A32, n_batch)
         out[prec]["batch"] = {"in_flight": batch, "solves": batch * n_batch,
                               "solves_per_s": batch * n_batch / bdt}
         del recs, rec
@@ -439,7 +443,21 @@ def rangefinder_c3(rla, torch, reps=3, batch=4):
     sv_err = float((srec.S[:top] / full[:top] - 1.0).abs().max())
     out["randsvd_bf16"] = {"seconds_per_solve": sdt, "solves_per_s": 1.0 / sdt,
                            f"max_rel_error_top{top}_singular_values": sv_err}
-    del srec
+    # RandSVD solves in flight as for the RangeFinder: the Jacobi SVD of one
+    # solve's small factor holds few CUs, the other solves' products fill the rest
+    srecs = [srec]
+    for i in range(1, batch):
+        rla.seed(7 + i)
+        srecs.append(rla.complete_approximator(sv, A32))
+        srecs[-1]._bf16 = srec._bf16
+    n_batch = 3 * reps
+    bdt = _in_flight(rla, torch, dev, srecs, A32, n_batch)
+    out["randsvd_bf16"]["batch"] = {"in_flight": batch, "solves": batch * n_batch,
+                                    "solves_per_s": batch * n_batch / bdt}
+    out["randsvd_over_rangefinder_seconds_per_solve"] = sdt / out["bf16"]["seconds_per_solve"]
+    out["randsvd_over_rangefinder_in_flight"] = (out["bf16"]["batch"]["solves_per_s"] /
+                                                 out["randsvd_bf16"]["batch"]["solves_per_s"])
+    del srecs, srec
     out["speedup_solves_per_s"] = out["plain"]["seconds_per_solve"] / out["bf16"]["seconds_per_solve"]
     out["batched_speedup_solves_per_s"] = (out["bf16"]["batch"]["solves_per_s"] /
                                            out["plain"]["batch"]["solves_per_s"])

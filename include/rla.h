@@ -278,33 +278,36 @@ int rla_sparse_apply_bell_split_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows,
 
 /* Row-group tiles: the same left product for tall Float32 X when M (rows x
  * in_rows, CSR ptr/idx with strictly ascending columns per row) has at most
- * 1024 rows and nonzeros of one magnitude (SparseSign, CountSketch).  The
- * lanes of a wavefront are 64 columns of X; a wavefront keeps the FP32 sums
- * of its output rows in registers while blocks of 384 rows of X pass through
- * LDS, each entry of M one 32-bit word read once for all 64 columns.  Each
- * workgroup sums a range of blocks; the ranges' sums are added in ascending
- * order in FP64, so the result is within rounding of rla_sparse_apply (not
- * identical).  Replaces the same mul! as rla_sparse_apply_bell
- * (sparse_sign.jl:339-348, count_sketch.jl:166-213).
+ * 4096 rows.  The lanes of a wavefront are 64 columns of X; a wavefront
+ * keeps the FP32 sums of its output rows in registers (16 wavefronts, at
+ * most 1024 rows, per workgroup; more rows take one workgroup per group of
+ * 1024) while blocks of 384 rows of X pass through LDS.  Each entry of M is
+ * read once for all 64 columns: with_values = 0 (nonzeros of one magnitude:
+ * SparseSign, CountSketch) one 32-bit word, its LDS offset and its sign;
+ * with_values = 1 (any values: FJLT's K) two, the offset and the value.
+ * Each workgroup sums a range of blocks; the ranges' sums are added in
+ * ascending order in FP64, so the result is within rounding of
+ * rla_sparse_apply (not identical).  Replaces the same mul! as
+ * rla_sparse_apply_bell (sparse_sign.jl:339-348, count_sketch.jl:166-213).
  *   rla_sparse_tile_shape: words per record and record count of the copy
- *     (RLA_ENOTSUP above 1024 rows or when a record would not fit the LDS,
+ *     (RLA_ENOTSUP above 4096 rows or when a record would not fit the LDS,
  *     RLA_EDOMAIN for columns out of range or not strictly ascending);
  *   rla_sparse_tile_build_f32: fills the copy (16-byte aligned, nrec *
- *     rec_words words) and returns the magnitude (RLA_EDOMAIN if the
- *     nonzeros do not share one);
+ *     rec_words words) and returns the magnitude (1 with values; RLA_EDOMAIN
+ *     without values if the nonzeros do not share one magnitude);
  *   rla_sparse_apply_tile_f32: C = alpha*M*X + beta*C for X column-major
  *     (x_rs == 1); the context workspace holds the ranges' sums, at most
  *     rla_sparse_apply_tile_bytes bytes. */
 int rla_sparse_tile_shape(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                          const int32_t* idx, int64_t* rec_words, int64_t* nrec);
+                          const int32_t* idx, int with_values, int64_t* rec_words, int64_t* nrec);
 int rla_sparse_tile_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                              const int32_t* idx, const float* val, int64_t rec_words,
-                              uint32_t* tiles, double* magnitude);
+                              const int32_t* idx, const float* val, int with_values,
+                              int64_t rec_words, uint32_t* tiles, double* magnitude);
 int rla_sparse_apply_tile_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs, size_t* bytes);
-int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
-                              const uint32_t* tiles, double magnitude, int64_t n_rhs,
-                              const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
-                              int64_t c_cs, double alpha, double beta);
+int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int with_values,
+                              int64_t rec_words, const uint32_t* tiles, double magnitude,
+                              int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                              int64_t c_rs, int64_t c_cs, double alpha, double beta);
 
 /* Agmon maximal-residual selection, the top-k step of sample_distribution!
  * for the Agmon distribution (src/Compressors/Distributions/agmon.jl:354-416):
@@ -440,6 +443,17 @@ int rla_fjlt_apply_bell_f64(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, i
                             const double* bval, const uint64_t* sign_chunks, double scale,
                             const double* X, int64_t x_rs, int64_t x_cs, double* Y, int64_t y_rs,
                             int64_t y_cs, double alpha, double beta);
+
+/* The same forward FJLT product for Float32 with K given by its row-group
+ * tile copy with values (rla_sparse_tile_* with with_values = 1, rows = s,
+ * in_rows = n_pad; s <= 4096): the FWHT of each chunk of columns, then the
+ * tile gather (its ranges' sums beside the transform in the context
+ * workspace).  Within rounding of rla_fjlt_apply (fjlt.jl:257-319). */
+int rla_fjlt_apply_tile_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rec_words, const uint32_t* tiles,
+                            const uint64_t* sign_chunks, double scale, const float* X,
+                            int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs,
+                            double alpha, double beta);
 
 /* ---- dense products and conversions ------------------------------------
  * Column-major GEMM C = alpha*op(A)*op(B) + beta*C (vendor library: the plain

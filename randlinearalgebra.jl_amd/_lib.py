@@ -95,11 +95,13 @@ _SIGS = {
     "rla_sparse_apply_bell_split_f64": [vp, i64, i64, i64, vp, vp, vp, i64, vp, i64, i64, vp, i64,
                                         i64, f64, f64],
     "rla_sparse_apply_bell_split_bytes": [i64, i64, i64, i64, vp],
-    "rla_sparse_tile_shape": [vp, i64, i64, vp, vp, vp, vp],
-    "rla_sparse_tile_build_f32": [vp, i64, i64, vp, vp, vp, i64, vp, vp],
+    "rla_sparse_tile_shape": [vp, i64, i64, vp, vp, i32, vp, vp],
+    "rla_sparse_tile_build_f32": [vp, i64, i64, vp, vp, vp, i32, i64, vp, vp],
     "rla_sparse_apply_tile_bytes": [i64, i64, i64, vp],
-    "rla_sparse_apply_tile_f32": [vp, i64, i64, i64, vp, f64, i64, vp, i64, i64, vp, i64, i64, f64,
-                                  f64],
+    "rla_sparse_apply_tile_f32": [vp, i64, i64, i32, i64, vp, f64, i64, vp, i64, i64, vp, i64, i64,
+                                  f64, f64],
+    "rla_fjlt_apply_tile_f32": [vp, i64, i64, i64, i64, i64, vp, vp, f64, vp, i64, i64, vp, i64,
+                                i64, f64, f64],
     "rla_fjlt_apply_bell_f32": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,
                                 i64, i64, f64, f64],
     "rla_fjlt_apply_bell_f64": [vp, i64, i64, i64, i64, i64, vp, vp, vp, vp, f64, vp, i64, i64, vp,

@@ -346,6 +346,9 @@ _BELL_CARRY_ROWS = 1024
 _TILE_MIN_ROWS = 1 << 19
 _TILE_MIN_COLS = 16
 _TILE_MAX_ROWS = 1024
+# FJLT's K (values carried) takes the tile gather up to this many rows (a
+# workgroup per 1024 of them)
+_FJLT_TILE_MAX_ROWS = 4096
 
 
 class _SparseOp:
@@ -461,31 +464,33 @@ class _SparseOp:
         self._ell[key] = out
         return out
 
-    def _tile_for(self, trans_m):
+    def _tile_for(self, trans_m, values=False):
         """Row-group tile copy of op(M) for Float32 X (built once per operator
         and cached): (words per record, records, magnitude), or None when M has
-        more than 1024 rows, nonzeros of several magnitudes, or a record would
-        not fit the LDS."""
-        key = ("tile", int(trans_m))
+        too many rows, nonzeros of several magnitudes (without values), or a
+        record would not fit the LDS."""
+        key = ("tile", int(trans_m), bool(values))
         if key in self._ell:
             return self._ell[key]
         rows, in_rows = (self.ncols, self.nrows) if trans_m else (self.nrows, self.ncols)
         out = None
-        if self._uniform and 0 < rows <= _TILE_MAX_ROWS:
+        limit = _FJLT_TILE_MAX_ROWS if values else _TILE_MAX_ROWS
+        if (values or self._uniform) and 0 < rows <= limit:
             dev = self.rowidx.device
             ctx = _lib.ctx(dev)
             ptr, idx = (self.colptr, self.rowidx) if trans_m else (self.rowptr, self.colidx)
             words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
             rc = _lib.call_rc("rla_sparse_tile_shape", ctx, rows, in_rows, _lib.ptr(ptr),
-                              _lib.ptr(idx), ctypes.byref(words), ctypes.byref(nrec))
+                              _lib.ptr(idx), int(values), ctypes.byref(words),
+                              ctypes.byref(nrec))
             if rc == 0:
                 val, val_t = self._vals_for(torch.float32)
                 v = val if trans_m else val_t
                 tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
                 mag = ctypes.c_double(0.0)
                 rc = _lib.call_rc("rla_sparse_tile_build_f32", ctx, rows, in_rows, _lib.ptr(ptr),
-                                  _lib.ptr(idx), _lib.ptr(v), words.value, _lib.ptr(tiles),
-                                  ctypes.byref(mag))
+                                  _lib.ptr(idx), _lib.ptr(v), int(values), words.value,
+                                  _lib.ptr(tiles), ctypes.byref(mag))
                 if rc == 0:
                     out = (words.value, tiles, mag.value)
         self._ell[key] = out
@@ -499,7 +504,7 @@ class _SparseOp:
             tile = self._tile_for(trans_m)
             if tile is not None:
                 words, tiles, mag = tile
-                _lib.call("rla_sparse_apply_tile_f32", _lib.ctx(X.t.device), C.rows, X.rows,
+                _lib.call("rla_sparse_apply_tile_f32", _lib.ctx(X.t.device), C.rows, X.rows, 0,
                           words, _lib.ptr(tiles), mag, X.cols, _lib.ptr(X.t), X.rs, X.cs,
                           _lib.ptr(C.t), C.rs, C.cs, alpha, beta)
                 return
@@ -943,6 +948,18 @@ class FJLTRecipe(CompressorRecipe):
     def _apply(self, adjoint, X, Y, alpha, beta):
         dt = X.dtype
         op = self.op
+        if (dt == torch.float32 and not adjoint and not op.exact and X.cols >= _TILE_MIN_COLS
+                and self.s <= _FJLT_TILE_MAX_ROWS):
+            tile = op._tile_for(0, values=True)
+            if tile is not None:
+                # forward: the gather reads K's entries, with their values, once
+                # for 64 columns of the transform
+                words, tiles, _ = tile
+                _lib.call("rla_fjlt_apply_tile_f32", _lib.ctx(X.t.device), self.dim,
+                          self.n_pad, self.s, X.cols, words, _lib.ptr(tiles),
+                          _lib.ptr(self.signs), self.scale, _lib.ptr(X.t), X.rs, X.cs,
+                          _lib.ptr(Y.t), Y.rs, Y.cs, alpha, beta)
+                return
         bell = op._bell_for(0, dt) if not adjoint else None
         if bell is not None:
             # forward: the gather reads K's rows from the block-interleaved copy

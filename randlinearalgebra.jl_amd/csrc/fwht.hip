@@ -508,11 +508,57 @@ static int fjlt_apply_bell(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, in
   return RLA_OK;
 }
 
+// Forward FJLT with K's row-group tile copy (values carried): each column
+// chunk's FWHT goes to the first part of the workspace, the ranges' sums of
+// the tile gather to the second; within rounding of the CSR gather.
+static int fjlt_apply_tile(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                           int64_t rec_words, const uint32_t* tiles, const uint64_t* signs,
+                           double scale, const float* X, int64_t x_rs, int64_t x_cs, float* Y,
+                           int64_t y_rs, int64_t y_cs, double alpha, double beta) {
+  RLA_CHECK_CTX(ctx);
+  if (!is_pow2(n_pad) || n_pad < 2) return fail(RLA_EDIM, "Size of vector must be power of 2.");
+  if (m < 1 || m > n_pad || s < 1 || ncols < 0)
+    return fail(RLA_EARG, "FJLT sizes must satisfy 1 <= m <= n_pad and s >= 1");
+  if (ncols == 0) return RLA_OK;
+  if (!tiles || !signs || !X || !Y) return fail(RLA_EARG, "FJLT apply: NULL pointer");
+  int rc = use_device(ctx);
+  if (rc) return rc;
+  const int L = ilog2(n_pad);
+  const int64_t cc = srht_chunk_cols<float>(n_pad, ncols);
+  const size_t p_bytes = (static_cast<size_t>(n_pad * cc) * sizeof(float) + 255) / 256 * 256;
+  // sums for a full chunk of columns or the narrower last one (more ranges)
+  size_t part_bytes = sparse_tile_part_bytes(s, n_pad, cc);
+  if (ncols % cc) part_bytes = std::max(part_bytes, sparse_tile_part_bytes(s, n_pad, ncols % cc));
+  void* ws = nullptr;
+  rc = workspace(ctx, p_bytes + part_bytes, &ws);
+  if (rc) return rc;
+  float* P = static_cast<float*>(ws);
+  float* part = reinterpret_cast<float*>(static_cast<char*>(ws) + p_bytes);
+  for (int64_t k0 = 0; k0 < ncols; k0 += cc) {
+    const int64_t kc = std::min(cc, ncols - k0);
+    rc = fwht_passes<float>(ctx, X + k0 * x_cs, x_rs, x_cs, m, P, n_pad, kc, L, signs, scale);
+    if (rc) return rc;
+    rc = sparse_apply_tile_part(ctx, s, n_pad, 1, rec_words, tiles, 1.0, kc, P, 1, n_pad,
+                                Y + k0 * y_cs, y_rs, y_cs, alpha, beta, part);
+    if (rc) return rc;
+  }
+  return RLA_OK;
+}
+
 }  // namespace rla
 
 using namespace rla;
 
 extern "C" {
+
+int rla_fjlt_apply_tile_f32(rla_ctx* ctx, int64_t m, int64_t n_pad, int64_t s, int64_t ncols,
+                            int64_t rec_words, const uint32_t* tiles,
+                            const uint64_t* sign_chunks, double scale, const float* X,
+                            int64_t x_rs, int64_t x_cs, float* Y, int64_t y_rs, int64_t y_cs,
+                            double alpha, double beta) {
+  return fjlt_apply_tile(ctx, m, n_pad, s, ncols, rec_words, tiles, sign_chunks, scale, X, x_rs,
+                         x_cs, Y, y_rs, y_cs, alpha, beta);
+}
 
 int rla_fjlt_apply_f32(rla_ctx* ctx, int adjoint, int64_t m, int64_t n_pad, int64_t s,
                        int64_t ncols, const int32_t* colptr, const int32_t* rowidx,

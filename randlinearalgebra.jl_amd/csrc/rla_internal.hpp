@@ -102,6 +102,15 @@ int workspace(rla_ctx* ctx, size_t bytes, void** out);
 // In-place inclusive prefix sum of a[0..n) on the context's stream.
 int scan_inclusive_i32(rla_ctx* ctx, int32_t* a, int64_t n);
 
+// Row-group tile apply of a Float32 X (sparse.hip, rla_sparse_apply_tile_f32)
+// with the ranges' sums in `part` (sparse_tile_part_bytes bytes), or in the
+// context workspace when part is NULL.
+int sparse_apply_tile_part(rla_ctx* ctx, int64_t rows, int64_t in_rows, int vals,
+                           int64_t rec_words, const uint32_t* tiles, double magnitude,
+                           int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                           int64_t c_rs, int64_t c_cs, double alpha, double beta, float* part);
+size_t sparse_tile_part_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs);
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }

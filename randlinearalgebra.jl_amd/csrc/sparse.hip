@@ -1278,7 +1278,7 @@ constexpr uint32_t kTgDummy = uint32_t(kTgJ) * 4u | kTgPlus;
 constexpr size_t kTgLdsX = size_t(64) * kTgStride * sizeof(float);
 constexpr size_t kTgLdsMax = 160 * 1024;
 constexpr size_t kTgSlack = 16;  // LDS past a record: the last step reads two words ahead
-constexpr int64_t kTgMaxRows = int64_t(kTgWaves) * 64;
+constexpr int64_t kTgMaxRows = 4096;  // four row groups of 1024
 constexpr int kTgWorkgroups = 256;  // one per CU
 
 struct TileShape {
@@ -1304,11 +1304,12 @@ static size_t tile_part_bytes(const TileShape& s) {
   return sizeof(float) * size_t(s.ncg) * size_t(s.g) * size_t(s.q) * size_t(s.rwg) * 64;
 }
 
-// pc[b * rows + i] = pairs of row i in block b (pc zeroed before); *bad set
+// pc[b * rows + i] = steps of row i in block b (pc zeroed before): entries,
+// or entry pairs without values; *bad set
 // when a row's columns are out of range or not strictly ascending.
 __global__ void tile_count_kernel(int64_t rows, int64_t in_rows, const int32_t* __restrict__ ptr,
-                                  const int32_t* __restrict__ idx, int32_t* __restrict__ pc,
-                                  int32_t* __restrict__ bad) {
+                                  const int32_t* __restrict__ idx, int vals,
+                                  int32_t* __restrict__ pc, int32_t* __restrict__ bad) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= rows) return;
   int64_t prev = -1, cur = -1;
@@ -1322,13 +1323,13 @@ __global__ void tile_count_kernel(int64_t rows, int64_t in_rows, const int32_t* 
     prev = c;
     const int64_t b = c / kTgJ;
     if (b != cur) {
-      if (cur >= 0) pc[cur * rows + i] = (cnt + 1) / 2;
+      if (cur >= 0) pc[cur * rows + i] = vals ? cnt : (cnt + 1) / 2;
       cur = b;
       cnt = 0;
     }
     ++cnt;
   }
-  if (cur >= 0) pc[cur * rows + i] = (cnt + 1) / 2;
+  if (cur >= 0) pc[cur * rows + i] = vals ? cnt : (cnt + 1) / 2;
 }
 
 // One workgroup per record: the maximum of the records' pair counts in *mx.
@@ -1373,8 +1374,9 @@ __global__ __launch_bounds__(kTgThreads) void tile_header_kernel(int64_t rows, i
   }
 }
 
-// Entries: thread per output row, walking its CSR row once.
-template <class T>
+// Entries: thread per output row, walking its CSR row once.  VAL: each step
+// is one entry, its offset word then its value; otherwise two sign words.
+template <class T, bool VAL>
 __global__ void tile_fill_kernel(int64_t rows, int rwg, int g_count, int h32, int64_t l32,
                                  const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                                  const T* __restrict__ val, uint32_t* __restrict__ tiles,
@@ -1393,7 +1395,7 @@ __global__ void tile_fill_kernel(int64_t rows, int rwg, int g_count, int h32, in
     const int64_t c = idx[p];
     const int64_t b = c / kTgJ;
     if (b != cur) {
-      if (cur >= 0 && (k & 1)) out[k] = kTgDummy;
+      if (!VAL && cur >= 0 && (k & 1)) out[k] = kTgDummy;
       cur = b;
       uint32_t* rec = tiles + (b * g_count + g) * l32;
       const uint16_t* h = reinterpret_cast<const uint16_t*>(rec);
@@ -1401,10 +1403,16 @@ __global__ void tile_fill_kernel(int64_t rows, int rwg, int g_count, int h32, in
       k = 0;
     }
     const T v = val[p];
-    bad |= !(v == mag || v == -mag);
-    out[k++] = uint32_t(c - b * kTgJ) * 4u | (v < T(0) ? kTgMinus : kTgPlus);
+    const uint32_t off = uint32_t(c - b * kTgJ) * 4u;
+    if (VAL) {
+      out[k++] = off;
+      out[k++] = __float_as_uint(static_cast<float>(v));
+    } else {
+      bad |= !(v == mag || v == -mag);
+      out[k++] = off | (v < T(0) ? kTgMinus : kTgPlus);
+    }
   }
-  if (k & 1) out[k] = kTgDummy;
+  if (!VAL && (k & 1)) out[k] = kTgDummy;
   if (bad) atomicOr(mixed, 1);
 }
 
@@ -1414,7 +1422,7 @@ __global__ void tile_zero_kernel(uint32_t* p, int64_t n) {
     p[e] = 0u;
 }
 
-template <int RW>
+template <int RW, bool VAL>
 __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
     int64_t in_rows, int64_t nblocks, int G, int Q, int NCG, int64_t n_rhs, int l32, int h32,
     const uint32_t* __restrict__ tiles, const float* __restrict__ X, int64_t x_cs,
@@ -1460,7 +1468,7 @@ __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
       sr[u] = k < n4 ? src[k] : make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  auto store = [&]() {
+  auto store = [&](int64_t b) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -1469,6 +1477,13 @@ __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
     for (int u = 0; u < 2; ++u) {
       const int k = threadIdx.x + u * kTgThreads;
       if (k < n4) reinterpret_cast<uint4*>(rec)[k] = sr[u];
+    }
+    // a record longer than the staging registers hold: the rest goes
+    // straight from global memory to LDS
+    if (n4 > 2 * kTgThreads) {
+      const uint4* src = reinterpret_cast<const uint4*>(tiles + (b * G + g) * int64_t(l32));
+      for (int k = 2 * kTgThreads + threadIdx.x; k < n4; k += kTgThreads)
+        reinterpret_cast<uint4*>(rec)[k] = src[k];
     }
   };
 
@@ -1479,7 +1494,7 @@ __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
   if (b0 < b1) load(b0);
   for (int64_t b = b0; b < b1; ++b) {
     __syncthreads();  // the previous block's gathers are done
-    store();
+    store(b);
     __syncthreads();
     if (b + 1 < b1) load(b + 1);
     const uint16_t* rp = reinterpret_cast<const uint16_t*>(rec) + w * RW;
@@ -1500,23 +1515,40 @@ __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
     for (int r = 0; r < RW; ++r) {
       const int p1 = r + 1 < RW ? __builtin_amdgcn_readlane(vp, r + 1) : nsteps;
       float a = acc[r];
-      for (; p + 2 <= p1; p += 2) {  // two steps, the reads alternating registers
-        const uint2 nb = ps[p + 1];
-        const float x0 = xat(nx.x), x1 = xat(nx.y);
-        a = __builtin_fmaf(sgn(nx.x), x0, a);
-        a = __builtin_fmaf(sgn(nx.y), x1, a);
-        nx = ps[p + 2];
-        const float x2 = xat(nb.x), x3 = xat(nb.y);
-        a = __builtin_fmaf(sgn(nb.x), x2, a);
-        a = __builtin_fmaf(sgn(nb.y), x3, a);
-      }
-      if (p < p1) {
-        const uint2 cu = nx;
-        nx = ps[p + 1];
-        const float x0 = xat(cu.x), x1 = xat(cu.y);
-        a = __builtin_fmaf(sgn(cu.x), x0, a);
-        a = __builtin_fmaf(sgn(cu.y), x1, a);
-        ++p;
+      if constexpr (VAL) {  // a step: one entry's offset and value
+        for (; p + 2 <= p1; p += 2) {
+          const uint2 nb = ps[p + 1];
+          const float x0 = xat(nx.x);
+          a = __builtin_fmaf(__uint_as_float(nx.y), x0, a);
+          nx = ps[p + 2];
+          const float x1 = xat(nb.x);
+          a = __builtin_fmaf(__uint_as_float(nb.y), x1, a);
+        }
+        if (p < p1) {
+          const uint2 cu = nx;
+          nx = ps[p + 1];
+          a = __builtin_fmaf(__uint_as_float(cu.y), xat(cu.x), a);
+          ++p;
+        }
+      } else {
+        for (; p + 2 <= p1; p += 2) {  // two steps, the reads alternating registers
+          const uint2 nb = ps[p + 1];
+          const float x0 = xat(nx.x), x1 = xat(nx.y);
+          a = __builtin_fmaf(sgn(nx.x), x0, a);
+          a = __builtin_fmaf(sgn(nx.y), x1, a);
+          nx = ps[p + 2];
+          const float x2 = xat(nb.x), x3 = xat(nb.y);
+          a = __builtin_fmaf(sgn(nb.x), x2, a);
+          a = __builtin_fmaf(sgn(nb.y), x3, a);
+        }
+        if (p < p1) {
+          const uint2 cu = nx;
+          nx = ps[p + 1];
+          const float x0 = xat(cu.x), x1 = xat(cu.y);
+          a = __builtin_fmaf(sgn(cu.x), x0, a);
+          a = __builtin_fmaf(sgn(cu.y), x1, a);
+          ++p;
+        }
       }
       acc[r] = a;
     }
@@ -1556,14 +1588,14 @@ __global__ void tile_combine_kernel(int64_t rows, int64_t n_rhs, int G, int Q, i
 }
 
 static int tile_shape_query(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                            const int32_t* idx, int64_t* rec_words, int64_t* nrec) {
+                            const int32_t* idx, int vals, int64_t* rec_words, int64_t* nrec) {
   RLA_CHECK_CTX(ctx);
   if (!rec_words || !nrec) return fail(RLA_EARG, "row-group tiles: NULL output");
   *rec_words = 0;
   *nrec = 0;
   if (rows < 1 || in_rows < 1)
     return fail(RLA_EARG, "row-group tiles: need rows >= 1 and in_rows >= 1");
-  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tiles: more than 1024 rows");
+  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tiles: more than 4096 rows");
   if (!ptr || !idx) return fail(RLA_EARG, "row-group tiles: NULL pointer");
   const TileShape sh = tile_shape(rows, in_rows, 1);
   if (sh.nblocks * int64_t(sh.g) > INT32_MAX)
@@ -1580,7 +1612,7 @@ static int tile_shape_query(rla_ctx* ctx, int64_t rows, int64_t in_rows, const i
   tile_zero_kernel<<<dim3(unsigned(std::min<int64_t>(4096, ceil_div(sh.nblocks * rows, 256)))),
                     256, 0, ctx->stream>>>(reinterpret_cast<uint32_t*>(pc), sh.nblocks * rows);
   tile_count_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
-      rows, in_rows, ptr, idx, pc, flags);
+      rows, in_rows, ptr, idx, vals ? 1 : 0, pc, flags);
   tile_stats_kernel<<<dim3(unsigned(sh.nblocks * sh.g)), kTgThreads, 0, ctx->stream>>>(
       rows, sh.rwg, sh.g, pc, flags + 1);
   RLA_LAUNCH_CHECK("tile_stats");
@@ -1600,13 +1632,13 @@ static int tile_shape_query(rla_ctx* ctx, int64_t rows, int64_t in_rows, const i
 
 template <class T>
 static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                      const int32_t* idx, const T* val, int64_t rec_words, uint32_t* tiles,
-                      double* magnitude) {
+                      const int32_t* idx, const T* val, int vals, int64_t rec_words,
+                      uint32_t* tiles, double* magnitude) {
   RLA_CHECK_CTX(ctx);
   if (!magnitude) return fail(RLA_EARG, "row-group tiles: magnitude is NULL");
   *magnitude = 0.0;
   if (rows < 1 || in_rows < 1 || rows > kTgMaxRows)
-    return fail(RLA_EARG, "row-group tiles: need 1 <= rows <= 1024 and in_rows >= 1");
+    return fail(RLA_EARG, "row-group tiles: need 1 <= rows <= 4096 and in_rows >= 1");
   if (!ptr || !idx || !val || !tiles) return fail(RLA_EARG, "row-group tiles: NULL pointer");
   const TileShape sh = tile_shape(rows, in_rows, 1);
   if (rec_words < sh.h32 + 4 || rec_words % 4 ||
@@ -1632,7 +1664,7 @@ static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t
   tile_zero_kernel<<<dim3(unsigned(std::min<int64_t>(4096, ceil_div(nrec * rec_words, 256)))),
                     256, 0, ctx->stream>>>(tiles, nrec * rec_words);
   tile_count_kernel<<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
-      rows, in_rows, ptr, idx, pc, flags);
+      rows, in_rows, ptr, idx, vals ? 1 : 0, pc, flags);
   tile_header_kernel<<<dim3(unsigned(nrec)), kTgThreads, 0, ctx->stream>>>(rows, sh.rwg, sh.g,
                                                                            rec_words, pc, tiles);
   RLA_LAUNCH_CHECK("tile_header");
@@ -1642,10 +1674,16 @@ static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t
   if (h[0])
     return fail(RLA_EDOMAIN,
                 "row-group tiles: columns out of range or not strictly ascending in a row");
+  if (vals) *magnitude = 1.0;
   if (nnz == 0) return RLA_OK;
-  tile_fill_kernel<T><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
-      rows, sh.rwg, sh.g, sh.h32, rec_words, ptr, idx, val, tiles, flags + 1);
+  if (vals)
+    tile_fill_kernel<T, true><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+        rows, sh.rwg, sh.g, sh.h32, rec_words, ptr, idx, val, tiles, flags + 1);
+  else
+    tile_fill_kernel<T, false><<<dim3(unsigned(ceil_div(rows, 256))), 256, 0, ctx->stream>>>(
+        rows, sh.rwg, sh.g, sh.h32, rec_words, ptr, idx, val, tiles, flags + 1);
   RLA_LAUNCH_CHECK("tile_fill");
+  if (vals) return RLA_OK;
   T v0 = T(0);
   RLA_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   RLA_HIP(hipMemcpyAsync(&v0, val, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
@@ -1655,32 +1693,48 @@ static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t
   return RLA_OK;
 }
 
-template <int RW>
+template <int RW, bool VAL>
 static int launch_tile(rla_ctx* ctx, const TileShape& sh, unsigned ids, int64_t in_rows,
                        int64_t n_rhs, int l32, const uint32_t* tiles, const float* X,
                        int64_t x_cs, float* part) {
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
-    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&tilegather_kernel<RW>),
+    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&tilegather_kernel<RW, VAL>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(kTgLdsMax));
   });
   RLA_HIP(attr_err);
   const size_t lds = kTgLdsX + size_t(l32) * 4 + kTgSlack;
-  tilegather_kernel<RW><<<dim3(ids), kTgThreads, lds, ctx->stream>>>(
+  tilegather_kernel<RW, VAL><<<dim3(ids), kTgThreads, lds, ctx->stream>>>(
       in_rows, sh.nblocks, sh.g, sh.q, sh.ncg, n_rhs, l32, sh.h32, tiles, X, x_cs, part);
   return RLA_OK;
 }
 
-static int sparse_apply_tile(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
-                             const uint32_t* tiles, double magnitude, int64_t n_rhs,
-                             const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
-                             int64_t c_cs, double alpha, double beta) {
+template <bool VAL>
+static int launch_tile_rw(rla_ctx* ctx, const TileShape& sh, unsigned g, int64_t in_rows,
+                          int64_t n_rhs, int l32, const uint32_t* tiles, const float* X,
+                          int64_t x_cs, float* part) {
+  switch (sh.rw) {
+    case 64: return launch_tile<64, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 32: return launch_tile<32, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 16: return launch_tile<16, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 8: return launch_tile<8, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    default: return launch_tile<4, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+  }
+}
+
+// part: the ranges' sums (sparse_tile_part_bytes bytes), or NULL for the
+// context workspace.
+int sparse_apply_tile_part(rla_ctx* ctx, int64_t rows, int64_t in_rows, int vals,
+                                int64_t rec_words, const uint32_t* tiles, double magnitude,
+                                int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs,
+                                float* C, int64_t c_rs, int64_t c_cs, double alpha, double beta,
+                                float* part) {
   RLA_CHECK_CTX(ctx);
   if (rows < 0 || n_rhs < 0) return fail(RLA_EARG, "negative dimensions");
   if (in_rows < 1) return fail(RLA_EARG, "row-group tile apply: need in_rows >= 1");
   if (rows == 0 || n_rhs == 0) return RLA_OK;
-  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tile apply: more than 1024 rows");
+  if (rows > kTgMaxRows) return fail(RLA_ENOTSUP, "row-group tile apply: more than 4096 rows");
   if (!tiles || !X || !C) return fail(RLA_EARG, "row-group tile apply: NULL pointer");
   if (x_rs != 1) return fail(RLA_EARG, "row-group tile apply needs X column-major (x_rs == 1)");
   const TileShape sh = tile_shape(rows, in_rows, n_rhs);
@@ -1694,19 +1748,16 @@ static int sparse_apply_tile(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_
     return fail(RLA_ENOTSUP, "row-group tile apply: grid too large");
   int rc = use_device(ctx);
   if (rc) return rc;
-  void* ws = nullptr;
-  rc = workspace(ctx, tile_part_bytes(sh), &ws);
-  if (rc) return rc;
-  float* part = static_cast<float*>(ws);
+  if (!part) {
+    void* ws = nullptr;
+    rc = workspace(ctx, tile_part_bytes(sh), &ws);
+    if (rc) return rc;
+    part = static_cast<float*>(ws);
+  }
   const unsigned g = static_cast<unsigned>(ids);
   const int l32 = static_cast<int>(rec_words);
-  switch (sh.rw) {
-    case 64: rc = launch_tile<64>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
-    case 32: rc = launch_tile<32>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
-    case 16: rc = launch_tile<16>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
-    case 8: rc = launch_tile<8>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
-    default: rc = launch_tile<4>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part); break;
-  }
+  rc = vals ? launch_tile_rw<true>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part)
+            : launch_tile_rw<false>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
   if (rc) return rc;
   RLA_LAUNCH_CHECK("tilegather");
   tile_combine_kernel<<<dim3(unsigned(ceil_div(rows * sh.ncg * 64, 256))), 256, 0,
@@ -1716,6 +1767,10 @@ static int sparse_apply_tile(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_
   return RLA_OK;
 }
 
+size_t sparse_tile_part_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs) {
+  if (rows <= 0 || n_rhs <= 0 || in_rows < 1) return 0;
+  return tile_part_bytes(tile_shape(rows, in_rows, n_rhs));
+}
 }  // namespace rla
 
 using namespace rla;
@@ -1876,29 +1931,30 @@ int rla_sparse_apply_bell_f64(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64
 }
 
 int rla_sparse_tile_shape(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                          const int32_t* idx, int64_t* rec_words, int64_t* nrec) {
-  return tile_shape_query(ctx, rows, in_rows, ptr, idx, rec_words, nrec);
+                          const int32_t* idx, int with_values, int64_t* rec_words, int64_t* nrec) {
+  return tile_shape_query(ctx, rows, in_rows, ptr, idx, with_values, rec_words, nrec);
 }
 
 int rla_sparse_tile_build_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t* ptr,
-                              const int32_t* idx, const float* val, int64_t rec_words,
-                              uint32_t* tiles, double* magnitude) {
-  return tile_build<float>(ctx, rows, in_rows, ptr, idx, val, rec_words, tiles, magnitude);
+                              const int32_t* idx, const float* val, int with_values,
+                              int64_t rec_words, uint32_t* tiles, double* magnitude) {
+  return tile_build<float>(ctx, rows, in_rows, ptr, idx, val, with_values, rec_words, tiles,
+                           magnitude);
 }
 
 int rla_sparse_apply_tile_bytes(int64_t rows, int64_t in_rows, int64_t n_rhs, size_t* bytes) {
   if (!bytes || rows < 0 || in_rows < 1 || n_rhs < 0)
     return fail(RLA_EARG, "rla_sparse_apply_tile_bytes: bad arguments");
-  *bytes = (rows == 0 || n_rhs == 0) ? 0 : tile_part_bytes(tile_shape(rows, in_rows, n_rhs));
+  *bytes = sparse_tile_part_bytes(rows, in_rows, n_rhs);
   return RLA_OK;
 }
 
-int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int64_t rec_words,
-                              const uint32_t* tiles, double magnitude, int64_t n_rhs,
-                              const float* X, int64_t x_rs, int64_t x_cs, float* C, int64_t c_rs,
-                              int64_t c_cs, double alpha, double beta) {
-  return sparse_apply_tile(ctx, rows, in_rows, rec_words, tiles, magnitude, n_rhs, X, x_rs, x_cs,
-                           C, c_rs, c_cs, alpha, beta);
+int rla_sparse_apply_tile_f32(rla_ctx* ctx, int64_t rows, int64_t in_rows, int with_values,
+                              int64_t rec_words, const uint32_t* tiles, double magnitude,
+                              int64_t n_rhs, const float* X, int64_t x_rs, int64_t x_cs, float* C,
+                              int64_t c_rs, int64_t c_cs, double alpha, double beta) {
+  return sparse_apply_tile_part(ctx, rows, in_rows, with_values, rec_words, tiles, magnitude,
+                                n_rhs, X, x_rs, x_cs, C, c_rs, c_cs, alpha, beta, nullptr);
 }
 
 }  // extern "C"

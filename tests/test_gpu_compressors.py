@@ -683,6 +683,33 @@ def test_fjlt_interleaved_forward_equals_csr_forward(rla, dev, dtype):
     assert torch.equal(C1, C3)
 
 
+@pytest.mark.parametrize("m,n,s", [(40000, 64, 512), (65536, 130, 2048), (3000, 16, 3000)])
+def test_fjlt_forward_takes_value_tiles(rla, dev, m, n, s):
+    # Float32 with at least 16 columns: the forward FJLT gathers K's entries,
+    # values carried, from its row-group tile copy (1-3 row groups), within
+    # 1e-5 of the CSR forward; the transpose keeps the CSR path
+    lib = rla._lib
+    print(f"fjlt tile m={m} n={n} s={s}", flush=True)
+    rla.seed(13)
+    A = rnd((m + 1, n), 6, np.float32)
+    Ad = colmajor(A, dev)[:m]
+    S = rla.complete_compressor(rla.FJLT(compression_dim=s, sparsity=0.01, type=F32), Ad)
+    C0 = rnd((s, n), 7, np.float32)
+    C1 = colmajor(C0, dev)
+    rla.mul(C1, S, Ad, 1.25, -0.5)
+    op = S.op
+    assert op._ell[("tile", 0, True)] is not None
+    val, val_t = op._vals_for(F32)
+    C2 = colmajor(C0, dev)
+    Av = rla.as_view(Ad)
+    lib.call("rla_fjlt_apply_f32", lib.ctx(dev), 0, S.dim, S.n_pad, S.s, n,
+             lib.ptr(op.colptr), lib.ptr(op.rowidx), lib.ptr(val), lib.ptr(op.rowptr),
+             lib.ptr(op.colidx), lib.ptr(val_t), lib.ptr(S.signs), S.scale, lib.ptr(Av.t),
+             Av.rs, Av.cs, lib.ptr(C2), 1, s, 1.25, -0.5)
+    d = (C1.double() - C2.double()).abs().max().item()
+    assert d <= 1e-5 * (C2.double().abs().max().item() + 1.0), d
+
+
 def _bell_layout(rowptr, colidx, vals, rows, in_rows, rb):
     """The block-interleaved arrays built on the host from the format's
     definition: per block b, slot s of row i at rows * (w_0+...+w_{b-1}) + s *
@@ -797,11 +824,12 @@ def test_sparse_block_interleaved_rejects_unsorted_rows(rla, dev):
 TILE_J = 384   # rows of X per block
 
 
-def _tile_layout(rowptr, colidx, val, rows, in_rows, words):
+def _tile_layout(rowptr, colidx, val, rows, in_rows, words, values=False):
     """The row-group tile copy from its definition: per (block, row group) a
-    16-bit step header (exclusive sums of each row's entries rounded up to
-    even, halved), then the rows' entry words (block-local row * 4 | +-1 as
-    a half-precision number << 16), odd rows padded with the zero row's word."""
+    16-bit step header (exclusive sums of each row's steps), then the rows'
+    steps.  Without values a step is two entry words (block-local row * 4 |
+    +-1 as a half-precision number << 16), odd rows padded with the zero
+    row's word; with values one entry (block-local row * 4, the FP32 value)."""
     rw = 4
     while rw < 64 and 16 * rw < rows:
         rw *= 2
@@ -819,9 +847,13 @@ def _tile_layout(rowptr, colidx, val, rows, in_rows, words):
         blk = c // TILE_J
         for b in np.unique(blk):
             sel = blk == b
-            w = (c[sel] - b * TILE_J).astype(np.uint32) * 4 | np.where(v[sel] < 0, minus, plus)
-            if w.size % 2:
-                w = np.append(w, dummy)
+            off = (c[sel] - b * TILE_J).astype(np.uint32) * 4
+            if values:
+                w = np.stack([off, v[sel].astype(np.float32).view(np.uint32)], 1).reshape(-1)
+            else:
+                w = off | np.where(v[sel] < 0, minus, plus)
+                if w.size % 2:
+                    w = np.append(w, dummy)
             per[(int(b), i)] = w.astype(np.uint32)
     for b in range(nb):
         for g in range(ng):
@@ -838,14 +870,15 @@ def _tile_layout(rowptr, colidx, val, rows, in_rows, words):
     return out.reshape(-1)
 
 
-def _tile_operator(dev, rows, in_rows, k, seed, mag=None):
+def _tile_operator(dev, rows, in_rows, k, seed, mag=None, normal=False):
     """CSR of an operator with k distinct rows per column and values
-    +-mag (mag = 1/sqrt(k) by default)."""
+    +-mag (mag = 1/sqrt(k) by default), or N(0,1) values."""
     g = np.random.default_rng(seed)
     mag = 1.0 / math.sqrt(k) if mag is None else mag
     cols = np.repeat(np.arange(in_rows), k)
     rws = np.concatenate([g.choice(rows, k, replace=False) for _ in range(in_rows)])
-    vals = np.where(g.integers(0, 2, cols.size) == 1, mag, -mag).astype(np.float32)
+    vals = (g.standard_normal(cols.size) if normal else
+            np.where(g.integers(0, 2, cols.size) == 1, mag, -mag)).astype(np.float32)
     order = np.lexsort((cols, rws))
     rws, cols, vals = rws[order], cols[order], vals[order]
     rowptr = np.zeros(rows + 1, np.int32)
@@ -855,39 +888,44 @@ def _tile_operator(dev, rows, in_rows, k, seed, mag=None):
     return rowptr, cols.astype(np.int32), vals, t(rowptr), t(cols.astype(np.int32)), t(vals)
 
 
+@pytest.mark.parametrize("values", [False, True])
 @pytest.mark.parametrize("rows,in_rows,ncols,k", [(1024, TILE_J * 20 + 77, 64, 8),
                                                   (1000, 50000, 70, 8), (300, 40001, 16, 3),
                                                   (17, 9000, 5, 2), (1, 5000, 3, 1),
                                                   (1024, 100000, 64, 1), (64, 1000, 129, 4),
-                                                  (1024, TILE_J, 64, 8)])
-def test_sparse_tile_apply_within_rounding(rla, dev, rows, in_rows, ncols, k):
+                                                  (1024, TILE_J, 64, 8), (2500, 30000, 65, 4),
+                                                  (4096, 20000, 64, 2)])
+def test_sparse_tile_apply_within_rounding(rla, dev, rows, in_rows, ncols, k, values):
     # the row-group tile copy against its definition (built on the host), and
     # its apply against the FP64 product: ragged last block, rows not a
     # multiple of a wavefront's, empty rows, 1-3 groups of 64 columns with a
-    # partial last one, one block, beta != 0, strided C
+    # partial last one, one block, 1-4 row groups, beta != 0, strided C;
+    # signs only (one magnitude) or N(0,1) values carried
     import ctypes
     import scipy.sparse as sp
     lib = rla._lib
     ctx = lib.ctx(dev)
-    print(f"tile case rows={rows} in_rows={in_rows} ncols={ncols} k={k}", flush=True)
-    rp_h, ci_h, v_h, rowptr, colidx, val = _tile_operator(dev, rows, in_rows, k, rows + ncols)
+    print(f"tile case rows={rows} in_rows={in_rows} ncols={ncols} k={k} values={values}",
+          flush=True)
+    rp_h, ci_h, v_h, rowptr, colidx, val = _tile_operator(dev, rows, in_rows, k, rows + ncols,
+                                                          normal=values)
     words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
     lib.call("rla_sparse_tile_shape", ctx, rows, in_rows, lib.ptr(rowptr), lib.ptr(colidx),
-             ctypes.byref(words), ctypes.byref(nrec))
+             int(values), ctypes.byref(words), ctypes.byref(nrec))
     tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
     mag = ctypes.c_double(0.0)
     lib.call("rla_sparse_tile_build_f32", ctx, rows, in_rows, lib.ptr(rowptr), lib.ptr(colidx),
-             lib.ptr(val), words.value, lib.ptr(tiles), ctypes.byref(mag))
-    assert mag.value == float(np.float32(1.0 / math.sqrt(k)))
-    want = _tile_layout(rp_h, ci_h, v_h, rows, in_rows, words.value)
+             lib.ptr(val), int(values), words.value, lib.ptr(tiles), ctypes.byref(mag))
+    assert mag.value == (1.0 if values else float(np.float32(1.0 / math.sqrt(k))))
+    want = _tile_layout(rp_h, ci_h, v_h, rows, in_rows, words.value, values)
     assert want.size == tiles.numel()
     assert np.array_equal(tiles.cpu().numpy().view(np.uint32), want)
     X = colmajor(rnd((in_rows, ncols), 3, np.float32), dev)
     C0 = colmajor(rnd((rows + 3, ncols), 4, np.float32), dev)[:rows]
     C = C0.clone()
-    lib.call("rla_sparse_apply_tile_f32", ctx, rows, in_rows, words.value, lib.ptr(tiles),
-             mag.value, ncols, lib.ptr(X), 1, in_rows, lib.ptr(C), C.stride(0), C.stride(1),
-             1.5, -0.5)
+    lib.call("rla_sparse_apply_tile_f32", ctx, rows, in_rows, int(values), words.value,
+             lib.ptr(tiles), mag.value, ncols, lib.ptr(X), 1, in_rows, lib.ptr(C), C.stride(0),
+             C.stride(1), 1.5, -0.5)
     S = sp.csr_matrix((v_h.astype(np.float64), ci_h, rp_h), shape=(rows, in_rows))
     ref = 1.5 * (S @ X.cpu().numpy().astype(np.float64)) - 0.5 * C0.cpu().numpy()
     d = np.abs(C.cpu().numpy() - ref).max()
@@ -899,35 +937,35 @@ def test_sparse_tile_rejects_what_it_cannot_take(rla, dev):
     lib = rla._lib
     ctx = lib.ctx(dev)
     words, nrec = ctypes.c_int64(0), ctypes.c_int64(0)
-    # more than 1024 rows
-    _, _, _, rowptr, colidx, val = _tile_operator(dev, 1025, 2000, 2, 1)
-    rc = lib.call_rc("rla_sparse_tile_shape", ctx, 1025, 2000, lib.ptr(rowptr), lib.ptr(colidx),
-                     ctypes.byref(words), ctypes.byref(nrec))
+    # more than 4096 rows
+    _, _, _, rowptr, colidx, val = _tile_operator(dev, 4097, 2000, 2, 1)
+    rc = lib.call_rc("rla_sparse_tile_shape", ctx, 4097, 2000, lib.ptr(rowptr), lib.ptr(colidx),
+                     0, ctypes.byref(words), ctypes.byref(nrec))
     assert rc == lib.RLA_ENOTSUP and words.value == 0
     # columns repeated / descending / out of range
     rp = torch.tensor([0, 2, 3], dtype=torch.int32, device=dev)
     for cols in ([3, 1, 0], [1, 1, 0], [0, 900, 2]):
         ci = torch.tensor(cols, dtype=torch.int32, device=dev)
-        rc = lib.call_rc("rla_sparse_tile_shape", ctx, 2, 800, lib.ptr(rp), lib.ptr(ci),
+        rc = lib.call_rc("rla_sparse_tile_shape", ctx, 2, 800, lib.ptr(rp), lib.ptr(ci), 0,
                          ctypes.byref(words), ctypes.byref(nrec))
         assert rc == lib.RLA_EDOMAIN
     # two magnitudes
     _, _, v_h, rowptr, colidx, val = _tile_operator(dev, 64, 3000, 2, 2)
     val[5] = 2 * val[5]
-    lib.call("rla_sparse_tile_shape", ctx, 64, 3000, lib.ptr(rowptr), lib.ptr(colidx),
+    lib.call("rla_sparse_tile_shape", ctx, 64, 3000, lib.ptr(rowptr), lib.ptr(colidx), 0,
              ctypes.byref(words), ctypes.byref(nrec))
     tiles = torch.empty(nrec.value * words.value, dtype=torch.int32, device=dev)
     mag = ctypes.c_double(-1.0)
     rc = lib.call_rc("rla_sparse_tile_build_f32", ctx, 64, 3000, lib.ptr(rowptr),
-                     lib.ptr(colidx), lib.ptr(val), words.value, lib.ptr(tiles),
+                     lib.ptr(colidx), lib.ptr(val), 0, words.value, lib.ptr(tiles),
                      ctypes.byref(mag))
     assert rc == lib.RLA_EDOMAIN and mag.value == 0.0
     # X not column-major, a record length that is not the copy's
     X = torch.zeros(3000, 4, dtype=F32, device=dev)
     C = torch.zeros(64, 4, dtype=F32, device=dev)
     for x_rs, w in ((4, words.value), (1, words.value + 2)):
-        rc = lib.call_rc("rla_sparse_apply_tile_f32", ctx, 64, 3000, w, lib.ptr(tiles), 1.0, 4,
-                         lib.ptr(X), x_rs, 1, lib.ptr(C), 1, 64, 1.0, 0.0)
+        rc = lib.call_rc("rla_sparse_apply_tile_f32", ctx, 64, 3000, 0, w, lib.ptr(tiles), 1.0,
+                         4, lib.ptr(X), x_rs, 1, lib.ptr(C), 1, 64, 1.0, 0.0)
         assert rc == lib.RLA_EARG
 
 
@@ -947,7 +985,7 @@ def test_tall_float32_left_apply_takes_row_group_tiles(rla, dev, kind, s):
     C0 = torch.randn(n, s, device=dev, dtype=F32).T
     C = C0.clone()
     rla.mul(C, S, Ad, 0.75, -2.0)
-    assert S.op._ell[("tile", 0)] is not None
+    assert S.op._ell[("tile", 0, False)] is not None
     rp = S.op.rowptr.cpu().numpy()
     M = sp.csr_matrix((S.op._vals_for(F32)[1].double().cpu().numpy(),
                        S.op.colidx.cpu().numpy(), rp), shape=(s, m))

@@ -280,3 +280,24 @@ def test_hbm_admission_account(rla):
     assert lib.rla_hbm_set_budget(dev, 0) == 0
     assert lib.rla_hbm_admit(-1, 1) == rla._lib.RLA_EARG
     assert lib.rla_hbm_set_budget(64, 1) == rla._lib.RLA_EARG
+
+
+def test_sparse_tile_workspace_query(rla):
+    # the row-group tile apply's workspace: FP32 sums of 64 columns for every
+    # (column group, row group of 16 * RW rows, range of 384-row blocks), with
+    # RW the smallest of 4..64 covering the rows in 16 wavefronts and about
+    # 256 workgroups in all; host arithmetic, no device
+    lib = rla._lib.load()
+    out = ctypes.c_size_t(0)
+
+    def q(rows, in_rows, n_rhs):
+        assert lib.rla_sparse_apply_tile_bytes(rows, in_rows, n_rhs, ctypes.byref(out)) == 0
+        return out.value
+
+    assert q(1024, 1000000, 64) == 256 * 1024 * 64 * 4          # 256 ranges, one group
+    assert q(2048, 65536, 512) == 8 * 2 * 16 * 1024 * 64 * 4    # 8 x 2 groups, 16 ranges
+    assert q(1, 5000, 3) == 14 * 64 * 64 * 4                    # RW = 4; 14 blocks, 14 ranges
+    assert q(300, 40001, 16) == 105 * 512 * 64 * 4              # RW = 32; 105 blocks
+    assert q(0, 1000, 64) == 0 and q(1024, 1000, 0) == 0
+    assert lib.rla_sparse_apply_tile_bytes(-1, 10, 1, ctypes.byref(out)) == rla._lib.RLA_EARG
+    assert lib.rla_sparse_apply_tile_bytes(4, 0, 1, ctypes.byref(out)) == rla._lib.RLA_EARG

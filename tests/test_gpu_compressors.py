@@ -973,7 +973,8 @@ def test_sparse_tile_rejects_what_it_cannot_take(rla, dev):
                                     ("count_sketch", 512)])
 def test_tall_float32_left_apply_takes_row_group_tiles(rla, dev, kind, s):
     # a Float32 operand of 2^19 + 1001 rows and 64 columns: the recipe's left
-    # apply takes the row-group tiles, within 1e-5 of the FP64 product
+    # apply takes the row-group tiles, within 1e-5 of the FP64 product, also
+    # after the operator is drawn again
     import scipy.sparse as sp
     m, n = (1 << 19) + 1001, 64
     print(f"tile recipe {kind} s={s}", flush=True)
@@ -984,10 +985,14 @@ def test_tall_float32_left_apply_takes_row_group_tiles(rla, dev, kind, s):
     S = rla.complete_compressor(tech, Ad)
     C0 = torch.randn(n, s, device=dev, dtype=F32).T
     C = C0.clone()
-    rla.mul(C, S, Ad, 0.75, -2.0)
-    assert S.op._ell[("tile", 0, False)] is not None
-    rp = S.op.rowptr.cpu().numpy()
-    M = sp.csr_matrix((S.op._vals_for(F32)[1].double().cpu().numpy(),
-                       S.op.colidx.cpu().numpy(), rp), shape=(s, m))
-    ref = 0.75 * (M @ Ad.cpu().numpy().astype(np.float64)) - 2.0 * C0.cpu().numpy()
-    assert np.abs(C.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max()
+    for redraw in (False, True):
+        if redraw:   # a new operator: the tile copy is rebuilt, not reused
+            rla.update_compressor(S)
+            C = C0.clone()
+        rla.mul(C, S, Ad, 0.75, -2.0)
+        assert S.op._ell[("tile", 0, False)] is not None
+        rp = S.op.rowptr.cpu().numpy()
+        M = sp.csr_matrix((S.op._vals_for(F32)[1].double().cpu().numpy(),
+                           S.op.colidx.cpu().numpy(), rp), shape=(s, m))
+        ref = 0.75 * (M @ Ad.cpu().numpy().astype(np.float64)) - 2.0 * C0.cpu().numpy()
+        assert np.abs(C.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max()

@@ -1422,7 +1422,7 @@ __global__ void tile_zero_kernel(uint32_t* p, int64_t n) {
     p[e] = 0u;
 }
 
-template <int RW, bool VAL>
+template <int RW, bool VAL, bool LONG>
 __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
     int64_t in_rows, int64_t nblocks, int G, int Q, int NCG, int64_t n_rhs, int l32, int h32,
     const uint32_t* __restrict__ tiles, const float* __restrict__ X, int64_t x_cs,
@@ -1478,9 +1478,9 @@ __global__ __launch_bounds__(kTgThreads) void tilegather_kernel(
       const int k = threadIdx.x + u * kTgThreads;
       if (k < n4) reinterpret_cast<uint4*>(rec)[k] = sr[u];
     }
-    // a record longer than the staging registers hold: the rest goes
+    // LONG: a record longer than the staging registers hold; the rest goes
     // straight from global memory to LDS
-    if (n4 > 2 * kTgThreads) {
+    if (LONG) {
       const uint4* src = reinterpret_cast<const uint4*>(tiles + (b * G + g) * int64_t(l32));
       for (int k = 2 * kTgThreads + threadIdx.x; k < n4; k += kTgThreads)
         reinterpret_cast<uint4*>(rec)[k] = src[k];
@@ -1693,33 +1693,39 @@ static int tile_build(rla_ctx* ctx, int64_t rows, int64_t in_rows, const int32_t
   return RLA_OK;
 }
 
-template <int RW, bool VAL>
+template <int RW, bool VAL, bool LONG>
 static int launch_tile(rla_ctx* ctx, const TileShape& sh, unsigned ids, int64_t in_rows,
                        int64_t n_rhs, int l32, const uint32_t* tiles, const float* X,
                        int64_t x_cs, float* part) {
   static std::once_flag once;
   static hipError_t attr_err = hipSuccess;
   std::call_once(once, [&] {
-    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&tilegather_kernel<RW, VAL>),
+    attr_err = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&tilegather_kernel<RW, VAL, LONG>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(kTgLdsMax));
   });
   RLA_HIP(attr_err);
   const size_t lds = kTgLdsX + size_t(l32) * 4 + kTgSlack;
-  tilegather_kernel<RW, VAL><<<dim3(ids), kTgThreads, lds, ctx->stream>>>(
+  tilegather_kernel<RW, VAL, LONG><<<dim3(ids), kTgThreads, lds, ctx->stream>>>(
       in_rows, sh.nblocks, sh.g, sh.q, sh.ncg, n_rhs, l32, sh.h32, tiles, X, x_cs, part);
   return RLA_OK;
 }
 
-template <bool VAL>
+template <bool VAL, bool LONG>
 static int launch_tile_rw(rla_ctx* ctx, const TileShape& sh, unsigned g, int64_t in_rows,
                           int64_t n_rhs, int l32, const uint32_t* tiles, const float* X,
                           int64_t x_cs, float* part) {
   switch (sh.rw) {
-    case 64: return launch_tile<64, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
-    case 32: return launch_tile<32, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
-    case 16: return launch_tile<16, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
-    case 8: return launch_tile<8, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
-    default: return launch_tile<4, VAL>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 64:
+      return launch_tile<64, VAL, LONG>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 32:
+      return launch_tile<32, VAL, LONG>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 16:
+      return launch_tile<16, VAL, LONG>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    case 8:
+      return launch_tile<8, VAL, LONG>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+    default:
+      return launch_tile<4, VAL, LONG>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
   }
 }
 
@@ -1756,8 +1762,14 @@ int sparse_apply_tile_part(rla_ctx* ctx, int64_t rows, int64_t in_rows, int vals
   }
   const unsigned g = static_cast<unsigned>(ids);
   const int l32 = static_cast<int>(rec_words);
-  rc = vals ? launch_tile_rw<true>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part)
-            : launch_tile_rw<false>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+  const bool lng = l32 > 8 * kTgThreads;  // more than 2 uint4 per thread to stage
+  if (vals)
+    rc = lng ? launch_tile_rw<true, true>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part)
+             : launch_tile_rw<true, false>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part);
+  else
+    rc = lng ? launch_tile_rw<false, true>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs, part)
+             : launch_tile_rw<false, false>(ctx, sh, g, in_rows, n_rhs, l32, tiles, X, x_cs,
+                                            part);
   if (rc) return rc;
   RLA_LAUNCH_CHECK("tilegather");
   tile_combine_kernel<<<dim3(unsigned(ceil_div(rows * sh.ncg * 64, 256))), 256, 0,
